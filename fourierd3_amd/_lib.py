@@ -1,0 +1,178 @@
+"""ctypes binding of ``libfd3hip.so`` (the C ABI declared in ``include/fd3.h``).
+
+The library is built in-tree by ``python -m fourierd3_amd.build``.  Loading
+fails loudly when it is missing: there is no fallback implementation.
+Status codes map to the exception types the reference raises for the same
+conditions (ValueError for bad arguments, NotImplementedError for options
+that are not built, RuntimeError for HIP failures).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libfd3hip.so"
+
+FD3_OK = 0
+FD3_ERR_ARG = -1
+FD3_ERR_UNSUPPORTED = -2
+FD3_ERR_HIP = -3
+FD3_ERR_WORKSPACE = -4
+
+FD3_INTERP_P3M = 0
+FD3_INTERP_LAGRANGE = 1
+FD3_DTYPE_F32 = 0
+FD3_DTYPE_F64 = 1
+FD3_NEED_GRAD = 1
+FD3_NEED_PARAMS = 2
+
+# Every symbol include/fd3.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "fd3_last_error", "fd3_version", "fd3_profile_enable", "fd3_profile_report",
+    "fd3_eval_edges_workspace", "fd3_eval_edges",
+    "fd3_eval_celllist_workspace", "fd3_eval_celllist",
+    "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
+    "fd3_spread", "fd3_gather",
+    "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
+    "fd3_green_host", "fd3_interp_weights_host",
+)
+
+
+class Model(C.Structure):
+    _fields_ = [
+        ("n_species", C.c_int32), ("n_rank", C.c_int32),
+        ("rcov", C.c_void_p), ("cnref", C.c_void_p), ("vq", C.c_void_p), ("eigs", C.c_void_p),
+        ("qq", C.c_void_p), ("cncorr", C.c_void_p),
+        ("s6", C.c_double), ("s8", C.c_double), ("a1", C.c_double), ("a2", C.c_double),
+        ("r_cut", C.c_double),
+    ]
+
+
+class Geom(C.Structure):
+    _fields_ = [
+        ("cell", C.c_double * 9), ("ns", C.c_int32 * 3), ("order", C.c_int32), ("interp", C.c_int32),
+    ]
+
+
+class Outputs(C.Structure):
+    _fields_ = [
+        ("energy", C.c_void_p), ("grad_pos", C.c_void_p), ("grad_shifts", C.c_void_p),
+        ("grad_cell", C.c_void_p), ("virial", C.c_void_p), ("grad_params", C.c_void_p),
+        ("cn", C.c_void_p),
+    ]
+
+
+_lib = None
+
+_P = C.c_void_p
+_SIG = {
+    "fd3_last_error": (C.c_char_p, []),
+    "fd3_version": (C.c_int, []),
+    "fd3_profile_enable": (C.c_int, [C.c_int]),
+    "fd3_profile_report": (C.c_int, [C.c_int, C.c_char_p, _P, _P, C.POINTER(C.c_int)]),
+    "fd3_eval_edges_workspace": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, C.c_int64, C.c_int,
+                                           C.POINTER(C.c_size_t)]),
+    "fd3_eval_edges": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, _P, C.c_int, _P, C.c_int64, _P, _P,
+                                 _P, C.c_int, C.c_int, _P, C.c_size_t, C.POINTER(Outputs), _P]),
+    "fd3_eval_celllist_workspace": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, C.c_int,
+                                              C.POINTER(C.c_size_t)]),
+    "fd3_eval_celllist": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, _P, C.c_int, _P, C.c_int, _P,
+                                    C.c_size_t, C.POINTER(Outputs), _P]),
+    "fd3_neighbours_workspace": (C.c_int, [C.c_int64, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_size_t)]),
+    "fd3_neighbours_count": (C.c_int, [C.c_int64, _P, C.c_int, C.POINTER(C.c_double), C.c_double, _P, C.c_size_t,
+                                       _P, _P]),
+    "fd3_neighbours_fill": (C.c_int, [C.c_int64, _P, C.c_int, C.POINTER(C.c_double), C.c_double, _P, C.c_size_t,
+                                      C.c_int64, _P, _P, _P, _P]),
+    "fd3_spread": (C.c_int, [C.POINTER(Geom), C.c_int32, C.c_int32, C.c_int64, _P, C.c_int, _P, _P, _P, _P]),
+    "fd3_gather": (C.c_int, [C.POINTER(Geom), C.c_int32, C.c_int32, C.c_int64, _P, C.c_int, _P, _P, _P, _P]),
+    "fd3_fft_workspace": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.POINTER(C.c_size_t)]),
+    "fd3_fft_r2c": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, _P, _P, _P, C.c_size_t, _P]),
+    "fd3_fft_c2r": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, _P, _P, _P, C.c_size_t, _P]),
+    "fd3_green_host": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, _P, _P, _P]),
+    "fd3_interp_weights_host": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, _P, _P, _P]),
+}
+
+
+def lib():
+    """Load (once) and return the ctypes handle; raises if the build is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -m fourierd3_amd.build` "
+                "(there is no non-HIP fallback)")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIG.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = h
+    return _lib
+
+
+def check(status: int) -> None:
+    if status == FD3_OK:
+        return
+    msg = lib().fd3_last_error().decode(errors="replace")
+    if status == FD3_ERR_ARG:
+        raise ValueError(msg)
+    if status == FD3_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if status == FD3_ERR_WORKSPACE:
+        raise RuntimeError("workspace too small: " + msg)
+    raise RuntimeError(msg)
+
+
+def ptr(t) -> int | None:
+    """Device (or host) address of a tensor, None for None."""
+    return None if t is None else t.data_ptr()
+
+
+def dtype_code(t) -> int:
+    import torch
+    if t.dtype == torch.float64:
+        return FD3_DTYPE_F64
+    if t.dtype == torch.float32:
+        return FD3_DTYPE_F32
+    raise ValueError(f"unsupported dtype {t.dtype}; expected float32 or float64")
+
+
+def make_geom(cell_A, ns, order: int, interp: int) -> Geom:
+    g = Geom()
+    flat = [float(v) for v in list(cell_A.reshape(-1))]
+    if len(flat) != 9:
+        raise ValueError(f"cell of shape {tuple(cell_A.shape)} should be of shape (3, 3)")
+    for i, v in enumerate(flat):
+        g.cell[i] = v
+    for i in range(3):
+        g.ns[i] = int(ns[i])
+    g.order = int(order)
+    g.interp = int(interp)
+    return g
+
+
+def cell_array(cell_A) -> "C.Array":
+    arr = (C.c_double * 9)()
+    for i, v in enumerate([float(v) for v in list(cell_A.reshape(-1))]):
+        arr[i] = v
+    return arr
+
+
+def profile_enable(on: bool = True) -> None:
+    check(lib().fd3_profile_enable(1 if on else 0))
+
+
+def profile_report(max_entries: int = 64) -> dict:
+    """{kernel name: (total ms, launches)} since profile_enable; synchronises."""
+    import numpy as np
+    names = C.create_string_buffer(64 * max_entries)
+    ms = np.zeros(max_entries, np.float64)
+    cnt = np.zeros(max_entries, np.int64)
+    n = C.c_int()
+    check(lib().fd3_profile_report(max_entries, names, ms.ctypes.data, cnt.ctypes.data, C.byref(n)))
+    raw = names.raw
+    out = {}
+    for i in range(n.value):
+        nm = raw[64 * i: 64 * i + 64].split(b"\0", 1)[0].decode()
+        out[nm] = (float(ms[i]), int(cnt[i]))
+    return out

@@ -1,0 +1,286 @@
+"""``FourierD3`` torch module backed by the gfx950 HIP library.
+
+Mirrors the reference module ``FourierD3`` (``src/fourierd3/core.py:14-583``):
+same constructor arguments, buffers (``species``, ``cell`` and ``volume`` in
+Bohr, ``eigs``, ``eigvecs``, ``v_q_reshaped``, ``rcov``, ``cnref``,
+``selfcont``), ``_update_cell(cell)`` and
+``forward(positions, edge_index, shifts) -> energy [Hartree]``.
+
+All arithmetic of the forward pass and of its derivatives runs in the HIP
+library (``libfd3hip.so``) through ``fd3_eval_edges``: the energy and, when
+any input requires grad, its exact gradients with respect to positions,
+shifts, the cell passed to ``_update_cell`` and (optionally) the BJ
+parameters, from hand-written adjoint kernels.  ``torch.autograd`` sees one
+``Function`` whose backward scales those gradients by the incoming gradient.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import List, Optional
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import tables as T
+
+A2B = T.ANGSTROM_TO_BOHR
+
+
+def _det3(m: torch.Tensor) -> torch.Tensor:
+    """Cofactor determinant (``src/fourierd3/utils.py:130-139``)."""
+    c00 = m[1, 1] * m[2, 2] - m[1, 2] * m[2, 1]
+    c10 = m[1, 2] * m[2, 0] - m[1, 0] * m[2, 2]
+    c20 = m[1, 0] * m[2, 1] - m[1, 1] * m[2, 0]
+    return m[0, 0] * c00 + m[0, 1] * c10 + m[0, 2] * c20
+
+
+class _EdgesEnergy(torch.autograd.Function):
+    """E(positions, shifts, cell, params) with gradients from the adjoint kernels."""
+
+    @staticmethod
+    def forward(ctx, positions, shifts, cell_A, params, model, edge_index):
+        need_grad = any(ctx.needs_input_grad[:3])
+        need_params = bool(ctx.needs_input_grad[3])
+        res = model._run_edges(positions, edge_index, shifts, cell_A, need_grad or need_params, need_params)
+        ctx.save_for_backward(res["grad_pos"], res["grad_shifts"], res["grad_cell"], res["grad_params"])
+        ctx.dtypes = (positions.dtype, shifts.dtype, cell_A.dtype, params.dtype)
+        ctx.param_device = params.device
+        return res["energy"][0].to(model.dtype)
+
+    @staticmethod
+    def backward(ctx, g):
+        gpos, gsh, gcell, gpar = ctx.saved_tensors
+        dp, ds, dc, dpr = ctx.dtypes
+        g64 = g.to(torch.float64)
+        out_pos = (gpos * g64).to(dp) if ctx.needs_input_grad[0] else None
+        out_sh = (gsh * g64).to(ds) if ctx.needs_input_grad[1] else None
+        out_cell = (gcell * g64).to(dc) if ctx.needs_input_grad[2] else None
+        out_par = (gpar.to(g64.device) * g64).to(dpr).to(ctx.param_device) if ctx.needs_input_grad[3] else None
+        return out_pos, out_sh, out_cell, out_par, None, None
+
+
+class FourierD3(torch.nn.Module):
+    """DFT-D3(BJ) dispersion energy by reciprocal-space summation (SPME/PME).
+
+    Args mirror ``src/fourierd3/core.py:50-68``.  ``method`` is 'spme'
+    (P3M B-splines, 1..5 nodes) or 'pme' (Lagrange, 2..7 nodes); ``cnfunc``
+    is 'smooth_cut'.  The C6 reference table is the synthetic stand-in of
+    ``fourierd3_amd.tables`` unless ``FD3_C6REF`` points at a real one.
+    ``ns_mesh`` (extension) overrides the mesh rule.
+    """
+
+    def __init__(self, species: List, cell, pbc: Optional[torch.Tensor] = None, mesh_spacing: float = 1.2,
+                 c6tol: float = 1, xcfunc: str = "pbe",
+                 device=torch.device("cuda" if torch.cuda.is_available() else "cpu"),
+                 dtype=torch.float32, method: str = "spme", params=None, cnfunc="smooth_cut",
+                 interpolation_nodes: int = 5, k_cutoff: float = 10.0, r_cut: float = 6.0,
+                 cncorr: Optional[torch.Tensor] = None, verbose=True, ns_mesh=None) -> None:
+        super().__init__()
+        device = torch.device(device)
+        self.device = device
+        self.dtype = dtype
+        self.method = method
+        self.mesh_spacing = mesh_spacing
+        self.interpolation_nodes = int(interpolation_nodes)
+        self.k_cutoff = k_cutoff
+        self.xcfunc = xcfunc
+        self.cnfunc = cnfunc
+        if pbc is not None:
+            assert bool(torch.as_tensor(pbc).all()), "particle-mesh only supports 3d pbc"
+        if verbose:
+            print("Assuming 3D PBC are satisfied")
+        if method not in ("spme", "pme", "ewald"):
+            raise ValueError(f"unknown method '{method}'; expected 'spme', 'pme' or 'ewald'")
+        if method == "ewald":
+            raise NotImplementedError("method='ewald' has no HIP implementation in this build; use 'spme' or 'pme'")
+        if cnfunc != "smooth_cut":
+            raise NotImplementedError(f"cnfunc='{cnfunc}' is not built; only 'smooth_cut' is")
+
+        z = np.asarray(species, dtype=np.int64)
+        uniq = np.unique(z)                                         # core.py:104
+        smap = {int(s): i for i, s in enumerate(uniq)}
+        compact = np.array([smap[int(s)] for s in z], dtype=np.int64)
+        self.species_unique = uniq
+        self.register_buffer("species", torch.tensor(compact, dtype=torch.long, device=device))
+        self._species_i32 = torch.tensor(compact, dtype=torch.int32, device=device)
+        self.n_species = len(uniq)
+
+        self.r_cut = r_cut * A2B                                      # core.py:121 (Bohr)
+        self._r_cut_A = float(r_cut)
+        self.register_buffer("angstrom_to_bohr", torch.tensor(A2B, dtype=dtype, device=device))
+
+        if params is not None:
+            prm = [float(v) for v in (params.tolist() if torch.is_tensor(params) else params)]
+        elif xcfunc == "pbe":
+            prm = list(T.PBE_PARAMS)
+        else:
+            raise NotImplementedError
+        # host-resident so that reading the values never synchronises the device
+        self.params = torch.tensor(prm, dtype=torch.float64)
+
+        eigs, vecs = T.decomp(uniq, c6tol, verbose)                   # core.py:139
+        self.register_buffer("eigs", torch.tensor(eigs, dtype=dtype, device=device))
+        self.register_buffer("eigvecs", torch.tensor(vecs, dtype=dtype, device=device))
+        self.n_rank = vecs.shape[1]
+        self.register_buffer("v_q_reshaped", self.eigvecs.view(self.n_species, 7, self.n_rank))
+        st = T.species_tables(uniq, prm)
+        self.register_buffer("rcov", torch.tensor(st["rcov"], dtype=dtype, device=device))
+        self.register_buffer("cnref", torch.tensor(st["cnref"], dtype=dtype, device=device))
+        self.register_buffer("selfcont", torch.tensor(st["selfcont"], dtype=dtype, device=device).view(-1, 1))
+        self.register_buffer("factor_cn", torch.tensor(4.0 / 3.0, dtype=dtype, device=device))
+        self.register_buffer("logit_scale", torch.tensor(-4.0, dtype=dtype, device=device))
+        # float32 tables the kernels read
+        f32 = dict(dtype=torch.float32, device=device)
+        self._t_rcov = torch.tensor(st["rcov"], **f32)
+        self._t_cnref = torch.tensor(st["cnref"], **f32).contiguous()
+        self._t_vq = torch.tensor(vecs, **f32).view(self.n_species, 7, self.n_rank).contiguous()
+        self._t_eigs = torch.tensor(eigs, **f32)
+        self._t_qq = torch.tensor(st["QQ"], **f32).contiguous()
+        if cncorr is not None:
+            cc = torch.as_tensor(cncorr)
+            self.cncorr = cc.to(device)
+            self._t_cncorr = cc.to(**f32).contiguous()
+        else:
+            self.cncorr = None
+            self._t_cncorr = None
+
+        cell_A = torch.as_tensor(np.asarray(cell, dtype=np.float64) if not torch.is_tensor(cell) else cell)
+        cell_A = cell_A.to(device=device, dtype=dtype)
+        cell_np = cell_A.detach().double().cpu().numpy()
+        if ns_mesh is not None:
+            self.ns_mesh = np.asarray(ns_mesh, dtype=np.int64)
+        else:
+            self.ns_mesh = T.get_ns_mesh(cell_np * A2B, mesh_spacing * A2B)   # core.py:186
+        if verbose:
+            print("Using mesh size", self.ns_mesh)
+        self._interp = L.FD3_INTERP_P3M if method == "spme" else L.FD3_INTERP_LAGRANGE
+        self._ws = None
+        self._update_cell(cell_A)
+
+    # ------------------------------------------------------------------
+    def _model_struct(self) -> L.Model:
+        m = L.Model()
+        m.n_species = self.n_species
+        m.n_rank = self.n_rank
+        m.rcov = L.ptr(self._t_rcov)
+        m.cnref = L.ptr(self._t_cnref)
+        m.vq = L.ptr(self._t_vq)
+        m.eigs = L.ptr(self._t_eigs)
+        m.qq = L.ptr(self._t_qq)
+        m.cncorr = L.ptr(self._t_cncorr)
+        p = [float(v) for v in self.params.detach().tolist()]
+        m.s6, m.s8, m.a1, m.a2 = p
+        m.r_cut = self._r_cut_A
+        return m
+
+    def _workspace(self, nbytes: int) -> torch.Tensor:
+        if self._ws is None or self._ws.numel() < nbytes:
+            self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    def _geom(self, cell_A_host: np.ndarray) -> L.Geom:
+        return L.make_geom(cell_A_host, self.ns_mesh, self.interpolation_nodes, self._interp)
+
+    def _update_cell(self, cell):
+        """``_update_cell`` (``src/fourierd3/core.py:244-267``): cell in Angstrom,
+        may require grad; the mesh size is not recomputed."""
+        cell = torch.as_tensor(cell)
+        if cell.shape != (3, 3):
+            raise ValueError(f"cell of shape {list(cell.shape)} should be of shape (3, 3)")
+        self._cell_A = cell
+        self.cell = cell * A2B
+        self.volume = torch.abs(_det3(self.cell))
+        self._cell_host = cell.detach().to(torch.float64).cpu().numpy().copy()
+
+    # ------------------------------------------------------------------
+    def _run_edges(self, positions, edge_index, shifts, cell_A, need_grad: bool, need_params: bool):
+        dev = self.device
+        pos = positions.detach()
+        if pos.device != dev:
+            raise ValueError(f"positions are on {pos.device}, the model on {dev}")
+        pos = pos.contiguous()
+        if pos.dim() != 2 or pos.shape[1] != 3:
+            raise ValueError(f"positions must be (n_atoms, 3), got {tuple(pos.shape)}")
+        n = pos.shape[0]
+        if n != self._species_i32.shape[0]:
+            raise ValueError(f"{n} positions for a model of {self._species_i32.shape[0]} atoms")
+        ei = edge_index.detach().to(device=dev, dtype=torch.long).contiguous()
+        if ei.dim() != 2 or ei.shape[0] != 2:
+            raise ValueError(f"edge_index must be (2, n_edges), got {tuple(ei.shape)}")
+        ne = ei.shape[1]
+        sh = shifts.detach().to(dev).contiguous()
+        if sh.shape != (ne, 3):
+            raise ValueError(f"shifts must be ({ne}, 3), got {tuple(sh.shape)}")
+        if pos.dtype not in (torch.float32, torch.float64):
+            pos = pos.double()
+        if sh.dtype not in (torch.float32, torch.float64):
+            sh = sh.double()
+        cell_host = cell_A.detach().to(torch.float64).cpu().numpy()
+        m = self._model_struct()
+        g = self._geom(cell_host)
+        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        nbytes = C.c_size_t()
+        L.check(L.lib().fd3_eval_edges_workspace(C.byref(m), C.byref(g), n, ne, flags, C.byref(nbytes)))
+        ws = self._workspace(nbytes.value)
+        f64 = dict(dtype=torch.float64, device=dev)
+        res = {
+            "energy": torch.empty(1, **f64),
+            "grad_pos": torch.empty((n, 3), **f64) if need_grad else torch.empty(0, **f64),
+            "grad_shifts": torch.empty((ne, 3), **f64) if need_grad else torch.empty(0, **f64),
+            "grad_cell": torch.empty((3, 3), **f64) if need_grad else torch.empty(0, **f64),
+            "virial": torch.empty((3, 3), **f64) if need_grad else torch.empty(0, **f64),
+            "grad_params": torch.empty(4, **f64) if need_params else torch.empty(0, **f64),
+            "cn": torch.empty(n, dtype=torch.float32, device=dev),
+        }
+        out = L.Outputs()
+        out.energy = L.ptr(res["energy"])
+        out.cn = L.ptr(res["cn"])
+        if need_grad:
+            out.grad_pos = L.ptr(res["grad_pos"])
+            out.grad_shifts = L.ptr(res["grad_shifts"])
+            out.grad_cell = L.ptr(res["grad_cell"])
+            out.virial = L.ptr(res["virial"])
+        if need_params:
+            out.grad_params = L.ptr(res["grad_params"])
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        L.check(L.lib().fd3_eval_edges(
+            C.byref(m), C.byref(g), n, L.ptr(pos), L.dtype_code(pos), L.ptr(self._species_i32), ne,
+            L.ptr(ei[0]), L.ptr(ei[1]), L.ptr(sh), L.dtype_code(sh), flags,
+            L.ptr(ws), ws.numel(), C.byref(out), stream))
+        return res
+
+    def forward(self, positions: torch.Tensor, edge_index: torch.Tensor = None,
+                shifts: torch.Tensor = None) -> torch.Tensor:
+        """Energy in Hartree of ``positions`` (Angstrom) with the CN neighbour
+        list ``edge_index`` (2, E) and lattice ``shifts`` (E, 3) in Angstrom
+        (``src/fourierd3/core.py:480-583``)."""
+        if edge_index is None or shifts is None:
+            raise ValueError("cnfunc='smooth_cut' needs edge_index and shifts")
+        return _EdgesEnergy.apply(positions, shifts, self._cell_A, self.params, self, edge_index)
+
+    # ------------------------------------------------------------------
+    def compute_cn_smooth(self, positions: torch.Tensor, edge_index: torch.Tensor, shifts: torch.Tensor,
+                          recalc: bool = False) -> torch.Tensor:
+        """Coordination numbers (``src/fourierd3/core.py:323-386``) from
+        positions and shifts in Bohr, computed by the HIP CN kernel.  With
+        ``recalc`` the cncorr correction is skipped, as in the reference."""
+        pos_A = positions.detach() / A2B
+        sh_A = shifts.detach() / A2B
+        saved = self._t_cncorr
+        if recalc:
+            self._t_cncorr = None
+        try:
+            res = self._run_edges(pos_A, edge_index, sh_A, self._cell_A, False, False)
+        finally:
+            self._t_cncorr = saved
+        return res["cn"].to(self.dtype)
+
+    def compute_c6_weights(self, cn: torch.Tensor) -> torch.Tensor:
+        """Softmax weights L_ip (``src/fourierd3/core.py:449-478``); a small
+        convenience helper on (N, 7) values, not used by ``forward``."""
+        refs = self.cnref[self.species]
+        logits = self.logit_scale * torch.square(cn.unsqueeze(1) - refs)
+        logits = logits.masked_fill(refs == -1, float("-inf"))
+        return torch.softmax(logits, dim=1)

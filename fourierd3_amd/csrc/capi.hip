@@ -1,0 +1,723 @@
+// C ABI of libfd3hip.so: validation, workspace layout, the two whole-path
+// pipelines and the component entry points.  See include/fd3.h.
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "fd3_internal.h"
+#include "fft_plan.h"
+#include "mesh_kernels.h"
+#include "neigh_kernels.h"
+
+namespace fd3 {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error("%s: %s", what, hipGetErrorString(e));
+  return FD3_ERR_HIP;
+}
+
+// ------------------------------------------------------------- profiling ---
+struct ProfRec {
+  const char* name;
+  hipEvent_t start, stop;
+};
+static bool g_prof = false;
+static std::vector<ProfRec> g_prof_recs;
+static std::vector<std::pair<const char*, hipEvent_t>> g_prof_open;
+
+bool prof_enabled() { return g_prof; }
+
+void prof_record(const char* name, hipStream_t s, bool start) {
+  hipEvent_t ev;
+  if (hipEventCreate(&ev) != hipSuccess) return;
+  hipEventRecord(ev, s);
+  if (start) {
+    g_prof_open.push_back({name, ev});
+  } else if (!g_prof_open.empty()) {
+    g_prof_recs.push_back({g_prof_open.back().first, g_prof_open.back().second, ev});
+    g_prof_open.pop_back();
+  } else {
+    hipEventDestroy(ev);
+  }
+}
+
+static void prof_clear() {
+  for (auto& r : g_prof_recs) {
+    hipEventDestroy(r.start);
+    hipEventDestroy(r.stop);
+  }
+  for (auto& o : g_prof_open) hipEventDestroy(o.second);
+  g_prof_recs.clear();
+  g_prof_open.clear();
+}
+
+void prof_set(bool on) {
+  prof_clear();
+  g_prof = on;
+}
+
+int prof_report(int max_entries, char* names, double* total_ms, int64_t* counts, int* n_entries) {
+  std::vector<std::string> keys;
+  std::vector<double> ms;
+  std::vector<int64_t> cnt;
+  for (auto& r : g_prof_recs) {
+    FD3_HIP(hipEventSynchronize(r.stop));
+    float t = 0.f;
+    FD3_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+    size_t k = 0;
+    while (k < keys.size() && keys[k] != r.name) ++k;
+    if (k == keys.size()) {
+      keys.push_back(r.name);
+      ms.push_back(0.0);
+      cnt.push_back(0);
+    }
+    ms[k] += t;
+    cnt[k] += 1;
+  }
+  const int n = (int)std::min<size_t>(keys.size(), (size_t)max_entries);
+  for (int i = 0; i < n; ++i) {
+    std::strncpy(names + 64 * i, keys[i].c_str(), 63);
+    names[64 * i + 63] = 0;
+    total_ms[i] = ms[i];
+    counts[i] = cnt[i];
+  }
+  *n_entries = n;
+  prof_clear();
+  return FD3_OK;
+}
+
+int make_geom(const fd3_geom* g, Geom* out) {
+  if (!g) {
+    set_error("geometry is NULL");
+    return FD3_ERR_ARG;
+  }
+  double c[9];
+  for (int i = 0; i < 9; ++i) {
+    c[i] = g->cell[i] * kA2B;
+    if (!std::isfinite(c[i])) {
+      set_error("cell entry %d is not finite", i);
+      return FD3_ERR_ARG;
+    }
+  }
+  const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
+                     c[2] * (c[3] * c[7] - c[4] * c[6]);
+  if (!(fabs(det) > 1e-12)) {
+    set_error("cell is singular (det = %g Bohr^3)", det);
+    return FD3_ERR_ARG;
+  }
+  double inv[9];
+  inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
+  inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
+  inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
+  inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
+  inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
+  inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
+  inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
+  inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
+  inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
+  for (int i = 0; i < 9; ++i) {
+    out->cell[i] = c[i];
+    out->inv[i] = inv[i];
+  }
+  for (int i = 0; i < 3; ++i)
+    for (int d = 0; d < 3; ++d) out->recip[i * 3 + d] = 2.0 * kPi * inv[d * 3 + i];
+  out->volume = fabs(det);
+  out->nx = g->ns[0];
+  out->ny = g->ns[1];
+  out->nz = g->ns[2];
+  out->order = g->order;
+  out->interp = g->interp;
+  if (g->interp != FD3_INTERP_P3M && g->interp != FD3_INTERP_LAGRANGE) {
+    set_error("unknown interpolation %d", g->interp);
+    return FD3_ERR_ARG;
+  }
+  if (g->interp == FD3_INTERP_P3M && (g->order < 1 || g->order > 5)) {
+    set_error("P3M (spme) interpolation supports 1..5 nodes, got %d", g->order);
+    return FD3_ERR_UNSUPPORTED;
+  }
+  if (g->interp == FD3_INTERP_LAGRANGE && (g->order < 2 || g->order > kMaxOrder)) {
+    set_error("Lagrange (pme) interpolation supports 2..%d nodes, got %d", kMaxOrder, g->order);
+    return FD3_ERR_UNSUPPORTED;
+  }
+  for (int i = 0; i < 3; ++i) {
+    const int n = g->ns[i];
+    if (n < 2 || n > 1024 || (n & (n - 1))) {
+      set_error("mesh size %d on axis %d is not a power of two in [2, 1024]", n, i);
+      return FD3_ERR_UNSUPPORTED;
+    }
+    if (n < g->order) {
+      set_error("mesh size %d on axis %d is smaller than the %d interpolation nodes", n, i, g->order);
+      return FD3_ERR_ARG;
+    }
+  }
+  return FD3_OK;
+}
+
+static int check_model(const fd3_model* m) {
+  if (!m) {
+    set_error("model is NULL");
+    return FD3_ERR_ARG;
+  }
+  if (m->n_species < 1 || m->n_rank < 1) {
+    set_error("n_species and n_rank must be positive, got %d and %d", m->n_species, m->n_rank);
+    return FD3_ERR_ARG;
+  }
+  if (m->n_species > 32 || m->n_rank > 32) {
+    set_error("this build supports up to 32 species and rank 32, got %d and %d", m->n_species, m->n_rank);
+    return FD3_ERR_UNSUPPORTED;
+  }
+  if (!m->rcov || !m->cnref || !m->vq || !m->eigs || !m->qq) {
+    set_error("model table pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  if (!(m->r_cut > 0.0)) {
+    set_error("r_cut must be positive, got %g", m->r_cut);
+    return FD3_ERR_ARG;
+  }
+  return FD3_OK;
+}
+
+// ------------------------------------------------------------ finalize ---
+struct FinalArgs {
+  const double* ks; int nks;     // (nks, 16)
+  const double* c6; int nc6;     // (nc6, 8)
+  const double* ga; int nga;     // (nga, 16) or null
+  const double* cnv; int ncnv;   // (ncnv, 8) or null
+  double volume;
+  double inv[9];
+  double* energy;
+  double* virial;
+  double* grad_cell;
+  double* grad_params;
+};
+
+__device__ inline double block_sum(double v, double* sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ void __launch_bounds__(256) k_finalize(FinalArgs a) {
+  __shared__ double sh[256];
+  __shared__ double tot[40];
+  // ks 0..10 -> tot[0..10]; c6 0..4 -> tot[11..15]; ga 0..8 -> tot[16..24]; cnv 0..5 -> tot[25..30]
+  for (int k = 0; k < 11; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.nks; b += blockDim.x) s += a.ks[(size_t)b * 16 + k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) tot[k] = s;
+  }
+  for (int k = 0; k < 5; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < a.nc6; b += blockDim.x) s += a.c6[(size_t)b * 8 + k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) tot[11 + k] = s;
+  }
+  for (int k = 0; k < 9; ++k) {
+    double s = 0.0;
+    if (a.ga)
+      for (int b = threadIdx.x; b < a.nga; b += blockDim.x) s += a.ga[(size_t)b * 16 + k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) tot[16 + k] = s;
+  }
+  for (int k = 0; k < 6; ++k) {
+    double s = 0.0;
+    if (a.cnv)
+      for (int b = threadIdx.x; b < a.ncnv; b += blockDim.x) s += a.cnv[(size_t)b * 8 + k];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) tot[25 + k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double om = a.volume;
+  const double ekp = -tot[0] / (2.0 * om);  // reciprocal-space energy
+  if (a.energy) a.energy[0] = ekp + 0.5 * tot[11];
+  // symmetric 3x3 from (xx, yy, zz, xy, xz, yz)
+  auto sym = [](const double* v, double* m) {
+    m[0] = v[0]; m[4] = v[1]; m[8] = v[2];
+    m[1] = m[3] = v[3]; m[2] = m[6] = v[4]; m[5] = m[7] = v[5];
+  };
+  double kv[9], cv[9];
+  sym(&tot[1], kv);
+  sym(&tot[25], cv);
+  if (a.virial) {
+    for (int i = 0; i < 9; ++i) a.virial[i] = kv[i] / (2.0 * om) + cv[i];
+    for (int i = 0; i < 3; ++i) a.virial[4 * i] -= ekp;
+  }
+  if (a.grad_cell) {
+    // dE/dC_cd = -sum_i f_ic g_id + (1/2Om) (C^-T kv)_cd - E_k (C^-T)_cd   (Bohr), then per Angstrom
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) {
+        double v = -tot[16 + c * 3 + d];
+        double t = 0.0;
+        for (int q = 0; q < 3; ++q) t += a.inv[q * 3 + c] * kv[q * 3 + d];
+        v += t / (2.0 * om) - ekp * a.inv[d * 3 + c];
+        a.grad_cell[c * 3 + d] = v * kA2B;
+      }
+  }
+  if (a.grad_params)
+    for (int k = 0; k < 4; ++k) a.grad_params[k] = -tot[7 + k] / (2.0 * om) + 0.5 * tot[12 + k];
+}
+
+// ------------------------------------------------------------ pipeline ---
+struct Common {
+  FftPlan plan;
+  Geom geom;
+  FftTables tab;
+  float* buf1;   // rho / spectrum / dE/drho
+  float2* buf2;  // intermediate A
+  float* cn_raw;
+  float* charges;
+  float* dEdcn;
+  double* gpos;
+  double *p_ks, *p_c6, *p_ga, *p_cn;
+  int n_c6, n_ga, n_cn;
+};
+
+static void carve_common(Arena& ar, Common& c, int64_t n, int r, int64_t n_cn_items) {
+  fft_tables_carve(c.plan, ar, &c.tab);
+  const size_t b1 = std::max(c.plan.mesh_real(), 2 * c.plan.mesh_complex());
+  c.buf1 = ar.take<float>(b1);
+  c.buf2 = ar.take<float2>(c.plan.mesh_complex());
+  c.cn_raw = ar.take<float>(std::max<int64_t>(n, 1));
+  c.charges = ar.take<float>(std::max<int64_t>(n * r, 1));
+  c.dEdcn = ar.take<float>(std::max<int64_t>(n, 1));
+  c.gpos = ar.take<double>(std::max<int64_t>(3 * n, 3));
+  c.n_c6 = c6_blocks(n);
+  c.n_ga = gather_blocks(n, c.geom.order);
+  c.n_cn = cn_blocks(n_cn_items);
+  c.p_ks = ar.take<double>((size_t)c.plan.ks_blocks * 16);
+  c.p_c6 = ar.take<double>((size_t)c.n_c6 * 8);
+  c.p_ga = ar.take<double>((size_t)c.n_ga * 16);
+  c.p_cn = ar.take<double>((size_t)c.n_cn * 8);
+}
+
+static int setup_common(const fd3_model* m, const fd3_geom* g, Common& c) {
+  int rc;
+  if ((rc = check_model(m))) return rc;
+  if ((rc = make_geom(g, &c.geom))) return rc;
+  if ((rc = fft_plan_make(g->ns, m->n_species, m->n_rank, &c.plan))) return rc;
+  return FD3_OK;
+}
+
+// Everything after the CN: C6 chain, spread, FFTs, k-space, adjoint.
+// pos4/spec_s in slot order; slot_atom null for the identity.
+static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4* pos4, const int32_t* spec_s,
+                         const int32_t* slot_atom, int flags, const fd3_outputs* out, hipStream_t s) {
+  int rc;
+  const int r = m->n_rank, nS = m->n_species;
+  const bool grad = flags & FD3_NEED_GRAD;
+  if ((rc = fft_tables_init(c.plan, c.tab, c.geom.order, c.geom.interp == FD3_INTERP_P3M, s))) return rc;
+  C6Args ca;
+  ca.n = n; ca.nS = nS; ca.r = r; ca.cn_raw = c.cn_raw; ca.spec_s = spec_s; ca.slot_atom = slot_atom;
+  ca.cnref = m->cnref; ca.vq = m->vq; ca.eigs = m->eigs; ca.qq = m->qq; ca.cncorr = m->cncorr;
+  ca.s6 = m->s6; ca.s8 = m->s8; ca.a1 = m->a1; ca.a2 = m->a2;
+  ca.charges = c.charges; ca.cn_out = out ? out->cn : nullptr; ca.partials = c.p_c6;
+  if ((rc = launch_c6_fwd(ca, s))) return rc;
+  if ((rc = launch_fill_f32(c.buf1, c.plan.mesh_real(), s))) return rc;
+  SpreadArgs sa;
+  sa.g = mesh_geom(c.geom); sa.n = n; sa.r = r; sa.pos4 = pos4; sa.raw_pos = nullptr; sa.raw_dtype = 0;
+  sa.spec_s = spec_s; sa.charges = c.charges; sa.rho = c.buf1;
+  if ((rc = launch_spread(sa, s))) return rc;
+  float2* spec = (float2*)c.buf1;
+  if ((rc = fft_forward_zy(c.plan, c.tab, c.buf1, c.buf2, spec, s))) return rc;
+  if ((rc = kspace_run(c.plan, c.tab, c.geom, spec, m->eigs, m->qq, m->s6, m->s8, m->a1, m->a2, grad ? 1 : 0,
+                       (flags & FD3_NEED_PARAMS) ? 1 : 0, c.p_ks, s)))
+    return rc;
+  if (grad) {
+    if ((rc = fft_inverse_yz(c.plan, c.tab, spec, c.buf2, c.buf1, s))) return rc;
+    GatherArgs ga;
+    ga.g = mesh_geom(c.geom); ga.n = n; ga.nS = nS; ga.r = r; ga.pos4 = pos4; ga.raw_pos = nullptr; ga.raw_dtype = 0;
+    ga.spec_s = spec_s; ga.charges = c.charges; ga.mesh = c.buf1; ga.cn_raw = c.cn_raw; ga.cnref = m->cnref;
+    ga.vq = m->vq; ga.eigs = m->eigs; ga.qq = m->qq; ga.cncorr = m->cncorr;
+    ga.s6 = m->s6; ga.s8 = m->s8; ga.a1 = m->a1; ga.a2 = m->a2;
+    ga.gpos = c.gpos; ga.dEdcn = c.dEdcn; ga.dEdc = nullptr; ga.partials = c.p_ga;
+    if (n > 0 && (rc = launch_gather_bwd(ga, s))) return rc;
+  }
+  return FD3_OK;
+}
+
+static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, const fd3_outputs* out,
+                        hipStream_t s) {
+  FinalArgs fa;
+  fa.ks = c.p_ks; fa.nks = c.plan.ks_blocks;
+  fa.c6 = c.p_c6; fa.nc6 = c.n_c6;
+  fa.ga = grad ? c.p_ga : nullptr; fa.nga = c.n_ga;
+  fa.cnv = (grad && have_cnv) ? c.p_cn : nullptr; fa.ncnv = c.n_cn;
+  fa.volume = c.geom.volume;
+  for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
+  fa.energy = out->energy;
+  fa.virial = grad ? out->virial : nullptr;
+  fa.grad_cell = grad ? out->grad_cell : nullptr;
+  fa.grad_params = params ? out->grad_params : nullptr;
+  ProfScope _ps("k_finalize", s);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, fa);
+  FD3_LAUNCH_CHECK("k_finalize");
+  return FD3_OK;
+}
+
+}  // namespace fd3
+
+using namespace fd3;
+
+extern "C" {
+
+const char* fd3_last_error(void) { return g_err.c_str(); }
+int fd3_version(void) { return 1; }
+
+int fd3_profile_enable(int on) {
+  fd3::prof_set(on != 0);
+  return FD3_OK;
+}
+
+int fd3_profile_report(int max_entries, char* names, double* total_ms, int64_t* counts, int* n_entries) {
+  return fd3::prof_report(max_entries, names, total_ms, counts, n_entries);
+}
+
+// ------------------------------------------------------------ edges path ---
+static int edges_layout(const fd3_model* m, const fd3_geom* g, int64_t n, int64_t ne, Common& c, Arena& ar,
+                        double4** pos4, int32_t** spec_s) {
+  int rc;
+  if ((rc = setup_common(m, g, c))) return rc;
+  if (n < 0 || ne < 0) {
+    set_error("negative atom or edge count");
+    return FD3_ERR_ARG;
+  }
+  carve_common(ar, c, n, m->n_rank, ne);
+  *pos4 = ar.take<double4>(std::max<int64_t>(n, 1));
+  *spec_s = ar.take<int32_t>(std::max<int64_t>(n, 1));
+  return FD3_OK;
+}
+
+int fd3_eval_edges_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, int64_t n_edges, int flags,
+                             size_t* bytes) {
+  (void)flags;
+  Common c;
+  Arena ar;
+  double4* p4;
+  int32_t* sp;
+  int rc = edges_layout(m, g, n_atoms, n_edges, c, ar, &p4, &sp);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n, const void* positions, int pos_dtype,
+                   const int32_t* species, int64_t ne, const int64_t* edge_src, const int64_t* edge_dst,
+                   const void* shifts, int shifts_dtype, int flags, void* workspace, size_t ws_bytes,
+                   const fd3_outputs* out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Common c;
+  Arena ar(workspace, ws_bytes);
+  double4* pos4;
+  int32_t* spec_s;
+  int rc = edges_layout(m, g, n, ne, c, ar, &pos4, &spec_s);
+  if (rc) return rc;
+  if (!ar.ok() || !workspace) {
+    set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
+    return FD3_ERR_WORKSPACE;
+  }
+  if (!out) {
+    set_error("outputs are NULL");
+    return FD3_ERR_ARG;
+  }
+  if ((n > 0 && (!positions || !species)) || (ne > 0 && (!edge_src || !edge_dst || !shifts))) {
+    set_error("input pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  const bool grad = flags & FD3_NEED_GRAD;
+  if ((rc = launch_prep_pos(positions, pos_dtype, species, n, pos4, spec_s, s))) return rc;
+  if ((rc = launch_fill_f32(c.cn_raw, std::max<int64_t>(n, 1), s))) return rc;
+  CnArgs cna;
+  cna.rcov = m->rcov; cna.rcut = (float)(m->r_cut * kA2B); cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
+  cna.gpos = c.gpos; cna.partials = c.p_cn;
+  EdgeArgs ea;
+  ea.n_edges = ne; ea.src = edge_src; ea.dst = edge_dst; ea.shifts = shifts; ea.shifts_dtype = shifts_dtype;
+  if ((rc = launch_cn_edges(ea, cna, pos4, spec_s, s))) return rc;
+  if ((rc = run_mesh_path(m, c, n, pos4, spec_s, nullptr, flags, out, s))) return rc;
+  if (grad) {
+    if ((rc = launch_cn_bwd_edges(ea, cna, pos4, spec_s, out->grad_shifts, s))) return rc;
+    if (out->grad_pos && (rc = launch_unpermute(c.gpos, nullptr, n, out->grad_pos, s))) return rc;
+  }
+  return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, true, out, s);
+}
+
+// --------------------------------------------------------- cell-list path ---
+static int cells_layout(const fd3_model* m, const fd3_geom* g, int64_t n, Common& c, CellGrid& cg, CellBuffers& cb,
+                        Arena& ar) {
+  int rc;
+  if ((rc = setup_common(m, g, c))) return rc;
+  if (n < 0) {
+    set_error("negative atom count");
+    return FD3_ERR_ARG;
+  }
+  if ((rc = cell_grid_make(g->cell, m->r_cut, &cg))) return rc;
+  carve_common(ar, c, n, m->n_rank, n);
+  cell_carve(cg, std::max<int64_t>(n, 1), ar, &cb);
+  return FD3_OK;
+}
+
+int fd3_eval_celllist_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, int flags, size_t* bytes) {
+  (void)flags;
+  Common c;
+  CellGrid cg;
+  CellBuffers cb;
+  Arena ar;
+  int rc = cells_layout(m, g, n_atoms, c, cg, cb, ar);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const void* positions, int pos_dtype,
+                      const int32_t* species, int flags, void* workspace, size_t ws_bytes, const fd3_outputs* out,
+                      void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Common c;
+  CellGrid cg;
+  CellBuffers cb;
+  Arena ar(workspace, ws_bytes);
+  int rc = cells_layout(m, g, n, c, cg, cb, ar);
+  if (rc) return rc;
+  if (!ar.ok() || !workspace) {
+    set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
+    return FD3_ERR_WORKSPACE;
+  }
+  if (!out) {
+    set_error("outputs are NULL");
+    return FD3_ERR_ARG;
+  }
+  if (n > 0 && (!positions || !species)) {
+    set_error("input pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  const bool grad = flags & FD3_NEED_GRAD;
+  if ((rc = cell_build(cg, cb, n, positions, pos_dtype, species, s))) return rc;
+  const CellArgs ca = cell_args(cg, cb, n);
+  CnArgs cna;
+  cna.rcov = m->rcov; cna.rcut = (float)(m->r_cut * kA2B); cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
+  cna.gpos = c.gpos; cna.partials = c.p_cn;
+  if ((rc = launch_cn_cells(ca, cna, s))) return rc;
+  if ((rc = run_mesh_path(m, c, n, cb.pos4, cb.spec_s, cb.slot_atom, flags, out, s))) return rc;
+  if (grad) {
+    if (n > 0 && (rc = launch_cn_bwd_cells(ca, cna, s))) return rc;
+    if (out->grad_pos && (rc = launch_unpermute(c.gpos, cb.slot_atom, n, out->grad_pos, s))) return rc;
+  }
+  fd3_outputs o2 = *out;
+  o2.grad_cell = nullptr;  // the cell-list path reports the strain derivative only
+  return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, n > 0, &o2, s);
+}
+
+// ------------------------------------------------------------ neighbours ---
+static int nb_layout(int64_t n, const double cell[9], double r_cut, CellGrid& cg, CellBuffers& cb, Arena& ar,
+                     int32_t** count_atom, int32_t** offset) {
+  if (n < 0) {
+    set_error("negative atom count");
+    return FD3_ERR_ARG;
+  }
+  int rc;
+  if ((rc = cell_grid_make(cell, r_cut, &cg))) return rc;
+  cell_carve(cg, std::max<int64_t>(n, 1), ar, &cb);
+  *count_atom = ar.take<int32_t>(std::max<int64_t>(n, 1));
+  *offset = ar.take<int32_t>(n + 1);
+  return FD3_OK;
+}
+
+int fd3_neighbours_workspace(int64_t n, const double cell[9], double r_cut, size_t* bytes) {
+  CellGrid cg;
+  CellBuffers cb;
+  Arena ar;
+  int32_t *ca, *of;
+  int rc = nb_layout(n, cell, r_cut, cg, cb, ar, &ca, &of);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+static int nb_prepare(int64_t n, const void* pos, int dtype, const double cell[9], double r_cut, void* ws,
+                      size_t ws_bytes, int64_t* total, CellGrid& cg, CellBuffers& cb, int32_t** offset,
+                      hipStream_t s) {
+  Arena ar(ws, ws_bytes);
+  int32_t* count_atom;
+  int rc = nb_layout(n, cell, r_cut, cg, cb, ar, &count_atom, offset);
+  if (rc) return rc;
+  if (!ar.ok() || !ws) {
+    set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
+    return FD3_ERR_WORKSPACE;
+  }
+  if (n == 0) {
+    if (total) return launch_fill_i64(total, 1, s);
+    return FD3_OK;
+  }
+  if ((rc = cell_build(cg, cb, n, pos, dtype, nullptr, s))) return rc;
+  const CellArgs ca = cell_args(cg, cb, n);
+  return nb_count(ca, count_atom, *offset, total, s);
+}
+
+int fd3_neighbours_count(int64_t n, const void* pos, int dtype, const double cell[9], double r_cut, void* ws,
+                         size_t ws_bytes, int64_t* n_pairs, void* stream) {
+  CellGrid cg;
+  CellBuffers cb;
+  int32_t* offset;
+  return nb_prepare(n, pos, dtype, cell, r_cut, ws, ws_bytes, n_pairs, cg, cb, &offset, (hipStream_t)stream);
+}
+
+int fd3_neighbours_fill(int64_t n, const void* pos, int dtype, const double cell[9], double r_cut, void* ws,
+                        size_t ws_bytes, int64_t capacity, int64_t* src, int64_t* dst, int32_t* shifts,
+                        void* stream) {
+  CellGrid cg;
+  CellBuffers cb;
+  int32_t* offset;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = nb_prepare(n, pos, dtype, cell, r_cut, ws, ws_bytes, nullptr, cg, cb, &offset, s);
+  if (rc || n == 0) return rc;
+  return nb_fill(cell_args(cg, cb, n), offset, capacity, src, dst, shifts, s);
+}
+
+// ------------------------------------------------------ spread / gather ---
+int fd3_spread(const fd3_geom* g, int32_t nS, int32_t r, int64_t n, const void* positions, int pos_dtype,
+               const int32_t* species, const float* charges, float* rho, void* stream) {
+  Geom geom;
+  int rc;
+  if ((rc = make_geom(g, &geom))) return rc;
+  if (nS < 1 || r < 1) {
+    set_error("n_species and n_rank must be positive");
+    return FD3_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const size_t M = (size_t)geom.nx * geom.ny * geom.nz;
+  if ((rc = launch_fill_f32(rho, M * nS * r, s))) return rc;
+  SpreadArgs sa;
+  sa.g = mesh_geom(geom); sa.n = n; sa.r = r; sa.pos4 = nullptr; sa.raw_pos = positions; sa.raw_dtype = pos_dtype;
+  sa.spec_s = species; sa.charges = charges; sa.rho = rho;
+  return launch_spread(sa, s);
+}
+
+int fd3_gather(const fd3_geom* g, int32_t nS, int32_t r, int64_t n, const void* positions, int pos_dtype,
+               const int32_t* species, const float* mesh, float* out, void* stream) {
+  Geom geom;
+  int rc;
+  if ((rc = make_geom(g, &geom))) return rc;
+  if (nS < 1 || r < 1) {
+    set_error("n_species and n_rank must be positive");
+    return FD3_ERR_ARG;
+  }
+  GatherArgs ga;
+  std::memset(&ga, 0, sizeof(ga));
+  ga.g = mesh_geom(geom); ga.n = n; ga.nS = nS; ga.r = r; ga.pos4 = nullptr; ga.raw_pos = positions;
+  ga.raw_dtype = pos_dtype; ga.spec_s = species; ga.mesh = mesh;
+  return launch_gather_plain(ga, out, (hipStream_t)stream);
+}
+
+// ------------------------------------------------------------------ FFT ---
+static int fft_layout(const int32_t ns[3], int32_t C, FftPlan& p, FftTables& t, Arena& ar, float2** A,
+                      float2** copy) {
+  if (C < 1) {
+    set_error("n_channels must be positive");
+    return FD3_ERR_ARG;
+  }
+  int rc;
+  if ((rc = fft_plan_make(ns, 1, C, &p))) return rc;
+  fft_tables_carve(p, ar, &t);
+  *A = ar.take<float2>(p.mesh_complex());
+  *copy = ar.take<float2>(p.mesh_complex());
+  return FD3_OK;
+}
+
+int fd3_fft_workspace(const int32_t ns[3], int32_t C, size_t* bytes) {
+  FftPlan p;
+  FftTables t;
+  Arena ar;
+  float2 *A, *cp;
+  int rc = fft_layout(ns, C, p, t, ar, &A, &cp);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+int fd3_fft_r2c(const int32_t ns[3], int32_t C, const float* rho, float* spec, void* ws, size_t ws_bytes,
+                void* stream) {
+  FftPlan p;
+  FftTables t;
+  Arena ar(ws, ws_bytes);
+  float2 *A, *cp;
+  int rc = fft_layout(ns, C, p, t, ar, &A, &cp);
+  if (rc) return rc;
+  if (!ar.ok() || !ws) {
+    set_error("FFT workspace too small");
+    return FD3_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = fft_tables_init(p, t, 1, 0, s))) return rc;
+  if ((rc = fft_forward_zy(p, t, rho, A, (float2*)spec, s))) return rc;
+  return fft_x(p, t, (float2*)spec, -1, s);
+}
+
+int fd3_fft_c2r(const int32_t ns[3], int32_t C, const float* spec, float* out, void* ws, size_t ws_bytes,
+                void* stream) {
+  FftPlan p;
+  FftTables t;
+  Arena ar(ws, ws_bytes);
+  float2 *A, *cp;
+  int rc = fft_layout(ns, C, p, t, ar, &A, &cp);
+  if (rc) return rc;
+  if (!ar.ok() || !ws) {
+    set_error("FFT workspace too small");
+    return FD3_ERR_WORKSPACE;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if ((rc = fft_tables_init(p, t, 1, 0, s))) return rc;
+  FD3_HIP(hipMemcpyAsync(cp, spec, p.mesh_complex() * sizeof(float2), hipMemcpyDeviceToDevice, s));
+  if ((rc = fft_x(p, t, cp, 1, s))) return rc;
+  return fft_inverse_yz(p, t, cp, A, out, s);
+}
+
+// ------------------------------------------------------------ host checks ---
+int fd3_green_host(double s6, double s8, double r0, double qq, int64_t n, const double* k, float* g, float* dg) {
+  PairK pk;
+  const float R = (float)r0;
+  pk.R = R;
+  pk.p6 = (float)s6 * float(2.0 * kPi * kPi / 3.0) / (R * R * R);
+  pk.p8 = 3.0f * (float)s8 * (float)qq * float(-kPi * kPi) / (R * R * R * R * R);
+  for (int64_t i = 0; i < n; ++i) green_eval(pk, (float)k[i], g + i, dg + i);
+  return FD3_OK;
+}
+
+int fd3_interp_weights_host(int32_t order, int32_t interp, int64_t n, const double* x, float* w, float* dw) {
+  if (order < 1 || order > kMaxOrder || (interp == FD3_INTERP_P3M && order > 5)) {
+    set_error("unsupported interpolation order %d", order);
+    return FD3_ERR_UNSUPPORTED;
+  }
+  float ww[kMaxOrder], dd[kMaxOrder];
+  for (int64_t i = 0; i < n; ++i) {
+    interp_weights<float>(order, interp, (float)x[i], ww, dd);
+    for (int j = 0; j < order; ++j) {
+      w[j * n + i] = ww[j];
+      dw[j * n + i] = dd[j];
+    }
+  }
+  return FD3_OK;
+}
+
+}  // extern "C"

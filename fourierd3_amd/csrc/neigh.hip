@@ -1,0 +1,497 @@
+// Cell-list neighbour search and the smooth-cutoff coordination number.
+//
+// Atoms are binned by their fractional coordinates wrapped into [0, 1): nb_a
+// bins along lattice vector a, with nb_a = max(1, floor(d_a / r_cut)) where
+// d_a is the spacing of the lattice planes normal to a.  A bin is then at
+// least r_cut thick, and every image within r_cut of an atom lies in the bins
+// b + d, |d_a| <= s_a = ceil(r_cut nb_a / d_a), with unwrapped bin b + d
+// mapping to bin (b + d) mod nb_a and lattice shift floor((b + d) / nb_a).
+// Thin boxes (d_a < r_cut) get nb_a = 1 and s_a > 1, i.e. several images of
+// the same atoms, including an atom's own images.  This enumerates exactly
+// matscipy's neighbour_list("ijS") pairs (src/fourierd3/calculator.py:103-109):
+// every (i, j, S) with |r_j + S cell - r_i| < r_cut except i == j, S == 0.
+//
+// The CN of src/fourierd3/core.py:348-386 sums, for each atom, over these
+// pairs, so no edge list is materialised on the calculator path.
+
+#include "fd3_internal.h"
+#include "neigh_kernels.h"
+
+namespace fd3 {
+
+// Smooth-cutoff CN term of one pair and its derivative wrt r (Bohr):
+//   r1 = (Rcov_ij + r_cut)/2, r' = max(r, r1)
+//   k  = 16 + (r' - r1)^2 / ((r_cut - r')^2 + 1e-6)
+//   f  = sigmoid(k (4/3 Rcov_ij / r - 1)),  f = 0 for r <= 1e-6
+__device__ inline void cn_term(float r, float rcs, float rcut, float* f, float* fp) {
+  if (!(r > 1e-6f)) {
+    *f = 0.f;
+    *fp = 0.f;
+    return;
+  }
+  const float r1 = 0.5f * rcs + 0.5f * rcut;
+  const bool far = r > r1;
+  const float rc = far ? r : r1;
+  const float dr = rc - r1, dc = rcut - rc;
+  const float den = dc * dc + 1e-6f;
+  const float k = 16.0f + dr * dr / den;
+  const float ratio = (4.0f / 3.0f) * rcs;
+  const float arg = ratio / r - 1.0f;
+  const float s = k * arg;
+  const float sig = 1.0f / (1.0f + expf(-s));
+  *f = sig;
+  const float dk = far ? (2.0f * dr / den + dr * dr * 2.0f * dc / (den * den)) : 0.0f;
+  const float ds = dk * arg - k * ratio / (r * r);
+  *fp = sig * (1.0f - sig) * ds;
+}
+
+__device__ inline int floordiv(int a, int b) {
+  int q = a / b;
+  if ((a % b) != 0 && ((a < 0) != (b < 0))) --q;
+  return q;
+}
+
+// Visit every neighbour image (t, S) of slot s within the cutoff.
+template <typename F>
+__device__ inline void for_each_neighbour(const CellArgs& c, int64_t s, F&& fn) {
+  const double4 ps = c.pos4[s];
+  const int lin = c.sbin[s];
+  const int bz = lin % c.nb[2], by = (lin / c.nb[2]) % c.nb[1], bx = lin / (c.nb[2] * c.nb[1]);
+  for (int dx = -c.sr[0]; dx <= c.sr[0]; ++dx) {
+    const int ux = bx + dx, wx = wrap(ux, c.nb[0]), Sx = floordiv(ux, c.nb[0]);
+    for (int dy = -c.sr[1]; dy <= c.sr[1]; ++dy) {
+      const int uy = by + dy, wy = wrap(uy, c.nb[1]), Sy = floordiv(uy, c.nb[1]);
+      for (int dz = -c.sr[2]; dz <= c.sr[2]; ++dz) {
+        const int uz = bz + dz, wz = wrap(uz, c.nb[2]), Sz = floordiv(uz, c.nb[2]);
+        const double shx = Sx * c.cell[0] + Sy * c.cell[3] + Sz * c.cell[6];
+        const double shy = Sx * c.cell[1] + Sy * c.cell[4] + Sz * c.cell[7];
+        const double shz = Sx * c.cell[2] + Sy * c.cell[5] + Sz * c.cell[8];
+        const bool zero = (Sx | Sy | Sz) == 0;
+        const int wb = (wx * c.nb[1] + wy) * c.nb[2] + wz;
+        const int t1 = c.bstart[wb + 1];
+        for (int t = c.bstart[wb]; t < t1; ++t) {
+          if (zero && t == s) continue;
+          const double4 pt = c.pos4[t];
+          const double vx = pt.x + shx - ps.x, vy = pt.y + shy - ps.y, vz = pt.z + shz - ps.z;
+          const double r2 = vx * vx + vy * vy + vz * vz;
+          if (r2 < c.rc2) fn(t, vx, vy, vz, r2, Sx, Sy, Sz);
+        }
+      }
+    }
+  }
+}
+
+// --------------------------------------------------------- build kernels ---
+__global__ void k_bin(CellArgs c, const void* pos, int dtype, int64_t n, double4* __restrict__ posw,
+                      int4* __restrict__ flo, int32_t* __restrict__ abin, int32_t* __restrict__ bcount) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double r[3];
+  load_pos(pos, dtype, i, r);
+  double fw[3];
+  int fl[3], b[3];
+  for (int d = 0; d < 3; ++d) {
+    const double f = r[0] * c.inv[0 * 3 + d] + r[1] * c.inv[1 * 3 + d] + r[2] * c.inv[2 * 3 + d];
+    const double flf = floor(f);
+    fl[d] = (int)flf;
+    fw[d] = f - flf;
+    if (fw[d] >= 1.0) {  // f just below an integer: f - floor(f) rounds up to 1
+      fw[d] = 0.0;
+      fl[d] += 1;
+    }
+    b[d] = min((int)(fw[d] * c.nb[d]), c.nb[d] - 1);
+  }
+  const double x = fw[0] * c.cell[0] + fw[1] * c.cell[3] + fw[2] * c.cell[6];
+  const double y = fw[0] * c.cell[1] + fw[1] * c.cell[4] + fw[2] * c.cell[7];
+  const double z = fw[0] * c.cell[2] + fw[1] * c.cell[5] + fw[2] * c.cell[8];
+  posw[i] = make_double4(x, y, z, 0.0);
+  flo[i] = make_int4(fl[0], fl[1], fl[2], 0);
+  const int lin = (b[0] * c.nb[1] + b[1]) * c.nb[2] + b[2];
+  abin[i] = lin;
+  atomicAdd(bcount + lin, 1);
+}
+
+// Exclusive scan of n ints into out[0..n] (out[n] = total); one block of 1024.
+__global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ out,
+                                                int64_t* __restrict__ total64) {
+  __shared__ int64_t part[1024];
+  const int t = threadIdx.x;
+  const int64_t chunk = (n + 1023) / 1024;
+  const int64_t lo = t * chunk, hi = min(n, lo + chunk);
+  int64_t s = 0;
+  for (int64_t i = lo; i < hi; ++i) s += in[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int64_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int64_t run = t ? part[t - 1] : 0;
+  for (int64_t i = lo; i < hi; ++i) {
+    out[i] = (int32_t)run;
+    run += in[i];
+  }
+  if (t == 1023) {
+    out[n] = (int32_t)part[1023];
+    if (total64) total64[0] = part[1023];
+  }
+}
+
+__global__ void k_scatter(const int32_t* __restrict__ abin, int64_t n, const int32_t* __restrict__ bstart,
+                          int32_t* __restrict__ bcur, int32_t* __restrict__ slot_atom) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = abin[i];
+  const int slot = bstart[b] + atomicAdd(bcur + b, 1);
+  slot_atom[slot] = (int32_t)i;
+}
+
+// Deterministic order inside each bin: ascending atom index.
+__global__ void k_sort_bins(const int32_t* __restrict__ bstart, int nbins, int32_t* __restrict__ slot_atom) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  const int lo = bstart[b], hi = bstart[b + 1];
+  for (int i = lo + 1; i < hi; ++i) {
+    const int v = slot_atom[i];
+    int j = i - 1;
+    while (j >= lo && slot_atom[j] > v) {
+      slot_atom[j + 1] = slot_atom[j];
+      --j;
+    }
+    slot_atom[j + 1] = v;
+  }
+}
+
+__global__ void k_fill_sorted(const int32_t* __restrict__ slot_atom, int64_t n, const double4* __restrict__ posw,
+                              const int4* __restrict__ flo, const int32_t* __restrict__ abin,
+                              const int32_t* __restrict__ species, double4* __restrict__ pos4,
+                              int4* __restrict__ flo_s, int32_t* __restrict__ sbin, int32_t* __restrict__ spec_s) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int i = slot_atom[s];
+  pos4[s] = posw[i];
+  flo_s[s] = flo[i];
+  sbin[s] = abin[i];
+  if (species) spec_s[s] = species[i];
+}
+
+// ----------------------------------------------------------- CN kernels ---
+__global__ void __launch_bounds__(256) k_cn_cells(CellArgs c, CnArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const float ri = a.rcov[c.spec_s[s]];
+  float cn = 0.f;
+  for_each_neighbour(c, s, [&](int t, double, double, double, double r2, int, int, int) {
+    float f, fp;
+    cn_term((float)sqrt(r2), ri + a.rcov[c.spec_s[t]], a.rcut, &f, &fp);
+    cn += f;
+  });
+  a.cn[s] = cn;
+}
+
+// dE/dr_s -= sum_t (g_s + g_t) f'(r) v / r ;  virial += sum_t g_s f'(r) v v^T / r
+__global__ void __launch_bounds__(256) k_cn_bwd_cells(CellArgs c, CnArgs a) {
+  __shared__ double red[16 * 6];
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  if (s < c.n) {
+    const float ri = a.rcov[c.spec_s[s]];
+    const float gs = a.dEdcn[s];
+    double fx = 0, fy = 0, fz = 0;
+    for_each_neighbour(c, s, [&](int t, double vx, double vy, double vz, double r2, int, int, int) {
+      float f, fp;
+      const float r = (float)sqrt(r2);
+      cn_term(r, ri + a.rcov[c.spec_s[t]], a.rcut, &f, &fp);
+      const double w = (double)(fp / r);
+      const double wt = w * (double)(gs + a.dEdcn[t]);
+      fx -= wt * vx; fy -= wt * vy; fz -= wt * vz;
+      const double wv = w * gs;
+      vir[0] += wv * vx * vx; vir[1] += wv * vy * vy; vir[2] += wv * vz * vz;
+      vir[3] += wv * vx * vy; vir[4] += wv * vx * vz; vir[5] += wv * vy * vz;
+    });
+    a.gpos[3 * s + 0] += fx;
+    a.gpos[3 * s + 1] += fy;
+    a.gpos[3 * s + 2] += fz;
+  }
+  block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
+}
+
+// Edge-list forms: pairs given by (src, dst, shift in Angstrom).
+__device__ inline void edge_vec(const EdgeArgs& e, int64_t k, const double4* pos4, double* v) {
+  const int64_t i = e.src[k], j = e.dst[k];
+  double sh[3];
+  load_pos(e.shifts, e.shifts_dtype, k, sh);  // Angstrom -> Bohr
+  const double4 pi = pos4[i], pj = pos4[j];
+  v[0] = pj.x - pi.x + sh[0];
+  v[1] = pj.y - pi.y + sh[1];
+  v[2] = pj.z - pi.z + sh[2];
+}
+
+__global__ void __launch_bounds__(256) k_cn_edges(EdgeArgs e, CnArgs a, const double4* __restrict__ pos4,
+                                                  const int32_t* __restrict__ spec) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= e.n_edges) return;
+  double v[3];
+  edge_vec(e, k, pos4, v);
+  const float r = (float)sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const int64_t i = e.src[k], j = e.dst[k];
+  float f, fp;
+  cn_term(r, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fp);
+  if (f != 0.f) atomicAdd(a.cn + i, f);
+}
+
+__global__ void __launch_bounds__(256) k_cn_bwd_edges(EdgeArgs e, CnArgs a, const double4* __restrict__ pos4,
+                                                      const int32_t* __restrict__ spec, double* __restrict__ gshift) {
+  __shared__ double red[16 * 6];
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double vir[6] = {0, 0, 0, 0, 0, 0};
+  if (k < e.n_edges) {
+    double v[3];
+    edge_vec(e, k, pos4, v);
+    const float r = (float)sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const int64_t i = e.src[k], j = e.dst[k];
+    float f, fp;
+    cn_term(r, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fp);
+    const double w = r > 0.f ? (double)(a.dEdcn[i] * fp / r) : 0.0;
+    const double g0 = w * v[0], g1 = w * v[1], g2 = w * v[2];  // dE/dv, Hartree/Bohr
+    if (w != 0.0) {
+      atomicAdd(a.gpos + 3 * j + 0, g0);
+      atomicAdd(a.gpos + 3 * j + 1, g1);
+      atomicAdd(a.gpos + 3 * j + 2, g2);
+      atomicAdd(a.gpos + 3 * i + 0, -g0);
+      atomicAdd(a.gpos + 3 * i + 1, -g1);
+      atomicAdd(a.gpos + 3 * i + 2, -g2);
+    }
+    if (gshift) {
+      gshift[3 * k + 0] = g0 * kA2B;
+      gshift[3 * k + 1] = g1 * kA2B;
+      gshift[3 * k + 2] = g2 * kA2B;
+    }
+    vir[0] = g0 * v[0]; vir[1] = g1 * v[1]; vir[2] = g2 * v[2];
+    vir[3] = g0 * v[1]; vir[4] = g0 * v[2]; vir[5] = g1 * v[2];
+  }
+  block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
+}
+
+// ----------------------------------------------- neighbour materialisation ---
+__global__ void k_nb_count(CellArgs c, int32_t* __restrict__ count_atom) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  int cnt = 0;
+  for_each_neighbour(c, s, [&](int, double, double, double, double, int, int, int) { ++cnt; });
+  count_atom[c.slot_atom[s]] = cnt;
+}
+
+__global__ void k_nb_fill(CellArgs c, const int32_t* __restrict__ offset, int64_t capacity, int64_t* __restrict__ src,
+                          int64_t* __restrict__ dst, int32_t* __restrict__ sh) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= c.n) return;
+  const int i = c.slot_atom[s];
+  int64_t o = offset[i];
+  const int4 fi = c.flo_s[s];
+  for_each_neighbour(c, s, [&](int t, double, double, double, double, int Sx, int Sy, int Sz) {
+    if (o < capacity) {
+      const int4 ft = c.flo_s[t];
+      src[o] = i;
+      dst[o] = c.slot_atom[t];
+      sh[3 * o + 0] = Sx - ft.x + fi.x;
+      sh[3 * o + 1] = Sy - ft.y + fi.y;
+      sh[3 * o + 2] = Sz - ft.z + fi.z;
+    }
+    ++o;
+  });
+}
+
+__device__ inline bool nb_less(int64_t j1, const int32_t* s1, int64_t j2, const int32_t* s2) {
+  if (j1 != j2) return j1 < j2;
+  for (int d = 0; d < 3; ++d)
+    if (s1[d] != s2[d]) return s1[d] < s2[d];
+  return false;
+}
+
+// Sort each atom's pairs by (j, S).
+__global__ void k_nb_sort(const int32_t* __restrict__ offset, int64_t n, int64_t capacity, int64_t* __restrict__ dst,
+                          int32_t* __restrict__ sh) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t lo = offset[i], hi = min((int64_t)offset[i + 1], capacity);
+  for (int64_t a = lo + 1; a < hi; ++a) {
+    const int64_t vj = dst[a];
+    const int32_t vs[3] = {sh[3 * a], sh[3 * a + 1], sh[3 * a + 2]};
+    int64_t b = a - 1;
+    while (b >= lo && nb_less(vj, vs, dst[b], sh + 3 * b)) {
+      dst[b + 1] = dst[b];
+      for (int d = 0; d < 3; ++d) sh[3 * (b + 1) + d] = sh[3 * b + d];
+      --b;
+    }
+    dst[b + 1] = vj;
+    for (int d = 0; d < 3; ++d) sh[3 * (b + 1) + d] = vs[d];
+  }
+}
+
+// ------------------------------------------------------------------ host ---
+int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g) {
+  double c[9];
+  for (int i = 0; i < 9; ++i) c[i] = cell_A[i] * kA2B;
+  const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
+                     c[2] * (c[3] * c[7] - c[4] * c[6]);
+  if (!(fabs(det) > 1e-12) || !std::isfinite(det)) {
+    set_error("cell is singular or not finite (det = %g)", det);
+    return FD3_ERR_ARG;
+  }
+  if (!(r_cut_A > 0.0)) {
+    set_error("r_cut must be positive, got %g", r_cut_A);
+    return FD3_ERR_ARG;
+  }
+  double inv[9];
+  inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
+  inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
+  inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
+  inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
+  inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
+  inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
+  inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
+  inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
+  inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
+  const double rc = r_cut_A * kA2B;
+  long long total = 1;
+  for (int a = 0; a < 3; ++a) {
+    // plane spacing normal to lattice vector a: 1 / |column a of cell^-1|
+    const double nrm = sqrt(inv[0 * 3 + a] * inv[0 * 3 + a] + inv[1 * 3 + a] * inv[1 * 3 + a] +
+                            inv[2 * 3 + a] * inv[2 * 3 + a]);
+    const double d = 1.0 / nrm;
+    int nb = (int)std::max(1.0, floor(d / rc));
+    nb = std::min(nb, 256);
+    g->nb[a] = nb;
+    g->sr[a] = (int)ceil(rc * nb / d - 1e-12);
+    if (g->sr[a] < 1) g->sr[a] = 1;
+    if (g->sr[a] > 64) {
+      set_error("box is too thin for r_cut along axis %d (%d images)", a, g->sr[a]);
+      return FD3_ERR_ARG;
+    }
+    total *= nb;
+  }
+  for (int i = 0; i < 9; ++i) {
+    g->cell[i] = c[i];
+    g->inv[i] = inv[i];
+  }
+  g->nbins = (int)total;
+  g->rc = rc;
+  g->rc2 = rc * rc;
+  return FD3_OK;
+}
+
+void cell_carve(const CellGrid& g, int64_t n, Arena& ar, CellBuffers* b) {
+  b->posw = ar.take<double4>(n);
+  b->flo = ar.take<int4>(n);
+  b->abin = ar.take<int32_t>(n);
+  b->bcount = ar.take<int32_t>(g.nbins);
+  b->bcur = ar.take<int32_t>(g.nbins);
+  b->bstart = ar.take<int32_t>(g.nbins + 1);
+  b->slot_atom = ar.take<int32_t>(n);
+  b->pos4 = ar.take<double4>(n);
+  b->flo_s = ar.take<int4>(n);
+  b->sbin = ar.take<int32_t>(n);
+  b->spec_s = ar.take<int32_t>(n);
+}
+
+CellArgs cell_args(const CellGrid& g, const CellBuffers& b, int64_t n) {
+  CellArgs c;
+  for (int i = 0; i < 9; ++i) {
+    c.cell[i] = g.cell[i];
+    c.inv[i] = g.inv[i];
+  }
+  for (int a = 0; a < 3; ++a) {
+    c.nb[a] = g.nb[a];
+    c.sr[a] = g.sr[a];
+  }
+  c.rc2 = g.rc2;
+  c.n = n;
+  c.pos4 = b.pos4;
+  c.sbin = b.sbin;
+  c.bstart = b.bstart;
+  c.spec_s = b.spec_s;
+  c.slot_atom = b.slot_atom;
+  c.flo_s = b.flo_s;
+  return c;
+}
+
+int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* pos, int dtype, const int32_t* species,
+               hipStream_t s) {
+  if (n <= 0) return FD3_OK;
+  int rc;
+  if ((rc = launch_fill_i32(b.bcount, g.nbins, s))) return rc;
+  if ((rc = launch_fill_i32(b.bcur, g.nbins, s))) return rc;
+  const CellArgs c = cell_args(g, b, n);
+  ProfScope _ps("k_celllist_build", s);
+  hipLaunchKernelGGL(k_bin, dim3(cdiv(n, 256)), dim3(256), 0, s, c, pos, dtype, n, b.posw, b.flo, b.abin, b.bcount);
+  FD3_LAUNCH_CHECK("k_bin");
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, b.bcount, (int64_t)g.nbins, b.bstart, (int64_t*)nullptr);
+  FD3_LAUNCH_CHECK("k_scan1");
+  hipLaunchKernelGGL(k_scatter, dim3(cdiv(n, 256)), dim3(256), 0, s, b.abin, n, b.bstart, b.bcur, b.slot_atom);
+  FD3_LAUNCH_CHECK("k_scatter");
+  hipLaunchKernelGGL(k_sort_bins, dim3(cdiv(g.nbins, 256)), dim3(256), 0, s, b.bstart, g.nbins, b.slot_atom);
+  FD3_LAUNCH_CHECK("k_sort_bins");
+  hipLaunchKernelGGL(k_fill_sorted, dim3(cdiv(n, 256)), dim3(256), 0, s, b.slot_atom, n, b.posw, b.flo, b.abin,
+                     species, b.pos4, b.flo_s, b.sbin, b.spec_s);
+  FD3_LAUNCH_CHECK("k_fill_sorted");
+  return FD3_OK;
+}
+
+int cn_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, 256)); }
+
+int launch_cn_cells(const CellArgs& c, const CnArgs& a, hipStream_t s) {
+  if (c.n <= 0) return FD3_OK;
+  ProfScope _ps("k_cn_cells", s);
+  hipLaunchKernelGGL(k_cn_cells, dim3(cn_blocks(c.n)), dim3(256), 0, s, c, a);
+  FD3_LAUNCH_CHECK("k_cn_cells");
+  return FD3_OK;
+}
+
+int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, hipStream_t s) {
+  ProfScope _ps("k_cn_bwd_cells", s);
+  hipLaunchKernelGGL(k_cn_bwd_cells, dim3(cn_blocks(c.n)), dim3(256), 0, s, c, a);
+  FD3_LAUNCH_CHECK("k_cn_bwd_cells");
+  return FD3_OK;
+}
+
+int launch_cn_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, hipStream_t s) {
+  if (e.n_edges <= 0) return FD3_OK;
+  ProfScope _ps("k_cn_edges", s);
+  hipLaunchKernelGGL(k_cn_edges, dim3(cdiv(e.n_edges, 256)), dim3(256), 0, s, e, a, pos4, spec);
+  FD3_LAUNCH_CHECK("k_cn_edges");
+  return FD3_OK;
+}
+
+int launch_cn_bwd_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, double* gshift,
+                        hipStream_t s) {
+  ProfScope _ps("k_cn_bwd_edges", s);
+  hipLaunchKernelGGL(k_cn_bwd_edges, dim3(cn_blocks(e.n_edges)), dim3(256), 0, s, e, a, pos4, spec, gshift);
+  FD3_LAUNCH_CHECK("k_cn_bwd_edges");
+  return FD3_OK;
+}
+
+int nb_count(const CellArgs& c, int32_t* count_atom, int32_t* offset, int64_t* total, hipStream_t s) {
+  if (c.n <= 0) {
+    return FD3_OK;
+  }
+  hipLaunchKernelGGL(k_nb_count, dim3(cdiv(c.n, 256)), dim3(256), 0, s, c, count_atom);
+  FD3_LAUNCH_CHECK("k_nb_count");
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, count_atom, c.n, offset, total);
+  FD3_LAUNCH_CHECK("k_scan1");
+  return FD3_OK;
+}
+
+int nb_fill(const CellArgs& c, const int32_t* offset, int64_t capacity, int64_t* src, int64_t* dst, int32_t* sh,
+            hipStream_t s) {
+  if (c.n <= 0) return FD3_OK;
+  hipLaunchKernelGGL(k_nb_fill, dim3(cdiv(c.n, 256)), dim3(256), 0, s, c, offset, capacity, src, dst, sh);
+  FD3_LAUNCH_CHECK("k_nb_fill");
+  hipLaunchKernelGGL(k_nb_sort, dim3(cdiv(c.n, 256)), dim3(256), 0, s, offset, c.n, capacity, dst, sh);
+  FD3_LAUNCH_CHECK("k_nb_sort");
+  return FD3_OK;
+}
+
+}  // namespace fd3

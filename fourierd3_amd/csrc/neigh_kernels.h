@@ -1,0 +1,80 @@
+// Argument blocks and launchers of the cell-list / CN kernels (neigh.hip).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+
+#include "fd3_internal.h"
+
+namespace fd3 {
+
+struct CellGrid {
+  double cell[9], inv[9];  // Bohr
+  int nb[3], sr[3];        // bins and search reach per lattice direction
+  int nbins;
+  double rc, rc2;          // cutoff (Bohr)
+};
+
+struct CellBuffers {
+  double4* posw;   // (n) caller order, wrapped positions (Bohr)
+  int4* flo;       // (n) caller order, floor of fractional coordinates
+  int32_t* abin;   // (n) caller order, bin of each atom
+  int32_t* bcount; // (nbins)
+  int32_t* bcur;   // (nbins)
+  int32_t* bstart; // (nbins + 1)
+  int32_t* slot_atom;  // (n) atom of each slot
+  double4* pos4;   // (n) slot order, wrapped positions
+  int4* flo_s;     // (n) slot order
+  int32_t* sbin;   // (n) slot order
+  int32_t* spec_s; // (n) slot order
+};
+
+struct CellArgs {
+  double cell[9], inv[9];
+  int nb[3], sr[3];
+  double rc2;
+  int64_t n;
+  const double4* pos4;
+  const int32_t* sbin;
+  const int32_t* bstart;
+  const int32_t* spec_s;
+  const int32_t* slot_atom;
+  const int4* flo_s;
+};
+
+struct CnArgs {
+  const float* rcov;  // (nS) Bohr
+  float rcut;         // Bohr
+  float* cn;          // (n) out (fwd)
+  const float* dEdcn; // (n) in (bwd)
+  double* gpos;       // (n, 3) in/out (bwd), Hartree/Bohr
+  double* partials;   // (blocks, 8) virial (bwd)
+};
+
+struct EdgeArgs {
+  int64_t n_edges;
+  const int64_t* src;
+  const int64_t* dst;
+  const void* shifts;  // (E, 3) Angstrom
+  int shifts_dtype;
+};
+
+int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g);
+void cell_carve(const CellGrid& g, int64_t n, Arena& ar, CellBuffers* b);
+CellArgs cell_args(const CellGrid& g, const CellBuffers& b, int64_t n);
+int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* pos, int dtype, const int32_t* species,
+               hipStream_t s);
+int cn_blocks(int64_t n);
+int launch_cn_cells(const CellArgs& c, const CnArgs& a, hipStream_t s);
+int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, hipStream_t s);
+int launch_cn_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, hipStream_t s);
+int launch_cn_bwd_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, double* gshift,
+                        hipStream_t s);
+int nb_count(const CellArgs& c, int32_t* count_atom, int32_t* offset, int64_t* total, hipStream_t s);
+int nb_fill(const CellArgs& c, const int32_t* offset, int64_t capacity, int64_t* src, int64_t* dst, int32_t* sh,
+            hipStream_t s);
+
+// implemented in mesh.hip
+int launch_fill_i32(int32_t* p, size_t n, hipStream_t s);
+
+}  // namespace fd3

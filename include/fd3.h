@@ -1,0 +1,169 @@
+/*
+ * fd3.h - C ABI of libfd3hip.so, the MI355X (gfx950) implementation of
+ * FourierD3's D3(BJ) energy / force / stress path.
+ *
+ * Conventions
+ *  - Every pointer marked [device] is device memory; everything else is host
+ *    memory read during the call only.
+ *  - Nothing here allocates device memory or synchronises the host: the caller
+ *    passes a workspace of the size the matching *_workspace function reports,
+ *    and every kernel is enqueued on `stream` (a hipStream_t; NULL = default
+ *    stream), so the calls can be captured in a HIP graph.
+ *  - Units at the boundary follow the reference's FourierD3.forward
+ *    (src/fourierd3/core.py:480-503): positions, shifts and cells in Angstrom,
+ *    energies in Hartree, gradients in Hartree/Angstrom.  Internally the
+ *    kernels work in Bohr (1 A = 1/0.52917726 Bohr, src/fourierd3/core.py:120).
+ *  - Return value: FD3_OK (0) or a negative FD3_ERR_* code; fd3_last_error()
+ *    returns a message for the calling thread.
+ */
+#ifndef FD3_H
+#define FD3_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FD3_OK 0
+#define FD3_ERR_ARG (-1)         /* bad shape / value  -> ValueError          */
+#define FD3_ERR_UNSUPPORTED (-2) /* option not built   -> NotImplementedError */
+#define FD3_ERR_HIP (-3)         /* HIP runtime error  -> RuntimeError        */
+#define FD3_ERR_WORKSPACE (-4)   /* workspace too small                       */
+
+#define FD3_INTERP_P3M 0      /* method='spme': centred cardinal B-splines, sinc^2p division */
+#define FD3_INTERP_LAGRANGE 1 /* method='pme' : Lagrange interpolation, no sinc division      */
+
+#define FD3_DTYPE_F32 0
+#define FD3_DTYPE_F64 1
+
+/* Flags of fd3_eval_* */
+#define FD3_NEED_GRAD 1    /* compute forces / virial / cell and shift gradients */
+#define FD3_NEED_PARAMS 2  /* compute dE/d(s6, s8, a1, a2)                        */
+
+/* Per-species model constants.  All arrays are float32 [device].
+ * Mirrors the buffers FourierD3.__init__ registers (src/fourierd3/core.py:139-182)
+ * and D3Potential's R0 / Q tables (src/fourierd3/pair_pot.py:49-89). */
+typedef struct fd3_model {
+  int32_t n_species; /* nS, 1..64                                         */
+  int32_t n_rank;    /* r, 1..64                                          */
+  const float* rcov;     /* (nS)      covalent radii, Bohr                */
+  const float* cnref;    /* (nS, 7)   reference CNs, -1 = unused          */
+  const float* vq;       /* (nS, 7, r) eigenvector rows V[Z, p, r]        */
+  const float* eigs;     /* (r)       eigenvalues lambda_r                */
+  const float* qq;       /* (nS, nS)  sqrt(Q_X) sqrt(Q_Y)                 */
+  const float* cncorr;   /* (nS, 2)   [b, a] of CN' = a CN + b, or NULL   */
+  double s6, s8, a1, a2; /* BJ parameters (src/fourierd3/core.py:165-170) */
+  double r_cut;          /* CN cutoff, Angstrom                           */
+} fd3_model;
+
+/* Geometry of one periodic box. */
+typedef struct fd3_geom {
+  double cell[9];  /* Angstrom, row i = lattice vector a_i                   */
+  int32_t ns[3];   /* mesh points per axis: powers of two, 2..1024            */
+  int32_t order;   /* interpolation nodes p: 1..5 (P3M), 2..7 (Lagrange)      */
+  int32_t interp;  /* FD3_INTERP_*                                            */
+} fd3_geom;
+
+/* Outputs of fd3_eval_*.  Any pointer may be NULL when not wanted. [device] */
+typedef struct fd3_outputs {
+  double* energy;      /* (1)    E in Hartree                                      */
+  double* grad_pos;    /* (N,3)  dE/dpositions, Hartree/A                          */
+  double* grad_shifts; /* (E,3)  dE/dshifts, Hartree/A (edge-list entry only)      */
+  double* grad_cell;   /* (3,3)  dE/dcell at fixed positions and shifts, Hartree/A  */
+  double* virial;      /* (3,3)  dE/dstrain for cell' = cell (1+eps)^T, r' = r (1+eps)^T, Hartree */
+  double* grad_params; /* (4)    dE/d(s6, s8, a1, a2), Hartree                     */
+  float* cn;           /* (N)    coordination numbers                              */
+} fd3_outputs;
+
+const char* fd3_last_error(void);
+int fd3_version(void);
+
+/* Per-kernel timing (no counterpart in the reference; used by the benchmark
+ * to measure kernel durations on the launch stream).  fd3_profile_enable(1)
+ * starts recording a HIP event pair around every kernel launch (do not enable
+ * it during graph capture); fd3_profile_report synchronises on the recorded
+ * events, writes up to max_entries kernel names (64 chars each into `names`),
+ * total milliseconds and launch counts, and clears the records. */
+int fd3_profile_enable(int on);
+int fd3_profile_report(int max_entries, char* names, double* total_ms, int64_t* counts, int* n_entries);
+
+/* ---- whole-path entry points ------------------------------------------------
+ * fd3_eval_edges: FourierD3.forward(positions, edge_index, shifts) with
+ *   cnfunc='smooth_cut' and method='spme'/'pme' (src/fourierd3/core.py:480-583):
+ *   energy and, with FD3_NEED_GRAD, its exact gradients with respect to
+ *   positions, shifts and cell (the reference obtains them by autograd).
+ *   edge_src/edge_dst are int64 [device] (rows of edge_index); shifts (E,3)
+ *   are lattice translations in Angstrom [device], dtype FD3_DTYPE_*.
+ * fd3_eval_celllist: FourierD3ASECalculator.calculate (src/fourierd3/calculator.py:125-192)
+ *   without a materialised neighbour list: the directed pairs within r_cut
+ *   (matscipy neighbour_list "ijS" semantics, src/fourierd3/calculator.py:94-123)
+ *   are enumerated on the fly from a cell list.  grad_shifts and grad_cell are
+ *   not produced; virial is.
+ * species: int32 [device] compact species ids 0..nS-1 (ascending atomic number,
+ *   src/fourierd3/core.py:104-114).  positions: (N,3) Angstrom [device].
+ */
+int fd3_eval_edges_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, int64_t n_edges,
+                             int flags, size_t* bytes);
+int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, const void* positions,
+                   int pos_dtype, const int32_t* species, int64_t n_edges, const int64_t* edge_src,
+                   const int64_t* edge_dst, const void* shifts, int shifts_dtype, int flags,
+                   void* workspace, size_t workspace_bytes, const fd3_outputs* out, void* stream);
+
+int fd3_eval_celllist_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, int flags,
+                                size_t* bytes);
+int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, const void* positions,
+                      int pos_dtype, const int32_t* species, int flags, void* workspace,
+                      size_t workspace_bytes, const fd3_outputs* out, void* stream);
+
+/* ---- component entry points (tests and callers that assemble their own path) ----
+ * Neighbour list (matscipy "ijS", src/fourierd3/calculator.py:103-109):
+ *   fd3_neighbours_count writes the number of directed pairs to *n_pairs [device int64];
+ *   fd3_neighbours_fill writes up to `capacity` pairs, sorted by (i, j, S),
+ *   as src/dst int64 and integer shifts S int32 (E,3).
+ */
+int fd3_neighbours_workspace(int64_t n_atoms, const double cell[9], double r_cut, size_t* bytes);
+int fd3_neighbours_count(int64_t n_atoms, const void* positions, int pos_dtype, const double cell[9],
+                         double r_cut, void* workspace, size_t workspace_bytes, int64_t* n_pairs,
+                         void* stream);
+int fd3_neighbours_fill(int64_t n_atoms, const void* positions, int pos_dtype, const double cell[9],
+                        double r_cut, void* workspace, size_t workspace_bytes, int64_t capacity,
+                        int64_t* src, int64_t* dst, int32_t* shifts, void* stream);
+
+/* Spread (MeshInterpolatorD3.points_to_mesh, src/fourierd3/mesh_interpolator_d3.py:74-138):
+ * rho[c, x, y, z] (c = species*r + rank, float32 [device], zeroed here) from
+ * per-atom charges (N, r) float32 placed in the channel block of the atom's species. */
+int fd3_spread(const fd3_geom* g, int32_t n_species, int32_t n_rank, int64_t n_atoms, const void* positions,
+               int pos_dtype, const int32_t* species, const float* charges, float* rho, void* stream);
+
+/* Gather (MeshInterpolatorD3.mesh_to_points, src/fourierd3/mesh_interpolator_d3.py:141-189):
+ * out[i, q] = sum_nodes W * mesh[species_i * r + q, node]   (N, r) float32 [device]. */
+int fd3_gather(const fd3_geom* g, int32_t n_species, int32_t n_rank, int64_t n_atoms, const void* positions,
+               int pos_dtype, const int32_t* species, const float* mesh, float* out, void* stream);
+
+/* 3-D real FFTs of C channels (torch.fft.rfftn / irfftn over the last three
+ * dims, src/fourierd3/kspace_filter_d3.py:55), unnormalised in both
+ * directions.  Spectrum layout: spec[c, kz, ky, kx] complex64 (kz = 0..nz/2),
+ * i.e. kx fastest.  fd3_fft_workspace gives the scratch size. */
+int fd3_fft_workspace(const int32_t ns[3], int32_t n_channels, size_t* bytes);
+int fd3_fft_r2c(const int32_t ns[3], int32_t n_channels, const float* rho, float* spec, void* workspace,
+                size_t workspace_bytes, void* stream);
+int fd3_fft_c2r(const int32_t ns[3], int32_t n_channels, const float* spec, float* out, void* workspace,
+                size_t workspace_bytes, void* stream);
+
+/* Green's function G_XY(|k|) (D3Potential.lr_from_k_sq, src/fourierd3/pair_pot.py:142-178)
+ * on n host values of |k| (Bohr^-1) for one species pair, host-side evaluation of
+ * the same code the kernels run (float32); also dG/d|k|. */
+int fd3_green_host(double s6, double s8, double r0, double qq, int64_t n, const double* k, float* g,
+                   float* dg_dk);
+
+/* 1-D interpolation weights (torch-pme MeshInterpolator._compute_1d_weights as
+ * used at src/fourierd3/core.py:555), host-side evaluation of the kernel code:
+ * for offsets x (n), w[j*n + i] and dw[j*n + i], j = 0..order-1. */
+int fd3_interp_weights_host(int32_t order, int32_t interp, int64_t n, const double* x, float* w, float* dw);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FD3_H */

@@ -1,0 +1,100 @@
+"""CPU tests of the host side: the C ABI library loads and exports every symbol
+include/fd3.h declares, status codes map to the reference's exception types,
+and the module/calculator surface mirrors the reference's constructor checks.
+No GPU kernel is launched here."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from fourierd3_amd import _lib as L
+from fourierd3_amd import tables as T
+from fourierd3_amd import workloads as W
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "fd3.h"
+
+
+def test_library_exports_every_declared_symbol():
+    decl = set(re.findall(r"\b(fd3_\w+)\s*\(", HEADER.read_text()))
+    assert decl, "no declarations parsed"
+    assert decl == set(L.EXPORTS)
+    h = L.lib()
+    for name in decl:
+        assert getattr(h, name) is not None
+    assert h.fd3_version() >= 1
+
+
+def test_status_codes_map_to_reference_exceptions():
+    h = L.lib()
+    g = L.make_geom(np.eye(3) * 10.0, [16, 12, 16], 5, L.FD3_INTERP_P3M)  # 12 is not a power of two
+    nbytes = C.c_size_t()
+    with pytest.raises(NotImplementedError, match="power of two"):
+        L.check(h.fd3_fft_workspace(g.ns, 1, C.byref(nbytes)))
+    m = L.Model()
+    m.n_species = 0
+    with pytest.raises(ValueError):
+        L.check(h.fd3_eval_edges_workspace(C.byref(m), C.byref(g), 10, 10, 1, C.byref(nbytes)))
+    sing = L.make_geom(np.zeros((3, 3)), [16, 16, 16], 5, 0)
+    with pytest.raises(ValueError, match="singular"):
+        L.check(h.fd3_spread(C.byref(sing), 1, 1, 0, None, 0, None, None, None, None))
+    arr = L.cell_array(np.eye(3) * 5.0)
+    with pytest.raises(ValueError, match="r_cut"):
+        L.check(h.fd3_neighbours_workspace(10, arr, -1.0, C.byref(nbytes)))
+
+
+def test_workspace_sizes_scale_with_mesh_and_atoms():
+    h = L.lib()
+    dummy = torch.zeros(4)
+    m = L.Model()
+    m.n_species, m.n_rank = 4, 2
+    for f in ("rcov", "cnref", "vq", "eigs", "qq"):
+        setattr(m, f, dummy.data_ptr())
+    m.s6, m.s8, m.a1, m.a2 = T.PBE_PARAMS
+    m.r_cut = 6.0
+    sizes = []
+    for n, ns in [(100, 16), (100, 32), (1000, 32)]:
+        g = L.make_geom(np.eye(3) * 20.0, [ns] * 3, 5, 0)
+        nb = C.c_size_t()
+        L.check(h.fd3_eval_celllist_workspace(C.byref(m), C.byref(g), n, 1, C.byref(nb)))
+        sizes.append(nb.value)
+    assert sizes[0] < sizes[1] < sizes[2]
+    # two complex meshes of C = 8 channels dominate: 2 * 8 * ns^2 (ns/2+1) * 8 bytes
+    assert sizes[1] > 2 * 8 * 32 * 32 * 17 * 8
+
+
+def test_module_constructor_checks():
+    from fourierd3_amd import FourierD3
+    box = W.sio2_box(30, seed=1)
+    with pytest.raises(NotImplementedError):
+        FourierD3(box.numbers, box.cell, device="cpu", method="ewald", verbose=False)
+    with pytest.raises(NotImplementedError):
+        FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d4", verbose=False)
+    with pytest.raises(NotImplementedError):
+        FourierD3(box.numbers, box.cell, device="cpu", xcfunc="b3lyp", verbose=False)
+    with pytest.raises(AssertionError):
+        FourierD3(box.numbers, box.cell, device="cpu", pbc=torch.tensor([True, True, False]), verbose=False)
+    m = FourierD3(box.numbers, box.cell, device="cpu", verbose=False)
+    assert list(m.ns_mesh) == list(T.get_ns_mesh(box.cell * T.ANGSTROM_TO_BOHR, 1.2 * T.ANGSTROM_TO_BOHR))
+    assert m.n_species == 2 and m.v_q_reshaped.shape == (2, 7, m.n_rank)
+    vol = abs(np.linalg.det(box.cell)) * T.ANGSTROM_TO_BOHR ** 3
+    assert float(m.volume) == pytest.approx(vol, rel=1e-6)
+    with pytest.raises(ValueError):
+        m._update_cell(torch.eye(2))
+
+
+def test_workload_generators():
+    b = W.hcno_box(2000, seed=3)
+    counts = {int(z): int(c) for z, c in zip(*np.unique(b.numbers, return_counts=True))}
+    assert counts == {1: 1000, 6: 500, 7: 150, 8: 350}
+    L_A = b.cell[0, 0]
+    d = b.positions[:, None, :] - b.positions[None, :, :]
+    d -= L_A * np.round(d / L_A)
+    r = np.sqrt((d ** 2).sum(-1)) + np.eye(len(d)) * 10
+    assert r.min() >= 1.0
+    t = W.strained(b, 0.1, seed=1)
+    f0 = b.positions @ np.linalg.inv(b.cell)
+    f1 = t.positions @ np.linalg.inv(t.cell)
+    assert np.allclose(f0, f1)

@@ -10,6 +10,7 @@
 
 #include "fd3_internal.h"
 #include "fft_plan.h"
+#include "fused.h"
 #include "mesh_kernels.h"
 #include "neigh_kernels.h"
 
@@ -218,6 +219,40 @@ __device__ inline double block_sum(double v, double* sh) {
   return r;
 }
 
+// First stage of the partial-sum reduction: rows [r0, r1) of a (rows, stride)
+// double array, columns 0..ncols-1, summed per block into out[block][col].
+// 256 threads = 16 row lanes x 16 columns; fixed order, so deterministic.
+__global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ src, int rows, int stride, int ncols,
+                                                     int rows_per_block, double* __restrict__ out) {
+  __shared__ double sh[16][17];
+  const int col = threadIdx.x & 15, lane = threadIdx.x >> 4;
+  const int r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
+  double s = 0.0;
+  if (col < ncols)
+    for (int r = r0 + lane; r < r1; r += 16) s += src[(size_t)r * stride + col];
+  sh[lane][col] = s;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    double t = 0.0;
+    for (int l = 0; l < 16; ++l) t += sh[l][threadIdx.x];
+    if (threadIdx.x < ncols) out[(size_t)blockIdx.x * stride + threadIdx.x] = t;
+  }
+}
+
+// Reduce (rows, stride) partials to at most 256 rows written to dst (room for
+// 256 * stride doubles); *src and *rows are redirected to the result.
+static int reduce_partials(const double** src, int* rows, int stride, int ncols, double* dst, hipStream_t s) {
+  if (*rows <= 256) return FD3_OK;
+  const int rpb = (*rows + 255) / 256;
+  const int nb = (*rows + rpb - 1) / rpb;
+  ProfScope _ps("k_reduce_rows", s);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(nb), dim3(256), 0, s, *src, *rows, stride, ncols, rpb, dst);
+  FD3_LAUNCH_CHECK("k_reduce_rows");
+  *src = dst;
+  *rows = nb;
+  return FD3_OK;
+}
+
 __global__ void __launch_bounds__(256) k_finalize(FinalArgs a) {
   __shared__ double sh[256];
   __shared__ double tot[40];
@@ -285,32 +320,34 @@ struct Common {
   FftPlan plan;
   Geom geom;
   FftTables tab;
-  float* buf1;   // rho / spectrum / dE/drho
-  float2* buf2;  // intermediate A
+  float* buf1;   // spectrum
+  float2* buf2;  // z-pass output A
+  ColBuffers cols;
   float* cn_raw;
   float* charges;
   float* dEdcn;
   double* gpos;
-  double *p_ks, *p_c6, *p_ga, *p_cn;
+  double *p_ks, *p_c6, *p_ga, *p_cn, *p_red;
   int n_c6, n_ga, n_cn;
 };
 
 static void carve_common(Arena& ar, Common& c, int64_t n, int r, int64_t n_cn_items) {
   fft_tables_carve(c.plan, ar, &c.tab);
-  const size_t b1 = std::max(c.plan.mesh_real(), 2 * c.plan.mesh_complex());
-  c.buf1 = ar.take<float>(b1);
-  c.buf2 = ar.take<float2>(c.plan.mesh_complex());
+  c.buf1 = ar.take<float>(2 * c.plan.mesh_complex());  // spectrum
+  c.buf2 = ar.take<float2>(c.plan.mesh_complex());     // z-pass output
+  col_carve(c.plan, n, ar, &c.cols);
   c.cn_raw = ar.take<float>(std::max<int64_t>(n, 1));
   c.charges = ar.take<float>(std::max<int64_t>(n * r, 1));
   c.dEdcn = ar.take<float>(std::max<int64_t>(n, 1));
   c.gpos = ar.take<double>(std::max<int64_t>(3 * n, 3));
   c.n_c6 = c6_blocks(n);
-  c.n_ga = gather_blocks(n, c.geom.order);
+  c.n_ga = c6_blocks(n);  // partials of k_c6_bwd
   c.n_cn = cn_blocks(n_cn_items);
   c.p_ks = ar.take<double>((size_t)c.plan.ks_blocks * 16);
   c.p_c6 = ar.take<double>((size_t)c.n_c6 * 8);
   c.p_ga = ar.take<double>((size_t)c.n_ga * 16);
   c.p_cn = ar.take<double>((size_t)c.n_cn * 8);
+  c.p_red = ar.take<double>(256 * 48);
 }
 
 static int setup_common(const fd3_model* m, const fd3_geom* g, Common& c) {
@@ -335,25 +372,30 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   ca.s6 = m->s6; ca.s8 = m->s8; ca.a1 = m->a1; ca.a2 = m->a2;
   ca.charges = c.charges; ca.cn_out = out ? out->cn : nullptr; ca.partials = c.p_c6;
   if ((rc = launch_c6_fwd(ca, s))) return rc;
-  if ((rc = launch_fill_f32(c.buf1, c.plan.mesh_real(), s))) return rc;
-  SpreadArgs sa;
-  sa.g = mesh_geom(c.geom); sa.n = n; sa.r = r; sa.pos4 = pos4; sa.raw_pos = nullptr; sa.raw_dtype = 0;
-  sa.spec_s = spec_s; sa.charges = c.charges; sa.rho = c.buf1;
-  if ((rc = launch_spread(sa, s))) return rc;
+  // atoms into mesh columns, then spreading fused into the forward z pass
+  FusedArgs fa;
+  fa.n = n; fa.nS = nS; fa.r = r;
+  fa.nx = c.geom.nx; fa.ny = c.geom.ny; fa.nz = c.geom.nz; fa.order = c.geom.order; fa.interp = c.geom.interp;
+  for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
+  fa.pos4 = pos4; fa.spec_s = spec_s; fa.charges = c.charges; fa.cb = c.cols;
+  if ((rc = col_sort(fa, s))) return rc;
   float2* spec = (float2*)c.buf1;
-  if ((rc = fft_forward_zy(c.plan, c.tab, c.buf1, c.buf2, spec, s))) return rc;
+  if ((rc = spread_r2c_z(fa, c.plan, c.tab, c.buf2, s))) return rc;
+  if ((rc = fft_y(c.plan, c.tab, c.buf2, spec, -1, s))) return rc;
   if ((rc = kspace_run(c.plan, c.tab, c.geom, spec, m->eigs, m->qq, m->s6, m->s8, m->a1, m->a2, grad ? 1 : 0,
                        (flags & FD3_NEED_PARAMS) ? 1 : 0, c.p_ks, s)))
     return rc;
   if (grad) {
-    if ((rc = fft_inverse_yz(c.plan, c.tab, spec, c.buf2, c.buf1, s))) return rc;
-    GatherArgs ga;
-    ga.g = mesh_geom(c.geom); ga.n = n; ga.nS = nS; ga.r = r; ga.pos4 = pos4; ga.raw_pos = nullptr; ga.raw_dtype = 0;
-    ga.spec_s = spec_s; ga.charges = c.charges; ga.mesh = c.buf1; ga.cn_raw = c.cn_raw; ga.cnref = m->cnref;
-    ga.vq = m->vq; ga.eigs = m->eigs; ga.qq = m->qq; ga.cncorr = m->cncorr;
-    ga.s6 = m->s6; ga.s8 = m->s8; ga.a1 = m->a1; ga.a2 = m->a2;
-    ga.gpos = c.gpos; ga.dEdcn = c.dEdcn; ga.dEdc = nullptr; ga.partials = c.p_ga;
-    if (n > 0 && (rc = launch_gather_bwd(ga, s))) return rc;
+    if ((rc = fft_y(c.plan, c.tab, spec, c.buf2, 1, s))) return rc;
+    if ((rc = c2r_z_gather(fa, c.plan, c.tab, c.buf2, s))) return rc;
+    C6BwdArgs ba;
+    ba.n = n; ba.nS = nS; ba.r = r; ba.nx = c.geom.nx; ba.ny = c.geom.ny; ba.nz = c.geom.nz;
+    for (int i = 0; i < 9; ++i) ba.inv[i] = c.geom.inv[i];
+    ba.pos4 = pos4; ba.spec_s = spec_s; ba.charges = c.charges; ba.accA = c.cols.accA; ba.accG = c.cols.accG;
+    ba.cn_raw = c.cn_raw; ba.cnref = m->cnref; ba.vq = m->vq; ba.eigs = m->eigs; ba.qq = m->qq;
+    ba.cncorr = m->cncorr; ba.s6 = m->s6; ba.s8 = m->s8; ba.a1 = m->a1; ba.a2 = m->a2;
+    ba.gpos = c.gpos; ba.dEdcn = c.dEdcn; ba.partials = c.p_ga;
+    if (n > 0 && (rc = launch_c6_bwd(ba, s))) return rc;
   }
   return FD3_OK;
 }
@@ -365,6 +407,11 @@ static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, 
   fa.c6 = c.p_c6; fa.nc6 = c.n_c6;
   fa.ga = grad ? c.p_ga : nullptr; fa.nga = c.n_ga;
   fa.cnv = (grad && have_cnv) ? c.p_cn : nullptr; fa.ncnv = c.n_cn;
+  int rc;
+  if ((rc = reduce_partials(&fa.ks, &fa.nks, 16, 11, c.p_red, s))) return rc;
+  if ((rc = reduce_partials(&fa.c6, &fa.nc6, 8, 5, c.p_red + 256 * 16, s))) return rc;
+  if (fa.ga && (rc = reduce_partials(&fa.ga, &fa.nga, 16, 9, c.p_red + 256 * 24, s))) return rc;
+  if (fa.cnv && (rc = reduce_partials(&fa.cnv, &fa.ncnv, 8, 6, c.p_red + 256 * 40, s))) return rc;
   fa.volume = c.geom.volume;
   for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
   fa.energy = out->energy;

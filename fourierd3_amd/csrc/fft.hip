@@ -17,84 +17,9 @@
 
 #include "fd3_internal.h"
 #include "fft_plan.h"
+#include "fft_lds.h"
 
 namespace fd3 {
-
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
-  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
-
-// In-place FFT of `nlines` lines of length n (power of two) held in LDS at
-// buf[l * ls + e].  tw[k * tws] = exp(-2 pi i k / n).  DIR = -1 forward
-// (exp(-i...)), +1 inverse (exp(+i...), unnormalised).  Requires
-// nlines * n / 4 <= MAXB * blockDim.x.  Ends with __syncthreads().
-template <int DIR, int MAXB>
-__device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __restrict__ tw, int tws) {
-  if (n <= 1) return;
-  const int log2n = __ffs(n) - 1;
-  const int nq = n >> 2;
-  const int total4 = nlines * nq;
-  int Ns = 1;
-  for (int s = 0; s < (log2n >> 1); ++s) {
-    float2 v0[MAXB], v1[MAXB], v2[MAXB], v3[MAXB];
-    int dst[MAXB];
-    const int stride = n / (4 * Ns);
-#pragma unroll
-    for (int c = 0; c < MAXB; ++c) {
-      const int b = threadIdx.x + c * blockDim.x;
-      if (b < total4) {
-        const int line = b / nq, j = b - line * nq;
-        float2* L = buf + line * ls;
-        const int k = j & (Ns - 1);
-        float2 a0 = L[j], a1 = L[j + nq], a2 = L[j + 2 * nq], a3 = L[j + 3 * nq];
-        if (k) {
-          const int t = stride * k * tws;
-          float2 w1 = tw[t], w2 = tw[2 * t], w3 = tw[3 * t];
-          if (DIR > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
-          a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3);
-        }
-        const float2 b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3);
-        const float2 d = csub(a1, a3);
-        // forward: b3 = -i d ; inverse: b3 = +i d
-        const float2 b3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
-        v0[c] = cadd(b0, b2); v2[c] = csub(b0, b2);
-        v1[c] = cadd(b1, b3); v3[c] = csub(b1, b3);
-        dst[c] = line * ls + ((j - k) << 2) + k;
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < MAXB; ++c) {
-      const int b = threadIdx.x + c * blockDim.x;
-      if (b < total4) {
-        float2* D = buf + dst[c];
-        D[0] = v0[c]; D[Ns] = v1[c]; D[2 * Ns] = v2[c]; D[3 * Ns] = v3[c];
-      }
-    }
-    __syncthreads();
-    Ns <<= 2;
-  }
-  if (log2n & 1) {  // final radix-2 stage, Ns = n/2: in place per butterfly
-    const int nh = n >> 1;
-    const int total2 = nlines * nh;
-    for (int b = threadIdx.x; b < total2; b += blockDim.x) {
-      const int line = b / nh, j = b - line * nh;
-      float2* L = buf + line * ls;
-      float2 w = tw[j * tws];
-      if (DIR > 0) w.y = -w.y;
-      const float2 a0 = L[j], a1 = cmul(L[j + nh], w);
-      L[j] = cadd(a0, a1);
-      L[j + nh] = csub(a0, a1);
-    }
-    __syncthreads();
-  }
-}
 
 // ------------------------------------------------------------ tables ---
 // tw_axis[k] = exp(-2 pi i k / n) for each axis; sinc_axis[m] = sinc(m/n)^(2p)
@@ -348,28 +273,33 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     if (a.use_sinc) sc = fmaxf(a.scx[kx] * a.scy[ky] * a.scz[kz], 1e-10f);
     const float isc = 1.0f / sc;
     float e_acc = 0.f, d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
+    // phi accumulates in P (this thread owns column (kyi, kx) of every line)
+    for (int l = 0; l < nS * RT; ++l) P[(l * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
+    // one Green's function evaluation per unordered species pair (G~ is symmetric)
     for (int X = 0; X < nS; ++X) {
-      float2 sx[8], ph[8];
-#pragma unroll
-      for (int qi = 0; qi < 8; ++qi) {
-        sx[qi] = qi < rt ? S[((X * RT + qi) * NK + kyi) * ls + kx] : make_float2(0.f, 0.f);
-        ph[qi] = make_float2(0.f, 0.f);
-      }
-      for (int Y = 0; Y < nS; ++Y) {
+      for (int Y = X; Y < nS; ++Y) {
         const PairK pk = pairs[X * nS + Y];
         float g, dg;
         green_eval(pk, kn, &g, &dg);
         const float gs = g * isc;
         float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
-#pragma unroll
-        for (int qi = 0; qi < 8; ++qi) {
-          if (qi < rt) {
-            const float2 sy = S[((Y * RT + qi) * NK + kyi) * ls + kx];
-            ph[qi].x += gs * sy.x;
-            ph[qi].y += gs * sy.y;
-            tre += a.eigs[q0 + qi] * (sx[qi].x * sy.x + sx[qi].y * sy.y);
+        for (int qi = 0; qi < rt; ++qi) {
+          const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
+          const int iy = ((Y * RT + qi) * NK + kyi) * ls + kx;
+          const float2 sx = S[ix], sy = S[iy];
+          float2 px = P[ix];
+          px.x += gs * sy.x;
+          px.y += gs * sy.y;
+          P[ix] = px;
+          if (Y != X) {
+            float2 py = P[iy];
+            py.x += gs * sx.x;
+            py.y += gs * sx.y;
+            P[iy] = py;
           }
+          tre += a.eigs[q0 + qi] * (sx.x * sy.x + sx.y * sy.y);
         }
+        if (Y != X) tre *= 2.0f;
         d_acc += dg * isc * tre;
         if (PARAMS) {
           // dG/ds6 = f6, dG/ds8 = 3 QQ f8, dG/dR0 for a1 (x sqrt(3QQ)) and a2
@@ -386,16 +316,16 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
           p_acc[3] += dgdR * isc * tre;
         }
       }
-#pragma unroll
-      for (int qi = 0; qi < 8; ++qi) {
-        if (qi < rt) {
-          const float lam = a.eigs[q0 + qi];
-          e_acc += lam * (sx[qi].x * ph[qi].x + sx[qi].y * ph[qi].y);
-          const float f = lam * a.neg_inv_vol;
-          P[((X * RT + qi) * NK + kyi) * ls + kx] = make_float2(f * ph[qi].x, f * ph[qi].y);
-        } else if (qi < RT) {
-          P[((X * RT + qi) * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
-        }
+    }
+    // energy sum_X sum_q lambda_q Re(S conj phi); P <- -(lambda_q / Omega) phi
+    for (int X = 0; X < nS; ++X) {
+      for (int qi = 0; qi < rt; ++qi) {
+        const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
+        const float lam = a.eigs[q0 + qi];
+        const float2 sx = S[ix], ph = P[ix];
+        e_acc += lam * (sx.x * ph.x + sx.y * ph.y);
+        const float f = lam * a.neg_inv_vol;
+        P[ix] = make_float2(f * ph.x, f * ph.y);
       }
     }
     acc[0] += (double)(w * e_acc);
@@ -539,6 +469,21 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
                        t.twz);
     FD3_LAUNCH_CHECK("k_c2r_z");
   }
+  return FD3_OK;
+}
+
+int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
+  const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2);
+  if (dir < 0) {
+    ProfScope _ps("k_c2c_y_fwd", s);
+    hipLaunchKernelGGL(k_c2c_y<-1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
+                       t.twy);
+  } else {
+    ProfScope _ps("k_c2c_y_inv", s);
+    hipLaunchKernelGGL(k_c2c_y<1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
+                       t.twy);
+  }
+  FD3_LAUNCH_CHECK("k_c2c_y");
   return FD3_OK;
 }
 

@@ -1,0 +1,126 @@
+// In-LDS line FFT and complex helpers shared by the FFT pass kernels.
+#pragma once
+
+#include "fd3_internal.h"
+
+namespace fd3 {
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+// In-place FFT of `nlines` lines of length n (power of two) held in LDS at
+// buf[l * ls + e].  tw[k * tws] = exp(-2 pi i k / n).  DIR = -1 forward
+// (exp(-i...)), +1 inverse (exp(+i...), unnormalised).  Requires
+// nlines * n / 4 <= MAXB * blockDim.x.  Ends with __syncthreads().
+template <int DIR, int MAXB>
+__device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __restrict__ tw, int tws) {
+  if (n <= 1) return;
+  const int log2n = __ffs(n) - 1;
+  const int nq = n >> 2;
+  const int total4 = nlines * nq;
+  int Ns = 1;
+  for (int s = 0; s < (log2n >> 1); ++s) {
+    float2 v0[MAXB], v1[MAXB], v2[MAXB], v3[MAXB];
+    int dst[MAXB];
+    const int stride = n / (4 * Ns);
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+      const int b = threadIdx.x + c * blockDim.x;
+      if (b < total4) {
+        const int line = b / nq, j = b - line * nq;
+        float2* L = buf + line * ls;
+        const int k = j & (Ns - 1);
+        float2 a0 = L[j], a1 = L[j + nq], a2 = L[j + 2 * nq], a3 = L[j + 3 * nq];
+        if (k) {
+          const int t = stride * k * tws;
+          float2 w1 = tw[t], w2 = tw[2 * t], w3 = tw[3 * t];
+          if (DIR > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+          a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3);
+        }
+        const float2 b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3);
+        const float2 d = csub(a1, a3);
+        // forward: b3 = -i d ; inverse: b3 = +i d
+        const float2 b3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+        v0[c] = cadd(b0, b2); v2[c] = csub(b0, b2);
+        v1[c] = cadd(b1, b3); v3[c] = csub(b1, b3);
+        dst[c] = line * ls + ((j - k) << 2) + k;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+      const int b = threadIdx.x + c * blockDim.x;
+      if (b < total4) {
+        float2* D = buf + dst[c];
+        D[0] = v0[c]; D[Ns] = v1[c]; D[2 * Ns] = v2[c]; D[3 * Ns] = v3[c];
+      }
+    }
+    __syncthreads();
+    Ns <<= 2;
+  }
+  if (log2n & 1) {  // final radix-2 stage, Ns = n/2: in place per butterfly
+    const int nh = n >> 1;
+    const int total2 = nlines * nh;
+    for (int b = threadIdx.x; b < total2; b += blockDim.x) {
+      const int line = b / nh, j = b - line * nh;
+      float2* L = buf + line * ls;
+      float2 w = tw[j * tws];
+      if (DIR > 0) w.y = -w.y;
+      const float2 a0 = L[j], a1 = cmul(L[j + nh], w);
+      L[j] = cadd(a0, a1);
+      L[j + nh] = csub(a0, a1);
+    }
+    __syncthreads();
+  }
+}
+
+// The z-pass pre/post-processing that turns a length-M complex FFT of a real
+// line of length nz = 2M (packed as z[m] = x[2m] + i x[2m+1]) into its
+// half spectrum X[0..M], and back.  L points at one LDS line of M+1 values;
+// threads t = 0..(M/2) of the line each handle the pair (k, M-k).
+__device__ inline void r2c_split(float2* L, int M, int k, const float2* __restrict__ twz) {
+  if (k == 0) {
+    const float2 z0 = L[0];
+    L[0] = make_float2(z0.x + z0.y, 0.0f);
+    L[M] = make_float2(z0.x - z0.y, 0.0f);
+  } else {
+    const int km = M - k;
+    const float2 zk = L[k], zm = L[km];
+    // E_k = (zk + conj zm)/2, O_k = -i (zk - conj zm)/2, X_k = E_k + W^k O_k
+    const float2 Ek = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+    const float2 Ok = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    const float2 Em = make_float2(0.5f * (zm.x + zk.x), 0.5f * (zm.y - zk.y));
+    const float2 Om = make_float2(0.5f * (zm.y + zk.y), -0.5f * (zm.x - zk.x));
+    const float2 Xk = cadd(Ek, cmul(twz[k], Ok));
+    const float2 Xm = cadd(Em, cmul(twz[km], Om));
+    L[k] = Xk;
+    L[km] = Xm;
+  }
+}
+
+// Inverse of r2c_split (unnormalised): Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) w^k,
+// w = exp(+2 pi i / nz); the imaginary parts of X[0] and X[M] are ignored.
+__device__ inline void c2r_merge(float2* L, int M, int k, const float2* __restrict__ twz) {
+  if (k == 0) {
+    const float a = L[0].x, b = L[M].x;
+    L[0] = make_float2(a + b, a - b);
+  } else {
+    const int km = M - k;
+    const float2 xk = L[k], xm = L[km];
+    const float2 Ak = make_float2(xk.x + xm.x, xk.y - xm.y);
+    const float2 Bk = cmulc(make_float2(xk.x - xm.x, xk.y + xm.y), twz[k]);
+    const float2 Am = make_float2(xm.x + xk.x, xm.y - xk.y);
+    const float2 Bm = cmulc(make_float2(xm.x - xk.x, xm.y + xk.y), twz[km]);
+    L[k] = make_float2(Ak.x - Bk.y, Ak.y + Bk.x);
+    L[km] = make_float2(Am.x - Bm.y, Am.y + Bm.x);
+  }
+}
+
+}  // namespace fd3

@@ -35,6 +35,8 @@ void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t);
 int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s);
 // rho (real) -> A -> spec; spec holds the y/z-transformed data (x not yet transformed)
 int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float2* A, float2* spec, hipStream_t s);
+// y pass alone: dir -1 A -> spec, dir +1 spec -> A
+int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s);
 // plain in-place x transform of spec (dir -1 forward, +1 inverse)
 int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream_t s);
 // spec (x already inverse-transformed) -> A -> out (real)

@@ -1,0 +1,67 @@
+// Spreading fused into the forward z pass and gathering fused into the inverse
+// z pass (fused.hip).
+#pragma once
+
+#include "fd3_internal.h"
+#include "fft_plan.h"
+
+namespace fd3 {
+
+// Atoms sorted into mesh columns: key = (species, x node 0, y node 0).
+struct ColBuffers {
+  int32_t* key;       // (n) slot order
+  int4* tmp_i;        // (n) slot order: node-0 indices (wrapped) and slot
+  float4* tmp_f;      // (n) slot order: per-axis offsets x of the stencil
+  int32_t* colcount;  // (nS * nx * ny)
+  int32_t* colcur;    // (nS * nx * ny)
+  int32_t* colstart;  // (nS * nx * ny + 1)
+  int4* rec_i;        // (n) column order
+  float4* rec_f;      // (n) column order
+  float* scharge;     // (n, r) column order
+  float* accA;        // (n, r) slot order: sum_nodes W dE/drho
+  float* accG;        // (n, 3) slot order: sum_q c_q sum_nodes dW/du dE/drho
+};
+
+void col_carve(const FftPlan& p, int64_t n, Arena& ar, ColBuffers* b);
+
+struct FusedArgs {
+  int64_t n;
+  int nS, r;
+  int nx, ny, nz, order, interp;
+  double inv[9];
+  const double4* pos4;
+  const int32_t* spec_s;
+  const float* charges;  // (n, r) slot order
+  ColBuffers cb;
+};
+
+// sort atoms into mesh columns (after the charges are known)
+int col_sort(const FusedArgs& a, hipStream_t s);
+// spreading + forward z pass: A[c][x][kz][y]
+int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s);
+// inverse z pass + gathering into cb.accA / cb.accG (zeroed here)
+int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const float2* A, hipStream_t s);
+
+struct C6BwdArgs {
+  int64_t n;
+  int nS, r, nx, ny, nz;
+  double inv[9];
+  const double4* pos4;
+  const int32_t* spec_s;
+  const float* charges;
+  const float* accA;
+  const float* accG;
+  const float* cn_raw;
+  const float* cnref;
+  const float* vq;
+  const float* eigs;
+  const float* qq;
+  const float* cncorr;
+  double s6, s8, a1, a2;
+  double* gpos;      // (n, 3) out: mesh part of dE/dr, Hartree/Bohr
+  float* dEdcn;      // (n) out
+  double* partials;  // (c6_blocks(n), 16): sum_i f_i (x) dE/dr_i
+};
+int launch_c6_bwd(const C6BwdArgs& a, hipStream_t s);
+
+}  // namespace fd3

@@ -1,0 +1,362 @@
+// B-spline spreading fused into the forward z pass, gathering fused into the
+// inverse z pass, and the per-atom adjoint of the CN -> C6 chain.
+//
+// A block of the z pass owns the lines (c, x, y0..y0+TY-1, all z) of one
+// channel c = species X * r + rank q.  Instead of reading a spread mesh it
+// builds those lines in LDS from the atoms of species X whose p x p x p
+// stencil touches them: atoms are sorted into mesh columns keyed by (X, x
+// node 0, y node 0), so the contributing atoms are the columns with x node 0
+// in {x - p + 1 .. x} and y node 0 in {y0 - p + 1 .. y0 + TY - 1}: at most two
+// contiguous runs per x node.  Each atom adds c_q Wx Wy Wz into the LDS lines
+// with LDS atomics; the block then runs the in-LDS z FFT and writes the half
+// spectrum, so the real mesh is never written to or read from HBM.
+//
+// The inverse z pass mirrors it: after the C2R transform the block holds the
+// real lines of dE/drho in LDS and the same atoms accumulate
+//   sum_nodes W dE/drho            (-> dE/dc, rank q)
+//   c_q sum_nodes dW/du_d dE/drho  (-> dE/du)
+// over the nodes in the block, added atomically to per-atom accumulators.
+// (points_to_mesh / mesh_to_points: src/fourierd3/mesh_interpolator_d3.py:99-189.)
+
+#include "fd3_internal.h"
+#include "fft_lds.h"
+#include "fused.h"
+#include "mesh_kernels.h"
+
+namespace fd3 {
+
+int launch_fill_i32(int32_t* p, size_t n, hipStream_t s);
+
+void col_carve(const FftPlan& p, int64_t n, Arena& ar, ColBuffers* b) {
+  const size_t ncol = (size_t)p.nS * p.nx * p.ny;
+  const int64_t nn = std::max<int64_t>(n, 1);
+  b->key = ar.take<int32_t>(nn);
+  b->tmp_i = ar.take<int4>(nn);
+  b->tmp_f = ar.take<float4>(nn);
+  b->colcount = ar.take<int32_t>(ncol);
+  b->colcur = ar.take<int32_t>(ncol);
+  b->colstart = ar.take<int32_t>(ncol + 1);
+  b->rec_i = ar.take<int4>(nn);
+  b->rec_f = ar.take<float4>(nn);
+  b->scharge = ar.take<float>(nn * p.r);
+  b->accA = ar.take<float>(nn * p.r);
+  b->accG = ar.take<float>(nn * 3);
+}
+
+// ------------------------------------------------------------ column sort ---
+__global__ void k_col_prep(FusedArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const double4 r = a.pos4[s];
+  const int X = a.spec_s[s];
+  const int ns[3] = {a.nx, a.ny, a.nz};
+  int b0[3];
+  float xo[3];
+  for (int d = 0; d < 3; ++d) {
+    const double f = r.x * a.inv[0 * 3 + d] + r.y * a.inv[1 * 3 + d] + r.z * a.inv[2 * 3 + d];
+    double x;
+    b0[d] = wrap(interp_base((double)ns[d] * f, a.order, &x), ns[d]);
+    xo[d] = (float)x;
+  }
+  const int key = (X * a.nx + b0[0]) * a.ny + b0[1];
+  a.cb.key[s] = key;
+  a.cb.tmp_i[s] = make_int4(b0[0], b0[1], b0[2], (int)s);
+  a.cb.tmp_f[s] = make_float4(xo[0], xo[1], xo[2], 0.f);
+  atomicAdd(a.cb.colcount + key, 1);
+}
+
+__global__ void k_col_scatter(FusedArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int key = a.cb.key[s];
+  const int pos = a.cb.colstart[key] + atomicAdd(a.cb.colcur + key, 1);
+  a.cb.rec_i[pos] = a.cb.tmp_i[s];
+  a.cb.rec_f[pos] = a.cb.tmp_f[s];
+  for (int q = 0; q < a.r; ++q) a.cb.scharge[(size_t)pos * a.r + q] = a.charges[s * a.r + q];
+}
+
+// Exclusive scan (defined in neigh.hip).
+__global__ void k_scan1(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ out, int64_t* __restrict__ total64);
+
+int col_sort(const FusedArgs& a, hipStream_t s) {
+  const size_t ncol = (size_t)a.nS * a.nx * a.ny;
+  int rc;
+  ProfScope _ps("k_col_sort", s);
+  if ((rc = launch_fill_i32(a.cb.colcount, ncol, s))) return rc;
+  if ((rc = launch_fill_i32(a.cb.colcur, ncol, s))) return rc;
+  if (a.n > 0) {
+    hipLaunchKernelGGL(k_col_prep, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
+    FD3_LAUNCH_CHECK("k_col_prep");
+  }
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, a.cb.colcount, (int64_t)ncol, a.cb.colstart,
+                     (int64_t*)nullptr);
+  FD3_LAUNCH_CHECK("k_scan1");
+  if (a.n > 0) {
+    hipLaunchKernelGGL(k_col_scatter, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
+    FD3_LAUNCH_CHECK("k_col_scatter");
+  }
+  return FD3_OK;
+}
+
+// Runs of the column-sorted atom list whose stencils touch plane x and rows
+// [y0, y0 + TY) of species X.  Built by thread 0 into shared memory.
+struct Segs {
+  int a0[2 * kMaxOrder];
+  int jx[2 * kMaxOrder];
+  int off[2 * kMaxOrder + 1];
+  int n;
+};
+
+__device__ inline void build_segments(Segs* sg, const int32_t* __restrict__ colstart, int X, int x, int y0, int TY,
+                                      int nx, int ny, int p) {
+  int ns = 0, off = 0;
+  for (int j = 0; j < p; ++j) {
+    const int bx0 = wrap(x - j, nx);
+    const int base = (X * nx + bx0) * ny;
+    int lo[2], hi[2], nr;
+    const int l = y0 - (p - 1), h = y0 + TY - 1;
+    if (TY + p - 1 >= ny) {
+      lo[0] = 0; hi[0] = ny - 1; nr = 1;
+    } else if (l < 0) {
+      lo[0] = l + ny; hi[0] = ny - 1; lo[1] = 0; hi[1] = h; nr = 2;
+    } else if (h >= ny) {
+      lo[0] = l; hi[0] = ny - 1; lo[1] = 0; hi[1] = h - ny; nr = 2;
+    } else {
+      lo[0] = l; hi[0] = h; nr = 1;
+    }
+    for (int k = 0; k < nr; ++k) {
+      const int a0 = colstart[base + lo[k]], a1 = colstart[base + hi[k] + 1];
+      if (a1 > a0) {
+        sg->a0[ns] = a0;
+        sg->jx[ns] = j;
+        sg->off[ns] = off;
+        off += a1 - a0;
+        ++ns;
+      }
+    }
+  }
+  sg->off[ns] = off;
+  sg->n = ns;
+}
+
+__device__ inline float pick(const float* w, int j, int p) {
+  float v = 0.f;
+  for (int i = 0; i < p; ++i)
+    if (i == j) v = w[i];
+  return v;
+}
+
+// ------------------------------------------------------ spread + z pass ---
+// grid (ny/TY, nx, C); LDS: TY lines of M+1 complex (= nz + 2 floats).
+__global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __restrict__ A, int TY,
+                                                      const float2* __restrict__ twz) {
+  extern __shared__ float2 lds[];
+  __shared__ Segs sg;
+  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = a.order;
+  const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
+  const int X = c / a.r, q = c - X * a.r;
+  for (int e = threadIdx.x; e < TY * ls; e += blockDim.x) lds[e] = make_float2(0.f, 0.f);
+  if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
+  __syncthreads();
+  float* ldsf = (float*)lds;
+  const int total = sg.off[sg.n];
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int k = 0;
+    while (i >= sg.off[k + 1]) ++k;
+    const int ai = sg.a0[k] + (i - sg.off[k]);
+    const int4 bi = a.cb.rec_i[ai];
+    const float4 xf = a.cb.rec_f[ai];
+    float w[kMaxOrder], dw[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder];
+    interp_weights<float>(p, a.interp, xf.x, w, dw);
+    const float wx = pick(w, sg.jx[k], p) * a.cb.scharge[(size_t)ai * a.r + q];
+    interp_weights<float>(p, a.interp, xf.y, wy, dw);
+    interp_weights<float>(p, a.interp, xf.z, wz, dw);
+    for (int jy = 0; jy < p; ++jy) {
+      int y = bi.y + jy;
+      if (y >= a.ny) y -= a.ny;
+      const int t = y - y0;
+      if (t < 0 || t >= TY) continue;
+      float* line = ldsf + 2 * t * ls;
+      const float v = wx * wy[jy];
+      for (int jz = 0; jz < p; ++jz) {
+        int z = bi.z + jz;
+        if (z >= nz) z -= nz;
+        atomicAdd(line + z, v * wz[jz]);
+      }
+    }
+  }
+  __syncthreads();
+  lds_fft<-1, kFftMaxB>(lds, M, TY, ls, twz, 2);
+  const int half = M >> 1;
+  for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
+    const int l = e / (half + 1), k = e - l * (half + 1);
+    r2c_split(lds + l * ls, M, k, twz);
+  }
+  __syncthreads();
+  float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
+  for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
+    const int kz = e / TY, t = e - kz * TY;
+    dst[(size_t)kz * a.ny + t] = lds[t * ls + kz];
+  }
+}
+
+// ------------------------------------------------------ z pass + gather ---
+__global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2* __restrict__ A, int TY,
+                                                      const float2* __restrict__ twz) {
+  extern __shared__ float2 lds[];
+  __shared__ Segs sg;
+  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = a.order;
+  const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
+  const int X = c / a.r, q = c - X * a.r;
+  const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
+  for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
+    const int kz = e / TY, t = e - kz * TY;
+    lds[t * ls + kz] = src[(size_t)kz * a.ny + t];
+  }
+  if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
+  __syncthreads();
+  const int half = M >> 1;
+  for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
+    const int l = e / (half + 1), k = e - l * (half + 1);
+    c2r_merge(lds + l * ls, M, k, twz);
+  }
+  __syncthreads();
+  lds_fft<1, kFftMaxB>(lds, M, TY, ls, twz, 2);
+  const float* ldsf = (const float*)lds;
+  const int total = sg.off[sg.n];
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int k = 0;
+    while (i >= sg.off[k + 1]) ++k;
+    const int ai = sg.a0[k] + (i - sg.off[k]);
+    const int4 bi = a.cb.rec_i[ai];
+    const float4 xf = a.cb.rec_f[ai];
+    float w[kMaxOrder], dw[kMaxOrder], wy[kMaxOrder], dwy[kMaxOrder], wz[kMaxOrder], dwz[kMaxOrder];
+    interp_weights<float>(p, a.interp, xf.x, w, dw);
+    const float wx = pick(w, sg.jx[k], p), dwx = pick(dw, sg.jx[k], p);
+    interp_weights<float>(p, a.interp, xf.y, wy, dwy);
+    interp_weights<float>(p, a.interp, xf.z, wz, dwz);
+    float acc = 0.f, g0 = 0.f, g1 = 0.f, g2 = 0.f;
+    for (int jy = 0; jy < p; ++jy) {
+      int y = bi.y + jy;
+      if (y >= a.ny) y -= a.ny;
+      const int t = y - y0;
+      if (t < 0 || t >= TY) continue;
+      const float* line = ldsf + 2 * t * ls;
+      float sz = 0.f, sdz = 0.f;
+      for (int jz = 0; jz < p; ++jz) {
+        int z = bi.z + jz;
+        if (z >= nz) z -= nz;
+        const float v = line[z];
+        sz += wz[jz] * v;
+        sdz += dwz[jz] * v;
+      }
+      acc += wy[jy] * sz;
+      g1 += dwy[jy] * sz;
+      g2 += wy[jy] * sdz;
+    }
+    const float cq = a.cb.scharge[(size_t)ai * a.r + q];
+    g0 = dwx * acc;
+    const int s = bi.w;
+    atomicAdd(a.cb.accA + (size_t)s * a.r + q, wx * acc);
+    atomicAdd(a.cb.accG + 3 * (size_t)s + 0, cq * g0);
+    atomicAdd(a.cb.accG + 3 * (size_t)s + 1, cq * wx * g1);
+    atomicAdd(a.cb.accG + 3 * (size_t)s + 2, cq * wx * g2);
+  }
+}
+
+int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s) {
+  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2);
+  ProfScope _ps("k_spread_r2c_z", s);
+  hipLaunchKernelGGL(k_spread_r2c_z, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, a, A, p.TY, t.twz);
+  FD3_LAUNCH_CHECK("k_spread_r2c_z");
+  return FD3_OK;
+}
+
+int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const float2* A, hipStream_t s) {
+  int rc;
+  if ((rc = launch_fill_f32(a.cb.accA, std::max<int64_t>(a.n, 1) * a.r, s))) return rc;
+  if ((rc = launch_fill_f32(a.cb.accG, std::max<int64_t>(a.n, 1) * 3, s))) return rc;
+  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2);
+  ProfScope _ps("k_c2r_z_gather", s);
+  hipLaunchKernelGGL(k_c2r_z_gather, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, a, A, p.TY, t.twz);
+  FD3_LAUNCH_CHECK("k_c2r_z_gather");
+  return FD3_OK;
+}
+
+// ------------------------------------------------------ C6 chain adjoint ---
+// Per atom, from the gathered sums:
+//   dE/dc_q  = accA_q + lambda_q c_q phi_XX(0)       (self term, src/fourierd3/core.py:547-550)
+//   dE/dr_b  = sum_d accG_d n_d cell^-1[b][d]
+//   dE/dCN   = sum_p L_p (g_p - sum_p' L_p' g_p') (-8)(CN' - ref_p) a_X,   g_p = sum_q dE/dc_q V[X,p,q]
+// (adjoint of src/fourierd3/core.py:469-478, 530-532 and of cncorr, :383-384).
+// Block partials (16 doubles): sum_i f_i (x) dE/dr_i, f = r cell^-1.
+__global__ void __launch_bounds__(256) k_c6_bwd(C6BwdArgs a) {
+  __shared__ double red[16 * 9];
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (s < a.n) {
+    const int X = a.spec_s[s];
+    const double ns[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
+    const double4 r = a.pos4[s];
+    double gr[3], f[3];
+    for (int d = 0; d < 3; ++d) f[d] = r.x * a.inv[0 * 3 + d] + r.y * a.inv[1 * 3 + d] + r.z * a.inv[2 * 3 + d];
+    for (int b = 0; b < 3; ++b) {
+      gr[b] = 0.0;
+      for (int d = 0; d < 3; ++d) gr[b] += (double)a.accG[3 * s + d] * ns[d] * a.inv[b * 3 + d];
+      a.gpos[3 * s + b] = gr[b];
+    }
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) part[i * 3 + j] = f[i] * gr[j];
+    const float qqx = a.qq[X * a.nS + X];
+    const float R = (float)(a.a1 * sqrt(3.0 * (double)qqx) + a.a2);
+    const float R2 = R * R, R6 = R2 * R2 * R2;
+    const float phi0 = (float)a.s6 / R6 + 3.0f * (float)a.s8 * qqx / (R6 * R2);
+    float cn = a.cn_raw[s];
+    float ca = 1.0f;
+    if (a.cncorr) {
+      ca = a.cncorr[2 * X + 1];
+      cn = ca * cn + a.cncorr[2 * X];
+    }
+    float L[kNRef];
+    float mx = -INFINITY;
+    for (int pp = 0; pp < kNRef; ++pp) {
+      const float ref = a.cnref[X * kNRef + pp];
+      const float d = cn - ref;
+      L[pp] = ref == -1.0f ? -INFINITY : -4.0f * d * d;
+      mx = fmaxf(mx, L[pp]);
+    }
+    float sum = 0.f;
+    for (int pp = 0; pp < kNRef; ++pp) {
+      L[pp] = L[pp] == -INFINITY ? 0.f : expf(L[pp] - mx);
+      sum += L[pp];
+    }
+    float gp[kNRef];
+    float gbar = 0.f;
+    for (int pp = 0; pp < kNRef; ++pp) {
+      L[pp] /= sum;
+      float g = 0.f;
+      for (int q = 0; q < a.r; ++q) {
+        const float dc = a.accA[s * a.r + q] + a.eigs[q] * a.charges[s * a.r + q] * phi0;
+        g += dc * a.vq[(X * kNRef + pp) * a.r + q];
+      }
+      gp[pp] = g;
+      gbar += L[pp] * g;
+    }
+    float dcn = 0.f;
+    for (int pp = 0; pp < kNRef; ++pp) {
+      const float ref = a.cnref[X * kNRef + pp];
+      if (ref != -1.0f) dcn += L[pp] * (gp[pp] - gbar) * (-8.0f) * (cn - ref);
+    }
+    a.dEdcn[s] = dcn * ca;
+  }
+  block_sum_store<9>(part, red, a.partials + (size_t)blockIdx.x * 16);
+}
+
+int launch_c6_bwd(const C6BwdArgs& a, hipStream_t s) {
+  ProfScope _ps("k_c6_bwd", s);
+  hipLaunchKernelGGL(k_c6_bwd, dim3(c6_blocks(a.n)), dim3(256), 0, s, a);
+  FD3_LAUNCH_CHECK("k_c6_bwd");
+  return FD3_OK;
+}
+
+}  // namespace fd3

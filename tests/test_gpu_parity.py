@@ -290,7 +290,9 @@ def test_edge_list_and_cell_list_paths_agree_at_headline_size():
     e = model(pos, torch.tensor(np.stack([i, j]), device=dev),
               torch.tensor(S, dtype=torch.float64, device=dev) @ torch.tensor(box.cell, device=dev))
     e.backward()
+    # two float32 evaluations with different summation orders (atomics vs
+    # per-atom sums): they agree to the per-result tolerance
     assert abs(float(e) * HARTREE_TO_EV - calc.results["energy"]) / box.n_atoms < 1e-7
-    assert np.abs(-pos.grad.cpu().numpy() * HARTREE_TO_EV - calc.results["forces"]).max() < 1e-6
+    assert np.abs(-pos.grad.cpu().numpy() * HARTREE_TO_EV - calc.results["forces"]).max() < F_TOL
     f = calc.results["forces"]
     print("[info] 100k: sum F", f.sum(0), "max |F|", np.abs(f).max(), flush=True)

@@ -42,6 +42,8 @@ def _args():
     ap.add_argument("--no-plain-gpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP events")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (reference kernel for HBM counters)")
     return ap.parse_args()
 
 
@@ -296,6 +298,13 @@ def main():
         sys.exit(2)
     info = run_gpu(args)
     model, box = info["model"], info["box"]
+    if args.calibrate_copy:
+        import torch
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     res = {
         "metric": METRIC,
         "value": box.n_atoms / info["dt"],

@@ -331,7 +331,7 @@ struct Common {
   int n_c6, n_ga, n_cn;
 };
 
-static void carve_common(Arena& ar, Common& c, int64_t n, int r, int64_t n_cn_items) {
+static void carve_common(Arena& ar, Common& c, int64_t n, int r, int cn_rows) {
   fft_tables_carve(c.plan, ar, &c.tab);
   c.buf1 = ar.take<float>(2 * c.plan.mesh_complex());  // spectrum
   c.buf2 = ar.take<float2>(c.plan.mesh_complex());     // z-pass output
@@ -342,7 +342,7 @@ static void carve_common(Arena& ar, Common& c, int64_t n, int r, int64_t n_cn_it
   c.gpos = ar.take<double>(std::max<int64_t>(3 * n, 3));
   c.n_c6 = c6_blocks(n);
   c.n_ga = c6_blocks(n);  // partials of k_c6_bwd
-  c.n_cn = cn_blocks(n_cn_items);
+  c.n_cn = std::max(cn_rows, 1);
   c.p_ks = ar.take<double>((size_t)c.plan.ks_blocks * 16);
   c.p_c6 = ar.take<double>((size_t)c.n_c6 * 8);
   c.p_ga = ar.take<double>((size_t)c.n_ga * 16);
@@ -451,7 +451,7 @@ static int edges_layout(const fd3_model* m, const fd3_geom* g, int64_t n, int64_
     set_error("negative atom or edge count");
     return FD3_ERR_ARG;
   }
-  carve_common(ar, c, n, m->n_rank, ne);
+  carve_common(ar, c, n, m->n_rank, cn_blocks(ne));
   *pos4 = ar.take<double4>(std::max<int64_t>(n, 1));
   *spec_s = ar.take<int32_t>(std::max<int64_t>(n, 1));
   return FD3_OK;
@@ -520,7 +520,7 @@ static int cells_layout(const fd3_model* m, const fd3_geom* g, int64_t n, Common
     return FD3_ERR_ARG;
   }
   if ((rc = cell_grid_make(g->cell, m->r_cut, &cg))) return rc;
-  carve_common(ar, c, n, m->n_rank, n);
+  carve_common(ar, c, n, m->n_rank, cg.nbins);
   cell_carve(cg, std::max<int64_t>(n, 1), ar, &cb);
   return FD3_OK;
 }
@@ -565,10 +565,10 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const vo
   CnArgs cna;
   cna.rcov = m->rcov; cna.rcut = (float)(m->r_cut * kA2B); cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
   cna.gpos = c.gpos; cna.partials = c.p_cn;
-  if ((rc = launch_cn_cells(ca, cna, s))) return rc;
+  if ((rc = launch_cn_cells(ca, cna, cg.nbins, s))) return rc;
   if ((rc = run_mesh_path(m, c, n, cb.pos4, cb.spec_s, cb.slot_atom, flags, out, s))) return rc;
   if (grad) {
-    if (n > 0 && (rc = launch_cn_bwd_cells(ca, cna, s))) return rc;
+    if (n > 0 && (rc = launch_cn_bwd_cells(ca, cna, cg.nbins, s))) return rc;
     if (out->grad_pos && (rc = launch_unpermute(c.gpos, cb.slot_atom, n, out->grad_pos, s))) return rc;
   }
   fd3_outputs o2 = *out;

@@ -15,6 +15,7 @@ struct ColBuffers {
   int32_t* colcount;  // (nS * nx * ny)
   int32_t* colcur;    // (nS * nx * ny)
   int32_t* colstart;  // (nS * nx * ny + 1)
+  int32_t* scantmp;   // scratch of the exclusive scan
   int4* rec_i;        // (n) column order
   float4* rec_f;      // (n) column order
   float* scharge;     // (n, r) column order

@@ -22,10 +22,9 @@
 #include "fft_lds.h"
 #include "fused.h"
 #include "mesh_kernels.h"
+#include "neigh_kernels.h"
 
 namespace fd3 {
-
-int launch_fill_i32(int32_t* p, size_t n, hipStream_t s);
 
 void col_carve(const FftPlan& p, int64_t n, Arena& ar, ColBuffers* b) {
   const size_t ncol = (size_t)p.nS * p.nx * p.ny;
@@ -36,6 +35,7 @@ void col_carve(const FftPlan& p, int64_t n, Arena& ar, ColBuffers* b) {
   b->colcount = ar.take<int32_t>(ncol);
   b->colcur = ar.take<int32_t>(ncol);
   b->colstart = ar.take<int32_t>(ncol + 1);
+  b->scantmp = ar.take<int32_t>(scan_tmp_elems((int64_t)ncol));
   b->rec_i = ar.take<int4>(nn);
   b->rec_f = ar.take<float4>(nn);
   b->scharge = ar.take<float>(nn * p.r);
@@ -75,8 +75,6 @@ __global__ void k_col_scatter(FusedArgs a) {
   for (int q = 0; q < a.r; ++q) a.cb.scharge[(size_t)pos * a.r + q] = a.charges[s * a.r + q];
 }
 
-// Exclusive scan (defined in neigh.hip).
-__global__ void k_scan1(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ out, int64_t* __restrict__ total64);
 
 int col_sort(const FusedArgs& a, hipStream_t s) {
   const size_t ncol = (size_t)a.nS * a.nx * a.ny;
@@ -88,9 +86,7 @@ int col_sort(const FusedArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_col_prep, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
     FD3_LAUNCH_CHECK("k_col_prep");
   }
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, a.cb.colcount, (int64_t)ncol, a.cb.colstart,
-                     (int64_t*)nullptr);
-  FD3_LAUNCH_CHECK("k_scan1");
+  if ((rc = scan_exclusive(a.cb.colcount, (int64_t)ncol, a.cb.colstart, a.cb.scantmp, nullptr, s))) return rc;
   if (a.n > 0) {
     hipLaunchKernelGGL(k_col_scatter, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
     FD3_LAUNCH_CHECK("k_col_scatter");

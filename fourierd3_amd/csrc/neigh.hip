@@ -139,6 +139,59 @@ __global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, 
   }
 }
 
+// Device-wide exclusive scan in three steps: per-block scans of 1024
+// elements, a scan of the block totals (k_scan1), and the offset add.
+__global__ void __launch_bounds__(256) k_scan_blocks(const int32_t* __restrict__ in, int64_t n,
+                                                     int32_t* __restrict__ out, int32_t* __restrict__ bsum) {
+  __shared__ int32_t sh[256];
+  const int64_t base = (int64_t)blockIdx.x * 1024 + 4 * threadIdx.x;
+  int32_t v[4];
+  int32_t t = 0;
+  for (int k = 0; k < 4; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    t += v[k];
+  }
+  sh[threadIdx.x] = t;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int32_t a = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += a;
+    __syncthreads();
+  }
+  int32_t run = sh[threadIdx.x] - t;
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 255) bsum[blockIdx.x] = sh[255];
+}
+
+__global__ void k_scan_add(int32_t* __restrict__ out, int64_t n, const int32_t* __restrict__ boff,
+                           int64_t nblocks, int64_t* __restrict__ total64) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] += boff[i / 1024];
+  if (i == 0) {
+    out[n] = boff[nblocks];
+    if (total64) total64[0] = boff[nblocks];
+  }
+}
+
+int scan_exclusive(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, int64_t* total, hipStream_t s) {
+  const int64_t nb = std::max<int64_t>(1, cdiv(n, 1024));
+  int32_t* bsum = tmp;
+  int32_t* boff = tmp + nb;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, in, n, out, bsum);
+  FD3_LAUNCH_CHECK("k_scan_blocks");
+  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, bsum, nb, boff, (int64_t*)nullptr);
+  FD3_LAUNCH_CHECK("k_scan1");
+  hipLaunchKernelGGL(k_scan_add, dim3(cdiv(std::max<int64_t>(n, 1), 256)), dim3(256), 0, s, out, n, boff, nb, total);
+  FD3_LAUNCH_CHECK("k_scan_add");
+  return FD3_OK;
+}
+
+size_t scan_tmp_elems(int64_t n) { return 2 * (size_t)std::max<int64_t>(1, cdiv(n, 1024)) + 1; }
+
 __global__ void k_scatter(const int32_t* __restrict__ abin, int64_t n, const int32_t* __restrict__ bstart,
                           int32_t* __restrict__ bcur, int32_t* __restrict__ slot_atom) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,44 +231,148 @@ __global__ void k_fill_sorted(const int32_t* __restrict__ slot_atom, int64_t n, 
 }
 
 // ----------------------------------------------------------- CN kernels ---
-__global__ void __launch_bounds__(256) k_cn_cells(CellArgs c, CnArgs a) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= c.n) return;
-  const float ri = a.rcov[c.spec_s[s]];
-  float cn = 0.f;
-  for_each_neighbour(c, s, [&](int t, double, double, double, double r2, int, int, int) {
-    float f, fp;
-    cn_term((float)sqrt(r2), ri + a.rcov[c.spec_s[t]], a.rcut, &f, &fp);
-    cn += f;
-  });
-  a.cn[s] = cn;
+// One block per home bin.  The images of the atoms in the neighbouring bins
+// (candidates) are staged in LDS with their lattice shift applied; 8 lanes
+// share each home atom and stride over the candidates, so every candidate is
+// read from HBM once per home bin instead of once per home atom.
+// Forward:  CN_s = sum_t f(r_st)
+// Backward: dE/dr_s -= sum_t (g_s + g_t) f'(r) v / r,
+//           virial  += sum_t g_s f'(r) v v^T / r        (directed pairs s -> t)
+// with g = dE/dCN (src/fourierd3/core.py:348-386 and its adjoint).
+constexpr int kCnCap = 1024;  // candidates staged per pass
+constexpr int kNbrMax = 343;  // neighbour bins of a single-pass plan (reach <= 3)
+constexpr int kCnLanes = 8;   // lanes per home atom
+
+struct NbrBin {
+  int wb;            // wrapped bin index
+  double sh[3];      // lattice shift vector (Bohr)
+  bool zero;         // shift is zero
+};
+
+__device__ inline NbrBin nbr_bin(const CellArgs& c, int b, int k) {
+  const int ny = 2 * c.sr[1] + 1, nz = 2 * c.sr[2] + 1;
+  const int dz = k % nz - c.sr[2], dy = (k / nz) % ny - c.sr[1], dx = k / (nz * ny) - c.sr[0];
+  const int bz = b % c.nb[2], by = (b / c.nb[2]) % c.nb[1], bx = b / (c.nb[2] * c.nb[1]);
+  const int ux = bx + dx, uy = by + dy, uz = bz + dz;
+  const int Sx = floordiv(ux, c.nb[0]), Sy = floordiv(uy, c.nb[1]), Sz = floordiv(uz, c.nb[2]);
+  NbrBin r;
+  r.wb = (wrap(ux, c.nb[0]) * c.nb[1] + wrap(uy, c.nb[1])) * c.nb[2] + wrap(uz, c.nb[2]);
+  for (int d = 0; d < 3; ++d) r.sh[d] = Sx * c.cell[d] + Sy * c.cell[3 + d] + Sz * c.cell[6 + d];
+  r.zero = (Sx | Sy | Sz) == 0;
+  return r;
 }
 
-// dE/dr_s -= sum_t (g_s + g_t) f'(r) v / r ;  virial += sum_t g_s f'(r) v v^T / r
-__global__ void __launch_bounds__(256) k_cn_bwd_cells(CellArgs c, CnArgs a) {
+template <bool BWD>
+__global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
+  __shared__ double4 cpos[kCnCap];  // image position (Bohr) and rcov of the candidate
+  __shared__ int cmeta[kCnCap];     // 2 * slot + (shift == 0)
+  __shared__ float cg[kCnCap];      // dE/dCN of the candidate (backward)
+  __shared__ int bcnt[kNbrMax + 1];
   __shared__ double red[16 * 6];
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double vir[6] = {0, 0, 0, 0, 0, 0};
-  if (s < c.n) {
-    const float ri = a.rcov[c.spec_s[s]];
-    const float gs = a.dEdcn[s];
-    double fx = 0, fy = 0, fz = 0;
-    for_each_neighbour(c, s, [&](int t, double vx, double vy, double vz, double r2, int, int, int) {
-      float f, fp;
-      const float r = (float)sqrt(r2);
-      cn_term(r, ri + a.rcov[c.spec_s[t]], a.rcut, &f, &fp);
-      const double w = (double)(fp / r);
-      const double wt = w * (double)(gs + a.dEdcn[t]);
-      fx -= wt * vx; fy -= wt * vy; fz -= wt * vz;
-      const double wv = w * gs;
-      vir[0] += wv * vx * vx; vir[1] += wv * vy * vy; vir[2] += wv * vz * vz;
-      vir[3] += wv * vx * vy; vir[4] += wv * vx * vz; vir[5] += wv * vy * vz;
-    });
-    a.gpos[3 * s + 0] += fx;
-    a.gpos[3 * s + 1] += fy;
-    a.gpos[3 * s + 2] += fz;
+  const int b = blockIdx.x;
+  const int hs = c.bstart[b], he = c.bstart[b + 1];
+  const int nbr = (2 * c.sr[0] + 1) * (2 * c.sr[1] + 1) * (2 * c.sr[2] + 1);
+  bool single = false;
+  if (nbr <= kNbrMax) {
+    for (int k = threadIdx.x; k < nbr; k += blockDim.x) {
+      const int wb = nbr_bin(c, b, k).wb;
+      bcnt[k + 1] = c.bstart[wb + 1] - c.bstart[wb];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      bcnt[0] = 0;
+      for (int k = 0; k < nbr; ++k) bcnt[k + 1] += bcnt[k];
+    }
+    __syncthreads();
+    single = bcnt[nbr] <= kCnCap;
   }
-  block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
+  float vir[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  // passes: the whole candidate list at once, or one neighbour bin (chunk) at a time
+  const int npass_bins = single ? 1 : nbr;
+  for (int kb = 0; kb < npass_bins; ++kb) {
+    int count, off0 = 0;
+    NbrBin fixed;
+    if (single) {
+      count = bcnt[nbr];
+    } else {
+      fixed = nbr_bin(c, b, kb);
+      count = c.bstart[fixed.wb + 1] - c.bstart[fixed.wb];
+    }
+    for (off0 = 0; off0 < count; off0 += kCnCap) {
+      const int nc = min(kCnCap, count - off0);
+      __syncthreads();
+      for (int e = threadIdx.x; e < nc; e += blockDim.x) {
+        NbrBin nbx;
+        int t;
+        if (single) {
+          int k = 0;
+          while (e >= bcnt[k + 1]) ++k;
+          nbx = nbr_bin(c, b, k);
+          t = c.bstart[nbx.wb] + (e - bcnt[k]);
+        } else {
+          nbx = fixed;
+          t = c.bstart[nbx.wb] + off0 + e;
+        }
+        const double4 p = c.pos4[t];
+        cpos[e] = make_double4(p.x + nbx.sh[0], p.y + nbx.sh[1], p.z + nbx.sh[2], (double)a.rcov[c.spec_s[t]]);
+        cmeta[e] = 2 * t + (nbx.zero ? 1 : 0);
+        if (BWD) cg[e] = a.dEdcn[t];
+      }
+      __syncthreads();
+      for (int h0 = hs; h0 < he; h0 += 256 / kCnLanes) {
+        const int s = h0 + threadIdx.x / kCnLanes, lane = threadIdx.x % kCnLanes;
+        float acc = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+        if (s < he) {
+          const double4 ps = c.pos4[s];
+          const float ri = a.rcov[c.spec_s[s]];
+          const float gs = BWD ? a.dEdcn[s] : 0.f;
+          for (int e = lane; e < nc; e += kCnLanes) {
+            const int m = cmeta[e];
+            if ((m & 1) && (m >> 1) == s) continue;  // the atom itself at zero shift
+            const double4 q = cpos[e];
+            const double vx = q.x - ps.x, vy = q.y - ps.y, vz = q.z - ps.z;
+            const double r2 = vx * vx + vy * vy + vz * vz;
+            if (!(r2 < c.rc2)) continue;
+            float f, fp;
+            const float r = (float)sqrt(r2);
+            cn_term(r, ri + (float)q.w, a.rcut, &f, &fp);
+            if (!BWD) {
+              acc += f;
+            } else {
+              const float w = fp / r;
+              const float wt = w * (gs + cg[e]);
+              const float fvx = (float)vx, fvy = (float)vy, fvz = (float)vz;
+              fx -= wt * fvx; fy -= wt * fvy; fz -= wt * fvz;
+              const float wv = w * gs;
+              vir[0] += wv * fvx * fvx; vir[1] += wv * fvy * fvy; vir[2] += wv * fvz * fvz;
+              vir[3] += wv * fvx * fvy; vir[4] += wv * fvx * fvz; vir[5] += wv * fvy * fvz;
+            }
+          }
+        }
+#pragma unroll
+        for (int o = kCnLanes / 2; o > 0; o >>= 1) {
+          acc += __shfl_xor(acc, o, kCnLanes);
+          fx += __shfl_xor(fx, o, kCnLanes);
+          fy += __shfl_xor(fy, o, kCnLanes);
+          fz += __shfl_xor(fz, o, kCnLanes);
+        }
+        if (s < he && lane == 0) {
+          if (!BWD) {
+            atomicAdd(a.cn + s, acc);
+          } else {
+            atomicAdd(a.gpos + 3 * s + 0, (double)fx);
+            atomicAdd(a.gpos + 3 * s + 1, (double)fy);
+            atomicAdd(a.gpos + 3 * s + 2, (double)fz);
+          }
+        }
+      }
+    }
+  }
+  if (BWD) {
+    double v[6];
+    for (int i = 0; i < 6; ++i) v[i] = vir[i];
+    block_sum_store<6>(v, red, a.partials + (size_t)blockIdx.x * 8);
+  }
 }
 
 // Edge-list forms: pairs given by (src, dst, shift in Angstrom).
@@ -442,17 +599,19 @@ int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* p
 
 int cn_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, 256)); }
 
-int launch_cn_cells(const CellArgs& c, const CnArgs& a, hipStream_t s) {
+int launch_cn_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   if (c.n <= 0) return FD3_OK;
   ProfScope _ps("k_cn_cells", s);
-  hipLaunchKernelGGL(k_cn_cells, dim3(cn_blocks(c.n)), dim3(256), 0, s, c, a);
+  int rc;
+  if ((rc = launch_fill_f32(a.cn, c.n, s))) return rc;
+  hipLaunchKernelGGL(k_cn_bins<false>, dim3(nbins), dim3(256), 0, s, c, a);
   FD3_LAUNCH_CHECK("k_cn_cells");
   return FD3_OK;
 }
 
-int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, hipStream_t s) {
+int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   ProfScope _ps("k_cn_bwd_cells", s);
-  hipLaunchKernelGGL(k_cn_bwd_cells, dim3(cn_blocks(c.n)), dim3(256), 0, s, c, a);
+  hipLaunchKernelGGL(k_cn_bins<true>, dim3(nbins), dim3(256), 0, s, c, a);
   FD3_LAUNCH_CHECK("k_cn_bwd_cells");
   return FD3_OK;
 }

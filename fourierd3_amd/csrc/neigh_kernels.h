@@ -65,8 +65,11 @@ CellArgs cell_args(const CellGrid& g, const CellBuffers& b, int64_t n);
 int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* pos, int dtype, const int32_t* species,
                hipStream_t s);
 int cn_blocks(int64_t n);
-int launch_cn_cells(const CellArgs& c, const CnArgs& a, hipStream_t s);
-int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, hipStream_t s);
+// one block per bin; the backward writes nbins rows of virial partials
+int launch_cn_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s);
+int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s);
+int scan_exclusive(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, int64_t* total, hipStream_t s);
+size_t scan_tmp_elems(int64_t n);
 int launch_cn_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, hipStream_t s);
 int launch_cn_bwd_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, double* gshift,
                         hipStream_t s);
@@ -76,5 +79,6 @@ int nb_fill(const CellArgs& c, const int32_t* offset, int64_t capacity, int64_t*
 
 // implemented in mesh.hip
 int launch_fill_i32(int32_t* p, size_t n, hipStream_t s);
+int launch_fill_f32(float* p, size_t n, hipStream_t s);
 
 }  // namespace fd3

@@ -53,35 +53,20 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ rho, fl
   extern __shared__ float2 lds[];
   const int M = nz >> 1, ls = M + 1;
   const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
+  float2* tws = lds + TY * ls;
+  stage_table(tws, twz, nz);
   const float2* src = (const float2*)rho + ((size_t)(c * nx + x) * ny + y0) * M;
   for (int e = threadIdx.x; e < TY * M; e += blockDim.x) {
     const int l = e / M, o = e - l * M;
     lds[l * ls + o] = src[e];
   }
   __syncthreads();
-  lds_fft<-1, kFftMaxB>(lds, M, TY, ls, twz, 2);
+  lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   // separate the even/odd halves: X[k] = E[k] + W_nz^k O[k]
   const int half = M >> 1;  // pairs k, M-k for k = 0..half
   for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
     const int l = e / (half + 1), k = e - l * (half + 1);
-    float2* L = lds + l * ls;
-    if (k == 0) {
-      const float2 z0 = L[0];
-      L[0] = make_float2(z0.x + z0.y, 0.0f);
-      L[M] = make_float2(z0.x - z0.y, 0.0f);
-    } else {
-      const int km = M - k;
-      const float2 zk = L[k], zm = L[km];
-      // E_k = (zk + conj zm)/2, O_k = -i (zk - conj zm)/2
-      const float2 Ek = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
-      const float2 Ok = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-      const float2 Em = make_float2(0.5f * (zm.x + zk.x), 0.5f * (zm.y - zk.y));
-      const float2 Om = make_float2(0.5f * (zm.y + zk.y), -0.5f * (zm.x - zk.x));
-      const float2 Xk = cadd(Ek, cmul(twz[k], Ok));
-      const float2 Xm = cadd(Em, cmul(twz[km], Om));
-      L[k] = Xk;
-      L[km] = Xm;
-    }
+    r2c_split(lds + l * ls, M, k, tws);
   }
   __syncthreads();
   float2* dst = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
@@ -98,33 +83,21 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
   extern __shared__ float2 lds[];
   const int M = nz >> 1, ls = M + 1;
   const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
+  float2* tws = lds + TY * ls;
+  stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
   for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
     const int kz = e / TY, t = e - kz * TY;
     lds[t * ls + kz] = src[(size_t)kz * ny + t];
   }
   __syncthreads();
-  // Z[k] = (X[k] + conj X[M-k]) + i (X[k] - conj X[M-k]) w^k,  w = exp(+2 pi i / nz)
   const int half = M >> 1;
   for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
     const int l = e / (half + 1), k = e - l * (half + 1);
-    float2* L = lds + l * ls;
-    if (k == 0) {
-      const float a = L[0].x, b = L[M].x;
-      L[0] = make_float2(a + b, a - b);
-    } else {
-      const int km = M - k;
-      const float2 xk = L[k], xm = L[km];
-      const float2 Ak = make_float2(xk.x + xm.x, xk.y - xm.y);
-      const float2 Bk = cmulc(make_float2(xk.x - xm.x, xk.y + xm.y), twz[k]);
-      const float2 Am = make_float2(xm.x + xk.x, xm.y - xk.y);
-      const float2 Bm = cmulc(make_float2(xm.x - xk.x, xm.y + xk.y), twz[km]);
-      L[k] = make_float2(Ak.x - Bk.y, Ak.y + Bk.x);
-      L[km] = make_float2(Am.x - Bm.y, Am.y + Bm.x);
-    }
+    c2r_merge(lds + l * ls, M, k, tws);
   }
   __syncthreads();
-  lds_fft<1, kFftMaxB>(lds, M, TY, ls, twz, 2);
+  lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   float2* dst = (float2*)out + ((size_t)(c * nx + x) * ny + y0) * M;
   for (int e = threadIdx.x; e < TY * M; e += blockDim.x) {
     const int l = e / M, o = e - l * M;
@@ -140,6 +113,8 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
   extern __shared__ float2 lds[];
   const int ls = ny + 1;
   const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
+  float2* tws = lds + TX * ls;
+  stage_table(tws, twy, ny);
   if (DIR < 0) {
     const float2* src = in + ((size_t)(c * nx + x0) * M1 + kz) * ny;
     for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
@@ -154,7 +129,7 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
     }
   }
   __syncthreads();
-  lds_fft<DIR, kFftMaxB>(lds, ny, TX, ls, twy, 1);
+  lds_fft<DIR, kFftMaxB>(lds, ny, TX, ls, tws, 1);
   if (DIR < 0) {
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
     for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
@@ -180,12 +155,14 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
   const size_t l0 = (size_t)blockIdx.x * LPB;
   const int nl = min(LPB, nlines - (int)l0);
   float2* base = spec + l0 * nx;
+  float2* tws = lds + LPB * ls;
+  stage_table(tws, twx, nx);
   for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
     const int l = e / nx, o = e - l * nx;
     lds[l * ls + o] = base[e];
   }
   __syncthreads();
-  lds_fft<DIR, kFftMaxB>(lds, nx, nl, ls, twx, 1);
+  lds_fft<DIR, kFftMaxB>(lds, nx, nl, ls, tws, 1);
   for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
     const int l = e / nx, o = e - l * nx;
     base[e] = lds[l * ls + o];
@@ -227,8 +204,10 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   const int ls = nx + 1;
   float2* S = lds;
   float2* P = lds + nlines * ls;
-  PairK* pairs = (PairK*)(P + nlines * ls);  // (nS, nS)
-  float* dR = (float*)(pairs + nS * nS);     // dR0_XY/da1 = sqrt(3 QQ)
+  float2* tws = P + nlines * ls;              // twiddles of the x lines
+  PairK* pairs = (PairK*)(tws + nx);          // (nS, nS)
+  float* dR = (float*)(pairs + nS * nS);      // dR0_XY/da1 = sqrt(3 QQ)
+  stage_table(tws, a.twx, nx);
   // per-pair constants
   for (int t = threadIdx.x; t < nS * nS; t += blockDim.x) {
     const float qq = a.qq[t];
@@ -253,7 +232,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     S[l * ls + kx] = v;
   }
   __syncthreads();
-  lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, a.twx, 1);
+  lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
 
   double acc[11];
 #pragma unroll
@@ -342,7 +321,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   block_sum_store<11>(acc, red, a.partials + (size_t)bid * 16);
   if (!a.need_grad) return;
-  lds_fft<1, kFftMaxB>(P, nx, nlines, ls, a.twx, 1);
+  lds_fft<1, kFftMaxB>(P, nx, nlines, ls, tws, 1);
   for (int e = threadIdx.x; e < nlines * nx; e += blockDim.x) {
     const int l = e / nx, kx = e - l * nx;
     const int kyi = l % NK, xq = l / NK, qi = xq % RT, X = xq / RT;
@@ -367,9 +346,10 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   p->M1 = p->M + 1;
   p->nS = nS; p->r = r; p->C = nS * r;
   const int budget = kFftLines;  // complex values per block for the pass kernels
-  p->TY = std::max(1, std::min(p->ny, budget / (p->M + 1)));
+  // lines per block: the register budget caps lines * length at kFftLines
+  p->TY = std::max(1, std::min(p->ny, budget / std::max(p->M, 1)));
   p->TY = 1 << (31 - __builtin_clz(p->TY));
-  p->TX = std::max(1, std::min(p->nx, budget / (p->ny + 1)));
+  p->TX = std::max(1, std::min(p->nx, budget / p->ny));
   p->TX = 1 << (31 - __builtin_clz(p->TX));
   p->LPB = std::max(1, budget / (p->nx + 1));
   // k-space: two LDS buffers of nS*RT*NK lines of nx+1 complex plus the
@@ -379,7 +359,7 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
     return FD3_ERR_UNSUPPORTED;
   }
   auto ks_bytes = [&](int rt_, int nk_) {
-    return (size_t)2 * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * nS * 16;
+    return (size_t)2 * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)p->nx * sizeof(float2) + (size_t)nS * nS * 16;
   };
   const size_t ks_budget = 62 * 1024;
   int RT = std::min(r, 8), NK = std::max(1, std::min(p->ny, 256 / p->nx));
@@ -425,14 +405,14 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
 
 int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float2* A, float2* spec, hipStream_t s) {
   {
-    const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2);
+    const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
     ProfScope _ps("k_r2c_z", s);
     hipLaunchKernelGGL(k_r2c_z, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, rho, A, p.nx, p.ny, p.nz, p.TY,
                        t.twz);
     FD3_LAUNCH_CHECK("k_r2c_z");
   }
   {
-    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2);
+    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
     ProfScope _ps("k_c2c_y_fwd", s);
     hipLaunchKernelGGL(k_c2c_y<-1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, A, spec, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
@@ -443,7 +423,7 @@ int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float
 
 int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream_t s) {
   const int nlines = p.C * p.M1 * p.ny;
-  const size_t sh = (size_t)p.LPB * (p.nx + 1) * sizeof(float2);
+  const size_t sh = (size_t)p.LPB * (p.nx + 1) * sizeof(float2) + (size_t)p.nx * sizeof(float2);
   const dim3 grid((nlines + p.LPB - 1) / p.LPB);
   ProfScope _ps("k_c2c_x", s);
   if (dir < 0)
@@ -456,14 +436,14 @@ int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream
 
 int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, float2* A, float* out, hipStream_t s) {
   {
-    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2);
+    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
     ProfScope _ps("k_c2c_y_inv", s);
     hipLaunchKernelGGL(k_c2c_y<1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, spec, A, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
     FD3_LAUNCH_CHECK("k_c2c_y inv");
   }
   {
-    const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2);
+    const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
     ProfScope _ps("k_c2r_z", s);
     hipLaunchKernelGGL(k_c2r_z, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, A, out, p.nx, p.ny, p.nz, p.TY,
                        t.twz);
@@ -473,7 +453,7 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
 }
 
 int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
-  const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2);
+  const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
   if (dir < 0) {
     ProfScope _ps("k_c2c_y_fwd", s);
     hipLaunchKernelGGL(k_c2c_y<-1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,

@@ -5,6 +5,12 @@
 
 namespace fd3 {
 
+// Copy a twiddle table of n entries into LDS (all threads; the caller's next
+// __syncthreads() publishes it).
+__device__ inline void stage_table(float2* dst, const float2* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
+}
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

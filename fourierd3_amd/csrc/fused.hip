@@ -151,6 +151,8 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
   const int nz = a.nz, M = nz >> 1, ls = M + 1, p = a.order;
   const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
   const int X = c / a.r, q = c - X * a.r;
+  float2* tws = lds + TY * ls;
+  stage_table(tws, twz, nz);
   for (int e = threadIdx.x; e < TY * ls; e += blockDim.x) lds[e] = make_float2(0.f, 0.f);
   if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
   __syncthreads();
@@ -182,11 +184,11 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
     }
   }
   __syncthreads();
-  lds_fft<-1, kFftMaxB>(lds, M, TY, ls, twz, 2);
+  lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   const int half = M >> 1;
   for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
     const int l = e / (half + 1), k = e - l * (half + 1);
-    r2c_split(lds + l * ls, M, k, twz);
+    r2c_split(lds + l * ls, M, k, tws);
   }
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
@@ -204,6 +206,8 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   const int nz = a.nz, M = nz >> 1, ls = M + 1, p = a.order;
   const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
   const int X = c / a.r, q = c - X * a.r;
+  float2* tws = lds + TY * ls;
+  stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
     const int kz = e / TY, t = e - kz * TY;
@@ -214,10 +218,10 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   const int half = M >> 1;
   for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
     const int l = e / (half + 1), k = e - l * (half + 1);
-    c2r_merge(lds + l * ls, M, k, twz);
+    c2r_merge(lds + l * ls, M, k, tws);
   }
   __syncthreads();
-  lds_fft<1, kFftMaxB>(lds, M, TY, ls, twz, 2);
+  lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   const float* ldsf = (const float*)lds;
   const int total = sg.off[sg.n];
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
@@ -261,7 +265,7 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
 }
 
 int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s) {
-  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2);
+  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_spread_r2c_z", s);
   hipLaunchKernelGGL(k_spread_r2c_z, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, a, A, p.TY, t.twz);
   FD3_LAUNCH_CHECK("k_spread_r2c_z");
@@ -272,7 +276,7 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
   int rc;
   if ((rc = launch_fill_f32(a.cb.accA, std::max<int64_t>(a.n, 1) * a.r, s))) return rc;
   if ((rc = launch_fill_f32(a.cb.accG, std::max<int64_t>(a.n, 1) * 3, s))) return rc;
-  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2);
+  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_c2r_z_gather", s);
   hipLaunchKernelGGL(k_c2r_z_gather, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, a, A, p.TY, t.twz);
   FD3_LAUNCH_CHECK("k_c2r_z_gather");

@@ -56,22 +56,15 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ rho, fl
   float2* tws = lds + TY * ls;
   stage_table(tws, twz, nz);
   const float2* src = (const float2*)rho + ((size_t)(c * nx + x) * ny + y0) * M;
-  for (int e = threadIdx.x; e < TY * M; e += blockDim.x) {
-    const int l = e / M, o = e - l * M;
-    lds[l * ls + o] = src[e];
-  }
+  const int lM = ilog2(M), lT = ilog2(TY);
+  for (int e = threadIdx.x; e < TY * M; e += blockDim.x) lds[(e >> lM) * ls + (e & (M - 1))] = src[e];
   __syncthreads();
   lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
-  // separate the even/odd halves: X[k] = E[k] + W_nz^k O[k]
-  const int half = M >> 1;  // pairs k, M-k for k = 0..half
-  for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
-    const int l = e / (half + 1), k = e - l * (half + 1);
-    r2c_split(lds + l * ls, M, k, tws);
-  }
+  z_pairs<true>(lds, TY, ls, M, tws);  // X[k] = E[k] + W_nz^k O[k]
   __syncthreads();
   float2* dst = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
   for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e / TY, t = e - kz * TY;
+    const int kz = e >> lT, t = e & (TY - 1);
     dst[(size_t)kz * ny + t] = lds[t * ls + kz];
   }
 }
@@ -86,23 +79,17 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
   float2* tws = lds + TY * ls;
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
+  const int lM = ilog2(M), lT = ilog2(TY);
   for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e / TY, t = e - kz * TY;
+    const int kz = e >> lT, t = e & (TY - 1);
     lds[t * ls + kz] = src[(size_t)kz * ny + t];
   }
   __syncthreads();
-  const int half = M >> 1;
-  for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
-    const int l = e / (half + 1), k = e - l * (half + 1);
-    c2r_merge(lds + l * ls, M, k, tws);
-  }
+  z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
   lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   float2* dst = (float2*)out + ((size_t)(c * nx + x) * ny + y0) * M;
-  for (int e = threadIdx.x; e < TY * M; e += blockDim.x) {
-    const int l = e / M, o = e - l * M;
-    dst[e] = lds[l * ls + o];
-  }
+  for (int e = threadIdx.x; e < TY * M; e += blockDim.x) dst[e] = lds[(e >> lM) * ls + (e & (M - 1))];
 }
 
 // ------------------------------------------------------------ y passes ---
@@ -115,16 +102,17 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
   const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
   float2* tws = lds + TX * ls;
   stage_table(tws, twy, ny);
+  const int lny = ilog2(ny), lTX = ilog2(TX);
   if (DIR < 0) {
     const float2* src = in + ((size_t)(c * nx + x0) * M1 + kz) * ny;
     for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int l = e / ny, y = e - l * ny;
+      const int l = e >> lny, y = e & (ny - 1);
       lds[l * ls + y] = src[(size_t)l * M1 * ny + y];
     }
   } else {
     const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
     for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int ky = e / TX, l = e - ky * TX;
+      const int ky = e >> lTX, l = e & (TX - 1);
       lds[l * ls + ky] = src[(size_t)ky * nx + l];
     }
   }
@@ -133,13 +121,13 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
   if (DIR < 0) {
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
     for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int ky = e / TX, l = e - ky * TX;
+      const int ky = e >> lTX, l = e & (TX - 1);
       dst[(size_t)ky * nx + l] = lds[l * ls + ky];
     }
   } else {
     float2* dst = out + ((size_t)(c * nx + x0) * M1 + kz) * ny;
     for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int l = e / ny, y = e - l * ny;
+      const int l = e >> lny, y = e & (ny - 1);
       dst[(size_t)l * M1 * ny + y] = lds[l * ls + y];
     }
   }
@@ -157,16 +145,11 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
   float2* base = spec + l0 * nx;
   float2* tws = lds + LPB * ls;
   stage_table(tws, twx, nx);
-  for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
-    const int l = e / nx, o = e - l * nx;
-    lds[l * ls + o] = base[e];
-  }
+  const int lnx = ilog2(nx);
+  for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) lds[(e >> lnx) * ls + (e & (nx - 1))] = base[e];
   __syncthreads();
   lds_fft<DIR, kFftMaxB>(lds, nx, nl, ls, tws, 1);
-  for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) {
-    const int l = e / nx, o = e - l * nx;
-    base[e] = lds[l * ls + o];
-  }
+  for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) base[e] = lds[(e >> lnx) * ls + (e & (nx - 1))];
 }
 
 // ------------------------------------------------- fused k-space kernel ---
@@ -202,6 +185,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   const int rt = min(RT, a.r - q0);
   const int nlines = nS * RT * NK;
   const int ls = nx + 1;
+  const int lnx = ilog2(nx), lNK = ilog2(NK), lRT = ilog2(RT);  // RT, NK, nx are powers of two
   float2* S = lds;
   float2* P = lds + nlines * ls;
   float2* tws = P + nlines * ls;              // twiddles of the x lines
@@ -222,8 +206,8 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   }
   // load lines: line l = (X*RT + qi)*NK + kyi
   for (int e = threadIdx.x; e < nlines * nx; e += blockDim.x) {
-    const int l = e / nx, kx = e - l * nx;
-    const int kyi = l % NK, xq = l / NK, qi = xq % RT, X = xq / RT;
+    const int l = e >> lnx, kx = e & (nx - 1);
+    const int kyi = l & (NK - 1), xq = l >> lNK, qi = xq & (RT - 1), X = xq >> lRT;
     float2 v = make_float2(0.f, 0.f);
     if (qi < rt) {
       const int c = X * a.r + q0 + qi;
@@ -240,7 +224,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
   const float mz = (float)kz;
   for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
-    const int kyi = t / nx, kx = t - kyi * nx;
+    const int kyi = t >> lnx, kx = t & (nx - 1);
     const int ky = ky0 + kyi;
     const float mx = (float)(kx < (nx + 1) / 2 ? kx : kx - nx);
     const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
@@ -323,8 +307,8 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   if (!a.need_grad) return;
   lds_fft<1, kFftMaxB>(P, nx, nlines, ls, tws, 1);
   for (int e = threadIdx.x; e < nlines * nx; e += blockDim.x) {
-    const int l = e / nx, kx = e - l * nx;
-    const int kyi = l % NK, xq = l / NK, qi = xq % RT, X = xq / RT;
+    const int l = e >> lnx, kx = e & (nx - 1);
+    const int kyi = l & (NK - 1), xq = l >> lNK, qi = xq & (RT - 1), X = xq >> lRT;
     if (qi < rt) {
       const int c = X * a.r + q0 + qi;
       a.spec[(((size_t)c * a.M1 + kz) * a.ny + ky0 + kyi) * nx + kx] = P[l * ls + kx];
@@ -362,11 +346,14 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
     return (size_t)2 * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)p->nx * sizeof(float2) + (size_t)nS * nS * 16;
   };
   const size_t ks_budget = 62 * 1024;
-  int RT = std::min(r, 8), NK = std::max(1, std::min(p->ny, 256 / p->nx));
+  // RT: ranks per block, a power of two (a last partial tile is padded with zero lines)
+  int RT = 1;
+  while (RT < std::min(r, 8)) RT <<= 1;
+  int NK = std::max(1, std::min(p->ny, 256 / p->nx));
   auto ks_fits = [&](int rt_, int nk_) {
     return ks_bytes(rt_, nk_) <= ks_budget && (size_t)nS * rt_ * nk_ * p->nx <= (size_t)kFftLines;
   };
-  while (RT > 1 && !ks_fits(RT, NK)) RT = (RT + 1) / 2;
+  while (RT > 1 && !ks_fits(RT, NK)) RT >>= 1;
   while (NK > 1 && !ks_fits(RT, NK)) NK >>= 1;
   if (!ks_fits(RT, NK)) {
     set_error("k-space tile of %d species x %d points does not fit in LDS", nS, p->nx);

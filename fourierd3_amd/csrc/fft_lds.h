@@ -30,6 +30,7 @@ __device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __
   if (n <= 1) return;
   const int log2n = __ffs(n) - 1;
   const int nq = n >> 2;
+  const int lognq = log2n - 2;
   const int total4 = nlines * nq;
   int Ns = 1;
   for (int s = 0; s < (log2n >> 1); ++s) {
@@ -40,7 +41,7 @@ __device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __
     for (int c = 0; c < MAXB; ++c) {
       const int b = threadIdx.x + c * blockDim.x;
       if (b < total4) {
-        const int line = b / nq, j = b - line * nq;
+        const int line = b >> lognq, j = b & (nq - 1);
         float2* L = buf + line * ls;
         const int k = j & (Ns - 1);
         float2 a0 = L[j], a1 = L[j + nq], a2 = L[j + 2 * nq], a3 = L[j + 3 * nq];
@@ -75,7 +76,7 @@ __device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __
     const int nh = n >> 1;
     const int total2 = nlines * nh;
     for (int b = threadIdx.x; b < total2; b += blockDim.x) {
-      const int line = b / nh, j = b - line * nh;
+      const int line = b >> (log2n - 1), j = b & (nh - 1);
       float2* L = buf + line * ls;
       float2 w = tw[j * tws];
       if (DIR > 0) w.y = -w.y;
@@ -86,6 +87,8 @@ __device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __
     __syncthreads();
   }
 }
+
+__device__ __forceinline__ int ilog2(int n) { return __ffs(n) - 1; }  // n a power of two
 
 // The z-pass pre/post-processing that turns a length-M complex FFT of a real
 // line of length nz = 2M (packed as z[m] = x[2m] + i x[2m+1]) into its
@@ -126,6 +129,25 @@ __device__ inline void c2r_merge(float2* L, int M, int k, const float2* __restri
     const float2 Bm = cmulc(make_float2(xm.x - xk.x, xm.y + xk.y), twz[km]);
     L[k] = make_float2(Ak.x - Bk.y, Ak.y + Bk.x);
     L[km] = make_float2(Am.x - Bm.y, Am.y + Bm.x);
+  }
+}
+
+// Apply r2c_split (R2C = true) or c2r_merge to every pair (k, M-k), k = 0..M/2,
+// of nlines LDS lines of stride ls (M a power of two).
+template <bool R2C>
+__device__ inline void z_pairs(float2* lds, int nlines, int ls, int M, const float2* __restrict__ tw) {
+  const int half = M >> 1;
+  if (half > 0) {
+    const int lh = ilog2(half);
+    for (int e = threadIdx.x; e < (nlines << lh); e += blockDim.x) {
+      const int l = e >> lh, k = e & (half - 1);
+      if (R2C) r2c_split(lds + l * ls, M, k, tw);
+      else c2r_merge(lds + l * ls, M, k, tw);
+    }
+  }
+  for (int l = threadIdx.x; l < nlines; l += blockDim.x) {  // k = M/2 (k = 0 when M = 1)
+    if (R2C) r2c_split(lds + l * ls, M, half, tw);
+    else c2r_merge(lds + l * ls, M, half, tw);
   }
 }
 

@@ -185,15 +185,12 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
   }
   __syncthreads();
   lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
-  const int half = M >> 1;
-  for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
-    const int l = e / (half + 1), k = e - l * (half + 1);
-    r2c_split(lds + l * ls, M, k, tws);
-  }
+  z_pairs<true>(lds, TY, ls, M, tws);
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
+  const int lT = ilog2(TY);
   for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e / TY, t = e - kz * TY;
+    const int kz = e >> lT, t = e & (TY - 1);
     dst[(size_t)kz * a.ny + t] = lds[t * ls + kz];
   }
 }
@@ -209,17 +206,14 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   float2* tws = lds + TY * ls;
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
+  const int lT = ilog2(TY);
   for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e / TY, t = e - kz * TY;
+    const int kz = e >> lT, t = e & (TY - 1);
     lds[t * ls + kz] = src[(size_t)kz * a.ny + t];
   }
   if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
   __syncthreads();
-  const int half = M >> 1;
-  for (int e = threadIdx.x; e < TY * (half + 1); e += blockDim.x) {
-    const int l = e / (half + 1), k = e - l * (half + 1);
-    c2r_merge(lds + l * ls, M, k, tws);
-  }
+  z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
   lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   const float* ldsf = (const float*)lds;

@@ -33,15 +33,16 @@ __device__ inline void cn_term(float r, float rcs, float rcut, float* f, float* 
   const bool far = r > r1;
   const float rc = far ? r : r1;
   const float dr = rc - r1, dc = rcut - rc;
-  const float den = dc * dc + 1e-6f;
-  const float k = 16.0f + dr * dr / den;
+  const float iden = __frcp_rn(dc * dc + 1e-6f);
+  const float k = 16.0f + dr * dr * iden;
   const float ratio = (4.0f / 3.0f) * rcs;
-  const float arg = ratio / r - 1.0f;
+  const float ir = __frcp_rn(r);
+  const float arg = ratio * ir - 1.0f;
   const float s = k * arg;
-  const float sig = 1.0f / (1.0f + expf(-s));
+  const float sig = __frcp_rn(1.0f + __expf(-s));
   *f = sig;
-  const float dk = far ? (2.0f * dr / den + dr * dr * 2.0f * dc / (den * den)) : 0.0f;
-  const float ds = dk * arg - k * ratio / (r * r);
+  const float dk = far ? 2.0f * dr * iden * (1.0f + dr * dc * iden) : 0.0f;
+  const float ds = dk * arg - k * ratio * ir * ir;
   *fp = sig * (1.0f - sig) * ds;
 }
 
@@ -264,14 +265,21 @@ __device__ inline NbrBin nbr_bin(const CellArgs& c, int b, int k) {
 
 template <bool BWD>
 __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
-  __shared__ double4 cpos[kCnCap];  // image position (Bohr) and rcov of the candidate
+  __shared__ float4 cpos[kCnCap];   // image position relative to the bin's first atom, rcov
   __shared__ int cmeta[kCnCap];     // 2 * slot + (shift == 0)
   __shared__ float cg[kCnCap];      // dE/dCN of the candidate (backward)
   __shared__ int bcnt[kNbrMax + 1];
   __shared__ double red[16 * 6];
   const int b = blockIdx.x;
   const int hs = c.bstart[b], he = c.bstart[b + 1];
+  if (hs == he) {  // empty bin (uniform across the block): zero virial partials
+    if (BWD && threadIdx.x < 8) a.partials[(size_t)blockIdx.x * 8 + threadIdx.x] = 0.0;
+    return;
+  }
   const int nbr = (2 * c.sr[0] + 1) * (2 * c.sr[1] + 1) * (2 * c.sr[2] + 1);
+  // origin of the block's float coordinates: differences stay within a few
+  // bin widths, so float keeps ~1e-7 relative precision on pair vectors
+  const double4 org = c.pos4[hs];
   bool single = false;
   if (nbr <= kNbrMax) {
     for (int k = threadIdx.x; k < nbr; k += blockDim.x) {
@@ -314,7 +322,8 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
           t = c.bstart[nbx.wb] + off0 + e;
         }
         const double4 p = c.pos4[t];
-        cpos[e] = make_double4(p.x + nbx.sh[0], p.y + nbx.sh[1], p.z + nbx.sh[2], (double)a.rcov[c.spec_s[t]]);
+        cpos[e] = make_float4((float)(p.x + nbx.sh[0] - org.x), (float)(p.y + nbx.sh[1] - org.y),
+                              (float)(p.z + nbx.sh[2] - org.z), a.rcov[c.spec_s[t]]);
         cmeta[e] = 2 * t + (nbx.zero ? 1 : 0);
         if (BWD) cg[e] = a.dEdcn[t];
       }
@@ -323,29 +332,30 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
         const int s = h0 + threadIdx.x / kCnLanes, lane = threadIdx.x % kCnLanes;
         float acc = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
         if (s < he) {
-          const double4 ps = c.pos4[s];
+          const double4 pd = c.pos4[s];
+          const float px = (float)(pd.x - org.x), py = (float)(pd.y - org.y), pz = (float)(pd.z - org.z);
           const float ri = a.rcov[c.spec_s[s]];
           const float gs = BWD ? a.dEdcn[s] : 0.f;
+          const float rc2 = (float)c.rc2;
           for (int e = lane; e < nc; e += kCnLanes) {
             const int m = cmeta[e];
             if ((m & 1) && (m >> 1) == s) continue;  // the atom itself at zero shift
-            const double4 q = cpos[e];
-            const double vx = q.x - ps.x, vy = q.y - ps.y, vz = q.z - ps.z;
-            const double r2 = vx * vx + vy * vy + vz * vz;
-            if (!(r2 < c.rc2)) continue;
+            const float4 q = cpos[e];
+            const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
+            const float r2 = vx * vx + vy * vy + vz * vz;
+            if (!(r2 < rc2)) continue;
             float f, fp;
-            const float r = (float)sqrt(r2);
-            cn_term(r, ri + (float)q.w, a.rcut, &f, &fp);
+            const float r = sqrtf(r2);
+            cn_term(r, ri + q.w, a.rcut, &f, &fp);
             if (!BWD) {
               acc += f;
             } else {
-              const float w = fp / r;
+              const float w = fp * __frcp_rn(r);
               const float wt = w * (gs + cg[e]);
-              const float fvx = (float)vx, fvy = (float)vy, fvz = (float)vz;
-              fx -= wt * fvx; fy -= wt * fvy; fz -= wt * fvz;
+              fx -= wt * vx; fy -= wt * vy; fz -= wt * vz;
               const float wv = w * gs;
-              vir[0] += wv * fvx * fvx; vir[1] += wv * fvy * fvy; vir[2] += wv * fvz * fvz;
-              vir[3] += wv * fvx * fvy; vir[4] += wv * fvx * fvz; vir[5] += wv * fvy * fvz;
+              vir[0] += wv * vx * vx; vir[1] += wv * vy * vy; vir[2] += wv * vz * vz;
+              vir[3] += wv * vx * vy; vir[4] += wv * vx * vz; vir[5] += wv * vy * vz;
             }
           }
         }

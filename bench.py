@@ -59,7 +59,9 @@ def kernel_bytes(name: str, plan: dict) -> float | None:
         "k_c2r_z": 8 * Mh * C + 4 * M * C,
         "k_fill": 4 * M * C,                     # zero the real mesh
         "k_spread": 4 * M * C,                   # mesh written once (accumulated in L2)
-        "k_gather_bwd": None,
+        # spreading fused into the z pass: the real mesh never reaches HBM
+        "k_spread_r2c_z": 8 * Mh * C + 36 * N,   # write half spectrum; read positions, species, charges
+        "k_c2r_z_gather": 8 * Mh * C + 36 * N,   # read half spectrum; per-atom records in, accumulators out
     }
     return table.get(name)
 

@@ -97,7 +97,7 @@ def synthetic_c6ref() -> np.ndarray:
     Entries whose reference is unused (cnref = -1) are zero.
     1/(w + w') = int_0^inf exp(-(w + w') t) dt is a Gram kernel, so every
     species block of ``decomp`` is symmetric positive semi-definite, as the real
-    Casimir-Polder table is.  With c6tol = 1 % the H/C/N/O block has rank 3.
+    Casimir-Polder table is.  With c6tol = 1 % the H/C/N/O block has rank 2 and O/Si rank 3.
     """
     cnref = load_cnref().astype(np.float64)
     nz = cnref.shape[0]

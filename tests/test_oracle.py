@@ -163,6 +163,78 @@ def test_neighbour_list_matches_naive_enumeration(case):
     assert len(got) == len(i)
 
 
+# ------------------------------------------------------------------ CN and C6 chain
+def _pair_box(numbers, pos_A, L_A=30.0, **kw):
+    """Isolated cluster in a cubic box large enough that no image is within r_cut."""
+    pos_A = np.asarray(pos_A, dtype=np.float64) + L_A / 2
+    cell = np.eye(3) * L_A
+    o = OracleFourierD3(numbers, cell, ns_mesh=(8, 8, 8), **kw)
+    i, j, S = neighbour_list(pos_A, cell, 6.0)
+    ei = torch.tensor(np.stack([i, j]))
+    sh = torch.tensor(S, dtype=torch.float64) @ torch.tensor(cell) * ANGSTROM_TO_BOHR
+    return o, o.cn_smooth(torch.tensor(pos_A) * ANGSTROM_TO_BOHR, ei, sh)
+
+
+def test_cn_is_one_half_at_four_thirds_of_the_covalent_distance():
+    # sigmoid(k (4/3 Rcov / r - 1)) = 1/2 at r = 4/3 Rcov for every steepness k
+    rcov = T.load_rcov().astype(np.float64)
+    d_A = 4.0 / 3.0 * (rcov[1] + rcov[6]) / ANGSTROM_TO_BOHR
+    _, cn = _pair_box([1, 6], [[0, 0, 0], [d_A, 0, 0]])
+    assert cn.numpy() == pytest.approx([0.5, 0.5], abs=1e-12)
+
+
+def test_cn_matches_d3_counting_function_and_adds_over_neighbours():
+    # below R_mid = (Rcov + r_cut)/2 the steepness is the D3 value k1 = 16, so each
+    # pair contributes Grimme's counting function 1 / (1 + exp(-16 (4/3 Rcov / r - 1)));
+    # a linear O-H-O trimer gives the middle atom the sum of its two bonds
+    rcov = T.load_rcov().astype(np.float64)
+    d = 1.05  # Angstrom
+    _, cn = _pair_box([8, 1, 8], [[-d, 0, 0], [0, 0, 0], [d, 0, 0]])
+    r = d * ANGSTROM_TO_BOHR
+    f_oh = 1.0 / (1.0 + math.exp(-16.0 * (4.0 / 3.0 * (rcov[1] + rcov[8]) / r - 1.0)))
+    f_oo = 1.0 / (1.0 + math.exp(-16.0 * (4.0 / 3.0 * (2 * rcov[8]) / (2 * r) - 1.0)))
+    assert float(cn[1]) == pytest.approx(2 * f_oh, rel=1e-12)
+    assert float(cn[0]) == pytest.approx(f_oh + f_oo, rel=1e-12)
+    assert float(cn[2]) == pytest.approx(float(cn[0]), rel=1e-14)
+
+
+def test_cn_decays_smoothly_to_zero_at_the_cutoff():
+    rcut_A = 6.0
+    ds = np.linspace(1.0, rcut_A - 1e-4, 200)
+    vals = np.array([float(_pair_box([6, 6], [[0, 0, 0], [d, 0, 0]])[1][0]) for d in ds])
+    assert np.all(np.diff(vals) <= 0)            # monotone in r
+    assert vals[-1] < 1e-12                      # vanishes at r_cut
+    # continuous at R_mid, where the steepness switches from 16 to the smooth-cut form
+    rcov = T.load_rcov().astype(np.float64)
+    rmid_A = 0.5 * (2 * rcov[6] * (1 / ANGSTROM_TO_BOHR) + rcut_A)
+    lo = float(_pair_box([6, 6], [[0, 0, 0], [rmid_A - 1e-7, 0, 0]])[1][0])
+    hi = float(_pair_box([6, 6], [[0, 0, 0], [rmid_A + 1e-7, 0, 0]])[1][0])
+    assert lo == pytest.approx(hi, abs=1e-8)  # slope x 4e-7 Bohr step is ~5e-10
+
+
+def test_c6_weights_are_a_masked_partition_of_unity():
+    o, _ = _pair_box([2, 6, 8], [[0, 0, 0], [3.0, 0, 0], [0, 3.0, 0]])
+    cn = torch.tensor([0.3, 2.2, 1.4], dtype=torch.float64)
+    L = o.c6_weights(cn)
+    assert torch.allclose(L.sum(1), torch.ones(3, dtype=torch.float64), atol=1e-15)
+    refs = o.cnref[o.species]
+    assert torch.all(L[refs == -1] == 0)
+    # helium has a single valid reference: weight exactly 1 for any CN
+    assert L[0, 0].item() == 1.0
+    # the heaviest weight sits on the reference CN nearest to the atom's CN
+    for a in (1, 2):
+        valid = (refs[a] != -1).nonzero().flatten()
+        near = valid[torch.argmin(torch.abs(refs[a, valid] - cn[a]))]
+        assert int(torch.argmax(L[a])) == int(near)
+
+
+def test_charges_reduce_to_the_eigenvector_row_of_a_single_reference_element():
+    o, _ = _pair_box([2, 8], [[0, 0, 0], [3.0, 0, 0]])
+    c = o.charges(torch.tensor([0.7, 1.2], dtype=torch.float64))
+    he = int(o.species[0])
+    assert torch.allclose(c[0], o.vq[he, 0], atol=1e-15)
+
+
 # ------------------------------------------------------------------ decomposition
 def test_decomposition_rank_search():
     types = [1, 6, 7, 8]
@@ -252,6 +324,34 @@ def test_supercell_energy_is_extensive():
     assert e2 == pytest.approx(8 * e1, rel=1e-11)
     assert np.allclose(f2[: len(f1)], f1, atol=1e-10)
     assert np.allclose(s2, s1, atol=1e-12)
+
+
+def test_energy_invariances():
+    """Translation by one mesh cell along a lattice vector, rotation of the whole
+    system and relabelling of atoms leave the SPME energy unchanged; forces and
+    stress transform as vectors and tensors.  A transposed reciprocal basis or
+    an off-by-one node index breaks one of these for a triclinic cell."""
+    box = W.strained(W.sio2_box(48, seed=5), 0.1, seed=5)
+    o = OracleFourierD3(box.numbers, box.cell, ns_mesh=(16, 16, 16))
+    e0, f0, s0 = o.calculate(box.positions, box.cell)
+    # one mesh cell along a_2
+    e1, f1, s1 = o.calculate(box.positions + box.cell[1] / 16, box.cell)
+    assert e1 == pytest.approx(e0, rel=1e-12)
+    assert np.allclose(f1, f0, atol=1e-11) and np.allclose(s1, s0, atol=1e-13)
+    # a proper rotation of positions and cell
+    rng = np.random.default_rng(3)
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    q *= np.sign(np.linalg.det(q))
+    e2, f2, s2 = o.calculate(box.positions @ q.T, box.cell @ q.T)
+    assert e2 == pytest.approx(e0, rel=1e-12)
+    assert np.allclose(f2, f0 @ q.T, atol=1e-11)
+    assert np.allclose(s2, q @ s0 @ q.T, atol=1e-13)
+    # relabelled atoms (new model, same species set)
+    perm = rng.permutation(box.n_atoms)
+    o3 = OracleFourierD3(box.numbers[perm], box.cell, ns_mesh=(16, 16, 16))
+    e3, f3, s3 = o3.calculate(box.positions[perm], box.cell)
+    assert e3 == pytest.approx(e0, rel=1e-12)
+    assert np.allclose(f3, f0[perm], atol=1e-11)
 
 
 def test_oracle_forces_and_stress_match_finite_differences():

@@ -747,7 +747,9 @@ int fd3_green_host(double s6, double s8, double r0, double qq, int64_t n, const 
   pk.R = R;
   pk.p6 = (float)s6 * float(2.0 * kPi * kPi / 3.0) / (R * R * R);
   pk.p8 = 3.0f * (float)s8 * (float)qq * float(-kPi * kPi) / (R * R * R * R * R);
-  for (int64_t i = 0; i < n; ++i) green_eval(pk, (float)k[i], g + i, dg + i);
+  float4 tab[3 * (kHtabN + 1)];  // the table the k-space kernel reads, built by the same code
+  for (int i = 0; i <= kHtabN; ++i) htab_interval(i, tab + 3 * i);
+  for (int64_t i = 0; i < n; ++i) green_eval_tab(pk, tab, (float)k[i], g + i, dg + i);
   return FD3_OK;
 }
 

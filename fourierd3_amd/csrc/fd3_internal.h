@@ -240,6 +240,91 @@ __host__ __device__ inline void green_eval(const PairK& pk, float k, float* g, f
   *dg = (pk.p6 * h6p + pk.p8 * h8p) * pk.R;
 }
 
+// h6 and h8 do not depend on the species pair, so the k-space kernel reads
+// them from a piecewise polynomial table instead of evaluating four
+// exponentials and three sine/cosine pairs per pair and k point.  Interval i
+// covers x in [i dx, (i+1) dx); with u = 2 (x/dx - i) - 1 in [-1, 1] each
+// function is the degree-5 polynomial interpolating it at the six Chebyshev
+// nodes of the interval (interpolation error < 1e-8 of h(0) at dx = 1/2).
+// Entry 3i .. 3i+2 holds {a6_0..a6_3}, {a6_4, a6_5, a8_0, a8_1}, {a8_2..a8_5},
+// monomial coefficients in u.  Interval kHtabN is all zeros and serves every
+// x >= kHtabXmax = 64, where |h6| < 1e-15 and |h8| < 4e-13 (h6(0) = 1,
+// h8(0) = -0.54): 1e-6 of float32 resolution, invisible in any sum.
+constexpr int kHtabN = 128;
+constexpr float kHtabDx = 0.5f;
+constexpr float kHtabXmax = kHtabN * kHtabDx;
+
+// h6(x), h8(x) in double for x > 0 (closed forms; the cancellation at small x
+// costs ~1e-16/x relative, < 1e-13 at the smallest table node).
+__host__ __device__ inline void h_exact(double x, double* h6, double* h8) {
+  const double s8 = 0.38268343236508977, c8 = 0.92387953251128674, sq3h = 0.86602540378443865;
+  const double e2 = exp(-0.5 * x), e1 = e2 * e2;
+  const double n6 = e1 - 2.0 * e2 * cos(1.04719755119659775 + sq3h * x);
+  const double n8 = exp(-x * s8) * cos(0.78539816339744831 + x * c8) + exp(-x * c8) * cos(2.35619449019234492 + x * s8);
+  *h6 = n6 / x;
+  *h8 = n8 / x;
+}
+
+// The three float4 table entries of interval i (0 <= i <= kHtabN).
+__host__ __device__ inline void htab_interval(int i, float4* out) {
+  if (i >= kHtabN) {
+    out[0] = out[1] = out[2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  double f6[6], f8[6], u[6];
+  for (int j = 0; j < 6; ++j) {
+    u[j] = cos((2 * j + 1) * kPi / 12.0);
+    h_exact(kHtabDx * (i + 0.5 * (u[j] + 1.0)), &f6[j], &f8[j]);
+  }
+  double a6[6], a8[6];
+  for (int f = 0; f < 2; ++f) {
+    const double* v = f ? f8 : f6;
+    double c[6];  // Chebyshev coefficients
+    for (int k = 0; k < 6; ++k) {
+      double s = 0.0;
+      for (int j = 0; j < 6; ++j) s += v[j] * cos(k * (2 * j + 1) * kPi / 12.0);
+      c[k] = s * (k ? 2.0 / 6.0 : 1.0 / 6.0);
+    }
+    // T0..T5 in monomials
+    double* a = f ? a8 : a6;
+    a[0] = c[0] - c[2] + c[4];
+    a[1] = c[1] - 3.0 * c[3] + 5.0 * c[5];
+    a[2] = 2.0 * c[2] - 8.0 * c[4];
+    a[3] = 4.0 * c[3] - 20.0 * c[5];
+    a[4] = 8.0 * c[4];
+    a[5] = 16.0 * c[5];
+  }
+  out[0] = make_float4((float)a6[0], (float)a6[1], (float)a6[2], (float)a6[3]);
+  out[1] = make_float4((float)a6[4], (float)a6[5], (float)a8[0], (float)a8[1]);
+  out[2] = make_float4((float)a8[2], (float)a8[3], (float)a8[4], (float)a8[5]);
+}
+
+// h6, h8 and their x-derivatives from the table (x >= 0).
+__host__ __device__ inline void green_h_tab(const float4* __restrict__ tab, float x, float* h6, float* h6p, float* h8,
+                                            float* h8p) {
+  const float fx = x * (1.0f / kHtabDx);
+  const int i = min((int)fx, kHtabN);
+  const float u = 2.0f * (fx - (float)i) - 1.0f;
+  const float4 A = tab[3 * i], B = tab[3 * i + 1], C = tab[3 * i + 2];
+  *h6 = A.x + u * (A.y + u * (A.z + u * (A.w + u * (B.x + u * B.y))));
+  *h8 = B.z + u * (B.w + u * (C.x + u * (C.y + u * (C.z + u * C.w))));
+  const float d6 = A.y + u * (2.0f * A.z + u * (3.0f * A.w + u * (4.0f * B.x + u * (5.0f * B.y))));
+  const float d8 = B.w + u * (2.0f * C.x + u * (3.0f * C.y + u * (4.0f * C.z + u * (5.0f * C.w))));
+  *h6p = d6 * (2.0f / kHtabDx);
+  *h8p = d8 * (2.0f / kHtabDx);
+}
+
+// G and dG/dk for |k| = k from the table.
+__host__ __device__ inline void green_eval_tab(const PairK& pk, const float4* __restrict__ tab, float k, float* g,
+                                               float* dg, float* h6o = nullptr, float* h6po = nullptr,
+                                               float* h8o = nullptr, float* h8po = nullptr) {
+  float h6, h6p, h8, h8p;
+  green_h_tab(tab, k * pk.R, &h6, &h6p, &h8, &h8p);
+  *g = pk.p6 * h6 + pk.p8 * h8;
+  *dg = (pk.p6 * h6p + pk.p8 * h8p) * pk.R;
+  if (h6o) { *h6o = h6; *h6po = h6p; *h8o = h8; *h8po = h8p; }
+}
+
 // ----------------------------------------------------------------- misc ---
 __device__ inline double warp_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

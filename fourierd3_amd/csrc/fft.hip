@@ -24,9 +24,10 @@ namespace fd3 {
 // ------------------------------------------------------------ tables ---
 // tw_axis[k] = exp(-2 pi i k / n) for each axis; sinc_axis[m] = sinc(m/n)^(2p)
 // with m the fftfreq integer of index m (rfftfreq for z).
-__global__ void k_init_tables(float2* twx, float2* twy, float2* twz, float* scx, float* scy, float* scz, int nx,
-                              int ny, int nz, int order, int use_sinc) {
+__global__ void k_init_tables(float2* twx, float2* twy, float2* twz, float* scx, float* scy, float* scz,
+                              float4* htab, int nx, int ny, int nz, int order, int use_sinc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t <= kHtabN) htab_interval(t, htab + 3 * t);
   auto tw = [](float2* dst, int k, int n) {
     double s, c;
     sincospi(-2.0 * (double)k / (double)n, &s, &c);
@@ -57,16 +58,15 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ rho, fl
   stage_table(tws, twz, nz);
   const float2* src = (const float2*)rho + ((size_t)(c * nx + x) * ny + y0) * M;
   const int lM = ilog2(M), lT = ilog2(TY);
-  for (int e = threadIdx.x; e < TY * M; e += blockDim.x) lds[(e >> lM) * ls + (e & (M - 1))] = src[e];
+  tile_to_lds<kCopyU>(lds, src, TY * M, [](int e) { return (size_t)e; },
+                     [=](int e) { return (e >> lM) * ls + (e & (M - 1)); });
   __syncthreads();
   lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   z_pairs<true>(lds, TY, ls, M, tws);  // X[k] = E[k] + W_nz^k O[k]
   __syncthreads();
   float2* dst = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
-  for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e >> lT, t = e & (TY - 1);
-    dst[(size_t)kz * ny + t] = lds[t * ls + kz];
-  }
+  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+                        [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
 }
 
 // A[c][x][kz][y0..y0+TY) -> out[c][x][y][0..nz) real (unnormalised inverse;
@@ -80,16 +80,15 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
   const int lM = ilog2(M), lT = ilog2(TY);
-  for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e >> lT, t = e & (TY - 1);
-    lds[t * ls + kz] = src[(size_t)kz * ny + t];
-  }
+  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+                     [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
   __syncthreads();
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
   lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   float2* dst = (float2*)out + ((size_t)(c * nx + x) * ny + y0) * M;
-  for (int e = threadIdx.x; e < TY * M; e += blockDim.x) dst[e] = lds[(e >> lM) * ls + (e & (M - 1))];
+  tile_from_lds<kCopyU>(dst, lds, TY * M, [](int e) { return (size_t)e; },
+                        [=](int e) { return (e >> lM) * ls + (e & (M - 1)); });
 }
 
 // ------------------------------------------------------------ y passes ---
@@ -105,31 +104,23 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
   const int lny = ilog2(ny), lTX = ilog2(TX);
   if (DIR < 0) {
     const float2* src = in + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int l = e >> lny, y = e & (ny - 1);
-      lds[l * ls + y] = src[(size_t)l * M1 * ny + y];
-    }
+    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (size_t)(e >> lny) * M1 * ny + (e & (ny - 1)); },
+                        [=](int e) { return (e >> lny) * ls + (e & (ny - 1)); });
   } else {
     const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int ky = e >> lTX, l = e & (TX - 1);
-      lds[l * ls + ky] = src[(size_t)ky * nx + l];
-    }
+    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (size_t)(e >> lTX) * nx + (e & (TX - 1)); },
+                        [=](int e) { return (e & (TX - 1)) * ls + (e >> lTX); });
   }
   __syncthreads();
   lds_fft<DIR, kFftMaxB>(lds, ny, TX, ls, tws, 1);
   if (DIR < 0) {
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int ky = e >> lTX, l = e & (TX - 1);
-      dst[(size_t)ky * nx + l] = lds[l * ls + ky];
-    }
+    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (size_t)(e >> lTX) * nx + (e & (TX - 1)); },
+                          [=](int e) { return (e & (TX - 1)) * ls + (e >> lTX); });
   } else {
     float2* dst = out + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    for (int e = threadIdx.x; e < TX * ny; e += blockDim.x) {
-      const int l = e >> lny, y = e & (ny - 1);
-      dst[(size_t)l * M1 * ny + y] = lds[l * ls + y];
-    }
+    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (size_t)(e >> lny) * M1 * ny + (e & (ny - 1)); },
+                          [=](int e) { return (e >> lny) * ls + (e & (ny - 1)); });
   }
 }
 
@@ -146,10 +137,12 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
   float2* tws = lds + LPB * ls;
   stage_table(tws, twx, nx);
   const int lnx = ilog2(nx);
-  for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) lds[(e >> lnx) * ls + (e & (nx - 1))] = base[e];
+  auto gi = [](int e) { return (size_t)e; };
+  auto li = [=](int e) { return (e >> lnx) * ls + (e & (nx - 1)); };
+  tile_to_lds<kCopyU>(lds, base, nl * nx, gi, li);
   __syncthreads();
   lds_fft<DIR, kFftMaxB>(lds, nx, nl, ls, tws, 1);
-  for (int e = threadIdx.x; e < nl * nx; e += blockDim.x) base[e] = lds[(e >> lnx) * ls + (e & (nx - 1))];
+  tile_from_lds<kCopyU>(base, lds, nl * nx, gi, li);
 }
 
 // ------------------------------------------------- fused k-space kernel ---
@@ -163,11 +156,13 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
 // inverse-transforms along x, leaving dE/drho's spectrum in place.
 // Per-block partial sums (double): [0] sum e, [1..6] sum w dsum k_a k_b/|k|
 // (xx,yy,zz,xy,xz,yz), [7..10] dE/d(s6,s8,a1,a2) spectral parts (unscaled).
+// G~ is evaluated once per unordered species pair and shared by all ranks.
 struct KspaceArgs {
   float2* spec;
   const float* eigs;
   const float* qq;      // (nS, nS)
   const float2* twx;
+  const float4* htab;
   const float* scx; const float* scy; const float* scz;
   double* partials;     // (gridDim, 16)
   float recip[9];       // rows b_i of 2 pi cell^-T (Bohr^-1)
@@ -176,9 +171,176 @@ struct KspaceArgs {
   int nx, ny, nz, M1, nS, r, RT, NK, use_sinc, need_grad, need_params;
 };
 
-template <bool PARAMS>
+// Per species pair: Green's function constants and the parameter-derivative
+// factors f6 = G's s6 coefficient / s6, f8 = its s8 coefficient / s8, dR0/da1.
+struct PairP {
+  float pf6, pf8, dR;
+};
+
+// Dynamic LDS of both k-space kernels: [h table | x twiddles | lines ... |
+// pairs], the table first so that its float4 entries are 16-byte aligned.
+__device__ inline void kspace_stage_consts(const KspaceArgs& a, float4* htab, float2* tws, PairK* pairs, PairP* pp) {
+  for (int i = threadIdx.x; i < 3 * (kHtabN + 1); i += blockDim.x) htab[i] = a.htab[i];
+  stage_table(tws, a.twx, a.nx);
+  const int nS = a.nS;
+  for (int t = threadIdx.x; t < nS * nS; t += blockDim.x) {
+    const float qq = a.qq[t];
+    const float sq = sqrtf(3.0f * qq);
+    const float R = a.a1 * sq + a.a2;
+    const float f6 = float(2.0 * kPi * kPi / 3.0) / (R * R * R);
+    const float f8 = 3.0f * qq * float(-kPi * kPi) / (R * R * R * R * R);
+    PairK pk;
+    pk.R = R;
+    pk.p6 = a.s6 * f6;
+    pk.p8 = a.s8 * f8;
+    pairs[t] = pk;
+    PairP q;
+    q.pf6 = f6; q.pf8 = f8; q.dR = sq;
+    pp[t] = q;
+  }
+}
+
+// k vector, |k|, 1/sinc^2p and the half-spectrum weight of column (kyi, kx).
+struct KPoint {
+  float kv0, kv1, kv2, kn, isc;
+};
+__device__ inline KPoint kspace_point(const KspaceArgs& a, int kx, int ky, int kz) {
+  const float mx = (float)(kx < (a.nx + 1) / 2 ? kx : kx - a.nx);
+  const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
+  const float mz = (float)kz;
+  KPoint p;
+  p.kv0 = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
+  p.kv1 = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
+  p.kv2 = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
+  p.kn = sqrtf(p.kv0 * p.kv0 + p.kv1 * p.kv1 + p.kv2 * p.kv2);
+  p.isc = 1.0f;
+  if (a.use_sinc) p.isc = __builtin_amdgcn_rcpf(fmaxf(a.scx[kx] * a.scy[ky] * a.scz[kz], 1e-10f));
+  return p;
+}
+
+// Derivative of G with respect to (s6, s8, a1, a2) at one pair, times t.
+__device__ inline void kspace_param_terms(const PairK& pk, const PairP& pp, float kn, float h6, float h6p, float h8,
+                                          float h8p, float t, float* p_acc) {
+  const float R = pk.R;
+  const float dgdR = pk.p6 * (-3.0f * h6 / R + kn * h6p) + pk.p8 * (-5.0f * h8 / R + kn * h8p);
+  p_acc[0] += pp.pf6 * h6 * t;
+  p_acc[1] += pp.pf8 * h8 * t;
+  p_acc[2] += dgdR * pp.dR * t;
+  p_acc[3] += dgdR * t;
+}
+
+__device__ inline void kspace_accumulate(double (&acc)[11], const KPoint& p, float w, float e_acc, float d_acc,
+                                         const float* p_acc, bool params) {
+  acc[0] += (double)(w * e_acc);
+  if (p.kn > 0.f) {
+    const double s = (double)(w * d_acc * p.isc) / (double)p.kn;
+    acc[1] += s * p.kv0 * p.kv0; acc[2] += s * p.kv1 * p.kv1; acc[3] += s * p.kv2 * p.kv2;
+    acc[4] += s * p.kv0 * p.kv1; acc[5] += s * p.kv0 * p.kv2; acc[6] += s * p.kv1 * p.kv2;
+  }
+  if (params) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[7 + i] += (double)(w * p_acc[i] * p.isc);
+  }
+}
+
+// nS = NS <= kKsRegMaxS species and RT ranks per block: each thread keeps the
+// NS*RT spectra of its k point in registers and writes phi back over them,
+// so the block needs a single LDS buffer.  Lines l = (X*RT + qi)*NK + kyi.
+template <int NS, int RT>
+__global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
+  extern __shared__ float4 lds4[];
+  __shared__ double red[16 * 16];
+  const int nx = a.nx, NK = a.NK;
+  const int ky0 = blockIdx.x * NK, kz = blockIdx.y, q0 = blockIdx.z * RT;
+  const int rt = min(RT, a.r - q0);
+  const int nlines = NS * RT * NK;
+  const int ls = nx + 1;
+  const int lnx = ilog2(nx), lNK = ilog2(NK);  // nx, NK powers of two
+  float4* htab = lds4;
+  float2* tws = (float2*)(htab + 3 * (kHtabN + 1));
+  float2* S = tws + nx;
+  PairK* pairs = (PairK*)(S + nlines * ls);
+  PairP* pp = (PairP*)(pairs + NS * NS);
+  kspace_stage_consts(a, htab, tws, pairs, pp);
+  // element e of the tile: line l = e / nx, column kx; lines of padded ranks
+  // (qi >= rt) are zero-filled and never stored
+  auto gofs = [=](int e) {
+    const int l = e >> lnx, kx = e & (nx - 1);
+    const int kyi = l & (NK - 1), xq = l >> lNK, qi = xq % RT, X = xq / RT;
+    return (((size_t)(X * a.r + q0 + qi) * a.M1 + kz) * a.ny + ky0 + kyi) * nx + kx;
+  };
+  auto lofs = [=](int e) { return (e >> lnx) * ls + (e & (nx - 1)); };
+  auto live = [=](int e) { return ((e >> lnx) >> lNK) % RT < rt; };
+  tile_to_lds<kCopyU>(S, a.spec, nlines * nx, gofs, lofs, live);
+  __syncthreads();
+  lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
+
+  float lam[RT];
+#pragma unroll
+  for (int q = 0; q < RT; ++q) lam[q] = q < rt ? a.eigs[q0 + q] : 0.0f;
+  const bool params = a.need_params;
+  double acc[11];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) acc[i] = 0.0;
+  const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
+  for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
+    const int kyi = t >> lnx, kx = t & (nx - 1);
+    const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
+    float2 sv[NS][RT], ph[NS][RT];
+#pragma unroll
+    for (int X = 0; X < NS; ++X)
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        sv[X][q] = S[((X * RT + q) * NK + kyi) * ls + kx];
+        ph[X][q] = make_float2(0.f, 0.f);
+      }
+    float d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int X = 0; X < NS; ++X) {
+#pragma unroll
+      for (int Y = X; Y < NS; ++Y) {
+        const PairK pk = pairs[X * NS + Y];
+        float g, dg, h6, h6p, h8, h8p;
+        green_eval_tab(pk, htab, kp.kn, &g, &dg, &h6, &h6p, &h8, &h8p);
+        const float gs = g * kp.isc;
+        float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
+#pragma unroll
+        for (int q = 0; q < RT; ++q) {
+          const float2 sx = sv[X][q], sy = sv[Y][q];
+          tre += lam[q] * (sx.x * sy.x + sx.y * sy.y);
+          ph[X][q].x += gs * sy.x;
+          ph[X][q].y += gs * sy.y;
+          if (Y != X) {
+            ph[Y][q].x += gs * sx.x;
+            ph[Y][q].y += gs * sx.y;
+          }
+        }
+        if (Y != X) tre *= 2.0f;
+        d_acc += dg * tre;
+        if (params) kspace_param_terms(pk, pp[X * NS + Y], kp.kn, h6, h6p, h8, h8p, tre, p_acc);
+      }
+    }
+    float e_acc = 0.f;
+#pragma unroll
+    for (int X = 0; X < NS; ++X)
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
+        const float f = lam[q] * a.neg_inv_vol;
+        S[((X * RT + q) * NK + kyi) * ls + kx] = make_float2(f * ph[X][q].x, f * ph[X][q].y);
+      }
+    kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, params);
+  }
+  const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  block_sum_store<11>(acc, red, a.partials + (size_t)bid * 16);  // ends with a barrier
+  if (!a.need_grad) return;
+  lds_fft<1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
+  tile_from_lds<kCopyU>(a.spec, S, nlines * nx, gofs, lofs, live);
+}
+
+// Any number of species: phi accumulates in a second LDS buffer P.
 __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
-  extern __shared__ float2 lds[];
+  extern __shared__ float4 lds4[];
   __shared__ double red[16 * 16];
   const int nx = a.nx, nS = a.nS, RT = a.RT, NK = a.NK;
   const int ky0 = blockIdx.x * NK, kz = blockIdx.y, q0 = blockIdx.z * RT;
@@ -186,65 +348,41 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   const int nlines = nS * RT * NK;
   const int ls = nx + 1;
   const int lnx = ilog2(nx), lNK = ilog2(NK), lRT = ilog2(RT);  // RT, NK, nx are powers of two
-  float2* S = lds;
-  float2* P = lds + nlines * ls;
-  float2* tws = P + nlines * ls;              // twiddles of the x lines
-  PairK* pairs = (PairK*)(tws + nx);          // (nS, nS)
-  float* dR = (float*)(pairs + nS * nS);      // dR0_XY/da1 = sqrt(3 QQ)
-  stage_table(tws, a.twx, nx);
-  // per-pair constants
-  for (int t = threadIdx.x; t < nS * nS; t += blockDim.x) {
-    const float qq = a.qq[t];
-    const float sq = sqrtf(3.0f * qq);
-    const float R = a.a1 * sq + a.a2;
-    PairK pk;
-    pk.R = R;
-    pk.p6 = a.s6 * float(2.0 * kPi * kPi / 3.0) / (R * R * R);
-    pk.p8 = 3.0f * a.s8 * qq * float(-kPi * kPi) / (R * R * R * R * R);
-    pairs[t] = pk;
-    dR[t] = sq;
-  }
-  // load lines: line l = (X*RT + qi)*NK + kyi
-  for (int e = threadIdx.x; e < nlines * nx; e += blockDim.x) {
+  float4* htab = lds4;
+  float2* tws = (float2*)(htab + 3 * (kHtabN + 1));
+  float2* S = tws + nx;
+  float2* P = S + nlines * ls;
+  PairK* pairs = (PairK*)(P + nlines * ls);
+  PairP* pp = (PairP*)(pairs + nS * nS);
+  kspace_stage_consts(a, htab, tws, pairs, pp);
+  auto gofs = [=](int e) {
     const int l = e >> lnx, kx = e & (nx - 1);
     const int kyi = l & (NK - 1), xq = l >> lNK, qi = xq & (RT - 1), X = xq >> lRT;
-    float2 v = make_float2(0.f, 0.f);
-    if (qi < rt) {
-      const int c = X * a.r + q0 + qi;
-      v = a.spec[(((size_t)c * a.M1 + kz) * a.ny + ky0 + kyi) * nx + kx];
-    }
-    S[l * ls + kx] = v;
-  }
+    return (((size_t)(X * a.r + q0 + qi) * a.M1 + kz) * a.ny + ky0 + kyi) * nx + kx;
+  };
+  auto lofs = [=](int e) { return (e >> lnx) * ls + (e & (nx - 1)); };
+  auto live = [=](int e) { return (((e >> lnx) >> lNK) & (RT - 1)) < rt; };
+  tile_to_lds<kCopyU>(S, a.spec, nlines * nx, gofs, lofs, live);
   __syncthreads();
   lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
 
+  const bool params = a.need_params;
   double acc[11];
 #pragma unroll
   for (int i = 0; i < 11; ++i) acc[i] = 0.0;
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
-  const float mz = (float)kz;
   for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
     const int kyi = t >> lnx, kx = t & (nx - 1);
-    const int ky = ky0 + kyi;
-    const float mx = (float)(kx < (nx + 1) / 2 ? kx : kx - nx);
-    const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
-    const float kv0 = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
-    const float kv1 = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
-    const float kv2 = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
-    const float kn = sqrtf(kv0 * kv0 + kv1 * kv1 + kv2 * kv2);
-    float sc = 1.0f;
-    if (a.use_sinc) sc = fmaxf(a.scx[kx] * a.scy[ky] * a.scz[kz], 1e-10f);
-    const float isc = 1.0f / sc;
-    float e_acc = 0.f, d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
+    const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
+    float d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
     // phi accumulates in P (this thread owns column (kyi, kx) of every line)
     for (int l = 0; l < nS * RT; ++l) P[(l * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
-    // one Green's function evaluation per unordered species pair (G~ is symmetric)
     for (int X = 0; X < nS; ++X) {
       for (int Y = X; Y < nS; ++Y) {
         const PairK pk = pairs[X * nS + Y];
-        float g, dg;
-        green_eval(pk, kn, &g, &dg);
-        const float gs = g * isc;
+        float g, dg, h6, h6p, h8, h8p;
+        green_eval_tab(pk, htab, kp.kn, &g, &dg, &h6, &h6p, &h8, &h8p);
+        const float gs = g * kp.isc;
         float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
         for (int qi = 0; qi < rt; ++qi) {
           const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
@@ -263,24 +401,12 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
           tre += a.eigs[q0 + qi] * (sx.x * sy.x + sx.y * sy.y);
         }
         if (Y != X) tre *= 2.0f;
-        d_acc += dg * isc * tre;
-        if (PARAMS) {
-          // dG/ds6 = f6, dG/ds8 = 3 QQ f8, dG/dR0 for a1 (x sqrt(3QQ)) and a2
-          float h6, h6p, h8, h8p;
-          const float R = pk.R;
-          green_h(kn * R, &h6, &h6p, &h8, &h8p);
-          const float f6 = float(2.0 * kPi * kPi / 3.0) / (R * R * R) * h6;
-          const float qq = a.qq[X * nS + Y];
-          const float f8q = 3.0f * qq * float(-kPi * kPi) / (R * R * R * R * R) * h8;
-          const float dgdR = pk.p6 * (-3.0f * h6 / R + kn * h6p) + pk.p8 * (-5.0f * h8 / R + kn * h8p);
-          p_acc[0] += f6 * isc * tre;
-          p_acc[1] += f8q * isc * tre;
-          p_acc[2] += dgdR * dR[X * nS + Y] * isc * tre;
-          p_acc[3] += dgdR * isc * tre;
-        }
+        d_acc += dg * tre;
+        if (params) kspace_param_terms(pk, pp[X * nS + Y], kp.kn, h6, h6p, h8, h8p, tre, p_acc);
       }
     }
     // energy sum_X sum_q lambda_q Re(S conj phi); P <- -(lambda_q / Omega) phi
+    float e_acc = 0.f;
     for (int X = 0; X < nS; ++X) {
       for (int qi = 0; qi < rt; ++qi) {
         const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
@@ -291,29 +417,13 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
         P[ix] = make_float2(f * ph.x, f * ph.y);
       }
     }
-    acc[0] += (double)(w * e_acc);
-    if (kn > 0.f) {
-      const double s = (double)(w * d_acc) / (double)kn;
-      acc[1] += s * kv0 * kv0; acc[2] += s * kv1 * kv1; acc[3] += s * kv2 * kv2;
-      acc[4] += s * kv0 * kv1; acc[5] += s * kv0 * kv2; acc[6] += s * kv1 * kv2;
-    }
-    if (PARAMS) {
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[7 + i] += (double)(w * p_acc[i]);
-    }
+    kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, params);
   }
   const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  block_sum_store<11>(acc, red, a.partials + (size_t)bid * 16);
+  block_sum_store<11>(acc, red, a.partials + (size_t)bid * 16);  // ends with a barrier
   if (!a.need_grad) return;
   lds_fft<1, kFftMaxB>(P, nx, nlines, ls, tws, 1);
-  for (int e = threadIdx.x; e < nlines * nx; e += blockDim.x) {
-    const int l = e >> lnx, kx = e & (nx - 1);
-    const int kyi = l & (NK - 1), xq = l >> lNK, qi = xq & (RT - 1), X = xq >> lRT;
-    if (qi < rt) {
-      const int c = X * a.r + q0 + qi;
-      a.spec[(((size_t)c * a.M1 + kz) * a.ny + ky0 + kyi) * nx + kx] = P[l * ls + kx];
-    }
-  }
+  tile_from_lds<kCopyU>(a.spec, P, nlines * nx, gofs, lofs, live);
 }
 
 // --------------------------------------------------------------- host ---
@@ -336,24 +446,31 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   p->TX = std::max(1, std::min(p->nx, budget / p->ny));
   p->TX = 1 << (31 - __builtin_clz(p->TX));
   p->LPB = std::max(1, budget / (p->nx + 1));
-  // k-space: two LDS buffers of nS*RT*NK lines of nx+1 complex plus the
-  // per-pair table, within 62 KB (the block also holds 2 KB of static LDS)
+  // k-space: the h table, the x twiddles, one (nS <= kKsRegMaxS) or two LDS
+  // buffers of nS*RT*NK lines of nx+1 complex, and the per-pair constants,
+  // within 62 KB (the block also holds 2 KB of static LDS)
   if (nS > 32) {
     set_error("the k-space kernel supports up to 32 species, got %d", nS);
     return FD3_ERR_UNSUPPORTED;
   }
+  p->ks_reg = nS <= kKsRegMaxS ? nS : 0;
+  const int nbuf = p->ks_reg ? 1 : 2;
   auto ks_bytes = [&](int rt_, int nk_) {
-    return (size_t)2 * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)p->nx * sizeof(float2) + (size_t)nS * nS * 16;
+    return (size_t)3 * (kHtabN + 1) * sizeof(float4) + (size_t)p->nx * sizeof(float2) +
+           (size_t)nbuf * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * nS * (sizeof(PairK) + 12);
   };
   const size_t ks_budget = 62 * 1024;
-  // RT: ranks per block, a power of two (a last partial tile is padded with zero lines)
+  // RT: ranks per block; the register variant takes any RT <= 4, the generic
+  // one a power of two <= 8 (a last partial tile is padded with zero lines)
   int RT = 1;
-  while (RT < std::min(r, 8)) RT <<= 1;
+  if (p->ks_reg) RT = std::min(r, 4);
+  else
+    while (RT < std::min(r, 8)) RT <<= 1;
   int NK = std::max(1, std::min(p->ny, 256 / p->nx));
   auto ks_fits = [&](int rt_, int nk_) {
     return ks_bytes(rt_, nk_) <= ks_budget && (size_t)nS * rt_ * nk_ * p->nx <= (size_t)kFftLines;
   };
-  while (RT > 1 && !ks_fits(RT, NK)) RT >>= 1;
+  while (RT > 1 && !ks_fits(RT, NK)) RT = p->ks_reg ? RT - 1 : RT >> 1;
   while (NK > 1 && !ks_fits(RT, NK)) NK >>= 1;
   if (!ks_fits(RT, NK)) {
     set_error("k-space tile of %d species x %d points does not fit in LDS", nS, p->nx);
@@ -379,13 +496,14 @@ void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t) {
   t->scx = ar.take<float>(p.nx);
   t->scy = ar.take<float>(p.ny);
   t->scz = ar.take<float>(p.M1);
+  t->htab = ar.take<float4>(3 * (kHtabN + 1));
 }
 
 int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s) {
-  const int n = std::max(std::max(p.nx, p.ny), p.nz);
+  const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz), kHtabN + 1);
   ProfScope _ps("k_init_tables", s);
   hipLaunchKernelGGL(k_init_tables, dim3((n + 255) / 256), dim3(256), 0, s, t.twx, t.twy, t.twz, t.scx, t.scy, t.scz,
-                     p.nx, p.ny, p.nz, order, use_sinc);
+                     t.htab, p.nx, p.ny, p.nz, order, use_sinc);
   FD3_LAUNCH_CHECK("k_init_tables");
   return FD3_OK;
 }
@@ -466,13 +584,20 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
   a.nx = p.nx; a.ny = p.ny; a.nz = p.nz; a.M1 = p.M1; a.nS = p.nS; a.r = p.r; a.RT = p.RT; a.NK = p.NK;
   a.use_sinc = g.interp == FD3_INTERP_P3M;
   a.need_grad = need_grad; a.need_params = need_params;
+  a.htab = t.htab;
   const size_t sh = p.ks_lds;
   const dim3 grid(p.ny / p.NK, p.M1, (p.r + p.RT - 1) / p.RT);
   ProfScope _ps("k_kspace", s);
-  if (need_params)
-    hipLaunchKernelGGL(k_kspace<true>, grid, dim3(256), sh, s, a);
-  else
-    hipLaunchKernelGGL(k_kspace<false>, grid, dim3(256), sh, s, a);
+#define FD3_KS(NS_, RT_) \
+  case NS_ * 8 + RT_: hipLaunchKernelGGL((k_kspace_reg<NS_, RT_>), grid, dim3(256), sh, s, a); break;
+  switch (p.ks_reg * 8 + p.RT) {
+    FD3_KS(1, 1) FD3_KS(1, 2) FD3_KS(1, 3) FD3_KS(1, 4)
+    FD3_KS(2, 1) FD3_KS(2, 2) FD3_KS(2, 3) FD3_KS(2, 4)
+    FD3_KS(3, 1) FD3_KS(3, 2) FD3_KS(3, 3) FD3_KS(3, 4)
+    FD3_KS(4, 1) FD3_KS(4, 2) FD3_KS(4, 3) FD3_KS(4, 4)
+    default: hipLaunchKernelGGL(k_kspace, grid, dim3(256), sh, s, a);
+  }
+#undef FD3_KS
   FD3_LAUNCH_CHECK("k_kspace");
   return FD3_OK;
 }

@@ -11,6 +11,51 @@ __device__ inline void stage_table(float2* dst, const float2* __restrict__ src, 
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+// Tile copies between global memory and LDS: element e (0 <= e < n) moves
+// between src/dst[gi(e)] and lds[li(e)]; elements with ok(e) false are read
+// as zero / not written.  A plain loop waits for each global load before its
+// LDS store, leaving one load per wave in flight; these issue U loads per
+// thread before the first store (and U LDS reads before the first global
+// store), which is what lets the pass kernels approach HBM bandwidth.
+struct AllElems {
+  __device__ bool operator()(int) const { return true; }
+};
+template <int U, typename GI, typename LI, typename OK = AllElems>
+__device__ __forceinline__ void tile_to_lds(float2* lds, const float2* __restrict__ src, int n, GI gi, LI li,
+                                            OK ok = OK()) {
+  for (int b = 0; b < n; b += U * (int)blockDim.x) {
+    float2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
+      v[u] = (e < n && ok(e)) ? src[gi(e)] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
+      if (e < n) lds[li(e)] = v[u];
+    }
+  }
+}
+template <int U, typename GI, typename LI, typename OK = AllElems>
+__device__ __forceinline__ void tile_from_lds(float2* __restrict__ dst, const float2* lds, int n, GI gi, LI li,
+                                              OK ok = OK()) {
+  for (int b = 0; b < n; b += U * (int)blockDim.x) {
+    float2 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
+      if (e < n) v[u] = lds[li(e)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
+      if (e < n && ok(e)) dst[gi(e)] = v[u];
+    }
+  }
+}
+constexpr int kCopyU = 16;  // loads in flight per thread (16 x 8 B)
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

@@ -19,6 +19,7 @@ struct FftPlan {
   int TX;                 // x lines per block of the y passes
   int LPB;                // lines per block of the plain x pass
   int RT, NK;             // ranks and ky rows per block of the k-space kernel
+  int ks_reg;             // k-space variant: nS (<= kKsRegMaxS) with S and phi in registers, 0 = generic
   int ks_blocks;          // grid size of the k-space kernel
   size_t ks_lds;          // dynamic LDS bytes of the k-space kernel
   size_t mesh_complex() const { return (size_t)C * nx * M1 * ny; }
@@ -28,7 +29,11 @@ struct FftPlan {
 struct FftTables {
   float2 *twx, *twy, *twz;
   float *scx, *scy, *scz;
+  float4* htab;  // 3 * (kHtabN + 1) entries: polynomial table of h6, h8
 };
+
+// Species counts for which the k-space kernel keeps a k point's spectra in registers.
+constexpr int kKsRegMaxS = 4;
 
 int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p);
 void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t);

@@ -188,11 +188,9 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
   z_pairs<true>(lds, TY, ls, M, tws);
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
-  const int lT = ilog2(TY);
-  for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e >> lT, t = e & (TY - 1);
-    dst[(size_t)kz * a.ny + t] = lds[t * ls + kz];
-  }
+  const int lT = ilog2(TY), ny = a.ny;
+  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+                        [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
 }
 
 // ------------------------------------------------------ z pass + gather ---
@@ -206,11 +204,9 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   float2* tws = lds + TY * ls;
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
-  const int lT = ilog2(TY);
-  for (int e = threadIdx.x; e < TY * (M + 1); e += blockDim.x) {
-    const int kz = e >> lT, t = e & (TY - 1);
-    lds[t * ls + kz] = src[(size_t)kz * a.ny + t];
-  }
+  const int lT = ilog2(TY), ny = a.ny;
+  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+                      [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
   if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
   __syncthreads();
   z_pairs<false>(lds, TY, ls, M, tws);

@@ -365,7 +365,10 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   int rc;
   const int r = m->n_rank, nS = m->n_species;
   const bool grad = flags & FD3_NEED_GRAD;
-  if ((rc = fft_tables_init(c.plan, c.tab, c.geom.order, c.geom.interp == FD3_INTERP_P3M, s))) return rc;
+  PairInit pi;
+  pi.qq = m->qq; pi.nS = nS;
+  pi.s6 = (float)m->s6; pi.s8 = (float)m->s8; pi.a1 = (float)m->a1; pi.a2 = (float)m->a2;
+  if ((rc = fft_tables_init(c.plan, c.tab, c.geom.order, c.geom.interp == FD3_INTERP_P3M, s, &pi))) return rc;
   C6Args ca;
   ca.n = n; ca.nS = nS; ca.r = r; ca.cn_raw = c.cn_raw; ca.spec_s = spec_s; ca.slot_atom = slot_atom;
   ca.cnref = m->cnref; ca.vq = m->vq; ca.eigs = m->eigs; ca.qq = m->qq; ca.cncorr = m->cncorr;
@@ -382,8 +385,8 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   float2* spec = (float2*)c.buf1;
   if ((rc = spread_r2c_z(fa, c.plan, c.tab, c.buf2, s))) return rc;
   if ((rc = fft_y(c.plan, c.tab, c.buf2, spec, -1, s))) return rc;
-  if ((rc = kspace_run(c.plan, c.tab, c.geom, spec, m->eigs, m->qq, m->s6, m->s8, m->a1, m->a2, grad ? 1 : 0,
-                       (flags & FD3_NEED_PARAMS) ? 1 : 0, c.p_ks, s)))
+  if ((rc = kspace_run(c.plan, c.tab, c.geom, spec, m->eigs, grad ? 1 : 0, (flags & FD3_NEED_PARAMS) ? 1 : 0, c.p_ks,
+                       s)))
     return rc;
   if (grad) {
     if ((rc = fft_y(c.plan, c.tab, spec, c.buf2, 1, s))) return rc;
@@ -742,14 +745,14 @@ int fd3_fft_c2r(const int32_t ns[3], int32_t C, const float* spec, float* out, v
 
 // ------------------------------------------------------------ host checks ---
 int fd3_green_host(double s6, double s8, double r0, double qq, int64_t n, const double* k, float* g, float* dg) {
-  PairK pk;
-  const float R = (float)r0;
-  pk.R = R;
-  pk.p6 = (float)s6 * float(2.0 * kPi * kPi / 3.0) / (R * R * R);
-  pk.p8 = 3.0f * (float)s8 * (float)qq * float(-kPi * kPi) / (R * R * R * R * R);
+  // R0 = a1 sqrt(3 qq) + a2 with a1 = 0, a2 = r0: the kernel's pair constants for this R0
+  const KsPair pk = ks_pair((float)qq, (float)s6, (float)s8, 0.0f, (float)r0);
   float4 tab[3 * (kHtabN + 1)];  // the table the k-space kernel reads, built by the same code
   for (int i = 0; i <= kHtabN; ++i) htab_interval(i, tab + 3 * i);
-  for (int64_t i = 0; i < n; ++i) green_eval_tab(pk, tab, (float)k[i], g + i, dg + i);
+  for (int64_t i = 0; i < n; ++i) {
+    float h6, d6, h8, d8;
+    green_eval_tab(pk, tab, (float)k[i], g + i, dg + i, &h6, &d6, &h8, &d8);
+  }
   return FD3_OK;
 }
 

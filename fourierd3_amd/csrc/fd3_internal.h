@@ -153,93 +153,13 @@ __host__ __device__ inline int wrap(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
-// Hardware exp / sin / cos on the GPU (v_exp_f32, v_sin_f32, v_cos_f32), the
-// libm versions when the same code is compiled for the host checks.
-__host__ __device__ inline float fd3_exp(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __expf(x);
-#else
-  return expf(x);
-#endif
-}
-__host__ __device__ inline void fd3_sincos(float x, float* s, float* c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  *s = __sinf(x);
-  *c = __cosf(x);
-#else
-  sincosf(x, s, c);
-#endif
-}
-
 // --------------------------------------------------------- Green's function ---
-// h6(x) = num6(x)/x and h8(x) = num8(x)/x of the analytic Fourier transforms
-// (src/fourierd3/pair_pot.py:152-175):
-//   f6^(k) = 2 pi^2 / (3 R^3) h6(kR),  num6 = e^-x - 2 e^-x/2 cos(pi/3 + sqrt3 x/2)
-//   f8^(k) =  -pi^2 / R^5     h8(kR),  num8 = e^-x sin(pi/8) cos(pi/4 + x cos(pi/8))
-//                                           + e^-x cos(pi/8) cos(3pi/4 + x sin(pi/8))
-// num6 and num8 are entire with num(0) = 0; below x = 1 their Maclaurin series
-// is summed (no cancellation), above it the closed form.  hp = dh/dx.
-__host__ __device__ inline void green_h(float x, float* h6, float* h6p, float* h8, float* h8p) {
-  if (x < 1.0f) {
-    // coefficients of x^k, k = 0..16 (exact zeros dropped)
-    const float c6[17] = {1.0f, 0.0f, -3.33333333333333315e-01f, 1.25e-01f, -1.66666666666666664e-02f, 0.0f,
-                          1.98412698412698413e-04f, 0.0f, -5.51146384479717850e-06f, 8.26719576719576754e-07f,
-                          -5.01042167708834404e-08f, 0.0f, 1.60590438368216133e-10f, 0.0f,
-                          -1.52943274636396328e-12f, 1.43384319971621551e-13f, -5.62291450869104120e-15f};
-    const float c8[17] = {-5.41196100146197012e-01f, 0.0f, 9.01993500243661595e-02f, 0.0f,
-                          -1.08880247073031376e-02f, 2.77777777777777788e-03f, -2.59238683507217581e-04f, 0.0f,
-                          1.49139136945049866e-06f, 0.0f, -1.35581033586408978e-08f, 0.0f,
-                          2.09821519285173499e-10f, -2.29414911954594490e-11f, 9.99150091834159524e-13f, 0.0f,
-                          -1.52154970177953047e-15f};
-    float a6 = c6[16], a8 = c8[16], d6 = 16.0f * c6[16], d8 = 16.0f * c8[16];
-    for (int k = 15; k >= 1; --k) {
-      a6 = a6 * x + c6[k];
-      a8 = a8 * x + c8[k];
-      d6 = d6 * x + float(k) * c6[k];
-      d8 = d8 * x + float(k) * c8[k];
-    }
-    *h6 = a6 * x + c6[0];
-    *h8 = a8 * x + c8[0];
-    *h6p = d6;
-    *h8p = d8;
-  } else {
-    // x >= 1: every term carries a factor exp(-x sin(pi/8)) or smaller, so the
-    // hardware exp/sin/cos (relative/phase error ~1e-6 here) are accurate enough.
-    const float s8 = 0.38268343236508977f, c8 = 0.92387953251128674f, sq3h = 0.86602540378443865f;
-    const float e2 = fd3_exp(-0.5f * x), e1 = e2 * e2;
-    float sp, cp;
-    fd3_sincos(1.04719755119659775f + sq3h * x, &sp, &cp);
-    const float n6 = e1 - 2.0f * e2 * cp;
-    const float n6p = -e1 + e2 * cp + 2.0f * sq3h * e2 * sp;
-    const float ea = fd3_exp(-x * s8), eb = fd3_exp(-x * c8);
-    float sa, ca, sb, cb;
-    fd3_sincos(0.78539816339744831f + x * c8, &sa, &ca);
-    fd3_sincos(2.35619449019234492f + x * s8, &sb, &cb);
-    const float n8 = ea * ca + eb * cb;
-    const float n8p = -s8 * ea * ca - c8 * ea * sa - c8 * eb * cb - s8 * eb * sb;
-    const float ix = 1.0f / x;
-    *h6 = n6 * ix;
-    *h8 = n8 * ix;
-    *h6p = (n6p - *h6) * ix;
-    *h8p = (n8p - *h8) * ix;
-  }
-}
-
-// Per-pair constants for G_XY(k) = s6 f6^(k; R) + 3 s8 QQ f8^(k; R).
-struct PairK {
-  float R;     // R0_XY, Bohr
-  float p6;    // s6 * 2 pi^2 / (3 R^3)
-  float p8;    // 3 s8 QQ * (-pi^2 / R^5)
-};
-
-// G and dG/dk for |k| = k.
-__host__ __device__ inline void green_eval(const PairK& pk, float k, float* g, float* dg) {
-  float h6, h6p, h8, h8p;
-  green_h(k * pk.R, &h6, &h6p, &h8, &h8p);
-  *g = pk.p6 * h6 + pk.p8 * h8;
-  *dg = (pk.p6 * h6p + pk.p8 * h8p) * pk.R;
-}
-
+// G_XY(k) = s6 f6^(k; R) + 3 s8 QQ f8^(k; R), R = R0_XY, with the analytic
+// Fourier transforms (src/fourierd3/pair_pot.py:152-175)
+//   f6^(k) = 2 pi^2 / (3 R^3) h6(kR),  h6 = [e^-x - 2 e^-x/2 cos(pi/3 + sqrt3 x/2)] / x
+//   f8^(k) =  -pi^2 / R^5     h8(kR),  h8 = [e^-x sin(pi/8) cos(pi/4 + x cos(pi/8))
+//                                           + e^-x cos(pi/8) cos(3pi/4 + x sin(pi/8))] / x
+// (h6(0) = 1, h8(0) = -sqrt2 sin(pi/8)).
 // h6 and h8 do not depend on the species pair, so the k-space kernel reads
 // them from a piecewise polynomial table instead of evaluating four
 // exponentials and three sine/cosine pairs per pair and k point.  Interval i
@@ -299,30 +219,61 @@ __host__ __device__ inline void htab_interval(int i, float4* out) {
   out[2] = make_float4((float)a8[2], (float)a8[3], (float)a8[4], (float)a8[5]);
 }
 
-// h6, h8 and their x-derivatives from the table (x >= 0).
-__host__ __device__ inline void green_h_tab(const float4* __restrict__ tab, float x, float* h6, float* h6p, float* h8,
-                                            float* h8p) {
+// h6, h8 and their derivatives with respect to the local coordinate u from
+// the table (x >= 0), by Horner's scheme carrying the derivative.
+__host__ __device__ inline void green_h_tab(const float4* __restrict__ tab, float x, float* h6, float* d6, float* h8,
+                                            float* d8) {
   const float fx = x * (1.0f / kHtabDx);
   const int i = min((int)fx, kHtabN);
   const float u = 2.0f * (fx - (float)i) - 1.0f;
   const float4 A = tab[3 * i], B = tab[3 * i + 1], C = tab[3 * i + 2];
-  *h6 = A.x + u * (A.y + u * (A.z + u * (A.w + u * (B.x + u * B.y))));
-  *h8 = B.z + u * (B.w + u * (C.x + u * (C.y + u * (C.z + u * C.w))));
-  const float d6 = A.y + u * (2.0f * A.z + u * (3.0f * A.w + u * (4.0f * B.x + u * (5.0f * B.y))));
-  const float d8 = B.w + u * (2.0f * C.x + u * (3.0f * C.y + u * (4.0f * C.z + u * (5.0f * C.w))));
-  *h6p = d6 * (2.0f / kHtabDx);
-  *h8p = d8 * (2.0f / kHtabDx);
+  float p = B.y, d = 0.f;
+  d = d * u + p; p = p * u + B.x;
+  d = d * u + p; p = p * u + A.w;
+  d = d * u + p; p = p * u + A.z;
+  d = d * u + p; p = p * u + A.y;
+  d = d * u + p; p = p * u + A.x;
+  *h6 = p; *d6 = d;
+  p = C.w; d = 0.f;
+  d = d * u + p; p = p * u + C.z;
+  d = d * u + p; p = p * u + C.y;
+  d = d * u + p; p = p * u + C.x;
+  d = d * u + p; p = p * u + B.w;
+  d = d * u + p; p = p * u + B.z;
+  *h8 = p; *d8 = d;
 }
 
-// G and dG/dk for |k| = k from the table.
-__host__ __device__ inline void green_eval_tab(const PairK& pk, const float4* __restrict__ tab, float k, float* g,
-                                               float* dg, float* h6o = nullptr, float* h6po = nullptr,
-                                               float* h8o = nullptr, float* h8po = nullptr) {
-  float h6, h6p, h8, h8p;
-  green_h_tab(tab, k * pk.R, &h6, &h6p, &h8, &h8p);
-  *g = pk.p6 * h6 + pk.p8 * h8;
-  *dg = (pk.p6 * h6p + pk.p8 * h8p) * pk.R;
-  if (h6o) { *h6o = h6; *h6po = h6p; *h8o = h8; *h8po = h8p; }
+// Per species pair constants of the k-space kernel.
+struct KsPair {
+  float R;         // R0_XY, Bohr
+  float p6, p8;    // G = p6 h6(kR) + p8 h8(kR): s6 2pi^2/(3R^3), 3 s8 QQ (-pi^2/R^5)
+  float q6, q8;    // dG/dk = q6 h6_u + q8 h8_u (u-derivatives of the table): p R (2/dx)
+  float pf6, pf8;  // dG/ds6 = pf6 h6, dG/ds8 = pf8 h8
+  float dR;        // dR0/da1 = sqrt(3 QQ)
+};
+
+__host__ __device__ inline KsPair ks_pair(float qq, float s6, float s8, float a1, float a2) {
+  const float sq = sqrtf(3.0f * qq);
+  const float R = a1 * sq + a2;
+  KsPair k;
+  k.R = R;
+  k.pf6 = float(2.0 * kPi * kPi / 3.0) / (R * R * R);
+  k.pf8 = 3.0f * qq * float(-kPi * kPi) / (R * R * R * R * R);
+  k.p6 = s6 * k.pf6;
+  k.p8 = s8 * k.pf8;
+  k.q6 = k.p6 * R * (2.0f / kHtabDx);
+  k.q8 = k.p8 * R * (2.0f / kHtabDx);
+  k.dR = sq;
+  return k;
+}
+
+// G and dG/dk at |k| = k from the table; h6, h8 and their u-derivatives
+// (dh/dx = dh/du * 2/dx) for the parameter derivatives.
+__host__ __device__ inline void green_eval_tab(const KsPair& pk, const float4* __restrict__ tab, float k, float* g,
+                                               float* dg, float* h6, float* d6, float* h8, float* d8) {
+  green_h_tab(tab, k * pk.R, h6, d6, h8, d8);
+  *g = pk.p6 * *h6 + pk.p8 * *h8;
+  *dg = pk.q6 * *d6 + pk.q8 * *d8;
 }
 
 // ----------------------------------------------------------------- misc ---

@@ -24,10 +24,13 @@ namespace fd3 {
 // ------------------------------------------------------------ tables ---
 // tw_axis[k] = exp(-2 pi i k / n) for each axis; sinc_axis[m] = sinc(m/n)^(2p)
 // with m the fftfreq integer of index m (rfftfreq for z).
+// Also the h table and, when pi.qq is set, the per-pair constants.
 __global__ void k_init_tables(float2* twx, float2* twy, float2* twz, float* scx, float* scy, float* scz,
-                              float4* htab, int nx, int ny, int nz, int order, int use_sinc) {
+                              float4* htab, KsPair* pc, PairInit pi, int nx, int ny, int nz, int order,
+                              int use_sinc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t <= kHtabN) htab_interval(t, htab + 3 * t);
+  if (pi.qq && t < pi.nS * pi.nS) pc[t] = ks_pair(pi.qq[t], pi.s6, pi.s8, pi.a1, pi.a2);
   auto tw = [](float2* dst, int k, int n) {
     double s, c;
     sincospi(-2.0 * (double)k / (double)n, &s, &c);
@@ -58,14 +61,14 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ rho, fl
   stage_table(tws, twz, nz);
   const float2* src = (const float2*)rho + ((size_t)(c * nx + x) * ny + y0) * M;
   const int lM = ilog2(M), lT = ilog2(TY);
-  tile_to_lds<kCopyU>(lds, src, TY * M, [](int e) { return (size_t)e; },
+  tile_to_lds<kCopyU>(lds, src, TY * M, [](int e) { return (unsigned)e; },
                      [=](int e) { return (e >> lM) * ls + (e & (M - 1)); });
   __syncthreads();
   lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   z_pairs<true>(lds, TY, ls, M, tws);  // X[k] = E[k] + W_nz^k O[k]
   __syncthreads();
   float2* dst = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
-  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
                         [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
 }
 
@@ -80,14 +83,14 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
   const int lM = ilog2(M), lT = ilog2(TY);
-  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
                      [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
   __syncthreads();
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
   lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   float2* dst = (float2*)out + ((size_t)(c * nx + x) * ny + y0) * M;
-  tile_from_lds<kCopyU>(dst, lds, TY * M, [](int e) { return (size_t)e; },
+  tile_from_lds<kCopyU>(dst, lds, TY * M, [](int e) { return (unsigned)e; },
                         [=](int e) { return (e >> lM) * ls + (e & (M - 1)); });
 }
 
@@ -104,22 +107,22 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
   const int lny = ilog2(ny), lTX = ilog2(TX);
   if (DIR < 0) {
     const float2* src = in + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (size_t)(e >> lny) * M1 * ny + (e & (ny - 1)); },
+    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (unsigned)(e >> lny) * M1 * ny + (e & (ny - 1)); },
                         [=](int e) { return (e >> lny) * ls + (e & (ny - 1)); });
   } else {
     const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (size_t)(e >> lTX) * nx + (e & (TX - 1)); },
+    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (unsigned)(e >> lTX) * nx + (e & (TX - 1)); },
                         [=](int e) { return (e & (TX - 1)) * ls + (e >> lTX); });
   }
   __syncthreads();
   lds_fft<DIR, kFftMaxB>(lds, ny, TX, ls, tws, 1);
   if (DIR < 0) {
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (size_t)(e >> lTX) * nx + (e & (TX - 1)); },
+    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (unsigned)(e >> lTX) * nx + (e & (TX - 1)); },
                           [=](int e) { return (e & (TX - 1)) * ls + (e >> lTX); });
   } else {
     float2* dst = out + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (size_t)(e >> lny) * M1 * ny + (e & (ny - 1)); },
+    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (unsigned)(e >> lny) * M1 * ny + (e & (ny - 1)); },
                           [=](int e) { return (e >> lny) * ls + (e & (ny - 1)); });
   }
 }
@@ -137,7 +140,7 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
   float2* tws = lds + LPB * ls;
   stage_table(tws, twx, nx);
   const int lnx = ilog2(nx);
-  auto gi = [](int e) { return (size_t)e; };
+  auto gi = [](int e) { return (unsigned)e; };
   auto li = [=](int e) { return (e >> lnx) * ls + (e & (nx - 1)); };
   tile_to_lds<kCopyU>(lds, base, nl * nx, gi, li);
   __syncthreads();
@@ -147,60 +150,63 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
 
 // ------------------------------------------------- fused k-space kernel ---
 // For one kz plane, NK consecutive ky rows and RT ranks [q0, q0+RT) of every
-// species: forward x FFT of the nS*RT*NK lines, then per k-point
+// species (a tile): forward x FFT of the nS*RT*NK lines, then per k-point
 //   phi_X^q = sum_Y G~_XY S_Y^q,  G~ = G(|k|) / sinc^2p (P3M only)
 //   e      += w sum_X sum_q lambda_q Re(S_X^q conj(phi_X^q))
 //   dsum   += w sum_XY dG~_XY/d|k| sum_q lambda_q Re(S_X^q conj(S_Y^q))
 // (w = 1 on the kz = 0 and nz/2 planes, 2 elsewhere: the half-spectrum weights
 // of src/fourierd3/pair_pot.py:209-218), writes -(lambda_q/Omega) phi back and
-// inverse-transforms along x, leaving dE/drho's spectrum in place.
+// inverse-transforms along x, leaving dE/drho's spectrum in place.  A block
+// runs KT tiles (consecutive ky rows) and reduces once.
 // Per-block partial sums (double): [0] sum e, [1..6] sum w dsum k_a k_b/|k|
 // (xx,yy,zz,xy,xz,yz), [7..10] dE/d(s6,s8,a1,a2) spectral parts (unscaled).
 // G~ is evaluated once per unordered species pair and shared by all ranks.
 struct KspaceArgs {
   float2* spec;
   const float* eigs;
-  const float* qq;      // (nS, nS)
+  const KsPair* pc;     // (nS, nS)
   const float2* twx;
   const float4* htab;
   const float* scx; const float* scy; const float* scz;
   double* partials;     // (gridDim, 16)
   float recip[9];       // rows b_i of 2 pi cell^-T (Bohr^-1)
-  float s6, s8, a1, a2;
   float neg_inv_vol;    // -1/Omega
-  int nx, ny, nz, M1, nS, r, RT, NK, use_sinc, need_grad, need_params;
-};
-
-// Per species pair: Green's function constants and the parameter-derivative
-// factors f6 = G's s6 coefficient / s6, f8 = its s8 coefficient / s8, dR0/da1.
-struct PairP {
-  float pf6, pf8, dR;
+  int nx, ny, nz, M1, nS, r, RT, NK, KT, use_sinc, need_grad, need_params;
 };
 
 // Dynamic LDS of both k-space kernels: [h table | x twiddles | lines ... |
-// pairs], the table first so that its float4 entries are 16-byte aligned.
-__device__ inline void kspace_stage_consts(const KspaceArgs& a, float4* htab, float2* tws, PairK* pairs, PairP* pp) {
+// line bases], the table first so that its float4 entries are 16-byte aligned.
+__device__ inline void kspace_stage_consts(const KspaceArgs& a, float4* htab, float2* tws) {
   for (int i = threadIdx.x; i < 3 * (kHtabN + 1); i += blockDim.x) htab[i] = a.htab[i];
   stage_table(tws, a.twx, a.nx);
-  const int nS = a.nS;
-  for (int t = threadIdx.x; t < nS * nS; t += blockDim.x) {
-    const float qq = a.qq[t];
-    const float sq = sqrtf(3.0f * qq);
-    const float R = a.a1 * sq + a.a2;
-    const float f6 = float(2.0 * kPi * kPi / 3.0) / (R * R * R);
-    const float f8 = 3.0f * qq * float(-kPi * kPi) / (R * R * R * R * R);
-    PairK pk;
-    pk.R = R;
-    pk.p6 = a.s6 * f6;
-    pk.p8 = a.s8 * f8;
-    pairs[t] = pk;
-    PairP q;
-    q.pf6 = f6; q.pf8 = f8; q.dR = sq;
-    pp[t] = q;
+}
+
+// Element offset in spec of every tile line l = (X*RT + qi)*NK + kyi, or -1
+// for the lines of ranks past the last (qi >= rt).
+__device__ inline void kspace_line_bases(const KspaceArgs& a, long long* lbase, int nlines, int RT, int rt, int ky0,
+                                         int kz, int q0) {
+  for (int l = threadIdx.x; l < nlines; l += blockDim.x) {
+    const int kyi = l % a.NK, xq = l / a.NK, qi = xq % RT, X = xq / RT;
+    lbase[l] = qi < rt ? (((long long)(X * a.r + q0 + qi) * a.M1 + kz) * a.ny + ky0 + kyi) * a.nx : -1LL;
   }
 }
 
-// k vector, |k|, 1/sinc^2p and the half-spectrum weight of column (kyi, kx).
+// Tile copies of the nlines x nx lines (LDS line stride nx + 1).
+__device__ inline void kspace_load(const KspaceArgs& a, float2* S, const long long* lbase, int nlines, int lnx) {
+  const int nx = a.nx;
+  tile_to_lds<kCopyU / 2>(
+      S, a.spec, nlines * nx, [=](int e) { return lbase[e >> lnx] + (e & (nx - 1)); },
+      [=](int e) { return e + (e >> lnx); }, [=](int e) { return lbase[e >> lnx] >= 0; });
+}
+__device__ inline void kspace_store(const KspaceArgs& a, const float2* S, const long long* lbase, int nlines,
+                                    int lnx) {
+  const int nx = a.nx;
+  tile_from_lds<kCopyU / 2>(
+      a.spec, S, nlines * nx, [=](int e) { return lbase[e >> lnx] + (e & (nx - 1)); },
+      [=](int e) { return e + (e >> lnx); }, [=](int e) { return lbase[e >> lnx] >= 0; });
+}
+
+// k vector, |k| and 1/sinc^2p of column (kx, ky).
 struct KPoint {
   float kv0, kv1, kv2, kn, isc;
 };
@@ -212,20 +218,21 @@ __device__ inline KPoint kspace_point(const KspaceArgs& a, int kx, int ky, int k
   p.kv0 = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
   p.kv1 = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
   p.kv2 = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
-  p.kn = sqrtf(p.kv0 * p.kv0 + p.kv1 * p.kv1 + p.kv2 * p.kv2);
+  p.kn = __builtin_amdgcn_sqrtf(p.kv0 * p.kv0 + p.kv1 * p.kv1 + p.kv2 * p.kv2);
   p.isc = 1.0f;
   if (a.use_sinc) p.isc = __builtin_amdgcn_rcpf(fmaxf(a.scx[kx] * a.scy[ky] * a.scz[kz], 1e-10f));
   return p;
 }
 
 // Derivative of G with respect to (s6, s8, a1, a2) at one pair, times t.
-__device__ inline void kspace_param_terms(const PairK& pk, const PairP& pp, float kn, float h6, float h6p, float h8,
-                                          float h8p, float t, float* p_acc) {
+__device__ inline void kspace_param_terms(const KsPair& pk, float kn, float h6, float d6, float h8, float d8, float t,
+                                          float* p_acc) {
   const float R = pk.R;
+  const float h6p = d6 * (2.0f / kHtabDx), h8p = d8 * (2.0f / kHtabDx);
   const float dgdR = pk.p6 * (-3.0f * h6 / R + kn * h6p) + pk.p8 * (-5.0f * h8 / R + kn * h8p);
-  p_acc[0] += pp.pf6 * h6 * t;
-  p_acc[1] += pp.pf8 * h8 * t;
-  p_acc[2] += dgdR * pp.dR * t;
+  p_acc[0] += pk.pf6 * h6 * t;
+  p_acc[1] += pk.pf8 * h8 * t;
+  p_acc[2] += dgdR * pk.dR * t;
   p_acc[3] += dgdR * t;
 }
 
@@ -233,7 +240,7 @@ __device__ inline void kspace_accumulate(double (&acc)[11], const KPoint& p, flo
                                          const float* p_acc, bool params) {
   acc[0] += (double)(w * e_acc);
   if (p.kn > 0.f) {
-    const double s = (double)(w * d_acc * p.isc) / (double)p.kn;
+    const double s = (double)(w * d_acc * p.isc * __builtin_amdgcn_rcpf(p.kn));
     acc[1] += s * p.kv0 * p.kv0; acc[2] += s * p.kv1 * p.kv1; acc[3] += s * p.kv2 * p.kv2;
     acc[4] += s * p.kv0 * p.kv1; acc[5] += s * p.kv0 * p.kv2; acc[6] += s * p.kv1 * p.kv2;
   }
@@ -243,99 +250,131 @@ __device__ inline void kspace_accumulate(double (&acc)[11], const KPoint& p, flo
   }
 }
 
-// nS = NS <= kKsRegMaxS species and RT ranks per block: each thread keeps the
+// Block sums of the 7 (or, with parameter gradients, 11) partials; the
+// unused parameter slots are written as zeros.  Ends with a barrier.
+__device__ inline void kspace_reduce(double (&acc)[11], double* red, const KspaceArgs& a, bool params) {
+  const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  double* out = a.partials + (size_t)bid * 16;
+  if (params) {
+    block_sum_store<11>(acc, red, out);
+  } else {
+    double v[7];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) v[i] = acc[i];
+    block_sum_store<7>(v, red, out);
+    if (threadIdx.x >= 7 && threadIdx.x < 11) out[threadIdx.x] = 0.0;
+  }
+}
+
+// nS = NS <= kKsRegMaxS species and RT ranks per tile: each thread keeps the
 // NS*RT spectra of its k point in registers and writes phi back over them,
 // so the block needs a single LDS buffer.  Lines l = (X*RT + qi)*NK + kyi.
-template <int NS, int RT>
+template <int NS, int RT, bool PARAMS>
 __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
+  constexpr int MAXB = ks_maxb(NS * RT);
+  constexpr int NP = NS * (NS + 1) / 2;  // unordered species pairs
   extern __shared__ float4 lds4[];
   __shared__ double red[16 * 16];
+  __shared__ KsPair pcl[NP];
   const int nx = a.nx, NK = a.NK;
-  const int ky0 = blockIdx.x * NK, kz = blockIdx.y, q0 = blockIdx.z * RT;
+  const int kz = blockIdx.y, q0 = blockIdx.z * RT;
   const int rt = min(RT, a.r - q0);
   const int nlines = NS * RT * NK;
   const int ls = nx + 1;
-  const int lnx = ilog2(nx), lNK = ilog2(NK);  // nx, NK powers of two
+  const int lnx = ilog2(nx);  // nx a power of two
   float4* htab = lds4;
   float2* tws = (float2*)(htab + 3 * (kHtabN + 1));
   float2* S = tws + nx;
-  PairK* pairs = (PairK*)(S + nlines * ls);
-  PairP* pp = (PairP*)(pairs + NS * NS);
-  kspace_stage_consts(a, htab, tws, pairs, pp);
-  // element e of the tile: line l = e / nx, column kx; lines of padded ranks
-  // (qi >= rt) are zero-filled and never stored
-  auto gofs = [=](int e) {
-    const int l = e >> lnx, kx = e & (nx - 1);
-    const int kyi = l & (NK - 1), xq = l >> lNK, qi = xq % RT, X = xq / RT;
-    return (((size_t)(X * a.r + q0 + qi) * a.M1 + kz) * a.ny + ky0 + kyi) * nx + kx;
-  };
-  auto lofs = [=](int e) { return (e >> lnx) * ls + (e & (nx - 1)); };
-  auto live = [=](int e) { return ((e >> lnx) >> lNK) % RT < rt; };
-  tile_to_lds<kCopyU>(S, a.spec, nlines * nx, gofs, lofs, live);
-  __syncthreads();
-  lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
+  long long* lbase = (long long*)(S + nlines * ls);
+  kspace_stage_consts(a, htab, tws);
+  for (int t = threadIdx.x; t < NP; t += blockDim.x) {  // pair t = (X, Y >= X) in row order
+    int X = 0, r0 = 0;
+    while (t >= r0 + NS - X) { r0 += NS - X; ++X; }
+    pcl[t] = a.pc[X * NS + X + (t - r0)];
+  }
 
   float lam[RT];
 #pragma unroll
   for (int q = 0; q < RT; ++q) lam[q] = q < rt ? a.eigs[q0 + q] : 0.0f;
-  const bool params = a.need_params;
   double acc[11];
 #pragma unroll
   for (int i = 0; i < 11; ++i) acc[i] = 0.0;
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
-  for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
-    const int kyi = t >> lnx, kx = t & (nx - 1);
-    const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
-    float2 sv[NS][RT], ph[NS][RT];
+  for (int it = 0; it < a.KT; ++it) {
+    const int ky0 = (blockIdx.x * a.KT + it) * NK;
+    if (it) __syncthreads();  // the previous tile's store has read S and lbase
+    kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
+    __syncthreads();
+    kspace_load(a, S, lbase, nlines, lnx);
+    __syncthreads();
+    lds_fft<-1, MAXB>(S, nx, nlines, ls, tws, 1);
+    for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
+      const int kyi = t >> lnx, kx = t & (nx - 1);
+      const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
+      // G~ and dG~/dk of every pair first, then the spectra
+      float gp[NP], dgp[NP], p_acc[4] = {0.f, 0.f, 0.f, 0.f};
+      float h6v[PARAMS ? NP : 1], d6v[PARAMS ? NP : 1], h8v[PARAMS ? NP : 1], d8v[PARAMS ? NP : 1];
 #pragma unroll
-    for (int X = 0; X < NS; ++X)
-#pragma unroll
-      for (int q = 0; q < RT; ++q) {
-        sv[X][q] = S[((X * RT + q) * NK + kyi) * ls + kx];
-        ph[X][q] = make_float2(0.f, 0.f);
+      for (int pi = 0; pi < NP; ++pi) {
+        float h6, d6, h8, d8;
+        green_eval_tab(pcl[pi], htab, kp.kn, &gp[pi], &dgp[pi], &h6, &d6, &h8, &d8);
+        gp[pi] *= kp.isc;
+        if (PARAMS) {
+          h6v[pi] = h6; d6v[pi] = d6; h8v[pi] = h8; d8v[pi] = d8;
+          asm volatile("" : "+v"(h6v[pi]), "+v"(d6v[pi]), "+v"(h8v[pi]), "+v"(d8v[pi]));
+        }
+        // pin g and dG/dk here: left to itself the compiler defers the
+        // derivative and keeps every pair's Horner intermediates live
+        asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
       }
-    float d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
+      float2 sv[NS][RT], ph[NS][RT];
 #pragma unroll
-    for (int X = 0; X < NS; ++X) {
-#pragma unroll
-      for (int Y = X; Y < NS; ++Y) {
-        const PairK pk = pairs[X * NS + Y];
-        float g, dg, h6, h6p, h8, h8p;
-        green_eval_tab(pk, htab, kp.kn, &g, &dg, &h6, &h6p, &h8, &h8p);
-        const float gs = g * kp.isc;
-        float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
+      for (int X = 0; X < NS; ++X)
 #pragma unroll
         for (int q = 0; q < RT; ++q) {
-          const float2 sx = sv[X][q], sy = sv[Y][q];
-          tre += lam[q] * (sx.x * sy.x + sx.y * sy.y);
-          ph[X][q].x += gs * sy.x;
-          ph[X][q].y += gs * sy.y;
-          if (Y != X) {
-            ph[Y][q].x += gs * sx.x;
-            ph[Y][q].y += gs * sx.y;
-          }
+          sv[X][q] = S[((X * RT + q) * NK + kyi) * ls + kx];
+          ph[X][q] = make_float2(0.f, 0.f);
         }
-        if (Y != X) tre *= 2.0f;
-        d_acc += dg * tre;
-        if (params) kspace_param_terms(pk, pp[X * NS + Y], kp.kn, h6, h6p, h8, h8p, tre, p_acc);
+      float d_acc = 0.f;
+#pragma unroll
+      for (int X = 0, pi = 0; X < NS; ++X) {
+#pragma unroll
+        for (int Y = X; Y < NS; ++Y, ++pi) {
+          const float gs = gp[pi];
+          float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
+#pragma unroll
+          for (int q = 0; q < RT; ++q) {
+            const float2 sx = sv[X][q], sy = sv[Y][q];
+            tre += lam[q] * (sx.x * sy.x + sx.y * sy.y);
+            ph[X][q].x += gs * sy.x;
+            ph[X][q].y += gs * sy.y;
+            if (Y != X) {
+              ph[Y][q].x += gs * sx.x;
+              ph[Y][q].y += gs * sx.y;
+            }
+          }
+          if (Y != X) tre *= 2.0f;
+          d_acc += dgp[pi] * tre;
+          if (PARAMS) kspace_param_terms(pcl[pi], kp.kn, h6v[pi], d6v[pi], h8v[pi], d8v[pi], tre, p_acc);
+        }
       }
+      float e_acc = 0.f;
+#pragma unroll
+      for (int X = 0; X < NS; ++X)
+#pragma unroll
+        for (int q = 0; q < RT; ++q) {
+          e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
+          const float f = lam[q] * a.neg_inv_vol;
+          S[((X * RT + q) * NK + kyi) * ls + kx] = make_float2(f * ph[X][q].x, f * ph[X][q].y);
+        }
+      kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, PARAMS);
     }
-    float e_acc = 0.f;
-#pragma unroll
-    for (int X = 0; X < NS; ++X)
-#pragma unroll
-      for (int q = 0; q < RT; ++q) {
-        e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
-        const float f = lam[q] * a.neg_inv_vol;
-        S[((X * RT + q) * NK + kyi) * ls + kx] = make_float2(f * ph[X][q].x, f * ph[X][q].y);
-      }
-    kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, params);
+    if (!a.need_grad) continue;
+    __syncthreads();
+    lds_fft<1, MAXB>(S, nx, nlines, ls, tws, 1);
+    kspace_store(a, S, lbase, nlines, lnx);
   }
-  const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  block_sum_store<11>(acc, red, a.partials + (size_t)bid * 16);  // ends with a barrier
-  if (!a.need_grad) return;
-  lds_fft<1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
-  tile_from_lds<kCopyU>(a.spec, S, nlines * nx, gofs, lofs, live);
+  kspace_reduce(acc, red, a, PARAMS);
 }
 
 // Any number of species: phi accumulates in a second LDS buffer P.
@@ -343,87 +382,85 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   extern __shared__ float4 lds4[];
   __shared__ double red[16 * 16];
   const int nx = a.nx, nS = a.nS, RT = a.RT, NK = a.NK;
-  const int ky0 = blockIdx.x * NK, kz = blockIdx.y, q0 = blockIdx.z * RT;
+  const int kz = blockIdx.y, q0 = blockIdx.z * RT;
   const int rt = min(RT, a.r - q0);
   const int nlines = nS * RT * NK;
   const int ls = nx + 1;
-  const int lnx = ilog2(nx), lNK = ilog2(NK), lRT = ilog2(RT);  // RT, NK, nx are powers of two
+  const int lnx = ilog2(nx);  // nx a power of two
   float4* htab = lds4;
   float2* tws = (float2*)(htab + 3 * (kHtabN + 1));
   float2* S = tws + nx;
   float2* P = S + nlines * ls;
-  PairK* pairs = (PairK*)(P + nlines * ls);
-  PairP* pp = (PairP*)(pairs + nS * nS);
-  kspace_stage_consts(a, htab, tws, pairs, pp);
-  auto gofs = [=](int e) {
-    const int l = e >> lnx, kx = e & (nx - 1);
-    const int kyi = l & (NK - 1), xq = l >> lNK, qi = xq & (RT - 1), X = xq >> lRT;
-    return (((size_t)(X * a.r + q0 + qi) * a.M1 + kz) * a.ny + ky0 + kyi) * nx + kx;
-  };
-  auto lofs = [=](int e) { return (e >> lnx) * ls + (e & (nx - 1)); };
-  auto live = [=](int e) { return (((e >> lnx) >> lNK) & (RT - 1)) < rt; };
-  tile_to_lds<kCopyU>(S, a.spec, nlines * nx, gofs, lofs, live);
-  __syncthreads();
-  lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
+  long long* lbase = (long long*)(P + nlines * ls);
+  kspace_stage_consts(a, htab, tws);
 
   const bool params = a.need_params;
   double acc[11];
 #pragma unroll
   for (int i = 0; i < 11; ++i) acc[i] = 0.0;
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
-  for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
-    const int kyi = t >> lnx, kx = t & (nx - 1);
-    const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
-    float d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
-    // phi accumulates in P (this thread owns column (kyi, kx) of every line)
-    for (int l = 0; l < nS * RT; ++l) P[(l * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
-    for (int X = 0; X < nS; ++X) {
-      for (int Y = X; Y < nS; ++Y) {
-        const PairK pk = pairs[X * nS + Y];
-        float g, dg, h6, h6p, h8, h8p;
-        green_eval_tab(pk, htab, kp.kn, &g, &dg, &h6, &h6p, &h8, &h8p);
-        const float gs = g * kp.isc;
-        float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
+  for (int it = 0; it < a.KT; ++it) {
+    const int ky0 = (blockIdx.x * a.KT + it) * NK;
+    if (it) __syncthreads();
+    kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
+    __syncthreads();
+    kspace_load(a, S, lbase, nlines, lnx);
+    __syncthreads();
+    lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
+    for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
+      const int kyi = t >> lnx, kx = t & (nx - 1);
+      const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
+      float d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
+      // phi accumulates in P (this thread owns column (kyi, kx) of every line)
+      for (int l = 0; l < nS * RT; ++l) P[(l * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
+      for (int X = 0; X < nS; ++X) {
+        for (int Y = X; Y < nS; ++Y) {
+          const KsPair pk = a.pc[X * nS + Y];
+          float g, dg, h6, d6, h8, d8;
+          green_eval_tab(pk, htab, kp.kn, &g, &dg, &h6, &d6, &h8, &d8);
+          const float gs = g * kp.isc;
+          float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
+          for (int qi = 0; qi < rt; ++qi) {
+            const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
+            const int iy = ((Y * RT + qi) * NK + kyi) * ls + kx;
+            const float2 sx = S[ix], sy = S[iy];
+            float2 px = P[ix];
+            px.x += gs * sy.x;
+            px.y += gs * sy.y;
+            P[ix] = px;
+            if (Y != X) {
+              float2 py = P[iy];
+              py.x += gs * sx.x;
+              py.y += gs * sx.y;
+              P[iy] = py;
+            }
+            tre += a.eigs[q0 + qi] * (sx.x * sy.x + sx.y * sy.y);
+          }
+          if (Y != X) tre *= 2.0f;
+          d_acc += dg * tre;
+          if (params) kspace_param_terms(pk, kp.kn, h6, d6, h8, d8, tre, p_acc);
+        }
+      }
+      // energy sum_X sum_q lambda_q Re(S conj phi); P <- -(lambda_q / Omega) phi
+      float e_acc = 0.f;
+      for (int X = 0; X < nS; ++X) {
         for (int qi = 0; qi < rt; ++qi) {
           const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
-          const int iy = ((Y * RT + qi) * NK + kyi) * ls + kx;
-          const float2 sx = S[ix], sy = S[iy];
-          float2 px = P[ix];
-          px.x += gs * sy.x;
-          px.y += gs * sy.y;
-          P[ix] = px;
-          if (Y != X) {
-            float2 py = P[iy];
-            py.x += gs * sx.x;
-            py.y += gs * sx.y;
-            P[iy] = py;
-          }
-          tre += a.eigs[q0 + qi] * (sx.x * sy.x + sx.y * sy.y);
+          const float lam = a.eigs[q0 + qi];
+          const float2 sx = S[ix], ph = P[ix];
+          e_acc += lam * (sx.x * ph.x + sx.y * ph.y);
+          const float f = lam * a.neg_inv_vol;
+          P[ix] = make_float2(f * ph.x, f * ph.y);
         }
-        if (Y != X) tre *= 2.0f;
-        d_acc += dg * tre;
-        if (params) kspace_param_terms(pk, pp[X * nS + Y], kp.kn, h6, h6p, h8, h8p, tre, p_acc);
       }
+      kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, params);
     }
-    // energy sum_X sum_q lambda_q Re(S conj phi); P <- -(lambda_q / Omega) phi
-    float e_acc = 0.f;
-    for (int X = 0; X < nS; ++X) {
-      for (int qi = 0; qi < rt; ++qi) {
-        const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
-        const float lam = a.eigs[q0 + qi];
-        const float2 sx = S[ix], ph = P[ix];
-        e_acc += lam * (sx.x * ph.x + sx.y * ph.y);
-        const float f = lam * a.neg_inv_vol;
-        P[ix] = make_float2(f * ph.x, f * ph.y);
-      }
-    }
-    kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, params);
+    if (!a.need_grad) continue;
+    __syncthreads();
+    lds_fft<1, kFftMaxB>(P, nx, nlines, ls, tws, 1);
+    kspace_store(a, P, lbase, nlines, lnx);
   }
-  const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  block_sum_store<11>(acc, red, a.partials + (size_t)bid * 16);  // ends with a barrier
-  if (!a.need_grad) return;
-  lds_fft<1, kFftMaxB>(P, nx, nlines, ls, tws, 1);
-  tile_from_lds<kCopyU>(a.spec, P, nlines * nx, gofs, lofs, live);
+  kspace_reduce(acc, red, a, params);
 }
 
 // --------------------------------------------------------------- host ---
@@ -447,8 +484,8 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   p->TX = 1 << (31 - __builtin_clz(p->TX));
   p->LPB = std::max(1, budget / (p->nx + 1));
   // k-space: the h table, the x twiddles, one (nS <= kKsRegMaxS) or two LDS
-  // buffers of nS*RT*NK lines of nx+1 complex, and the per-pair constants,
-  // within 62 KB (the block also holds 2 KB of static LDS)
+  // buffers of nS*RT*NK lines of nx+1 complex, and the line bases, within
+  // 62 KB (the block also holds 2 KB of static LDS)
   if (nS > 32) {
     set_error("the k-space kernel supports up to 32 species, got %d", nS);
     return FD3_ERR_UNSUPPORTED;
@@ -457,10 +494,10 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   const int nbuf = p->ks_reg ? 1 : 2;
   auto ks_bytes = [&](int rt_, int nk_) {
     return (size_t)3 * (kHtabN + 1) * sizeof(float4) + (size_t)p->nx * sizeof(float2) +
-           (size_t)nbuf * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * nS * (sizeof(PairK) + 12);
+           (size_t)nbuf * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * rt_ * nk_ * sizeof(long long);
   };
   const size_t ks_budget = 62 * 1024;
-  // RT: ranks per block; the register variant takes any RT <= 4, the generic
+  // RT: ranks per tile; the register variant takes any RT <= 4, the generic
   // one a power of two <= 8 (a last partial tile is padded with zero lines)
   int RT = 1;
   if (p->ks_reg) RT = std::min(r, 4);
@@ -468,7 +505,8 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
     while (RT < std::min(r, 8)) RT <<= 1;
   int NK = std::max(1, std::min(p->ny, 256 / p->nx));
   auto ks_fits = [&](int rt_, int nk_) {
-    return ks_bytes(rt_, nk_) <= ks_budget && (size_t)nS * rt_ * nk_ * p->nx <= (size_t)kFftLines;
+    const int maxb = p->ks_reg ? ks_maxb(nS * rt_) : kFftMaxB;
+    return ks_bytes(rt_, nk_) <= ks_budget && (size_t)nS * rt_ * nk_ * p->nx <= (size_t)1024 * maxb;
   };
   while (RT > 1 && !ks_fits(RT, NK)) RT = p->ks_reg ? RT - 1 : RT >> 1;
   while (NK > 1 && !ks_fits(RT, NK)) NK >>= 1;
@@ -478,7 +516,12 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   }
   p->RT = RT; p->NK = NK;
   p->ks_lds = ks_bytes(RT, NK);
-  p->ks_blocks = (p->ny / NK) * p->M1 * ((r + RT - 1) / RT);
+  // tiles per block: up to 4 ky tiles, keeping at least ~8k blocks in flight
+  const int zb = (r + RT - 1) / RT;
+  int KT = 1;
+  while (KT < 4 && (p->ny / NK) % (2 * KT) == 0 && (size_t)(p->ny / NK / (2 * KT)) * p->M1 * zb >= 8192) KT <<= 1;
+  p->KT = KT;
+  p->ks_blocks = (p->ny / NK / KT) * p->M1 * zb;
   // register budget of lds_fft<.., kFftMaxB>: lines * n <= kFftLines
   const size_t lim = kFftLines;
   if ((size_t)p->TY * p->M > lim || (size_t)p->TX * p->ny > lim || (size_t)p->LPB * p->nx > lim ||
@@ -497,13 +540,17 @@ void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t) {
   t->scy = ar.take<float>(p.ny);
   t->scz = ar.take<float>(p.M1);
   t->htab = ar.take<float4>(3 * (kHtabN + 1));
+  t->pc = ar.take<KsPair>((size_t)p.nS * p.nS);
 }
 
-int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s) {
-  const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz), kHtabN + 1);
+int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s,
+                    const PairInit* pairs) {
+  PairInit pi{};
+  if (pairs) pi = *pairs;
+  const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz), std::max(kHtabN + 1, pi.nS * pi.nS));
   ProfScope _ps("k_init_tables", s);
   hipLaunchKernelGGL(k_init_tables, dim3((n + 255) / 256), dim3(256), 0, s, t.twx, t.twy, t.twz, t.scx, t.scy, t.scz,
-                     t.htab, p.nx, p.ny, p.nz, order, use_sinc);
+                     t.htab, t.pc, pi, p.nx, p.ny, p.nz, order, use_sinc);
   FD3_LAUNCH_CHECK("k_init_tables");
   return FD3_OK;
 }
@@ -572,24 +619,24 @@ int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, i
   return FD3_OK;
 }
 
-int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, const float* qq,
-               double s6, double s8, double a1, double a2, int need_grad, int need_params, double* partials,
-               hipStream_t s) {
+int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
+               int need_params, double* partials, hipStream_t s) {
   KspaceArgs a;
-  a.spec = spec; a.eigs = eigs; a.qq = qq; a.twx = t.twx;
+  a.spec = spec; a.eigs = eigs; a.pc = t.pc; a.twx = t.twx; a.htab = t.htab;
   a.scx = t.scx; a.scy = t.scy; a.scz = t.scz; a.partials = partials;
   for (int i = 0; i < 9; ++i) a.recip[i] = (float)g.recip[i];
-  a.s6 = (float)s6; a.s8 = (float)s8; a.a1 = (float)a1; a.a2 = (float)a2;
   a.neg_inv_vol = (float)(-1.0 / g.volume);
-  a.nx = p.nx; a.ny = p.ny; a.nz = p.nz; a.M1 = p.M1; a.nS = p.nS; a.r = p.r; a.RT = p.RT; a.NK = p.NK;
+  a.nx = p.nx; a.ny = p.ny; a.nz = p.nz; a.M1 = p.M1; a.nS = p.nS; a.r = p.r; a.RT = p.RT; a.NK = p.NK; a.KT = p.KT;
   a.use_sinc = g.interp == FD3_INTERP_P3M;
   a.need_grad = need_grad; a.need_params = need_params;
-  a.htab = t.htab;
   const size_t sh = p.ks_lds;
-  const dim3 grid(p.ny / p.NK, p.M1, (p.r + p.RT - 1) / p.RT);
+  const dim3 grid(p.ny / p.NK / p.KT, p.M1, (p.r + p.RT - 1) / p.RT);
   ProfScope _ps("k_kspace", s);
-#define FD3_KS(NS_, RT_) \
-  case NS_ * 8 + RT_: hipLaunchKernelGGL((k_kspace_reg<NS_, RT_>), grid, dim3(256), sh, s, a); break;
+#define FD3_KS(NS_, RT_)                                                                         \
+  case NS_ * 8 + RT_:                                                                            \
+    if (need_params) hipLaunchKernelGGL((k_kspace_reg<NS_, RT_, true>), grid, dim3(256), sh, s, a); \
+    else hipLaunchKernelGGL((k_kspace_reg<NS_, RT_, false>), grid, dim3(256), sh, s, a);            \
+    break;
   switch (p.ks_reg * 8 + p.RT) {
     FD3_KS(1, 1) FD3_KS(1, 2) FD3_KS(1, 3) FD3_KS(1, 4)
     FD3_KS(2, 1) FD3_KS(2, 2) FD3_KS(2, 3) FD3_KS(2, 4)

@@ -16,7 +16,9 @@ __device__ inline void stage_table(float2* dst, const float2* __restrict__ src, 
 // as zero / not written.  A plain loop waits for each global load before its
 // LDS store, leaving one load per wave in flight; these issue U loads per
 // thread before the first store (and U LDS reads before the first global
-// store), which is what lets the pass kernels approach HBM bandwidth.
+// store), which is what lets the pass kernels approach HBM bandwidth.  The
+// loads are unconditional (indices clamped into the tile, dead elements read
+// src[0]) so that no load sits in a branch the compiler must wait inside.
 struct AllElems {
   __device__ bool operator()(int) const { return true; }
 };
@@ -25,15 +27,17 @@ __device__ __forceinline__ void tile_to_lds(float2* lds, const float2* __restric
                                             OK ok = OK()) {
   for (int b = 0; b < n; b += U * (int)blockDim.x) {
     float2 v[U];
+    bool live[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
-      v[u] = (e < n && ok(e)) ? src[gi(e)] : make_float2(0.f, 0.f);
+      const int e = min(b + u * (int)blockDim.x + (int)threadIdx.x, n - 1);
+      live[u] = ok(e);
+      v[u] = src[live[u] ? gi(e) : 0];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
-      if (e < n) lds[li(e)] = v[u];
+      if (e < n) lds[li(e)] = live[u] ? v[u] : make_float2(0.f, 0.f);
     }
   }
 }
@@ -43,10 +47,7 @@ __device__ __forceinline__ void tile_from_lds(float2* __restrict__ dst, const fl
   for (int b = 0; b < n; b += U * (int)blockDim.x) {
     float2 v[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
-      if (e < n) v[u] = lds[li(e)];
-    }
+    for (int u = 0; u < U; ++u) v[u] = lds[li(min(b + u * (int)blockDim.x + (int)threadIdx.x, n - 1))];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = b + u * (int)blockDim.x + (int)threadIdx.x;

@@ -18,7 +18,8 @@ struct FftPlan {
   int TY;                 // y lines per block of the z passes
   int TX;                 // x lines per block of the y passes
   int LPB;                // lines per block of the plain x pass
-  int RT, NK;             // ranks and ky rows per block of the k-space kernel
+  int RT, NK;             // ranks and ky rows per tile of the k-space kernel
+  int KT;                 // tiles (of NK ky rows) per k-space block
   int ks_reg;             // k-space variant: nS (<= kKsRegMaxS) with S and phi in registers, 0 = generic
   int ks_blocks;          // grid size of the k-space kernel
   size_t ks_lds;          // dynamic LDS bytes of the k-space kernel
@@ -30,14 +31,27 @@ struct FftTables {
   float2 *twx, *twy, *twz;
   float *scx, *scy, *scz;
   float4* htab;  // 3 * (kHtabN + 1) entries: polynomial table of h6, h8
+  KsPair* pc;    // (nS, nS) per-pair constants of the k-space kernel
 };
+
+// Inputs of the per-pair constants (device table qq = sqrt(Q_X) sqrt(Q_Y)).
+struct PairInit {
+  const float* qq;
+  int nS;
+  float s6, s8, a1, a2;
+};
+
+// Butterflies per thread held by the register k-space variant's line FFTs:
+// its tiles hold nS * RT * NK * nx <= 1024 * ks_maxb(nS * RT) values.
+constexpr int ks_maxb(int lines_per_row) { return lines_per_row <= 8 ? 2 : 4; }
 
 // Species counts for which the k-space kernel keeps a k point's spectra in registers.
 constexpr int kKsRegMaxS = 4;
 
 int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p);
 void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t);
-int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s);
+int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s,
+                    const PairInit* pairs = nullptr);
 // rho (real) -> A -> spec; spec holds the y/z-transformed data (x not yet transformed)
 int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float2* A, float2* spec, hipStream_t s);
 // y pass alone: dir -1 A -> spec, dir +1 spec -> A
@@ -47,8 +61,8 @@ int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream
 // spec (x already inverse-transformed) -> A -> out (real)
 int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, float2* A, float* out, hipStream_t s);
 // fused forward-x / k-space / inverse-x; partials has p.ks_blocks * 16 doubles
-int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, const float* qq,
-               double s6, double s8, double a1, double a2, int need_grad, int need_params, double* partials,
-               hipStream_t s);
+// (the pair constants come from fft_tables_init with a PairInit)
+int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
+               int need_params, double* partials, hipStream_t s);
 
 }  // namespace fd3

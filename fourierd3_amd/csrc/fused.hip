@@ -189,7 +189,7 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
-  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
                         [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
 }
 
@@ -205,7 +205,7 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
-  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (size_t)(e >> lT) * ny + (e & (TY - 1)); },
+  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
                       [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
   if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
   __syncthreads();

@@ -67,56 +67,61 @@ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
-// In-place FFT of `nlines` lines of length n (power of two) held in LDS at
-// buf[l * ls + e].  tw[k * tws] = exp(-2 pi i k / n).  DIR = -1 forward
-// (exp(-i...)), +1 inverse (exp(+i...), unnormalised).  Requires
-// nlines * n / 4 <= MAXB * blockDim.x.  Ends with __syncthreads().
+// In-place FFT of `nlines` lines of length n (power of two, n <= 4 blockDim.x)
+// held in LDS at buf[l * ls + e].  tw[k * tws] = exp(-2 pi i k / n).  DIR = -1
+// forward (exp(-i...)), +1 inverse (exp(+i...), unnormalised).  Stockham
+// radix-4 stages (+ a final radix-2 stage for odd log2 n).  Thread t owns
+// butterfly column j = t % (n/4) of lines t / (n/4) + c * (blockDim.x / (n/4)),
+// c < MAXB, so its twiddles and offsets are shared by its MAXB butterflies.
+// Requires nlines * n / 4 <= MAXB * blockDim.x.  Ends with __syncthreads().
 template <int DIR, int MAXB>
 __device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __restrict__ tw, int tws) {
   if (n <= 1) return;
   const int log2n = __ffs(n) - 1;
-  const int nq = n >> 2;
-  const int lognq = log2n - 2;
-  const int total4 = nlines * nq;
-  int Ns = 1;
-  for (int s = 0; s < (log2n >> 1); ++s) {
-    float2 v0[MAXB], v1[MAXB], v2[MAXB], v3[MAXB];
-    int dst[MAXB];
-    const int stride = n / (4 * Ns);
+  if (log2n >= 2) {
+    const int nq = n >> 2, lognq = log2n - 2;
+    const int j = threadIdx.x & (nq - 1);
+    const int l0 = threadIdx.x >> lognq;
+    const int lstep = blockDim.x >> lognq;
+    float2* const base = buf + l0 * ls;
+    const int dl = lstep * ls;
+    int Ns = 1, lNs = 0;
+    for (int s = 0; s < (log2n >> 1); ++s) {
+      const int k = j & (Ns - 1);
+      float2 w1 = make_float2(1.f, 0.f), w2 = w1, w3 = w1;
+      if (s > 0) {  // stage 0 has k = 0 everywhere
+        const int t = (k << (lognq - lNs)) * tws;  // k * n / (4 Ns)
+        w1 = tw[t]; w2 = tw[2 * t]; w3 = tw[3 * t];
+        if (DIR > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+      }
+      const int dst0 = ((j - k) << 2) + k;
+      float2 v0[MAXB], v1[MAXB], v2[MAXB], v3[MAXB];
 #pragma unroll
-    for (int c = 0; c < MAXB; ++c) {
-      const int b = threadIdx.x + c * blockDim.x;
-      if (b < total4) {
-        const int line = b >> lognq, j = b & (nq - 1);
-        float2* L = buf + line * ls;
-        const int k = j & (Ns - 1);
-        float2 a0 = L[j], a1 = L[j + nq], a2 = L[j + 2 * nq], a3 = L[j + 3 * nq];
-        if (k) {
-          const int t = stride * k * tws;
-          float2 w1 = tw[t], w2 = tw[2 * t], w3 = tw[3 * t];
-          if (DIR > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
-          a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3);
+      for (int c = 0; c < MAXB; ++c) {
+        if (l0 + c * lstep < nlines) {
+          const float2* L = base + c * dl;
+          float2 a0 = L[j], a1 = L[j + nq], a2 = L[j + 2 * nq], a3 = L[j + 3 * nq];
+          if (s > 0) { a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3); }
+          const float2 b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3);
+          const float2 d = csub(a1, a3);
+          // forward: b3 = -i d ; inverse: b3 = +i d
+          const float2 b3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+          v0[c] = cadd(b0, b2); v2[c] = csub(b0, b2);
+          v1[c] = cadd(b1, b3); v3[c] = csub(b1, b3);
         }
-        const float2 b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3);
-        const float2 d = csub(a1, a3);
-        // forward: b3 = -i d ; inverse: b3 = +i d
-        const float2 b3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
-        v0[c] = cadd(b0, b2); v2[c] = csub(b0, b2);
-        v1[c] = cadd(b1, b3); v3[c] = csub(b1, b3);
-        dst[c] = line * ls + ((j - k) << 2) + k;
       }
-    }
-    __syncthreads();
+      __syncthreads();
 #pragma unroll
-    for (int c = 0; c < MAXB; ++c) {
-      const int b = threadIdx.x + c * blockDim.x;
-      if (b < total4) {
-        float2* D = buf + dst[c];
-        D[0] = v0[c]; D[Ns] = v1[c]; D[2 * Ns] = v2[c]; D[3 * Ns] = v3[c];
+      for (int c = 0; c < MAXB; ++c) {
+        if (l0 + c * lstep < nlines) {
+          float2* D = base + c * dl + dst0;
+          D[0] = v0[c]; D[Ns] = v1[c]; D[2 * Ns] = v2[c]; D[3 * Ns] = v3[c];
+        }
       }
+      __syncthreads();
+      Ns <<= 2;
+      lNs += 2;
     }
-    __syncthreads();
-    Ns <<= 2;
   }
   if (log2n & 1) {  // final radix-2 stage, Ns = n/2: in place per butterfly
     const int nh = n >> 1;

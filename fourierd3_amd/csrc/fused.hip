@@ -95,40 +95,56 @@ int col_sort(const FusedArgs& a, hipStream_t s) {
 }
 
 // Runs of the column-sorted atom list whose stencils touch plane x and rows
-// [y0, y0 + TY) of species X.  Built by thread 0 into shared memory.
+// [y0, y0 + TY) of species X: for each of the p x nodes j, the columns with
+// y node 0 in [y0 - p + 1, y0 + TY - 1] (mod ny) form at most two contiguous
+// runs.  Built in two steps so that the up to 2p colstart lookups are issued
+// by 2p threads at once (segs_lookup, before a barrier) and compacted by one
+// thread (segs_compact, before the next barrier).
 struct Segs {
+  int ca0[2 * kMaxOrder], cn[2 * kMaxOrder];  // candidate runs (start, length)
   int a0[2 * kMaxOrder];
   int jx[2 * kMaxOrder];
   int off[2 * kMaxOrder + 1];
   int n;
 };
 
-__device__ inline void build_segments(Segs* sg, const int32_t* __restrict__ colstart, int X, int x, int y0, int TY,
-                                      int nx, int ny, int p) {
+__device__ inline void segs_lookup(Segs* sg, const int32_t* __restrict__ colstart, int X, int x, int y0, int TY,
+                                   int nx, int ny, int p) {
+  const int t = threadIdx.x;
+  if (t >= 2 * p) return;
+  const int j = t >> 1, k = t & 1;
+  const int bx0 = wrap(x - j, nx);
+  const int base = (X * nx + bx0) * ny;
+  int lo[2], hi[2], nr;
+  const int l = y0 - (p - 1), h = y0 + TY - 1;
+  if (TY + p - 1 >= ny) {
+    lo[0] = 0; hi[0] = ny - 1; nr = 1;
+  } else if (l < 0) {
+    lo[0] = l + ny; hi[0] = ny - 1; lo[1] = 0; hi[1] = h; nr = 2;
+  } else if (h >= ny) {
+    lo[0] = l; hi[0] = ny - 1; lo[1] = 0; hi[1] = h - ny; nr = 2;
+  } else {
+    lo[0] = l; hi[0] = h; nr = 1;
+  }
+  int a0 = 0, cnt = 0;
+  if (k < nr) {
+    a0 = colstart[base + lo[k]];
+    cnt = colstart[base + hi[k] + 1] - a0;
+  }
+  sg->ca0[t] = a0;
+  sg->cn[t] = cnt;
+}
+
+__device__ inline void segs_compact(Segs* sg, int p) {
+  if (threadIdx.x != 0) return;
   int ns = 0, off = 0;
-  for (int j = 0; j < p; ++j) {
-    const int bx0 = wrap(x - j, nx);
-    const int base = (X * nx + bx0) * ny;
-    int lo[2], hi[2], nr;
-    const int l = y0 - (p - 1), h = y0 + TY - 1;
-    if (TY + p - 1 >= ny) {
-      lo[0] = 0; hi[0] = ny - 1; nr = 1;
-    } else if (l < 0) {
-      lo[0] = l + ny; hi[0] = ny - 1; lo[1] = 0; hi[1] = h; nr = 2;
-    } else if (h >= ny) {
-      lo[0] = l; hi[0] = ny - 1; lo[1] = 0; hi[1] = h - ny; nr = 2;
-    } else {
-      lo[0] = l; hi[0] = h; nr = 1;
-    }
-    for (int k = 0; k < nr; ++k) {
-      const int a0 = colstart[base + lo[k]], a1 = colstart[base + hi[k] + 1];
-      if (a1 > a0) {
-        sg->a0[ns] = a0;
-        sg->jx[ns] = j;
-        sg->off[ns] = off;
-        off += a1 - a0;
-        ++ns;
-      }
+  for (int t = 0; t < 2 * p; ++t) {
+    if (sg->cn[t] > 0) {
+      sg->a0[ns] = sg->ca0[t];
+      sg->jx[ns] = t >> 1;
+      sg->off[ns] = off;
+      off += sg->cn[t];
+      ++ns;
     }
   }
   sg->off[ns] = off;
@@ -153,8 +169,10 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
   const int X = c / a.r, q = c - X * a.r;
   float2* tws = lds + TY * ls;
   stage_table(tws, twz, nz);
+  segs_lookup(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
   for (int e = threadIdx.x; e < TY * ls; e += blockDim.x) lds[e] = make_float2(0.f, 0.f);
-  if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
+  __syncthreads();
+  segs_compact(&sg, p);
   __syncthreads();
   float* ldsf = (float*)lds;
   const int total = sg.off[sg.n];
@@ -205,10 +223,11 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
+  segs_lookup(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
   tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
                       [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
-  if (threadIdx.x == 0) build_segments(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
   __syncthreads();
+  segs_compact(&sg, p);
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
   lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);

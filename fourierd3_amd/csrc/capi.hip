@@ -253,35 +253,27 @@ static int reduce_partials(const double** src, int* rows, int stride, int ncols,
   return FD3_OK;
 }
 
-__global__ void __launch_bounds__(256) k_finalize(FinalArgs a) {
-  __shared__ double sh[256];
-  __shared__ double tot[40];
+// One block of 1024 threads: wave w sums columns w and w + 16 of the 31
+// partial columns (rows <= 256 after reduce_partials), thread 0 assembles.
+__global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
+  __shared__ double tot[32];
   // ks 0..10 -> tot[0..10]; c6 0..4 -> tot[11..15]; ga 0..8 -> tot[16..24]; cnv 0..5 -> tot[25..30]
-  for (int k = 0; k < 11; ++k) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < a.nks; b += blockDim.x) s += a.ks[(size_t)b * 16 + k];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) tot[k] = s;
-  }
-  for (int k = 0; k < 5; ++k) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < a.nc6; b += blockDim.x) s += a.c6[(size_t)b * 8 + k];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) tot[11 + k] = s;
-  }
-  for (int k = 0; k < 9; ++k) {
-    double s = 0.0;
-    if (a.ga)
-      for (int b = threadIdx.x; b < a.nga; b += blockDim.x) s += a.ga[(size_t)b * 16 + k];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) tot[16 + k] = s;
-  }
-  for (int k = 0; k < 6; ++k) {
-    double s = 0.0;
-    if (a.cnv)
-      for (int b = threadIdx.x; b < a.ncnv; b += blockDim.x) s += a.cnv[(size_t)b * 8 + k];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) tot[25 + k] = s;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int k = w; k < 31; k += 16) {
+    const double* src;
+    int rows, stride, col;
+    if (k < 11) { src = a.ks; rows = a.nks; stride = 16; col = k; }
+    else if (k < 16) { src = a.c6; rows = a.nc6; stride = 8; col = k - 11; }
+    else if (k < 25) { src = a.ga; rows = a.ga ? a.nga : 0; stride = 16; col = k - 16; }
+    else { src = a.cnv; rows = a.cnv ? a.ncnv : 0; stride = 8; col = k - 25; }
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int b = lane + 64 * i;
+      if (b < rows) v[i] = src[(size_t)b * stride + col];
+    }
+    const double sum = warp_sum((v[0] + v[1]) + (v[2] + v[3]));
+    if (lane == 0) tot[k] = sum;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -422,7 +414,7 @@ static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, 
   fa.grad_cell = grad ? out->grad_cell : nullptr;
   fa.grad_params = params ? out->grad_params : nullptr;
   ProfScope _ps("k_finalize", s);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, fa);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, fa);
   FD3_LAUNCH_CHECK("k_finalize");
   return FD3_OK;
 }

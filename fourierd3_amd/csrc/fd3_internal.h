@@ -227,15 +227,13 @@ __host__ __device__ inline void green_h_tab(const float4* __restrict__ tab, floa
   const int i = min((int)fx, kHtabN);
   const float u = 2.0f * (fx - (float)i) - 1.0f;
   const float4 A = tab[3 * i], B = tab[3 * i + 1], C = tab[3 * i + 2];
-  float p = B.y, d = 0.f;
-  d = d * u + p; p = p * u + B.x;
+  float p = B.y * u + B.x, d = B.y;
   d = d * u + p; p = p * u + A.w;
   d = d * u + p; p = p * u + A.z;
   d = d * u + p; p = p * u + A.y;
   d = d * u + p; p = p * u + A.x;
   *h6 = p; *d6 = d;
-  p = C.w; d = 0.f;
-  d = d * u + p; p = p * u + C.z;
+  p = C.w * u + C.z; d = C.w;
   d = d * u + p; p = p * u + C.y;
   d = d * u + p; p = p * u + C.x;
   d = d * u + p; p = p * u + B.w;

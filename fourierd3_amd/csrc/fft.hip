@@ -61,15 +61,13 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ rho, fl
   stage_table(tws, twz, nz);
   const float2* src = (const float2*)rho + ((size_t)(c * nx + x) * ny + y0) * M;
   const int lM = ilog2(M), lT = ilog2(TY);
-  tile_to_lds<kCopyU>(lds, src, TY * M, [](int e) { return (unsigned)e; },
-                     [=](int e) { return (e >> lM) * ls + (e & (M - 1)); });
+  tile2_to_lds<kCopyU>(lds, src, TileShape{TY, lM, 1, ls, 1}, StridedLines{M});
   __syncthreads();
   lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   z_pairs<true>(lds, TY, ls, M, tws);  // X[k] = E[k] + W_nz^k O[k]
   __syncthreads();
   float2* dst = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
-  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
-                        [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
+  tile2_from_lds<kCopyU>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
 }
 
 // A[c][x][kz][y0..y0+TY) -> out[c][x][y][0..nz) real (unnormalised inverse;
@@ -83,15 +81,13 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
   stage_table(tws, twz, nz);
   const float2* src = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
   const int lM = ilog2(M), lT = ilog2(TY);
-  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
-                     [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
+  tile2_to_lds<kCopyU>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
   __syncthreads();
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
   lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
   float2* dst = (float2*)out + ((size_t)(c * nx + x) * ny + y0) * M;
-  tile_from_lds<kCopyU>(dst, lds, TY * M, [](int e) { return (unsigned)e; },
-                        [=](int e) { return (e >> lM) * ls + (e & (M - 1)); });
+  tile2_from_lds<kCopyU>(dst, lds, TileShape{TY, lM, 1, ls, 1}, StridedLines{M});
 }
 
 // ------------------------------------------------------------ y passes ---
@@ -107,23 +103,19 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
   const int lny = ilog2(ny), lTX = ilog2(TX);
   if (DIR < 0) {
     const float2* src = in + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (unsigned)(e >> lny) * M1 * ny + (e & (ny - 1)); },
-                        [=](int e) { return (e >> lny) * ls + (e & (ny - 1)); });
+    tile2_to_lds<kCopyU>(lds, src, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
   } else {
     const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    tile_to_lds<kCopyU>(lds, src, TX * ny, [=](int e) { return (unsigned)(e >> lTX) * nx + (e & (TX - 1)); },
-                        [=](int e) { return (e & (TX - 1)) * ls + (e >> lTX); });
+    tile2_to_lds<kCopyU>(lds, src, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
   }
   __syncthreads();
   lds_fft<DIR, kFftMaxB>(lds, ny, TX, ls, tws, 1);
   if (DIR < 0) {
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (unsigned)(e >> lTX) * nx + (e & (TX - 1)); },
-                          [=](int e) { return (e & (TX - 1)) * ls + (e >> lTX); });
+    tile2_from_lds<kCopyU>(dst, lds, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
   } else {
     float2* dst = out + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile_from_lds<kCopyU>(dst, lds, TX * ny, [=](int e) { return (unsigned)(e >> lny) * M1 * ny + (e & (ny - 1)); },
-                          [=](int e) { return (e >> lny) * ls + (e & (ny - 1)); });
+    tile2_from_lds<kCopyU>(dst, lds, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
   }
 }
 
@@ -140,12 +132,11 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
   float2* tws = lds + LPB * ls;
   stage_table(tws, twx, nx);
   const int lnx = ilog2(nx);
-  auto gi = [](int e) { return (unsigned)e; };
-  auto li = [=](int e) { return (e >> lnx) * ls + (e & (nx - 1)); };
-  tile_to_lds<kCopyU>(lds, base, nl * nx, gi, li);
+  const TileShape ts{nl, lnx, 1, ls, 1};
+  tile2_to_lds<kCopyU>(lds, base, ts, StridedLines{nx});
   __syncthreads();
   lds_fft<DIR, kFftMaxB>(lds, nx, nl, ls, tws, 1);
-  tile_from_lds<kCopyU>(base, lds, nl * nx, gi, li);
+  tile2_from_lds<kCopyU>(base, lds, ts, StridedLines{nx});
 }
 
 // ------------------------------------------------- fused k-space kernel ---
@@ -191,19 +182,14 @@ __device__ inline void kspace_line_bases(const KspaceArgs& a, long long* lbase, 
   }
 }
 
-// Tile copies of the nlines x nx lines (LDS line stride nx + 1).
+// Tile copies of the nlines x nx lines (LDS line stride nx + 1); line l of
+// the tile starts at spec + lbase[l] (-1: a padded rank, zero-filled).
 __device__ inline void kspace_load(const KspaceArgs& a, float2* S, const long long* lbase, int nlines, int lnx) {
-  const int nx = a.nx;
-  tile_to_lds<kCopyU / 2>(
-      S, a.spec, nlines * nx, [=](int e) { return lbase[e >> lnx] + (e & (nx - 1)); },
-      [=](int e) { return e + (e >> lnx); }, [=](int e) { return lbase[e >> lnx] >= 0; });
+  tile2_to_lds<kCopyU / 2>(S, a.spec, TileShape{nlines, lnx, 1, a.nx + 1, 1}, LineTable{lbase});
 }
 __device__ inline void kspace_store(const KspaceArgs& a, const float2* S, const long long* lbase, int nlines,
                                     int lnx) {
-  const int nx = a.nx;
-  tile_from_lds<kCopyU / 2>(
-      a.spec, S, nlines * nx, [=](int e) { return lbase[e >> lnx] + (e & (nx - 1)); },
-      [=](int e) { return e + (e >> lnx); }, [=](int e) { return lbase[e >> lnx] >= 0; });
+  tile2_from_lds<kCopyU / 2>(a.spec, S, TileShape{nlines, lnx, 1, a.nx + 1, 1}, LineTable{lbase});
 }
 
 // k vector, |k| and 1/sinc^2p of column (kx, ky).

@@ -1,6 +1,8 @@
 // In-LDS line FFT and complex helpers shared by the FFT pass kernels.
 #pragma once
 
+#include <type_traits>
+
 #include "fd3_internal.h"
 
 namespace fd3 {
@@ -11,50 +13,144 @@ __device__ inline void stage_table(float2* dst, const float2* __restrict__ src, 
   for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
-// Tile copies between global memory and LDS: element e (0 <= e < n) moves
-// between src/dst[gi(e)] and lds[li(e)]; elements with ok(e) false are read
-// as zero / not written.  A plain loop waits for each global load before its
-// LDS store, leaving one load per wave in flight; these issue U loads per
-// thread before the first store (and U LDS reads before the first global
-// store), which is what lets the pass kernels approach HBM bandwidth.  The
-// loads are unconditional (indices clamped into the tile, dead elements read
-// src[0]) so that no load sits in a branch the compiler must wait inside.
-struct AllElems {
-  __device__ bool operator()(int) const { return true; }
+// Tile copies between global memory and LDS.  A tile is nl lines of
+// len = 2^llen elements; element (l, c) sits at g[gl(l) + c * gcs] in global
+// memory and at lds[l * lls + c * lcs].  Element number e = k * blockDim.x + t
+// (row-major over (l, c)) of thread t splits into a per-thread part and a part
+// that is uniform over the block for each k, so the per-element address work
+// is a few adds.  A plain copy loop waits for each global load before its LDS
+// store, leaving one load per wave in flight; these issue U loads per thread
+// before the first store (and U LDS reads before the first global store).
+// Loads are unconditional (dead elements read g[0]) so that none sits in a
+// branch the compiler has to wait inside.  gl(l) < 0 marks a dead line: it is
+// read as zeros and not written.
+struct StridedLines {  // line l at l * gls
+  long long gls;
 };
-template <int U, typename GI, typename LI, typename OK = AllElems>
-__device__ __forceinline__ void tile_to_lds(float2* lds, const float2* __restrict__ src, int n, GI gi, LI li,
-                                            OK ok = OK()) {
-  for (int b = 0; b < n; b += U * (int)blockDim.x) {
+struct LineTable {  // line l at base[l], -1 for a dead line (zero-filled, not written)
+  const long long* base;
+};
+struct TileShape {
+  int nl, llen;  // lines, log2 line length
+  int gcs;       // global column stride
+  int lls, lcs;  // LDS line and column strides
+};
+
+// All tile copies run in blocks of kTileThreads threads.
+constexpr int kTileThreads = 256;
+constexpr int kTileLog = 8;
+
+// Element k * kTileThreads + t of a tile (row-major over (line, column)):
+// thread t owns line lt + lu(k), column ct + cu(k), where lt, ct depend on t
+// only and lu, cu on k only (uniform over the block).
+struct TileIter {
+  bool wide;  // a line spans several k
+  int lt, ct, nk, llen;
+  __device__ TileIter(const TileShape& t) : wide(t.llen > kTileLog), llen(t.llen) {
+    const int tid = threadIdx.x;
+    lt = wide ? 0 : tid >> t.llen;
+    ct = wide ? tid : tid & ((1 << t.llen) - 1);
+    nk = ((t.nl << t.llen) + kTileThreads - 1) >> kTileLog;
+  }
+  __device__ int lu(int k) const { return wide ? k >> (llen - kTileLog) : k << (kTileLog - llen); }
+  __device__ int cu(int k) const { return wide ? (k & ((1 << (llen - kTileLog)) - 1)) << kTileLog : 0; }
+};
+
+template <int U>
+__device__ __forceinline__ void tile2_to_lds(float2* lds, const float2* __restrict__ g, TileShape t, StridedLines gl) {
+  const TileIter it(t);
+  // per-thread base, kept opaque so the uniform part below stays scalar
+  const float2* gt = g + ((long long)it.lt * gl.gls + (long long)it.ct * t.gcs);
+  asm volatile("" : "+v"(gt));
+  const int lt0 = it.lt * t.lls + it.ct * t.lcs;
+  for (int k0 = 0; k0 < it.nk; k0 += U) {
     float2 v[U];
-    bool live[U];
+    bool in[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = min(b + u * (int)blockDim.x + (int)threadIdx.x, n - 1);
-      live[u] = ok(e);
-      v[u] = src[live[u] ? gi(e) : 0];
+      const int k = k0 + u;
+      const int lu = it.lu(k), cu = it.cu(k);
+      in[u] = k < it.nk && it.lt + lu < t.nl;
+      const long long uo = (long long)lu * gl.gls + (long long)cu * t.gcs;
+      v[u] = in[u] ? gt[uo] : g[0];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
-      if (e < n) lds[li(e)] = live[u] ? v[u] : make_float2(0.f, 0.f);
+      const int k = k0 + u;
+      if (in[u]) lds[lt0 + it.lu(k) * t.lls + it.cu(k) * t.lcs] = v[u];
     }
   }
 }
-template <int U, typename GI, typename LI, typename OK = AllElems>
-__device__ __forceinline__ void tile_from_lds(float2* __restrict__ dst, const float2* lds, int n, GI gi, LI li,
-                                              OK ok = OK()) {
-  for (int b = 0; b < n; b += U * (int)blockDim.x) {
+
+template <int U>
+__device__ __forceinline__ void tile2_from_lds(float2* __restrict__ g, const float2* lds, TileShape t, StridedLines gl) {
+  const TileIter it(t);
+  float2* gt = g + ((long long)it.lt * gl.gls + (long long)it.ct * t.gcs);
+  asm volatile("" : "+v"(gt));
+  const int lt0 = it.lt * t.lls + it.ct * t.lcs;
+  for (int k0 = 0; k0 < it.nk; k0 += U) {
     float2 v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) v[u] = lds[li(min(b + u * (int)blockDim.x + (int)threadIdx.x, n - 1))];
+    bool in[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int e = b + u * (int)blockDim.x + (int)threadIdx.x;
-      if (e < n && ok(e)) dst[gi(e)] = v[u];
+      const int k = k0 + u;
+      in[u] = k < it.nk && it.lt + it.lu(k) < t.nl;
+      v[u] = lds[in[u] ? lt0 + it.lu(k) * t.lls + it.cu(k) * t.lcs : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      if (in[u]) gt[(long long)it.lu(k) * gl.gls + (long long)it.cu(k) * t.gcs] = v[u];
     }
   }
 }
+
+template <int U>
+__device__ __forceinline__ void tile2_to_lds(float2* lds, const float2* __restrict__ g, TileShape t, LineTable gl) {
+  const TileIter it(t);
+  const int lt0 = it.lt * t.lls + it.ct * t.lcs;
+  for (int k0 = 0; k0 < it.nk; k0 += U) {
+    float2 v[U];
+    bool in[U], live[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      const int l = it.lt + it.lu(k);
+      in[u] = k < it.nk && l < t.nl;
+      const long long b = in[u] ? gl.base[l] : -1;
+      live[u] = b >= 0;
+      v[u] = g[live[u] ? b + (long long)(it.ct + it.cu(k)) * t.gcs : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      if (in[u]) lds[lt0 + it.lu(k) * t.lls + it.cu(k) * t.lcs] = live[u] ? v[u] : make_float2(0.f, 0.f);
+    }
+  }
+}
+
+template <int U>
+__device__ __forceinline__ void tile2_from_lds(float2* __restrict__ g, const float2* lds, TileShape t, LineTable gl) {
+  const TileIter it(t);
+  const int lt0 = it.lt * t.lls + it.ct * t.lcs;
+  for (int k0 = 0; k0 < it.nk; k0 += U) {
+    float2 v[U];
+    long long go[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = k0 + u;
+      const int l = it.lt + it.lu(k);
+      const bool in = k < it.nk && l < t.nl;
+      const long long b = in ? gl.base[l] : -1;
+      go[u] = b >= 0 ? b + (long long)(it.ct + it.cu(k)) * t.gcs : -1;
+      v[u] = lds[in ? lt0 + it.lu(k) * t.lls + it.cu(k) * t.lcs : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (go[u] >= 0) g[go[u]] = v[u];
+  }
+}
+
 constexpr int kCopyU = 16;  // loads in flight per thread (16 x 8 B)
 
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -207,8 +207,7 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
-  tile_from_lds<kCopyU>(dst, lds, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
-                        [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
+  tile2_from_lds<kCopyU>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
 }
 
 // ------------------------------------------------------ z pass + gather ---
@@ -224,8 +223,7 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
   segs_lookup(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
-  tile_to_lds<kCopyU>(lds, src, TY * (M + 1), [=](int e) { return (unsigned)(e >> lT) * ny + (e & (TY - 1)); },
-                      [=](int e) { return (e & (TY - 1)) * ls + (e >> lT); });
+  tile2_to_lds<kCopyU>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
   __syncthreads();
   segs_compact(&sg, p);
   z_pairs<false>(lds, TY, ls, M, tws);

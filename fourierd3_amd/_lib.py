@@ -21,6 +21,7 @@ FD3_ERR_WORKSPACE = -4
 
 FD3_INTERP_P3M = 0
 FD3_INTERP_LAGRANGE = 1
+FD3_INTERP_EWALD = 2
 FD3_DTYPE_F32 = 0
 FD3_DTYPE_F64 = 1
 FD3_NEED_GRAD = 1
@@ -31,6 +32,7 @@ EXPORTS = (
     "fd3_last_error", "fd3_version", "fd3_profile_enable", "fd3_profile_report",
     "fd3_eval_edges_workspace", "fd3_eval_edges",
     "fd3_eval_celllist_workspace", "fd3_eval_celllist",
+    "fd3_eval_celllist_batch_workspace", "fd3_eval_celllist_batch",
     "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
     "fd3_spread", "fd3_gather",
     "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
@@ -78,6 +80,10 @@ _SIG = {
                                               C.POINTER(C.c_size_t)]),
     "fd3_eval_celllist": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, _P, C.c_int, _P, C.c_int, _P,
                                     C.c_size_t, C.POINTER(Outputs), _P]),
+    "fd3_eval_celllist_batch_workspace": (C.c_int, [C.POINTER(Model), C.c_int32, C.POINTER(Geom),
+                                                    C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_size_t)]),
+    "fd3_eval_celllist_batch": (C.c_int, [C.POINTER(Model), C.c_int32, C.POINTER(Geom), C.POINTER(C.c_int64), _P,
+                                          C.c_int, _P, C.c_int, _P, C.c_size_t, C.POINTER(Outputs), _P]),
     "fd3_neighbours_workspace": (C.c_int, [C.c_int64, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_size_t)]),
     "fd3_neighbours_count": (C.c_int, [C.c_int64, _P, C.c_int, C.POINTER(C.c_double), C.c_double, _P, C.c_size_t,
                                        _P, _P]),

@@ -146,6 +146,82 @@ class FourierD3ASECalculator(_Base):
             L.ptr(model._species_i32), flags, L.ptr(self._ws), self._ws.numel(), C.byref(o), stream))
         return out
 
+    def evaluate_batch(self, positions: torch.Tensor, species: torch.Tensor, offsets, cells,
+                       need_grad: bool = True, need_params: bool = False) -> dict:
+        """E+F+stress of many boxes in one call (``fd3_eval_celllist_batch``).
+
+        positions (N_total, 3) Angstrom and species (N_total,) int32 compact
+        ids (positions in the model's ascending atomic-number order) are
+        device tensors packed box after box; ``offsets`` (B + 1) are the atom
+        offsets and ``cells`` the B host (3, 3) cells in Angstrom.  Each box
+        gets its own mesh by the reference's rule for its cell.  Returns
+        device tensors energy (B,), grad_pos (N_total, 3), virial (B, 3, 3)
+        and grad_params (B, 4) in Hartree units, with no host sync."""
+        model = self._model
+        offs = np.asarray(offsets, dtype=np.int64)
+        nb = len(offs) - 1
+        geoms = (L.Geom * max(nb, 1))()
+        a2b = self.angstrom_to_bohr
+        for b in range(nb):
+            cell = np.asarray(cells[b], dtype=np.float64).reshape(3, 3)
+            ns = T.get_ns_mesh(cell * a2b, self.mesh_spacing * a2b)
+            geoms[b] = L.make_geom(cell, ns, model.interpolation_nodes, model._interp)
+        m = model._model_struct()
+        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        offs_c = (C.c_int64 * len(offs))(*[int(v) for v in offs])
+        nbytes = C.c_size_t()
+        L.check(L.lib().fd3_eval_celllist_batch_workspace(C.byref(m), nb, geoms, offs_c, flags, C.byref(nbytes)))
+        if self._ws is None or self._ws.numel() < nbytes.value:
+            self._ws = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        out = {"energy": torch.empty(nb, **f64), "grad_pos": torch.empty((int(offs[-1]), 3), **f64),
+               "virial": torch.empty((nb, 3, 3), **f64), "grad_params": torch.empty((nb, 4), **f64)}
+        outs = (L.Outputs * max(nb, 1))()
+        for b in range(nb):
+            o = outs[b]
+            o.energy = L.ptr(out["energy"]) + 8 * b
+            if need_grad:
+                o.grad_pos = L.ptr(out["grad_pos"]) + 24 * int(offs[b])
+                o.virial = L.ptr(out["virial"]) + 72 * b
+            if need_params:
+                o.grad_params = L.ptr(out["grad_params"]) + 32 * b
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(L.lib().fd3_eval_celllist_batch(
+            C.byref(m), nb, geoms, offs_c, L.ptr(positions), L.dtype_code(positions), L.ptr(species), flags,
+            L.ptr(self._ws), self._ws.numel(), outs, stream))
+        return out
+
+    def calculate_batch(self, atoms_list) -> list:
+        """Energy (eV), forces (eV/A) and stress (eV/A^3) of every ``atoms`` in
+        ``atoms_list`` (one species set, the one this calculator's model was
+        built for, or built here from the first entry) with one batched call."""
+        if self._model is None:
+            self._build_model(atoms_list[0])
+        model = self._model
+        zmap = {int(z): i for i, z in enumerate(model.species_unique)}
+        offs = [0]
+        pos, spec, cells = [], [], []
+        for at in atoms_list:
+            z = np.asarray(at.numbers)
+            if any(int(v) not in zmap for v in np.unique(z)):
+                raise ValueError("every box of a batch must use the species set of the model")
+            pos.append(np.asarray(at.positions, dtype=np.float64))
+            spec.append(np.array([zmap[int(v)] for v in z], dtype=np.int32))
+            cells.append(_cell_array(at))
+            offs.append(offs[-1] + len(z))
+        p = torch.tensor(np.concatenate(pos), device=self.device)
+        sp = torch.tensor(np.concatenate(spec), device=self.device)
+        res = self.evaluate_batch(p, sp, offs, cells)
+        e = (res["energy"] * self.HARTREE_TO_EV).cpu().numpy()
+        f = (-res["grad_pos"] * self.HARTREE_TO_EV).cpu().numpy()
+        v = (res["virial"] * self.HARTREE_TO_EV).cpu().numpy()
+        out = []
+        for b, cell in enumerate(cells):
+            vol_b = abs(np.linalg.det(cell)) * self.angstrom_to_bohr ** 3
+            out.append({"energy": float(e[b]), "forces": f[offs[b]:offs[b + 1]],
+                        "stress": v[b] / vol_b * self.angstrom_to_bohr ** 3})
+        return out
+
     def calculate(self, atoms=None, properties=None, system_changes=all_changes):
         """Energy (eV), forces (eV/A) and 3x3 stress (eV/A^3) of ``atoms``
         (``src/fourierd3/calculator.py:125-192``)."""

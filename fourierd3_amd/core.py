@@ -93,8 +93,6 @@ class FourierD3(torch.nn.Module):
             print("Assuming 3D PBC are satisfied")
         if method not in ("spme", "pme", "ewald"):
             raise ValueError(f"unknown method '{method}'; expected 'spme', 'pme' or 'ewald'")
-        if method == "ewald":
-            raise NotImplementedError("method='ewald' has no HIP implementation in this build; use 'spme' or 'pme'")
         if cnfunc != "smooth_cut":
             raise NotImplementedError(f"cnfunc='{cnfunc}' is not built; only 'smooth_cut' is")
 
@@ -151,11 +149,14 @@ class FourierD3(torch.nn.Module):
         cell_np = cell_A.detach().double().cpu().numpy()
         if ns_mesh is not None:
             self.ns_mesh = np.asarray(ns_mesh, dtype=np.int64)
+        elif method == "ewald":
+            # k grid of the direct sum: ceil(k_cutoff |a_i| / 2 pi), cell in Bohr (core.py:228-230)
+            self.ns_mesh = np.ceil(k_cutoff * np.linalg.norm(cell_np * A2B, axis=1) / (2 * math.pi)).astype(np.int64)
         else:
             self.ns_mesh = T.get_ns_mesh(cell_np * A2B, mesh_spacing * A2B)   # core.py:186
         if verbose:
             print("Using mesh size", self.ns_mesh)
-        self._interp = L.FD3_INTERP_P3M if method == "spme" else L.FD3_INTERP_LAGRANGE
+        self._interp = {"spme": L.FD3_INTERP_P3M, "pme": L.FD3_INTERP_LAGRANGE, "ewald": L.FD3_INTERP_EWALD}[method]
         self._ws = None
         self._update_cell(cell_A)
 

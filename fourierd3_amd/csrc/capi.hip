@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "fd3_internal.h"
+#include "ewald.h"
 #include "fft_plan.h"
 #include "fused.h"
 #include "mesh_kernels.h"
@@ -143,6 +144,7 @@ int make_geom(const fd3_geom* g, Geom* out) {
   out->nz = g->ns[2];
   out->order = g->order;
   out->interp = g->interp;
+  if (g->interp == FD3_INTERP_EWALD) return FD3_OK;  // k grid sizes: checked by ewald_plan_make
   if (g->interp != FD3_INTERP_P3M && g->interp != FD3_INTERP_LAGRANGE) {
     set_error("unknown interpolation %d", g->interp);
     return FD3_ERR_ARG;
@@ -315,6 +317,8 @@ struct Common {
   float* buf1;   // spectrum
   float2* buf2;  // z-pass output A
   ColBuffers cols;
+  int32_t* ew_perm;  // Ewald: atoms by species
+  int32_t* ew_seg;
   float* cn_raw;
   float* charges;
   float* dEdcn;
@@ -325,8 +329,12 @@ struct Common {
 
 static void carve_common(Arena& ar, Common& c, int64_t n, int r, int cn_rows) {
   fft_tables_carve(c.plan, ar, &c.tab);
-  c.buf1 = ar.take<float>(2 * c.plan.mesh_complex());  // spectrum
-  c.buf2 = ar.take<float2>(c.plan.mesh_complex());     // z-pass output
+  // spectrum and z-pass output (mesh methods), or S and phi (Ewald)
+  const size_t nbuf = c.geom.interp == FD3_INTERP_EWALD ? ewald_buffer_elems(c.plan) : c.plan.mesh_complex();
+  c.buf1 = ar.take<float>(2 * nbuf);
+  c.buf2 = ar.take<float2>(nbuf);
+  c.ew_perm = ar.take<int32_t>(std::max<int64_t>(n, 1));
+  c.ew_seg = ar.take<int32_t>(c.plan.nS + 1);
   col_carve(c.plan, n, ar, &c.cols);
   c.cn_raw = ar.take<float>(std::max<int64_t>(n, 1));
   c.charges = ar.take<float>(std::max<int64_t>(n * r, 1));
@@ -346,6 +354,7 @@ static int setup_common(const fd3_model* m, const fd3_geom* g, Common& c) {
   int rc;
   if ((rc = check_model(m))) return rc;
   if ((rc = make_geom(g, &c.geom))) return rc;
+  if (g->interp == FD3_INTERP_EWALD) return ewald_plan_make(g->ns, m->n_species, m->n_rank, &c.plan);
   if ((rc = fft_plan_make(g->ns, m->n_species, m->n_rank, &c.plan))) return rc;
   return FD3_OK;
 }
@@ -367,6 +376,29 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   ca.s6 = m->s6; ca.s8 = m->s8; ca.a1 = m->a1; ca.a2 = m->a2;
   ca.charges = c.charges; ca.cn_out = out ? out->cn : nullptr; ca.partials = c.p_c6;
   if ((rc = launch_c6_fwd(ca, s))) return rc;
+  if (c.geom.interp == FD3_INTERP_EWALD) {
+    EwaldArgs ea;
+    ea.n = n; ea.nS = nS; ea.r = r; ea.nx = c.geom.nx; ea.ny = c.geom.ny; ea.nz = c.geom.nz;
+    for (int i = 0; i < 9; ++i) { ea.inv[i] = c.geom.inv[i]; ea.recip[i] = c.geom.recip[i]; }
+    ea.volume = c.geom.volume;
+    ea.pos4 = pos4; ea.spec_s = spec_s; ea.charges = c.charges; ea.eigs = m->eigs;
+    ea.pc = c.tab.pc; ea.htab = c.tab.htab;
+    ea.S = (float2*)c.buf1; ea.phi = c.buf2; ea.perm = c.ew_perm; ea.seg = c.ew_seg;
+    ea.partials = c.p_ks; ea.accA = c.cols.accA; ea.accG = c.cols.accG;
+    ea.need_params = (flags & FD3_NEED_PARAMS) ? 1 : 0;
+    if ((rc = ewald_run(ea, grad, s))) return rc;
+    if (grad && n > 0) {
+      C6BwdArgs ba;
+      ba.n = n; ba.nS = nS; ba.r = r; ba.nx = 1; ba.ny = 1; ba.nz = 1;  // accG holds dE/df
+      for (int i = 0; i < 9; ++i) ba.inv[i] = c.geom.inv[i];
+      ba.pos4 = pos4; ba.spec_s = spec_s; ba.charges = c.charges; ba.accA = c.cols.accA; ba.accG = c.cols.accG;
+      ba.cn_raw = c.cn_raw; ba.cnref = m->cnref; ba.vq = m->vq; ba.eigs = m->eigs; ba.qq = m->qq;
+      ba.cncorr = m->cncorr; ba.s6 = m->s6; ba.s8 = m->s8; ba.a1 = m->a1; ba.a2 = m->a2;
+      ba.gpos = c.gpos; ba.dEdcn = c.dEdcn; ba.partials = c.p_ga;
+      if ((rc = launch_c6_bwd(ba, s))) return rc;
+    }
+    return FD3_OK;
+  }
   // atoms into mesh columns, then spreading fused into the forward z pass
   FusedArgs fa;
   fa.n = n; fa.nS = nS; fa.r = r;
@@ -569,6 +601,60 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const vo
   fd3_outputs o2 = *out;
   o2.grad_cell = nullptr;  // the cell-list path reports the strain derivative only
   return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, n > 0, &o2, s);
+}
+
+// ---------------------------------------------------------- batched path ---
+static int batch_check(int32_t nb, const fd3_geom* geoms, const int64_t* offs) {
+  if (nb < 0 || (nb > 0 && (!geoms || !offs))) {
+    set_error("bad batch description (n_boxes %d)", nb);
+    return FD3_ERR_ARG;
+  }
+  if (nb > 0 && offs[0] != 0) {
+    set_error("atom_offsets[0] must be 0");
+    return FD3_ERR_ARG;
+  }
+  for (int32_t b = 0; b < nb; ++b)
+    if (offs[b + 1] < offs[b]) {
+      set_error("atom_offsets must be non-decreasing (box %d)", b);
+      return FD3_ERR_ARG;
+    }
+  return FD3_OK;
+}
+
+int fd3_eval_celllist_batch_workspace(const fd3_model* m, int32_t n_boxes, const fd3_geom* geoms,
+                                      const int64_t* atom_offsets, int flags, size_t* bytes) {
+  int rc;
+  if ((rc = batch_check(n_boxes, geoms, atom_offsets))) return rc;
+  size_t mx = 256;
+  for (int32_t b = 0; b < n_boxes; ++b) {
+    size_t sz = 0;
+    if ((rc = fd3_eval_celllist_workspace(m, &geoms[b], atom_offsets[b + 1] - atom_offsets[b], flags, &sz))) return rc;
+    mx = std::max(mx, sz);
+  }
+  *bytes = mx;
+  return FD3_OK;
+}
+
+int fd3_eval_celllist_batch(const fd3_model* m, int32_t n_boxes, const fd3_geom* geoms, const int64_t* atom_offsets,
+                            const void* positions, int pos_dtype, const int32_t* species, int flags, void* workspace,
+                            size_t workspace_bytes, const fd3_outputs* outs, void* stream) {
+  int rc;
+  if ((rc = batch_check(n_boxes, geoms, atom_offsets))) return rc;
+  if (n_boxes > 0 && !outs) {
+    set_error("outputs are NULL");
+    return FD3_ERR_ARG;
+  }
+  const size_t esz = pos_dtype == FD3_DTYPE_F64 ? sizeof(double) : sizeof(float);
+  for (int32_t b = 0; b < n_boxes; ++b) {
+    const int64_t a0 = atom_offsets[b], n = atom_offsets[b + 1] - a0;
+    const void* pb = positions ? (const char*)positions + (size_t)a0 * 3 * esz : nullptr;
+    const int32_t* sb = species ? species + a0 : nullptr;
+    // boxes run one after another on the stream, so they share the workspace
+    if ((rc = fd3_eval_celllist(m, &geoms[b], n, pb, pos_dtype, sb, flags, workspace, workspace_bytes, &outs[b],
+                                stream)))
+      return rc;
+  }
+  return FD3_OK;
 }
 
 // ------------------------------------------------------------ neighbours ---

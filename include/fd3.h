@@ -34,6 +34,9 @@ extern "C" {
 
 #define FD3_INTERP_P3M 0      /* method='spme': centred cardinal B-splines, sinc^2p division */
 #define FD3_INTERP_LAGRANGE 1 /* method='pme' : Lagrange interpolation, no sinc division      */
+#define FD3_INTERP_EWALD 2    /* method='ewald': structure-factor sum over the full k grid
+                                 m_d in fftfreq(ns_d) * ns_d (src/fourierd3/core.py:225-242,
+                                 563-577); ns 1..4096 per axis, rank <= 32, nS * rank <= 64 */
 
 #define FD3_DTYPE_F32 0
 #define FD3_DTYPE_F64 1
@@ -61,7 +64,7 @@ typedef struct fd3_model {
 /* Geometry of one periodic box. */
 typedef struct fd3_geom {
   double cell[9];  /* Angstrom, row i = lattice vector a_i                   */
-  int32_t ns[3];   /* mesh points per axis: powers of two, 2..1024            */
+  int32_t ns[3];   /* mesh points per axis: powers of two, 2..1024 (Ewald: k grid) */
   int32_t order;   /* interpolation nodes p: 1..5 (P3M), 2..7 (Lagrange)      */
   int32_t interp;  /* FD3_INTERP_*                                            */
 } fd3_geom;
@@ -116,6 +119,23 @@ int fd3_eval_celllist_workspace(const fd3_model* m, const fd3_geom* g, int64_t n
 int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, const void* positions,
                       int pos_dtype, const int32_t* species, int flags, void* workspace,
                       size_t workspace_bytes, const fd3_outputs* out, void* stream);
+
+/* fd3_eval_celllist_batch: n_boxes independent periodic boxes that share one
+ *   model (one species set): box b owns atoms [atom_offsets[b], atom_offsets[b+1])
+ *   of the packed positions (N_total, 3) and species (N_total) [device], has its
+ *   own geometry geoms[b] (its own cell and mesh) and writes outs[b].  Same
+ *   results as n_boxes fd3_eval_celllist calls; atom_offsets (n_boxes + 1),
+ *   geoms and outs are host arrays.  The boxes run in order on `stream` and
+ *   share one workspace of the size fd3_eval_celllist_batch_workspace reports
+ *   (the largest single-box workspace).  No counterpart in the reference (it
+ *   evaluates one box per call); this is the batched form of
+ *   src/fourierd3/calculator.py:125-192 for many small cells.
+ */
+int fd3_eval_celllist_batch_workspace(const fd3_model* m, int32_t n_boxes, const fd3_geom* geoms,
+                                      const int64_t* atom_offsets, int flags, size_t* bytes);
+int fd3_eval_celllist_batch(const fd3_model* m, int32_t n_boxes, const fd3_geom* geoms, const int64_t* atom_offsets,
+                            const void* positions, int pos_dtype, const int32_t* species, int flags, void* workspace,
+                            size_t workspace_bytes, const fd3_outputs* outs, void* stream);
 
 /* ---- component entry points (tests and callers that assemble their own path) ----
  * Neighbour list (matscipy "ijS", src/fourierd3/calculator.py:103-109):

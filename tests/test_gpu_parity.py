@@ -149,15 +149,25 @@ def _errors(e, f, s, e_ref, f_ref, s_ref, n):
     return abs(e - e_ref) / n, float(np.abs(f - f_ref).max()), float(np.abs(s - s_ref).max())
 
 
-@pytest.mark.parametrize("case", ["sio2_648", "sio2_thin_triclinic", "hcno_small_triclinic"])
+@pytest.mark.parametrize("case", ["sio2_648", "sio2_thin_triclinic", "hcno_small_triclinic", "carbon_only",
+                                  "mixed_z5", "mixed_z10"])
 def test_calculator_matches_oracle(case):
+    """Species counts 1..4 run the register-resident k-space kernel (with rank
+    tiles 1..4), 5 and more the generic one."""
     _say("calculator", case)
     if case == "sio2_648":
         box = W.sio2_box()
     elif case == "sio2_thin_triclinic":
         box = W.strained(W.sio2_box(60, seed=7), 0.1, seed=7)
-    else:
+    elif case == "hcno_small_triclinic":
         box = W.strained(W.hcno_box(400, seed=8), 0.1, seed=8)
+    elif case == "carbon_only":
+        b = W.hcno_box(200, seed=9)
+        box = W.strained(W.Box(numbers=np.full(b.n_atoms, 6), positions=b.positions, cell=b.cell), 0.1, seed=9)
+    elif case == "mixed_z5":
+        box = W.strained(W.mixed_z_box(250, zmax=5, seed=10), 0.1, seed=10)
+    else:
+        box = W.strained(W.mixed_z_box(300, zmax=10, seed=11), 0.1, seed=11)
     calc = FourierD3ASECalculator(device=_dev())
     atoms = _atoms(box)
     calc.calculate(atoms)
@@ -296,3 +306,80 @@ def test_edge_list_and_cell_list_paths_agree_at_headline_size():
     assert np.abs(-pos.grad.cpu().numpy() * HARTREE_TO_EV - calc.results["forces"]).max() < F_TOL
     f = calc.results["forces"]
     print("[info] 100k: sum F", f.sum(0), "max |F|", np.abs(f).max(), flush=True)
+
+
+def test_batched_boxes_match_single_box_calls():
+    """fd3_eval_celllist_batch on boxes of different sizes, cells and meshes
+    gives each box exactly what its own fd3_eval_celllist call gives."""
+    _say("batch of 7 H/C/N/O boxes")
+    boxes = [W.strained(W.hcno_box(n, seed=20 + i), 0.1, seed=20 + i)
+             for i, n in enumerate([40, 120, 333, 64, 900, 250, 75])]
+    boxes = [b for b in boxes if set(np.unique(b.numbers)) == {1, 6, 7, 8}]
+    calc = FourierD3ASECalculator(device=_dev())
+    res = calc.calculate_batch([_atoms(b) for b in boxes])
+    for b, r in zip(boxes, res):
+        one = FourierD3ASECalculator(device=_dev())
+        one.calculate(_atoms(b))
+        ref = one.results
+        # per-atom sums use float atomics, whose order varies between runs:
+        # forces agree to float rounding, far inside the parity tolerance
+        assert r["energy"] == pytest.approx(ref["energy"], rel=1e-9, abs=1e-9)
+        assert np.allclose(r["forces"], ref["forces"], rtol=0, atol=1e-6)
+        assert np.allclose(r["stress"], ref["stress"], rtol=0, atol=1e-9)
+    # and one of them against the oracle
+    o = OracleFourierD3(boxes[2].numbers, boxes[2].cell)
+    e_ref, f_ref, s_ref = o.calculate(boxes[2].positions, boxes[2].cell)
+    de, df, ds = _errors(res[2]["energy"], res[2]["forces"], res[2]["stress"], e_ref, f_ref, s_ref, boxes[2].n_atoms)
+    print(f"[errors] batch box 2: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
+@pytest.mark.parametrize("case", ["sio2_triclinic", "hcno_triclinic"])
+def test_ewald_calculator_matches_oracle(case):
+    """method='ewald': the direct structure-factor sum over the full k grid."""
+    _say("ewald calculator", case)
+    if case == "sio2_triclinic":
+        box = W.strained(W.sio2_box(60, seed=30), 0.1, seed=30)
+    else:
+        box = W.strained(W.hcno_box(150, seed=31), 0.1, seed=31)
+    calc = FourierD3ASECalculator(device=_dev(), method="ewald", k_cutoff=10.0)
+    calc.calculate(_atoms(box))
+    o = OracleFourierD3(box.numbers, box.cell, method="ewald", k_cutoff=10.0)
+    assert list(o.ns) == list(calc._model.ns_mesh)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"], e_ref, f_ref,
+                         s_ref, box.n_atoms)
+    print(f"[errors] ewald {case} k grid {list(o.ns)}: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
+def test_ewald_module_autograd_and_param_gradients():
+    """Edge-list path with method='ewald': autograd through a strain tensor and
+    dE/d(s6, s8, a1, a2) against the oracle."""
+    _say("ewald module")
+    box = W.strained(W.sio2_box(48, seed=32), 0.08, seed=32)
+    i, j, S = neighbour_list(box.positions, box.cell, 6.0)
+    dev = _dev()
+    model = FourierD3(box.numbers, box.cell, device=dev, dtype=torch.float64, method="ewald", k_cutoff=8.0,
+                      verbose=False)
+    model.params.requires_grad_(True)
+    cell = torch.tensor(box.cell, dtype=torch.float64, device=dev)
+    strain = torch.zeros(3, 3, dtype=torch.float64, device=dev, requires_grad=True)
+    scell = cell + torch.einsum("ab,Ab->Aa", strain, cell)
+    model._update_cell(scell)
+    pos = torch.tensor(box.positions, dtype=torch.float64, device=dev, requires_grad=True)
+    spos = pos + torch.einsum("ab,ib->ia", strain, pos)
+    ei = torch.tensor(np.stack([i, j]), dtype=torch.long, device=dev)
+    e = model(spos, ei, torch.tensor(S, dtype=torch.float64, device=dev) @ scell) * HARTREE_TO_EV
+    e.backward()
+    forces = (-pos.grad).cpu().numpy()
+    stress = (strain.grad / model.volume * ANGSTROM_TO_BOHR ** 3).detach().cpu().numpy()
+    o = OracleFourierD3(box.numbers, box.cell, method="ewald", k_cutoff=8.0)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell, edges=(i, j, S))
+    de, df, ds = _errors(float(e), forces, stress, e_ref, f_ref, s_ref, box.n_atoms)
+    print(f"[errors] ewald module: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+    got = model.params.grad.cpu().numpy()
+    _, ref = o.param_gradient(box.positions, box.cell, edges=(i, j, S))
+    print("[errors] ewald dE/dparams", got, ref, flush=True)
+    assert np.allclose(got, ref, rtol=1e-4, atol=1e-6 * box.n_atoms)

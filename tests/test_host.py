@@ -68,10 +68,14 @@ def test_workspace_sizes_scale_with_mesh_and_atoms():
 def test_module_constructor_checks():
     from fourierd3_amd import FourierD3
     box = W.sio2_box(30, seed=1)
-    with pytest.raises(NotImplementedError):
-        FourierD3(box.numbers, box.cell, device="cpu", method="ewald", verbose=False)
+    with pytest.raises(ValueError):
+        FourierD3(box.numbers, box.cell, device="cpu", method="p3m", verbose=False)
     with pytest.raises(NotImplementedError):
         FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d4", verbose=False)
+    # Ewald: the k grid is ceil(k_cutoff |a_i| / 2 pi) with the cell in Bohr (core.py:228-230)
+    me = FourierD3(box.numbers, box.cell, device="cpu", method="ewald", k_cutoff=4.0, verbose=False)
+    la = np.linalg.norm(box.cell * T.ANGSTROM_TO_BOHR, axis=1)
+    assert list(me.ns_mesh) == [int(np.ceil(4.0 * v / (2 * np.pi))) for v in la]
     with pytest.raises(NotImplementedError):
         FourierD3(box.numbers, box.cell, device="cpu", xcfunc="b3lyp", verbose=False)
     with pytest.raises(AssertionError):

@@ -180,8 +180,8 @@ static int check_model(const fd3_model* m) {
     set_error("n_species and n_rank must be positive, got %d and %d", m->n_species, m->n_rank);
     return FD3_ERR_ARG;
   }
-  if (m->n_species > 32 || m->n_rank > 32) {
-    set_error("this build supports up to 32 species and rank 32, got %d and %d", m->n_species, m->n_rank);
+  if (m->n_species > 64 || m->n_rank > 32) {
+    set_error("this build supports up to 64 species and rank 32, got %d and %d", m->n_species, m->n_rank);
     return FD3_ERR_UNSUPPORTED;
   }
   if (!m->rcov || !m->cnref || !m->vq || !m->eigs || !m->qq) {
@@ -317,6 +317,7 @@ struct Common {
   float* buf1;   // spectrum
   float2* buf2;  // z-pass output A
   ColBuffers cols;
+  float2* k0;        // (C) k = 0 coefficients of dE/drho
   int32_t* ew_perm;  // Ewald: atoms by species
   int32_t* ew_seg;
   float* cn_raw;
@@ -333,6 +334,7 @@ static void carve_common(Arena& ar, Common& c, int64_t n, int r, int cn_rows) {
   const size_t nbuf = c.geom.interp == FD3_INTERP_EWALD ? ewald_buffer_elems(c.plan) : c.plan.mesh_complex();
   c.buf1 = ar.take<float>(2 * nbuf);
   c.buf2 = ar.take<float2>(nbuf);
+  c.k0 = ar.take<float2>(c.plan.C);
   c.ew_perm = ar.take<int32_t>(std::max<int64_t>(n, 1));
   c.ew_seg = ar.take<int32_t>(c.plan.nS + 1);
   col_carve(c.plan, n, ar, &c.cols);
@@ -392,6 +394,7 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
       ba.n = n; ba.nS = nS; ba.r = r; ba.nx = 1; ba.ny = 1; ba.nz = 1;  // accG holds dE/df
       for (int i = 0; i < 9; ++i) ba.inv[i] = c.geom.inv[i];
       ba.pos4 = pos4; ba.spec_s = spec_s; ba.charges = c.charges; ba.accA = c.cols.accA; ba.accG = c.cols.accG;
+      ba.k0 = nullptr;  // the Ewald adjoint sums every k directly
       ba.cn_raw = c.cn_raw; ba.cnref = m->cnref; ba.vq = m->vq; ba.eigs = m->eigs; ba.qq = m->qq;
       ba.cncorr = m->cncorr; ba.s6 = m->s6; ba.s8 = m->s8; ba.a1 = m->a1; ba.a2 = m->a2;
       ba.gpos = c.gpos; ba.dEdcn = c.dEdcn; ba.partials = c.p_ga;
@@ -410,7 +413,7 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   if ((rc = spread_r2c_z(fa, c.plan, c.tab, c.buf2, s))) return rc;
   if ((rc = fft_y(c.plan, c.tab, c.buf2, spec, -1, s))) return rc;
   if ((rc = kspace_run(c.plan, c.tab, c.geom, spec, m->eigs, grad ? 1 : 0, (flags & FD3_NEED_PARAMS) ? 1 : 0, c.p_ks,
-                       s)))
+                       c.k0, s)))
     return rc;
   if (grad) {
     if ((rc = fft_y(c.plan, c.tab, spec, c.buf2, 1, s))) return rc;
@@ -419,6 +422,7 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
     ba.n = n; ba.nS = nS; ba.r = r; ba.nx = c.geom.nx; ba.ny = c.geom.ny; ba.nz = c.geom.nz;
     for (int i = 0; i < 9; ++i) ba.inv[i] = c.geom.inv[i];
     ba.pos4 = pos4; ba.spec_s = spec_s; ba.charges = c.charges; ba.accA = c.cols.accA; ba.accG = c.cols.accG;
+    ba.k0 = c.k0;
     ba.cn_raw = c.cn_raw; ba.cnref = m->cnref; ba.vq = m->vq; ba.eigs = m->eigs; ba.qq = m->qq;
     ba.cncorr = m->cncorr; ba.s6 = m->s6; ba.s8 = m->s8; ba.a1 = m->a1; ba.a2 = m->a2;
     ba.gpos = c.gpos; ba.dEdcn = c.dEdcn; ba.partials = c.p_ga;

@@ -160,6 +160,12 @@ struct KspaceArgs {
   const float4* htab;
   const float* scx; const float* scy; const float* scz;
   double* partials;     // (gridDim, 16)
+  // dE/drho's k = 0 coefficient per channel.  It is a constant on the mesh:
+  // it adds exactly itself to every atom's sum_nodes W dE/drho and nothing to
+  // the force sums (sum_nodes dW/du = 0), so it is passed to the C6 adjoint
+  // directly and zeroed in the spectrum, which keeps its float rounding out of
+  // the forces.
+  float2* k0;           // (C)
   float recip[9];       // rows b_i of 2 pi cell^-T (Bohr^-1)
   float neg_inv_vol;    // -1/Omega
   int nx, ny, nz, M1, nS, r, RT, NK, KT, use_sinc, need_grad, need_params;
@@ -351,7 +357,12 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
         for (int q = 0; q < RT; ++q) {
           e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
           const float f = lam[q] * a.neg_inv_vol;
-          S[((X * RT + q) * NK + kyi) * ls + kx] = make_float2(f * ph[X][q].x, f * ph[X][q].y);
+          float2 v = make_float2(f * ph[X][q].x, f * ph[X][q].y);
+          if (kz == 0 && ky0 + kyi == 0 && kx == 0) {  // k = 0: kept out of the mesh (see KspaceArgs::k0)
+            if (q < rt) a.k0[X * a.r + q0 + q] = v;
+            v = make_float2(0.f, 0.f);
+          }
+          S[((X * RT + q) * NK + kyi) * ls + kx] = v;
         }
       kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, PARAMS);
     }
@@ -392,7 +403,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     __syncthreads();
     kspace_load(a, S, lbase, nlines, lnx);
     __syncthreads();
-    lds_fft<-1, kFftMaxB>(S, nx, nlines, ls, tws, 1);
+    lds_fft<-1, kKsGenericMaxB>(S, nx, nlines, ls, tws, 1);
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
@@ -436,14 +447,19 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
           const float2 sx = S[ix], ph = P[ix];
           e_acc += lam * (sx.x * ph.x + sx.y * ph.y);
           const float f = lam * a.neg_inv_vol;
-          P[ix] = make_float2(f * ph.x, f * ph.y);
+          float2 v = make_float2(f * ph.x, f * ph.y);
+          if (kz == 0 && ky0 + kyi == 0 && kx == 0) {  // k = 0: kept out of the mesh (see KspaceArgs::k0)
+            a.k0[X * a.r + q0 + qi] = v;
+            v = make_float2(0.f, 0.f);
+          }
+          P[ix] = v;
         }
       }
       kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, params);
     }
     if (!a.need_grad) continue;
     __syncthreads();
-    lds_fft<1, kFftMaxB>(P, nx, nlines, ls, tws, 1);
+    lds_fft<1, kKsGenericMaxB>(P, nx, nlines, ls, tws, 1);
     kspace_store(a, P, lbase, nlines, lnx);
   }
   kspace_reduce(acc, red, a, params);
@@ -472,17 +488,14 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   // k-space: the h table, the x twiddles, one (nS <= kKsRegMaxS) or two LDS
   // buffers of nS*RT*NK lines of nx+1 complex, and the line bases, within
   // 62 KB (the block also holds 2 KB of static LDS)
-  if (nS > 32) {
-    set_error("the k-space kernel supports up to 32 species, got %d", nS);
-    return FD3_ERR_UNSUPPORTED;
-  }
   p->ks_reg = nS <= kKsRegMaxS ? nS : 0;
   const int nbuf = p->ks_reg ? 1 : 2;
+  // the generic variant may take most of gfx950's 160 KB of LDS per workgroup
+  const size_t ks_budget = p->ks_reg ? (size_t)62 * 1024 : (size_t)kKsGenericLds;
   auto ks_bytes = [&](int rt_, int nk_) {
     return (size_t)3 * (kHtabN + 1) * sizeof(float4) + (size_t)p->nx * sizeof(float2) +
            (size_t)nbuf * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * rt_ * nk_ * sizeof(long long);
   };
-  const size_t ks_budget = 62 * 1024;
   // RT: ranks per tile; the register variant takes any RT <= 4, the generic
   // one a power of two <= 8 (a last partial tile is padded with zero lines)
   int RT = 1;
@@ -491,13 +504,13 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
     while (RT < std::min(r, 8)) RT <<= 1;
   int NK = std::max(1, std::min(p->ny, 256 / p->nx));
   auto ks_fits = [&](int rt_, int nk_) {
-    const int maxb = p->ks_reg ? ks_maxb(nS * rt_) : kFftMaxB;
+    const int maxb = p->ks_reg ? ks_maxb(nS * rt_) : kKsGenericMaxB;
     return ks_bytes(rt_, nk_) <= ks_budget && (size_t)nS * rt_ * nk_ * p->nx <= (size_t)1024 * maxb;
   };
   while (RT > 1 && !ks_fits(RT, NK)) RT = p->ks_reg ? RT - 1 : RT >> 1;
   while (NK > 1 && !ks_fits(RT, NK)) NK >>= 1;
   if (!ks_fits(RT, NK)) {
-    set_error("k-space tile of %d species x %d points does not fit in LDS", nS, p->nx);
+    set_error("k-space tile of %d species x %d points exceeds the LDS of a workgroup", nS, p->nx);
     return FD3_ERR_UNSUPPORTED;
   }
   p->RT = RT; p->NK = NK;
@@ -510,8 +523,7 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   p->ks_blocks = (p->ny / NK / KT) * p->M1 * zb;
   // register budget of lds_fft<.., kFftMaxB>: lines * n <= kFftLines
   const size_t lim = kFftLines;
-  if ((size_t)p->TY * p->M > lim || (size_t)p->TX * p->ny > lim || (size_t)p->LPB * p->nx > lim ||
-      (size_t)nS * RT * NK * p->nx > lim) {
+  if ((size_t)p->TY * p->M > lim || (size_t)p->TX * p->ny > lim || (size_t)p->LPB * p->nx > lim) {
     set_error("internal FFT tiling exceeds the per-thread register budget");
     return FD3_ERR_UNSUPPORTED;
   }
@@ -606,8 +618,9 @@ int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, i
 }
 
 int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
-               int need_params, double* partials, hipStream_t s) {
+               int need_params, double* partials, float2* k0, hipStream_t s) {
   KspaceArgs a;
+  a.k0 = k0;
   a.spec = spec; a.eigs = eigs; a.pc = t.pc; a.twx = t.twx; a.htab = t.htab;
   a.scx = t.scx; a.scy = t.scy; a.scz = t.scz; a.partials = partials;
   for (int i = 0; i < 9; ++i) a.recip[i] = (float)g.recip[i];
@@ -628,7 +641,10 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
     FD3_KS(2, 1) FD3_KS(2, 2) FD3_KS(2, 3) FD3_KS(2, 4)
     FD3_KS(3, 1) FD3_KS(3, 2) FD3_KS(3, 3) FD3_KS(3, 4)
     FD3_KS(4, 1) FD3_KS(4, 2) FD3_KS(4, 3) FD3_KS(4, 4)
-    default: hipLaunchKernelGGL(k_kspace, grid, dim3(256), sh, s, a);
+    default:
+      if (sh > 64 * 1024) FD3_HIP(hipFuncSetAttribute((const void*)k_kspace,
+                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+      hipLaunchKernelGGL(k_kspace, grid, dim3(256), sh, s, a);
   }
 #undef FD3_KS
   FD3_LAUNCH_CHECK("k_kspace");

@@ -47,6 +47,11 @@ constexpr int ks_maxb(int lines_per_row) { return lines_per_row <= 8 ? 2 : 4; }
 
 // Species counts for which the k-space kernel keeps a k point's spectra in registers.
 constexpr int kKsRegMaxS = 4;
+// The generic k-space kernel: LDS budget (of the 160 KB a gfx950 workgroup may
+// use; 2 KB of static LDS come on top) and butterflies per thread of its line
+// FFTs (tiles of nS * RT * NK * nx <= 1024 * kKsGenericMaxB values).
+constexpr int kKsGenericLds = 150 * 1024;
+constexpr int kKsGenericMaxB = 8;
 
 int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p);
 void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t);
@@ -62,7 +67,8 @@ int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream
 int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, float2* A, float* out, hipStream_t s);
 // fused forward-x / k-space / inverse-x; partials has p.ks_blocks * 16 doubles
 // (the pair constants come from fft_tables_init with a PairInit)
+// k0 (C): the k = 0 coefficients of dE/drho, zeroed in the spectrum.
 int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
-               int need_params, double* partials, hipStream_t s);
+               int need_params, double* partials, float2* k0, hipStream_t s);
 
 }  // namespace fd3

@@ -52,6 +52,7 @@ struct C6BwdArgs {
   const float* charges;
   const float* accA;
   const float* accG;
+  const float2* k0;  // (nS * r) or NULL: k = 0 part of dE/dc, added to accA
   const float* cn_raw;
   const float* cnref;
   const float* vq;

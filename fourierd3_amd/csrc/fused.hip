@@ -343,7 +343,8 @@ __global__ void __launch_bounds__(256) k_c6_bwd(C6BwdArgs a) {
       L[pp] /= sum;
       float g = 0.f;
       for (int q = 0; q < a.r; ++q) {
-        const float dc = a.accA[s * a.r + q] + a.eigs[q] * a.charges[s * a.r + q] * phi0;
+        float dc = a.accA[s * a.r + q] + a.eigs[q] * a.charges[s * a.r + q] * phi0;
+        if (a.k0) dc += a.k0[X * a.r + q].x;
         g += dc * a.vq[(X * kNRef + pp) * a.r + q];
       }
       gp[pp] = g;

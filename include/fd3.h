@@ -49,7 +49,7 @@ extern "C" {
  * Mirrors the buffers FourierD3.__init__ registers (src/fourierd3/core.py:139-182)
  * and D3Potential's R0 / Q tables (src/fourierd3/pair_pot.py:49-89). */
 typedef struct fd3_model {
-  int32_t n_species; /* nS, 1..32                                         */
+  int32_t n_species; /* nS, 1..64                                         */
   int32_t n_rank;    /* r, 1..32                                          */
   const float* rcov;     /* (nS)      covalent radii, Bohr                */
   const float* cnref;    /* (nS, 7)   reference CNs, -1 = unused          */

@@ -87,8 +87,9 @@ def ns_mesh_rule(cell_bohr: np.ndarray, h_bohr: float) -> np.ndarray:
 def green_pair(k, R, Q, s6, s8):
     """G(k) = s6 f6^(k; R) + 3 s8 Q f8^(k; R) with the two branches of
     ``src/fourierd3/pair_pot.py:142-178`` (Taylor form only where k R < 1e-15)."""
-    k = torch.as_tensor(k, dtype=DT)
-    R = torch.as_tensor(R, dtype=DT)
+    dt = k.dtype if torch.is_tensor(k) and k.dtype in (torch.float32, torch.float64) else DT
+    k = torch.as_tensor(k, dtype=dt)
+    R = torch.as_tensor(R, dtype=dt)
     pi = math.pi
     small = k * R < 1e-15
     ks = torch.where(small, torch.ones_like(k), k)
@@ -143,7 +144,10 @@ class OracleFourierD3:
     """FP64 restatement of ``FourierD3`` (``src/fourierd3/core.py:14-583``)."""
 
     def __init__(self, species, cell, mesh_spacing=1.2, c6tol=1.0, params=None, method="spme",
-                 interpolation_nodes=5, k_cutoff=10.0, r_cut=6.0, cncorr=None, ns_mesh=None):
+                 interpolation_nodes=5, k_cutoff=10.0, r_cut=6.0, cncorr=None, ns_mesh=None, dtype=None):
+        # float64 by default; float32 restates the reference at its own default
+        # dtype (src/fourierd3/core.py:60), used to size float32 tolerances
+        self.DT = DT if dtype is None else dtype
         cnref_t, rcov_t, sqq_t = _tables()
         z = np.asarray(species, dtype=np.int64)
         uniq = np.unique(z)                                       # core.py:104
@@ -154,16 +158,16 @@ class OracleFourierD3:
         self.method = method
         self.p = int(interpolation_nodes)
         self.r_cut = r_cut * ANGSTROM_TO_BOHR                     # core.py:121
-        self.cncorr = None if cncorr is None else torch.as_tensor(cncorr, dtype=DT)
+        self.cncorr = None if cncorr is None else torch.as_tensor(cncorr, dtype=self.DT)
         e, v = oracle_decomp(uniq, c6tol)                          # core.py:139
-        self.eigs = torch.tensor(e, dtype=DT)
+        self.eigs = torch.tensor(e, dtype=self.DT)
         self.n_rank = len(e)
-        self.vq = torch.tensor(v, dtype=DT).view(self.n_species, 7, self.n_rank)   # core.py:148
-        self.rcov = torch.tensor(rcov_t[uniq], dtype=DT)           # core.py:152
-        self.cnref = torch.tensor(cnref_t[uniq], dtype=DT)         # core.py:155
+        self.vq = torch.tensor(v, dtype=self.DT).view(self.n_species, 7, self.n_rank)   # core.py:148
+        self.rcov = torch.tensor(rcov_t[uniq], dtype=self.DT)           # core.py:152
+        self.cnref = torch.tensor(cnref_t[uniq], dtype=self.DT)         # core.py:155
         p = PBE if params is None else tuple(float(x) for x in params)
-        self.params = torch.tensor(p, dtype=DT)
-        sq = torch.tensor(sqq_t[uniq], dtype=DT)
+        self.params = torch.tensor(p, dtype=self.DT)
+        sq = torch.tensor(sqq_t[uniq], dtype=self.DT)
         self.QQ = torch.outer(sq, sq)                              # pair_pot.py:55
         self.R0 = self.params[2] * torch.sqrt(3 * self.QQ) + self.params[3]   # pair_pot.py:64
         d = torch.diagonal(self.R0)
@@ -178,7 +182,7 @@ class OracleFourierD3:
     # -- cell ---------------------------------------------------------------
     def update_cell(self, cell_A: torch.Tensor):
         """``_update_cell`` (core.py:244-267); ``cell_A`` may require grad."""
-        self.cell = cell_A.to(DT) * ANGSTROM_TO_BOHR
+        self.cell = cell_A.to(self.DT) * ANGSTROM_TO_BOHR
         self.volume = torch.abs(torch.linalg.det(self.cell))
 
     # -- Green's function ---------------------------------------------------
@@ -195,9 +199,9 @@ class OracleFourierD3:
         """torch-pme ``_generate_kvectors``: rows of 2 pi cell^-T times integer m."""
         recip = 2 * math.pi * torch.linalg.inv(self.cell).T
         nx, ny, nz = (int(v) for v in self.ns)
-        mx = torch.fft.fftfreq(nx, dtype=DT) * nx
-        my = torch.fft.fftfreq(ny, dtype=DT) * ny
-        mz = (torch.fft.fftfreq(nz, dtype=DT) if full else torch.fft.rfftfreq(nz, dtype=DT)) * nz
+        mx = torch.fft.fftfreq(nx, dtype=self.DT) * nx
+        my = torch.fft.fftfreq(ny, dtype=self.DT) * ny
+        mz = (torch.fft.fftfreq(nz, dtype=self.DT) if full else torch.fft.rfftfreq(nz, dtype=self.DT)) * nz
         k = (mx.view(-1, 1, 1, 1) * recip[0] + my.view(1, -1, 1, 1) * recip[1]
              + mz.view(1, 1, -1, 1) * recip[2])
         return k, (mx, my, mz)
@@ -215,7 +219,7 @@ class OracleFourierD3:
         kcn = 16.0 + (rc_ - r1) ** 2 / ((self.r_cut - rc_) ** 2 + 1e-6)
         contrib = torch.sigmoid(kcn * ((4.0 / 3.0) * rcs / rs - 1.0))
         contrib = torch.where(r > 1e-6, contrib, torch.zeros_like(contrib))
-        cn = torch.zeros(pos_B.shape[0], dtype=DT).index_add(0, src, contrib)
+        cn = torch.zeros(pos_B.shape[0], dtype=self.DT).index_add(0, src, contrib)
         if self.cncorr is not None:
             cn = self.cncorr[self.species, 1] * cn + self.cncorr[self.species, 0]
         return cn
@@ -239,7 +243,7 @@ class OracleFourierD3:
     def spread(self, pos_B, c):
         """``points_to_mesh`` (mesh_interpolator_d3.py:99-138) on u = n (r cell^-1)."""
         nx, ny, nz = (int(v) for v in self.ns)
-        ns_t = torch.tensor([nx, ny, nz], dtype=DT)
+        ns_t = torch.tensor([nx, ny, nz], dtype=self.DT)
         u = (pos_B @ torch.linalg.inv(self.cell)) * ns_t
         w, idx = interpolation_weights(u, self.p, "P3M" if self.method == "spme" else "Lagrange")
         idx = torch.remainder(idx, torch.tensor([nx, ny, nz]))
@@ -249,10 +253,10 @@ class OracleFourierD3:
         wgeo = (w[a, :, 0] * w[b, :, 1] * w[g, :, 2]).t()                   # (N, p^3)
         lin = (idx[g, :, 2] + nz * (idx[b, :, 1] + ny * idx[a, :, 0])).t().flatten()
         n, r = c.shape
-        chan = torch.zeros(n, self.n_species, r, dtype=DT)
+        chan = torch.zeros(n, self.n_species, r, dtype=self.DT)
         chan = chan.index_put((torch.arange(n), self.species), c)
         vals = (chan.reshape(n, 1, -1) * wgeo.unsqueeze(2)).flatten(0, 1)
-        rho = torch.zeros(nx * ny * nz, self.n_species * r, dtype=DT).index_add(0, lin, vals)
+        rho = torch.zeros(nx * ny * nz, self.n_species * r, dtype=self.DT).index_add(0, lin, vals)
         return rho.t().reshape(self.n_species, r, nx, ny, nz)
 
     def kfilter(self, params=None):
@@ -266,7 +270,7 @@ class OracleFourierD3:
                   * torch.sinc(mz / nz)[None, None, :]) ** (2 * self.p)
             sc = torch.where(sc < 1e-10, torch.full_like(sc, 1e-10), sc)
             g = g / sc
-        w = torch.full((g.shape[-1],), 2.0, dtype=DT)
+        w = torch.full((g.shape[-1],), 2.0, dtype=self.DT)
         w[0] = 1.0
         w[-1] = 1.0
         return g * w
@@ -286,7 +290,7 @@ class OracleFourierD3:
         g = self.green(torch.linalg.norm(k, dim=-1), params)                # (nS, nS, nk)
         arg = k @ pos_B.T                                                   # (nk, N)
         n = pos_B.shape[0]
-        chan = torch.zeros(n, self.n_species, c.shape[1], dtype=DT).index_put((torch.arange(n), self.species), c)
+        chan = torch.zeros(n, self.n_species, c.shape[1], dtype=self.DT).index_put((torch.arange(n), self.species), c)
         cs = torch.einsum("izr,ki->rzk", chan, torch.cos(arg))
         sn = torch.einsum("izr,ki->rzk", chan, torch.sin(arg))
         w = torch.sqrt(torch.abs(self.eigs)).view(-1, 1, 1)
@@ -296,8 +300,8 @@ class OracleFourierD3:
     # -- energy ----------------------------------------------------------------
     def energy(self, pos_A, edge_index, shifts_A, params=None):
         """``forward`` (core.py:480-583), returns Hartree."""
-        pos_B = pos_A.to(DT) * ANGSTROM_TO_BOHR
-        cn = self.cn_smooth(pos_B, edge_index, shifts_A.to(DT) * ANGSTROM_TO_BOHR)
+        pos_B = pos_A.to(self.DT) * ANGSTROM_TO_BOHR
+        cn = self.cn_smooth(pos_B, edge_index, shifts_A.to(self.DT) * ANGSTROM_TO_BOHR)
         c = self.charges(cn)
         if params is None:
             vs = self.vself(c)
@@ -319,14 +323,14 @@ class OracleFourierD3:
         if edges is None:
             edges = neighbour_list(positions_A, cell_np, self.r_cut / ANGSTROM_TO_BOHR)
         i, j, S = edges
-        cell = torch.tensor(cell_np, dtype=DT)
-        strain = torch.zeros(3, 3, dtype=DT, requires_grad=True)
+        cell = torch.tensor(cell_np, dtype=self.DT)
+        strain = torch.zeros(3, 3, dtype=self.DT, requires_grad=True)
         scell = cell + torch.einsum("ab,Ab->Aa", strain, cell)
         self.update_cell(scell)
-        pos = torch.tensor(positions_A, dtype=DT, requires_grad=True)
+        pos = torch.tensor(positions_A, dtype=self.DT, requires_grad=True)
         spos = pos + torch.einsum("ab,ib->ia", strain, pos)
         ei = torch.tensor(np.stack([i, j]), dtype=torch.long)
-        sh = torch.tensor(S, dtype=DT) @ scell
+        sh = torch.tensor(S, dtype=self.DT) @ scell
         e = self.energy(spos, ei, sh, params) * HARTREE_TO_EV
         e.backward()
         stress = strain.grad / self.volume.detach() * ANGSTROM_TO_BOHR ** 3
@@ -346,7 +350,7 @@ class OracleFourierD3:
         self.update_cell(cell)
         prm = self.params.clone().requires_grad_(True)
         ei = torch.tensor(np.stack([i, j]), dtype=torch.long)
-        sh = torch.tensor(S, dtype=DT) @ cell
+        sh = torch.tensor(S, dtype=self.DT) @ cell
         e = self.energy(torch.tensor(positions_A), ei, sh, prm) * HARTREE_TO_EV
         e.backward()
         return float(e.detach()), prm.grad.numpy().copy()

@@ -430,3 +430,51 @@ def test_many_species_box_at_full_size_is_translation_invariant():
     assert abs(r1["energy"] - r2["energy"]) / n < E_TOL
     assert np.abs(r1["forces"] - r2["forces"]).max() < F_TOL + 2e-5 * fmax
     assert np.abs(r1["stress"] - r2["stress"]).max() < S_TOL
+
+
+def test_compute_cn_smooth_and_c6_weights_match_oracle():
+    """FourierD3.compute_cn_smooth (Bohr inputs, src/fourierd3/core.py:323-386)
+    and compute_c6_weights against the oracle, on a strained box."""
+    _say("compute_cn_smooth")
+    box = W.strained(W.hcno_box(300, seed=40), 0.1, seed=40)
+    i, j, S = neighbour_list(box.positions, box.cell, 6.0)
+    dev = _dev()
+    model = FourierD3(box.numbers, box.cell, device=dev, dtype=torch.float64, verbose=False)
+    pB = torch.tensor(box.positions, dtype=torch.float64) * ANGSTROM_TO_BOHR
+    shB = torch.tensor(S, dtype=torch.float64) @ torch.tensor(box.cell) * ANGSTROM_TO_BOHR
+    ei = torch.tensor(np.stack([i, j]))
+    cn = model.compute_cn_smooth(pB.to(dev), ei.to(dev), shB.to(dev)).cpu().numpy()
+    o = OracleFourierD3(box.numbers, box.cell)
+    cn_ref = o.cn_smooth(pB, ei, shB).numpy()
+    print(f"[errors] cn max {np.abs(cn - cn_ref).max():.3e}", flush=True)
+    assert np.abs(cn - cn_ref).max() < 1e-5  # float32 sums of ~100 terms of size <= 1
+    w = model.compute_c6_weights(torch.tensor(cn_ref, device=dev)).cpu().numpy()
+    w_ref = o.c6_weights(torch.tensor(cn_ref)).numpy()
+    assert np.abs(w - w_ref).max() < 1e-6
+
+
+_GOLDEN = sorted(__import__("glob").glob(__import__("os").path.join(__import__("os").path.dirname(__file__),
+                                                                     "golden", "*.npz")))
+
+
+@pytest.mark.skipif(bool(__import__("os").environ.get("FD3_C6REF")),
+                    reason="fixtures are tied to the built-in C6 table")
+@pytest.mark.parametrize("path", _GOLDEN, ids=lambda p: p.split("/")[-1][:-4])
+def test_hip_path_matches_golden_fixtures(path):
+    """The HIP calculator against the committed FP64 fixtures."""
+    name = path.split("/")[-1][:-4]
+    _say("golden", name)
+    d = np.load(path)
+    kw = {}
+    if "pme4" in name:
+        kw = {"method": "pme", "interpolation_nodes": 4}
+    elif "ewald" in name:
+        kw = {"method": "ewald", "k_cutoff": 6.0}
+    calc = FourierD3ASECalculator(device=_dev(), **kw)
+    calc.calculate(SimpleNamespace(numbers=d["numbers"], positions=d["positions"], cell=d["cell"],
+                                   pbc=np.array([True] * 3)))
+    assert list(calc._model.ns_mesh) == list(d["ns"])
+    de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"],
+                         float(d["energy"]), d["forces"], d["stress"], len(d["numbers"]))
+    print(f"[errors] golden {name}: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL

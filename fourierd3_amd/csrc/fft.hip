@@ -91,10 +91,13 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
 }
 
 // ------------------------------------------------------------ y passes ---
-// forward: A[c][x][kz][0..ny) (x in tile) -> B[c][kz][ky][x]; grid (nx/TX, M+1, C)
-template <int DIR>
-__global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, float2* __restrict__ out, int nx,
-                                               int ny, int M1, int TX, const float2* __restrict__ twy) {
+// forward: A[c][x][kz][0..ny) (x in tile) -> B[c][kz][ky][x]; grid (nx/TX, M+1, C);
+// blocks of 2^LOG threads.
+template <int DIR, int LOG>
+__global__ void __launch_bounds__(1 << LOG) k_c2c_y(const float2* __restrict__ in, float2* __restrict__ out, int nx,
+                                                    int ny, int M1, int TX, const float2* __restrict__ twy) {
+  constexpr int U = kFftLines >> LOG;     // a whole tile in one batch of loads
+  constexpr int MAXB = kFftLines >> (LOG + 2);
   extern __shared__ float2 lds[];
   const int ls = ny + 1;
   const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
@@ -103,19 +106,19 @@ __global__ void __launch_bounds__(256) k_c2c_y(const float2* __restrict__ in, fl
   const int lny = ilog2(ny), lTX = ilog2(TX);
   if (DIR < 0) {
     const float2* src = in + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile2_to_lds<kCopyU>(lds, src, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
+    tile2_to_lds<U, LOG>(lds, src, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
   } else {
     const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    tile2_to_lds<kCopyU>(lds, src, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
+    tile2_to_lds<U, LOG>(lds, src, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
   }
   __syncthreads();
-  lds_fft<DIR, kFftMaxB>(lds, ny, TX, ls, tws, 1);
+  lds_fft<DIR, MAXB>(lds, ny, TX, ls, tws, 1);
   if (DIR < 0) {
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    tile2_from_lds<kCopyU>(dst, lds, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
+    tile2_from_lds<U, LOG>(dst, lds, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
   } else {
     float2* dst = out + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile2_from_lds<kCopyU>(dst, lds, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
+    tile2_from_lds<U, LOG>(dst, lds, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
   }
 }
 
@@ -466,6 +469,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
 }
 
 // --------------------------------------------------------------- host ---
+constexpr int kYLog = 9;  // y passes: 512-thread blocks
 int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   for (int i = 0; i < 3; ++i) {
     const int n = ns[i];
@@ -564,7 +568,7 @@ int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float
   {
     const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
     ProfScope _ps("k_c2c_y_fwd", s);
-    hipLaunchKernelGGL(k_c2c_y<-1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, A, spec, p.nx, p.ny, p.M1, p.TX,
+    hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, A, spec, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
     FD3_LAUNCH_CHECK("k_c2c_y fwd");
   }
@@ -588,7 +592,7 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
   {
     const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
     ProfScope _ps("k_c2c_y_inv", s);
-    hipLaunchKernelGGL(k_c2c_y<1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, spec, A, p.nx, p.ny, p.M1, p.TX,
+    hipLaunchKernelGGL((k_c2c_y<1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, spec, A, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
     FD3_LAUNCH_CHECK("k_c2c_y inv");
   }
@@ -606,11 +610,11 @@ int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, i
   const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
   if (dir < 0) {
     ProfScope _ps("k_c2c_y_fwd", s);
-    hipLaunchKernelGGL(k_c2c_y<-1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
+    hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
   } else {
     ProfScope _ps("k_c2c_y_inv", s);
-    hipLaunchKernelGGL(k_c2c_y<1>, dim3(p.nx / p.TX, p.M1, p.C), dim3(256), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
+    hipLaunchKernelGGL((k_c2c_y<1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
   }
   FD3_LAUNCH_CHECK("k_c2c_y");

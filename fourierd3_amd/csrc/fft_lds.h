@@ -36,29 +36,29 @@ struct TileShape {
   int lls, lcs;  // LDS line and column strides
 };
 
-// All tile copies run in blocks of kTileThreads threads.
-constexpr int kTileThreads = 256;
+// Tile copies run in blocks of 2^LOG threads (256 unless stated).
 constexpr int kTileLog = 8;
 
-// Element k * kTileThreads + t of a tile (row-major over (line, column)):
-// thread t owns line lt + lu(k), column ct + cu(k), where lt, ct depend on t
-// only and lu, cu on k only (uniform over the block).
+// Element k * 2^LOG + t of a tile (row-major over (line, column)): thread t
+// owns line lt + lu(k), column ct + cu(k), where lt, ct depend on t only and
+// lu, cu on k only (uniform over the block).
+template <int LOG>
 struct TileIter {
   bool wide;  // a line spans several k
   int lt, ct, nk, llen;
-  __device__ TileIter(const TileShape& t) : wide(t.llen > kTileLog), llen(t.llen) {
+  __device__ TileIter(const TileShape& t) : wide(t.llen > LOG), llen(t.llen) {
     const int tid = threadIdx.x;
     lt = wide ? 0 : tid >> t.llen;
     ct = wide ? tid : tid & ((1 << t.llen) - 1);
-    nk = ((t.nl << t.llen) + kTileThreads - 1) >> kTileLog;
+    nk = ((t.nl << t.llen) + (1 << LOG) - 1) >> LOG;
   }
-  __device__ int lu(int k) const { return wide ? k >> (llen - kTileLog) : k << (kTileLog - llen); }
-  __device__ int cu(int k) const { return wide ? (k & ((1 << (llen - kTileLog)) - 1)) << kTileLog : 0; }
+  __device__ int lu(int k) const { return wide ? k >> (llen - LOG) : k << (LOG - llen); }
+  __device__ int cu(int k) const { return wide ? (k & ((1 << (llen - LOG)) - 1)) << LOG : 0; }
 };
 
-template <int U>
+template <int U, int LOG = kTileLog>
 __device__ __forceinline__ void tile2_to_lds(float2* lds, const float2* __restrict__ g, TileShape t, StridedLines gl) {
-  const TileIter it(t);
+  const TileIter<LOG> it(t);
   // per-thread base, kept opaque so the uniform part below stays scalar
   const float2* gt = g + ((long long)it.lt * gl.gls + (long long)it.ct * t.gcs);
   asm volatile("" : "+v"(gt));
@@ -82,9 +82,9 @@ __device__ __forceinline__ void tile2_to_lds(float2* lds, const float2* __restri
   }
 }
 
-template <int U>
+template <int U, int LOG = kTileLog>
 __device__ __forceinline__ void tile2_from_lds(float2* __restrict__ g, const float2* lds, TileShape t, StridedLines gl) {
-  const TileIter it(t);
+  const TileIter<LOG> it(t);
   float2* gt = g + ((long long)it.lt * gl.gls + (long long)it.ct * t.gcs);
   asm volatile("" : "+v"(gt));
   const int lt0 = it.lt * t.lls + it.ct * t.lcs;
@@ -105,9 +105,9 @@ __device__ __forceinline__ void tile2_from_lds(float2* __restrict__ g, const flo
   }
 }
 
-template <int U>
+template <int U, int LOG = kTileLog>
 __device__ __forceinline__ void tile2_to_lds(float2* lds, const float2* __restrict__ g, TileShape t, LineTable gl) {
-  const TileIter it(t);
+  const TileIter<LOG> it(t);
   const int lt0 = it.lt * t.lls + it.ct * t.lcs;
   for (int k0 = 0; k0 < it.nk; k0 += U) {
     float2 v[U];
@@ -129,9 +129,9 @@ __device__ __forceinline__ void tile2_to_lds(float2* lds, const float2* __restri
   }
 }
 
-template <int U>
+template <int U, int LOG = kTileLog>
 __device__ __forceinline__ void tile2_from_lds(float2* __restrict__ g, const float2* lds, TileShape t, LineTable gl) {
-  const TileIter it(t);
+  const TileIter<LOG> it(t);
   const int lt0 = it.lt * t.lls + it.ct * t.lcs;
   for (int k0 = 0; k0 < it.nk; k0 += U) {
     float2 v[U];

@@ -159,8 +159,12 @@ __device__ inline float pick(const float* w, int j, int p) {
 }
 
 // ------------------------------------------------------ spread + z pass ---
+// fused z passes: 512-thread blocks for the spread (measured faster), 256 for
+// the gather (its 100 VGPRs leave no room for more waves)
+constexpr int kZLogSpread = 9, kZLogGather = 8;
 // grid (ny/TY, nx, C); LDS: TY lines of M+1 complex (= nz + 2 floats).
-__global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __restrict__ A, int TY,
+template <int LOG>
+__global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* __restrict__ A, int TY,
                                                       const float2* __restrict__ twz) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
@@ -202,16 +206,17 @@ __global__ void __launch_bounds__(256) k_spread_r2c_z(FusedArgs a, float2* __res
     }
   }
   __syncthreads();
-  lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
+  lds_fft<-1, (kFftLines >> (LOG + 2))>(lds, M, TY, ls, tws, 2);
   z_pairs<true>(lds, TY, ls, M, tws);
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
-  tile2_from_lds<kCopyU>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
+  tile2_from_lds<(kFftLines >> LOG) + 1, LOG>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
 }
 
 // ------------------------------------------------------ z pass + gather ---
-__global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2* __restrict__ A, int TY,
+template <int LOG>
+__global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const float2* __restrict__ A, int TY,
                                                       const float2* __restrict__ twz) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
@@ -223,12 +228,12 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
   segs_lookup(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
-  tile2_to_lds<kCopyU>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
+  tile2_to_lds<(kFftLines >> LOG) + 1, LOG>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
   __syncthreads();
   segs_compact(&sg, p);
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
-  lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
+  lds_fft<1, (kFftLines >> (LOG + 2))>(lds, M, TY, ls, tws, 2);
   const float* ldsf = (const float*)lds;
   const int total = sg.off[sg.n];
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
@@ -274,7 +279,8 @@ __global__ void __launch_bounds__(256) k_c2r_z_gather(FusedArgs a, const float2*
 int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s) {
   const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_spread_r2c_z", s);
-  hipLaunchKernelGGL(k_spread_r2c_z, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, a, A, p.TY, t.twz);
+  hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread>), dim3(p.ny / p.TY, p.nx, p.C), dim3(1 << kZLogSpread), sh, s, a, A, p.TY,
+                     t.twz);
   FD3_LAUNCH_CHECK("k_spread_r2c_z");
   return FD3_OK;
 }
@@ -285,7 +291,8 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
   if ((rc = launch_fill_f32(a.cb.accG, std::max<int64_t>(a.n, 1) * 3, s))) return rc;
   const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_c2r_z_gather", s);
-  hipLaunchKernelGGL(k_c2r_z_gather, dim3(p.ny / p.TY, p.nx, p.C), dim3(256), sh, s, a, A, p.TY, t.twz);
+  hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather>), dim3(p.ny / p.TY, p.nx, p.C), dim3(1 << kZLogGather), sh, s, a, A, p.TY,
+                     t.twz);
   FD3_LAUNCH_CHECK("k_c2r_z_gather");
   return FD3_OK;
 }

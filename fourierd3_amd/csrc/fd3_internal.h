@@ -69,7 +69,7 @@ int make_geom(const fd3_geom* g, Geom* out);  // validates, converts to Bohr
 // M_p(u - node) written out as polynomials in x; Lagrange weights are the
 // Lagrange basis through the p nodes.  dw = d w / d u.
 template <typename T>
-__host__ __device__ inline void interp_weights(int p, int interp, T x, T* w, T* dw) {
+__host__ __device__ __forceinline__ void interp_weights(int p, int interp, T x, T* w, T* dw) {
   if (interp == FD3_INTERP_P3M) {
     const T x2 = x * x, x3 = x2 * x, x4 = x3 * x;
     switch (p) {

@@ -161,14 +161,14 @@ __device__ inline float pick(const float* w, int j, int p) {
 // ------------------------------------------------------ spread + z pass ---
 // fused z passes: 512-thread blocks for the spread (measured faster), 256 for
 // the gather (its 100 VGPRs leave no room for more waves)
-constexpr int kZLogSpread = 9, kZLogGather = 8;
+constexpr int kZLogSpread = 9, kZLogGather = 9;
 // grid (ny/TY, nx, C); LDS: TY lines of M+1 complex (= nz + 2 floats).
-template <int LOG>
+template <int LOG, int P>  // P > 0: compile-time interpolation order
 __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* __restrict__ A, int TY,
                                                       const float2* __restrict__ twz) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
-  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = a.order;
+  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = P > 0 ? P : a.order;
   const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
   const int X = c / a.r, q = c - X * a.r;
   float2* tws = lds + TY * ls;
@@ -215,12 +215,12 @@ __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* 
 }
 
 // ------------------------------------------------------ z pass + gather ---
-template <int LOG>
+template <int LOG, int P>  // P > 0: compile-time interpolation order
 __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const float2* __restrict__ A, int TY,
                                                       const float2* __restrict__ twz) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
-  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = a.order;
+  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = P > 0 ? P : a.order;
   const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
   const int X = c / a.r, q = c - X * a.r;
   float2* tws = lds + TY * ls;
@@ -279,8 +279,9 @@ __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const fl
 int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s) {
   const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_spread_r2c_z", s);
-  hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread>), dim3(p.ny / p.TY, p.nx, p.C), dim3(1 << kZLogSpread), sh, s, a, A, p.TY,
-                     t.twz);
+  const dim3 grid(p.ny / p.TY, p.nx, p.C), block(1 << kZLogSpread);
+  if (a.order == 5) hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 5>), grid, block, sh, s, a, A, p.TY, t.twz);
+  else hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 0>), grid, block, sh, s, a, A, p.TY, t.twz);
   FD3_LAUNCH_CHECK("k_spread_r2c_z");
   return FD3_OK;
 }
@@ -291,8 +292,9 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
   if ((rc = launch_fill_f32(a.cb.accG, std::max<int64_t>(a.n, 1) * 3, s))) return rc;
   const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_c2r_z_gather", s);
-  hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather>), dim3(p.ny / p.TY, p.nx, p.C), dim3(1 << kZLogGather), sh, s, a, A, p.TY,
-                     t.twz);
+  const dim3 grid(p.ny / p.TY, p.nx, p.C), block(1 << kZLogGather);
+  if (a.order == 5) hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 5>), grid, block, sh, s, a, A, p.TY, t.twz);
+  else hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 0>), grid, block, sh, s, a, A, p.TY, t.twz);
   FD3_LAUNCH_CHECK("k_c2r_z_gather");
   return FD3_OK;
 }

@@ -47,23 +47,26 @@ bool prof_enabled() { return g_prof; }
 void prof_record(const char* name, hipStream_t s, bool start) {
   hipEvent_t ev;
   if (hipEventCreate(&ev) != hipSuccess) return;
-  hipEventRecord(ev, s);
+  if (hipEventRecord(ev, s) != hipSuccess) {
+    (void)hipEventDestroy(ev);
+    return;
+  }
   if (start) {
     g_prof_open.push_back({name, ev});
   } else if (!g_prof_open.empty()) {
     g_prof_recs.push_back({g_prof_open.back().first, g_prof_open.back().second, ev});
     g_prof_open.pop_back();
   } else {
-    hipEventDestroy(ev);
+    (void)hipEventDestroy(ev);
   }
 }
 
 static void prof_clear() {
   for (auto& r : g_prof_recs) {
-    hipEventDestroy(r.start);
-    hipEventDestroy(r.stop);
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
   }
-  for (auto& o : g_prof_open) hipEventDestroy(o.second);
+  for (auto& o : g_prof_open) (void)hipEventDestroy(o.second);
   g_prof_recs.clear();
   g_prof_open.clear();
 }

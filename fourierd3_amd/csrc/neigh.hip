@@ -14,6 +14,8 @@
 // The CN of src/fourierd3/core.py:348-386 sums, for each atom, over these
 // pairs, so no edge list is materialised on the calculator path.
 
+#include <climits>
+
 #include "fd3_internal.h"
 #include "neigh_kernels.h"
 
@@ -202,19 +204,26 @@ __global__ void k_scatter(const int32_t* __restrict__ abin, int64_t n, const int
   slot_atom[slot] = (int32_t)i;
 }
 
-// Deterministic order inside each bin: ascending atom index.
-__global__ void k_sort_bins(const int32_t* __restrict__ bstart, int nbins, int32_t* __restrict__ slot_atom) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nbins) return;
-  const int lo = bstart[b], hi = bstart[b + 1];
-  for (int i = lo + 1; i < hi; ++i) {
-    const int v = slot_atom[i];
-    int j = i - 1;
-    while (j >= lo && slot_atom[j] > v) {
-      slot_atom[j + 1] = slot_atom[j];
-      --j;
+// Deterministic order inside each bin: ascending atom index.  One wave per
+// bin: each atom's slot is lo + (number of atoms of the bin with a smaller
+// index), counted against 64 atoms at a time through lane shuffles.  Reads the
+// scattered (arbitrary) order from `in`, writes the sorted one to `out`.
+__global__ void __launch_bounds__(256) k_sort_bins(const int32_t* __restrict__ bstart, int nbins,
+                                                   const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= nbins) return;  // uniform per wave
+  const int lo = bstart[b], cnt = bstart[b + 1] - lo;
+  for (int i0 = 0; i0 < cnt; i0 += 64) {
+    const bool mine = i0 + lane < cnt;
+    const int vi = mine ? in[lo + i0 + lane] : INT_MAX;
+    int rank = 0;
+    for (int j0 = 0; j0 < cnt; j0 += 64) {
+      const int vj = j0 + lane < cnt ? in[lo + j0 + lane] : INT_MAX;
+      const int m = min(64, cnt - j0);
+      for (int t = 0; t < m; ++t) rank += __shfl(vj, t) < vi ? 1 : 0;
     }
-    slot_atom[j + 1] = v;
+    if (mine) out[lo + rank] = vi;
   }
 }
 
@@ -597,9 +606,10 @@ int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* p
   FD3_LAUNCH_CHECK("k_bin");
   hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, b.bcount, (int64_t)g.nbins, b.bstart, (int64_t*)nullptr);
   FD3_LAUNCH_CHECK("k_scan1");
-  hipLaunchKernelGGL(k_scatter, dim3(cdiv(n, 256)), dim3(256), 0, s, b.abin, n, b.bstart, b.bcur, b.slot_atom);
+  // scatter into sbin (free until k_fill_sorted), then sort each bin into slot_atom
+  hipLaunchKernelGGL(k_scatter, dim3(cdiv(n, 256)), dim3(256), 0, s, b.abin, n, b.bstart, b.bcur, b.sbin);
   FD3_LAUNCH_CHECK("k_scatter");
-  hipLaunchKernelGGL(k_sort_bins, dim3(cdiv(g.nbins, 256)), dim3(256), 0, s, b.bstart, g.nbins, b.slot_atom);
+  hipLaunchKernelGGL(k_sort_bins, dim3(cdiv(g.nbins, 4)), dim3(256), 0, s, b.bstart, g.nbins, b.sbin, b.slot_atom);
   FD3_LAUNCH_CHECK("k_sort_bins");
   hipLaunchKernelGGL(k_fill_sorted, dim3(cdiv(n, 256)), dim3(256), 0, s, b.slot_atom, n, b.posw, b.flo, b.abin,
                      species, b.pos4, b.flo_s, b.sbin, b.spec_s);

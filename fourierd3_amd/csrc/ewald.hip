@@ -156,8 +156,8 @@ __global__ void __launch_bounds__(256) k_ewald_sf(EwaldArgs a) {
           }
         d_acc += (double)(dg * tre);
         if (a.need_params) {
-          const float R = pk.R;
-          const float h6p = d6 * (2.0f / kHtabDx), h8p = d8 * (2.0f / kHtabDx);
+          const float R = pk.Rs * kHtabDx;
+          const float h6p = d6 * kHtabInvDx, h8p = d8 * kHtabInvDx;
           const float dgdR = pk.p6 * (-3.0f * h6 / R + kn * h6p) + pk.p8 * (-5.0f * h8 / R + kn * h8p);
           p_acc[0] += (double)(pk.pf6 * h6 * tre);
           p_acc[1] += (double)(pk.pf8 * h8 * tre);

@@ -163,16 +163,18 @@ __host__ __device__ inline int wrap(int i, int n) {
 // h6 and h8 do not depend on the species pair, so the k-space kernel reads
 // them from a piecewise polynomial table instead of evaluating four
 // exponentials and three sine/cosine pairs per pair and k point.  Interval i
-// covers x in [i dx, (i+1) dx); with u = 2 (x/dx - i) - 1 in [-1, 1] each
-// function is the degree-5 polynomial interpolating it at the six Chebyshev
-// nodes of the interval (interpolation error < 1e-8 of h(0) at dx = 1/2).
-// Entry 3i .. 3i+2 holds {a6_0..a6_3}, {a6_4, a6_5, a8_0, a8_1}, {a8_2..a8_5},
-// monomial coefficients in u.  Interval kHtabN is all zeros and serves every
+// covers x in [i dx, (i+1) dx); on it each function is the degree-5
+// polynomial interpolating it at the six Chebyshev nodes of the interval
+// (interpolation error < 1e-8 of h(0) at dx = 1/2), stored as monomial
+// coefficients in the local coordinate w = x/dx - i in [0, 1), which the
+// device gets with one fract.  Entry 3i .. 3i+2 holds {a6_0..a6_3},
+// {a6_4, a6_5, a8_0, a8_1}, {a8_2..a8_5}.  Interval kHtabN is all zeros and serves every
 // x >= kHtabXmax = 64, where |h6| < 1e-15 and |h8| < 4e-13 (h6(0) = 1,
 // h8(0) = -0.54): 1e-6 of float32 resolution, invisible in any sum.
 constexpr int kHtabN = 128;
 constexpr float kHtabDx = 0.5f;
 constexpr float kHtabXmax = kHtabN * kHtabDx;
+constexpr float kHtabInvDx = 1.0f / kHtabDx;  // dh/dx = kHtabInvDx dh/dw
 
 // h6(x), h8(x) in double for x > 0 (closed forms; the cancellation at small x
 // costs ~1e-16/x relative, < 1e-13 at the smallest table node).
@@ -205,47 +207,60 @@ __host__ __device__ inline void htab_interval(int i, float4* out) {
       for (int j = 0; j < 6; ++j) s += v[j] * cos(k * (2 * j + 1) * kPi / 12.0);
       c[k] = s * (k ? 2.0 / 6.0 : 1.0 / 6.0);
     }
-    // T0..T5 in monomials
+    // T0..T5 in monomials of u = 2w - 1
+    double m[6];
+    m[0] = c[0] - c[2] + c[4];
+    m[1] = c[1] - 3.0 * c[3] + 5.0 * c[5];
+    m[2] = 2.0 * c[2] - 8.0 * c[4];
+    m[3] = 4.0 * c[3] - 20.0 * c[5];
+    m[4] = 8.0 * c[4];
+    m[5] = 16.0 * c[5];
+    // then in monomials of w: sum_k m_k (2w - 1)^k = sum_j w^j 2^j sum_k m_k C(k,j) (-1)^(k-j)
+    const double binom[6][6] = {{1, 0, 0, 0, 0, 0},  {1, 1, 0, 0, 0, 0},   {1, 2, 1, 0, 0, 0},
+                                {1, 3, 3, 1, 0, 0},  {1, 4, 6, 4, 1, 0},   {1, 5, 10, 10, 5, 1}};
     double* a = f ? a8 : a6;
-    a[0] = c[0] - c[2] + c[4];
-    a[1] = c[1] - 3.0 * c[3] + 5.0 * c[5];
-    a[2] = 2.0 * c[2] - 8.0 * c[4];
-    a[3] = 4.0 * c[3] - 20.0 * c[5];
-    a[4] = 8.0 * c[4];
-    a[5] = 16.0 * c[5];
+    for (int j = 0; j < 6; ++j) {
+      double t = 0.0;
+      for (int k = j; k < 6; ++k) t += m[k] * binom[k][j] * ((k - j) & 1 ? -1.0 : 1.0);
+      a[j] = t * (double)(1 << j);
+    }
   }
   out[0] = make_float4((float)a6[0], (float)a6[1], (float)a6[2], (float)a6[3]);
   out[1] = make_float4((float)a6[4], (float)a6[5], (float)a8[0], (float)a8[1]);
   out[2] = make_float4((float)a8[2], (float)a8[3], (float)a8[4], (float)a8[5]);
 }
 
-// h6, h8 and their derivatives with respect to the local coordinate u from
-// the table (x >= 0), by Horner's scheme carrying the derivative.
-__host__ __device__ inline void green_h_tab(const float4* __restrict__ tab, float x, float* h6, float* d6, float* h8,
+// h6, h8 and their derivatives with respect to the local coordinate w from
+// the table at x = fx dx (fx >= 0), by Horner's scheme carrying the derivative.
+__host__ __device__ inline void green_h_tab(const float4* __restrict__ tab, float fx, float* h6, float* d6, float* h8,
                                             float* d8) {
-  const float fx = x * (1.0f / kHtabDx);
-  const int i = min((int)fx, kHtabN);
-  const float u = 2.0f * (fx - (float)i) - 1.0f;
-  const float4 A = tab[3 * i], B = tab[3 * i + 1], C = tab[3 * i + 2];
-  float p = B.y * u + B.x, d = B.y;
-  d = d * u + p; p = p * u + A.w;
-  d = d * u + p; p = p * u + A.z;
-  d = d * u + p; p = p * u + A.y;
-  d = d * u + p; p = p * u + A.x;
+  const int i = (int)fminf(fx, (float)kHtabN);
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float w = __builtin_amdgcn_fractf(fx);
+#else
+  const float w = fx - floorf(fx);
+#endif
+  const float4* e = (const float4*)((const char*)tab + (unsigned)i * (unsigned)(3 * sizeof(float4)));
+  const float4 A = e[0], B = e[1], C = e[2];
+  float p = B.y * w + B.x, d = B.y;
+  d = d * w + p; p = p * w + A.w;
+  d = d * w + p; p = p * w + A.z;
+  d = d * w + p; p = p * w + A.y;
+  d = d * w + p; p = p * w + A.x;
   *h6 = p; *d6 = d;
-  p = C.w * u + C.z; d = C.w;
-  d = d * u + p; p = p * u + C.y;
-  d = d * u + p; p = p * u + C.x;
-  d = d * u + p; p = p * u + B.w;
-  d = d * u + p; p = p * u + B.z;
+  p = C.w * w + C.z; d = C.w;
+  d = d * w + p; p = p * w + C.y;
+  d = d * w + p; p = p * w + C.x;
+  d = d * w + p; p = p * w + B.w;
+  d = d * w + p; p = p * w + B.z;
   *h8 = p; *d8 = d;
 }
 
 // Per species pair constants of the k-space kernel.
 struct KsPair {
-  float R;         // R0_XY, Bohr
+  float Rs;        // R0_XY / dx (R0 in Bohr): x / dx = k Rs
   float p6, p8;    // G = p6 h6(kR) + p8 h8(kR): s6 2pi^2/(3R^3), 3 s8 QQ (-pi^2/R^5)
-  float q6, q8;    // dG/dk = q6 h6_u + q8 h8_u (u-derivatives of the table): p R (2/dx)
+  float q6, q8;    // dG/dk = q6 h6_w + q8 h8_w (w-derivatives of the table): p R / dx
   float pf6, pf8;  // dG/ds6 = pf6 h6, dG/ds8 = pf8 h8
   float dR;        // dR0/da1 = sqrt(3 QQ)
 };
@@ -254,22 +269,22 @@ __host__ __device__ inline KsPair ks_pair(float qq, float s6, float s8, float a1
   const float sq = sqrtf(3.0f * qq);
   const float R = a1 * sq + a2;
   KsPair k;
-  k.R = R;
+  k.Rs = R * kHtabInvDx;
   k.pf6 = float(2.0 * kPi * kPi / 3.0) / (R * R * R);
   k.pf8 = 3.0f * qq * float(-kPi * kPi) / (R * R * R * R * R);
   k.p6 = s6 * k.pf6;
   k.p8 = s8 * k.pf8;
-  k.q6 = k.p6 * R * (2.0f / kHtabDx);
-  k.q8 = k.p8 * R * (2.0f / kHtabDx);
+  k.q6 = k.p6 * R * kHtabInvDx;
+  k.q8 = k.p8 * R * kHtabInvDx;
   k.dR = sq;
   return k;
 }
 
-// G and dG/dk at |k| = k from the table; h6, h8 and their u-derivatives
-// (dh/dx = dh/du * 2/dx) for the parameter derivatives.
+// G and dG/dk at |k| = k from the table; h6, h8 and their w-derivatives
+// (dh/dx = dh/dw / dx) for the parameter derivatives.
 __host__ __device__ inline void green_eval_tab(const KsPair& pk, const float4* __restrict__ tab, float k, float* g,
                                                float* dg, float* h6, float* d6, float* h8, float* d8) {
-  green_h_tab(tab, k * pk.R, h6, d6, h8, d8);
+  green_h_tab(tab, k * pk.Rs, h6, d6, h8, d8);
   *g = pk.p6 * *h6 + pk.p8 * *h8;
   *dg = pk.q6 * *d6 + pk.q8 * *d8;
 }

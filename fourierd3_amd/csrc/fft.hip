@@ -191,16 +191,6 @@ __device__ inline void kspace_line_bases(const KspaceArgs& a, long long* lbase, 
   }
 }
 
-// Tile copies of the nlines x nx lines (LDS line stride nx + 1); line l of
-// the tile starts at spec + lbase[l] (-1: a padded rank, zero-filled).
-__device__ inline void kspace_load(const KspaceArgs& a, float2* S, const long long* lbase, int nlines, int lnx) {
-  tile2_to_lds<kCopyU / 2>(S, a.spec, TileShape{nlines, lnx, 1, a.nx + 1, 1}, LineTable{lbase});
-}
-__device__ inline void kspace_store(const KspaceArgs& a, const float2* S, const long long* lbase, int nlines,
-                                    int lnx) {
-  tile2_from_lds<kCopyU / 2>(a.spec, S, TileShape{nlines, lnx, 1, a.nx + 1, 1}, LineTable{lbase});
-}
-
 // k vector, |k| and 1/sinc^2p of column (kx, ky).
 struct KPoint {
   float kv0, kv1, kv2, kn, isc;
@@ -222,8 +212,8 @@ __device__ inline KPoint kspace_point(const KspaceArgs& a, int kx, int ky, int k
 // Derivative of G with respect to (s6, s8, a1, a2) at one pair, times t.
 __device__ inline void kspace_param_terms(const KsPair& pk, float kn, float h6, float d6, float h8, float d8, float t,
                                           float* p_acc) {
-  const float R = pk.R;
-  const float h6p = d6 * (2.0f / kHtabDx), h8p = d8 * (2.0f / kHtabDx);
+  const float R = pk.Rs * kHtabDx;
+  const float h6p = d6 * kHtabInvDx, h8p = d8 * kHtabInvDx;
   const float dgdR = pk.p6 * (-3.0f * h6 / R + kn * h6p) + pk.p8 * (-5.0f * h8 / R + kn * h8p);
   p_acc[0] += pk.pf6 * h6 * t;
   p_acc[1] += pk.pf8 * h8 * t;
@@ -231,9 +221,11 @@ __device__ inline void kspace_param_terms(const KsPair& pk, float kn, float h6, 
   p_acc[3] += dgdR * t;
 }
 
+// e_acc, d_acc and p_acc are sums over G, not G~: the 1/sinc^2p factor of
+// the k point is applied here.
 __device__ inline void kspace_accumulate(double (&acc)[11], const KPoint& p, float w, float e_acc, float d_acc,
                                          const float* p_acc, bool params) {
-  acc[0] += (double)(w * e_acc);
+  acc[0] += (double)(w * e_acc * p.isc);
   if (p.kn > 0.f) {
     const double s = (double)(w * d_acc * p.isc * __builtin_amdgcn_rcpf(p.kn));
     acc[1] += s * p.kv0 * p.kv0; acc[2] += s * p.kv1 * p.kv1; acc[3] += s * p.kv2 * p.kv2;
@@ -297,12 +289,10 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
   for (int it = 0; it < a.KT; ++it) {
     const int ky0 = (blockIdx.x * a.KT + it) * NK;
-    if (it) __syncthreads();  // the previous tile's store has read S and lbase
+    if (it) __syncthreads();  // the previous tile is done with S and lbase
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
     __syncthreads();
-    kspace_load(a, S, lbase, nlines, lnx);
-    __syncthreads();
-    lds_fft<-1, MAXB>(S, nx, nlines, ls, tws, 1);
+    lds_fft<-1, MAXB>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
@@ -313,7 +303,6 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
       for (int pi = 0; pi < NP; ++pi) {
         float h6, d6, h8, d8;
         green_eval_tab(pcl[pi], htab, kp.kn, &gp[pi], &dgp[pi], &h6, &d6, &h8, &d8);
-        gp[pi] *= kp.isc;
         if (PARAMS) {
           h6v[pi] = h6; d6v[pi] = d6; h8v[pi] = h8; d8v[pi] = d8;
           asm volatile("" : "+v"(h6v[pi]), "+v"(d6v[pi]), "+v"(h8v[pi]), "+v"(d8v[pi]));
@@ -353,26 +342,30 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
           if (PARAMS) kspace_param_terms(pcl[pi], kp.kn, h6v[pi], d6v[pi], h8v[pi], d8v[pi], tre, p_acc);
         }
       }
-      float e_acc = 0.f;
+      // ph holds sum_Y G S_Y; the 1/sinc^2p of G~ goes into f and the accumulation
+      float e_acc = 0.f, f[RT];
+#pragma unroll
+      for (int q = 0; q < RT; ++q) f[q] = lam[q] * a.neg_inv_vol * kp.isc;
 #pragma unroll
       for (int X = 0; X < NS; ++X)
 #pragma unroll
         for (int q = 0; q < RT; ++q) {
           e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
-          const float f = lam[q] * a.neg_inv_vol;
-          float2 v = make_float2(f * ph[X][q].x, f * ph[X][q].y);
-          if (kz == 0 && ky0 + kyi == 0 && kx == 0) {  // k = 0: kept out of the mesh (see KspaceArgs::k0)
-            if (q < rt) a.k0[X * a.r + q0 + q] = v;
-            v = make_float2(0.f, 0.f);
-          }
-          S[((X * RT + q) * NK + kyi) * ls + kx] = v;
+          S[((X * RT + q) * NK + kyi) * ls + kx] = make_float2(f[q] * ph[X][q].x, f[q] * ph[X][q].y);
         }
       kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, PARAMS);
     }
+    if (kz == 0 && ky0 == 0 && threadIdx.x == 0) {  // k = 0 (this thread's column 0): kept out of the mesh
+      for (int l = 0; l < NS * RT; ++l) {
+        float2* sp = &S[l * NK * ls];
+        const int X = l / RT, q = l - X * RT;
+        if (q < rt) a.k0[X * a.r + q0 + q] = *sp;
+        *sp = make_float2(0.f, 0.f);
+      }
+    }
     if (!a.need_grad) continue;
     __syncthreads();
-    lds_fft<1, MAXB>(S, nx, nlines, ls, tws, 1);
-    kspace_store(a, S, lbase, nlines, lnx);
+    lds_fft<1, MAXB>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
   kspace_reduce(acc, red, a, PARAMS);
 }
@@ -404,9 +397,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     if (it) __syncthreads();
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
     __syncthreads();
-    kspace_load(a, S, lbase, nlines, lnx);
-    __syncthreads();
-    lds_fft<-1, kKsGenericMaxB>(S, nx, nlines, ls, tws, 1);
+    lds_fft<-1, kKsGenericMaxB>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
@@ -418,7 +409,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
           const KsPair pk = a.pc[X * nS + Y];
           float g, dg, h6, d6, h8, d8;
           green_eval_tab(pk, htab, kp.kn, &g, &dg, &h6, &d6, &h8, &d8);
-          const float gs = g * kp.isc;
+          const float gs = g;  // G, not G~: 1/sinc^2p is applied to phi and the sums below
           float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
           for (int qi = 0; qi < rt; ++qi) {
             const int ix = ((X * RT + qi) * NK + kyi) * ls + kx;
@@ -441,7 +432,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
           if (params) kspace_param_terms(pk, kp.kn, h6, d6, h8, d8, tre, p_acc);
         }
       }
-      // energy sum_X sum_q lambda_q Re(S conj phi); P <- -(lambda_q / Omega) phi
+      // energy sum_X sum_q lambda_q Re(S conj phi); P <- -(lambda_q / Omega) phi / sinc^2p
       float e_acc = 0.f;
       for (int X = 0; X < nS; ++X) {
         for (int qi = 0; qi < rt; ++qi) {
@@ -449,7 +440,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
           const float lam = a.eigs[q0 + qi];
           const float2 sx = S[ix], ph = P[ix];
           e_acc += lam * (sx.x * ph.x + sx.y * ph.y);
-          const float f = lam * a.neg_inv_vol;
+          const float f = lam * a.neg_inv_vol * kp.isc;
           float2 v = make_float2(f * ph.x, f * ph.y);
           if (kz == 0 && ky0 + kyi == 0 && kx == 0) {  // k = 0: kept out of the mesh (see KspaceArgs::k0)
             a.k0[X * a.r + q0 + qi] = v;
@@ -462,8 +453,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     }
     if (!a.need_grad) continue;
     __syncthreads();
-    lds_fft<1, kKsGenericMaxB>(P, nx, nlines, ls, tws, 1);
-    kspace_store(a, P, lbase, nlines, lnx);
+    lds_fft<1, kKsGenericMaxB>(P, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
   kspace_reduce(acc, red, a, params);
 }

@@ -163,17 +163,55 @@ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
+// Where lds_fft takes its input and puts its output: the LDS lines themselves
+// (LdsLines), or lines in global memory through a line table (GlobalLines:
+// line l at g + base[l]; base[l] < 0 is a dead line, read as zeros and not
+// written).  A global input is read straight into the first stage's
+// butterflies and a global output written straight from the last stage's, so
+// the lines never pass through LDS on their way in or out.
+struct LineIO {
+  float2* g;
+  const long long* base;
+};
+struct LdsLines : LineIO {  // members unused
+  __device__ LdsLines() : LineIO{nullptr, nullptr} {}
+};
+struct GlobalLines : LineIO {
+  __device__ GlobalLines(float2* g_, const long long* base_) : LineIO{g_, base_} {}
+};
+
 // In-place FFT of `nlines` lines of length n (power of two, n <= 4 blockDim.x)
 // held in LDS at buf[l * ls + e].  tw[k * tws] = exp(-2 pi i k / n).  DIR = -1
 // forward (exp(-i...)), +1 inverse (exp(+i...), unnormalised).  Stockham
 // radix-4 stages (+ a final radix-2 stage for odd log2 n).  Thread t owns
 // butterfly column j = t % (n/4) of lines t / (n/4) + c * (blockDim.x / (n/4)),
 // c < MAXB, so its twiddles and offsets are shared by its MAXB butterflies.
-// Requires nlines * n / 4 <= MAXB * blockDim.x.  Ends with __syncthreads().
-template <int DIR, int MAXB>
-__device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __restrict__ tw, int tws) {
-  if (n <= 1) return;
+// Requires nlines * n / 4 <= MAXB * blockDim.x.  With a GlobalLines input the
+// caller need not fill buf; with a GlobalLines output buf is left holding
+// intermediate data.  Ends with __syncthreads().
+template <int DIR, int MAXB, class IN = LdsLines, class OUT = LdsLines>
+__device__ __forceinline__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __restrict__ tw, int tws,
+                                        IN in = {},
+                        OUT out = {}) {
+  constexpr bool kGin = std::is_same<IN, GlobalLines>::value;
+  constexpr bool kGout = std::is_same<OUT, GlobalLines>::value;
   const int log2n = __ffs(n) - 1;
+  if (kGin && log2n < 2) {  // no radix-4 stage to read the input: copy it in
+    for (int e = threadIdx.x; e < (nlines << log2n); e += blockDim.x) {
+      const int l = e >> log2n, i = e & (n - 1);
+      if constexpr (kGin) {
+        const long long b = in.base[l];
+        buf[l * ls + i] = b >= 0 ? in.g[b + i] : make_float2(0.f, 0.f);
+      }
+    }
+    __syncthreads();
+  }
+  if (n <= 1) {
+    if constexpr (kGout)
+      for (int l = threadIdx.x; l < nlines; l += blockDim.x)
+        if (out.base[l] >= 0) out.g[out.base[l]] = buf[l * ls];
+    return;
+  }
   if (log2n >= 2) {
     const int nq = n >> 2, lognq = log2n - 2;
     const int j = threadIdx.x & (nq - 1);
@@ -181,43 +219,88 @@ __device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __
     const int lstep = blockDim.x >> lognq;
     float2* const base = buf + l0 * ls;
     const int dl = lstep * ls;
-    int Ns = 1, lNs = 0;
-    for (int s = 0; s < (log2n >> 1); ++s) {
+    // One radix-4 stage of span Ns = 4^s; GI: read the input from global
+    // (stage 0 only), GO: write the output to global (the last stage, Ns =
+    // n/4, k = j, outputs j + m n/4).  Both are template constants so that
+    // no stage mixes LDS and global stores.
+    auto stage = [&](int Ns, int lNs, auto gi, auto go) __attribute__((always_inline)) {
+      constexpr bool GI = decltype(gi)::value, GO = decltype(go)::value;
       const int k = j & (Ns - 1);
       float2 w1 = make_float2(1.f, 0.f), w2 = w1, w3 = w1;
-      if (s > 0) {  // stage 0 has k = 0 everywhere
+      if (Ns > 1) {  // stage 0 has k = 0 everywhere
         const int t = (k << (lognq - lNs)) * tws;  // k * n / (4 Ns)
         w1 = tw[t]; w2 = tw[2 * t]; w3 = tw[3 * t];
         if (DIR > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
       }
-      const int dst0 = ((j - k) << 2) + k;
-      float2 v0[MAXB], v1[MAXB], v2[MAXB], v3[MAXB];
+      float2 a[MAXB][4];
+      bool live[MAXB];
 #pragma unroll
       for (int c = 0; c < MAXB; ++c) {
-        if (l0 + c * lstep < nlines) {
-          const float2* L = base + c * dl;
-          float2 a0 = L[j], a1 = L[j + nq], a2 = L[j + 2 * nq], a3 = L[j + 3 * nq];
-          if (s > 0) { a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3); }
-          const float2 b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3);
-          const float2 d = csub(a1, a3);
-          // forward: b3 = -i d ; inverse: b3 = +i d
-          const float2 b3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
-          v0[c] = cadd(b0, b2); v2[c] = csub(b0, b2);
-          v1[c] = cadd(b1, b3); v3[c] = csub(b1, b3);
+        const int l = l0 + c * lstep;
+        if constexpr (GI) {
+          // every load issued before the first use; dead lines read line 0
+          const long long b = l < nlines ? in.base[l] : -1;
+          live[c] = b >= 0;
+          const float2* src = in.g + (live[c] ? b : 0) + j;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) a[c][m] = src[m * nq];
+        } else {
+          live[c] = l < nlines;
+          const float2* L = live[c] ? base + c * dl + j : buf;  // dead: read line 0
+#pragma unroll
+          for (int m = 0; m < 4; ++m) a[c][m] = L[m * nq];
+        }
+      }
+      float2 v[MAXB][4];
+#pragma unroll
+      for (int c = 0; c < MAXB; ++c) {
+        float2 a0 = a[c][0], a1 = a[c][1], a2 = a[c][2], a3 = a[c][3];
+        if (GI && !live[c]) a0 = a1 = a2 = a3 = make_float2(0.f, 0.f);
+        if (Ns > 1) { a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3); }
+        const float2 b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3);
+        const float2 d = csub(a1, a3);
+        // forward: b3 = -i d ; inverse: b3 = +i d
+        const float2 b3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+        v[c][0] = cadd(b0, b2); v[c][2] = csub(b0, b2);
+        v[c][1] = cadd(b1, b3); v[c][3] = csub(b1, b3);
+      }
+      if constexpr (GO) {
+#pragma unroll
+        for (int c = 0; c < MAXB; ++c) {
+          const int l = l0 + c * lstep;
+          const long long b = l < nlines ? out.base[l] : -1;
+          if (b >= 0) {
+            float2* dst = out.g + b + j;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) dst[m * nq] = v[c][m];
+          }
+        }
+      } else {
+        if (!GI) __syncthreads();  // every LDS read of this stage is done
+        const int dst0 = ((j - k) << 2) + k;
+#pragma unroll
+        for (int c = 0; c < MAXB; ++c) {
+          if (l0 + c * lstep < nlines) {
+            float2* D = base + c * dl + dst0;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) D[m * Ns] = v[c][m];
+          }
         }
       }
       __syncthreads();
-#pragma unroll
-      for (int c = 0; c < MAXB; ++c) {
-        if (l0 + c * lstep < nlines) {
-          float2* D = base + c * dl + dst0;
-          D[0] = v0[c]; D[Ns] = v1[c]; D[2 * Ns] = v2[c]; D[3 * Ns] = v3[c];
-        }
-      }
-      __syncthreads();
-      Ns <<= 2;
-      lNs += 2;
-    }
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    const int S4 = log2n >> 1;
+    const bool go_last = kGout && !(log2n & 1);  // the last radix-4 stage writes global
+    const int nmid_end = go_last ? S4 - 1 : S4;  // stages [1, nmid_end) are LDS to LDS
+    if (S4 == 1 && go_last)
+      stage(1, 0, std::integral_constant<bool, kGin>{}, T{});
+    else
+      stage(1, 0, std::integral_constant<bool, kGin>{}, F{});
+    int Ns = 4, lNs = 2;
+    for (int s = 1; s < nmid_end; ++s, Ns <<= 2, lNs += 2) stage(Ns, lNs, F{}, F{});
+    if (go_last && S4 > 1) stage(Ns, lNs, F{}, T{});
   }
   if (log2n & 1) {  // final radix-2 stage, Ns = n/2: in place per butterfly
     const int nh = n >> 1;
@@ -228,8 +311,16 @@ __device__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __
       float2 w = tw[j * tws];
       if (DIR > 0) w.y = -w.y;
       const float2 a0 = L[j], a1 = cmul(L[j + nh], w);
-      L[j] = cadd(a0, a1);
-      L[j + nh] = csub(a0, a1);
+      if constexpr (kGout) {
+        const long long gb = out.base[line];
+        if (gb >= 0) {
+          out.g[gb + j] = cadd(a0, a1);
+          out.g[gb + j + nh] = csub(a0, a1);
+        }
+      } else {
+        L[j] = cadd(a0, a1);
+        L[j + nh] = csub(a0, a1);
+      }
     }
     __syncthreads();
   }

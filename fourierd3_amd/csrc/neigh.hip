@@ -242,19 +242,26 @@ __global__ void k_fill_sorted(const int32_t* __restrict__ slot_atom, int64_t n, 
 
 // ----------------------------------------------------------- CN kernels ---
 // One block per home bin.  The images of the atoms in the neighbouring bins
-// (candidates) are staged in LDS with their lattice shift applied; 8 lanes
-// share each home atom and stride over the candidates, so every candidate is
-// read from HBM once per home bin instead of once per home atom.
+// (candidates) are staged in LDS with their lattice shift applied, so every
+// candidate is read from HBM once per home bin instead of once per home atom.
+// Each wave then takes home atoms in turn: its 64 lanes test |r| < cutoff
+// for up to 512 candidates (a few instructions each), compact the hits into a
+// list in LDS with a ballot, and evaluate the CN term on that list with all
+// lanes busy; the home atom's sums stay in registers until the wave moves to
+// the next atom.  (Evaluating the term inside the distance loop would run it
+// for almost every candidate: one of 64 lanes nearly always has a hit.)
 // Forward:  CN_s = sum_t f(r_st)
 // Backward: dE/dr_s -= sum_t (g_s + g_t) f'(r) v / r,
 //           virial  += sum_t g_s f'(r) v v^T / r        (directed pairs s -> t)
 // with g = dE/dCN (src/fourierd3/core.py:348-386 and its adjoint).
-constexpr int kCnCap = 1024;  // candidates staged per pass
-constexpr int kNbrMax = 343;  // neighbour bins of a single-pass plan (reach <= 3)
-constexpr int kCnLanes = 8;   // lanes per home atom
+constexpr int kCnCap = 1024;   // candidates staged per pass
+constexpr int kNbrMax = 343;   // neighbour bins of a single-pass plan (reach <= 3)
+constexpr int kCnHomes = 128;  // home atoms per pass
+constexpr int kCnStep = 8;     // ballot steps (of 64 candidates) per hit list
 
 struct NbrBin {
   int wb;            // wrapped bin index
+  int S[3];          // lattice shift in cells
   double sh[3];      // lattice shift vector (Bohr)
   bool zero;         // shift is zero
 };
@@ -267,17 +274,34 @@ __device__ inline NbrBin nbr_bin(const CellArgs& c, int b, int k) {
   const int Sx = floordiv(ux, c.nb[0]), Sy = floordiv(uy, c.nb[1]), Sz = floordiv(uz, c.nb[2]);
   NbrBin r;
   r.wb = (wrap(ux, c.nb[0]) * c.nb[1] + wrap(uy, c.nb[1])) * c.nb[2] + wrap(uz, c.nb[2]);
+  r.S[0] = Sx; r.S[1] = Sy; r.S[2] = Sz;
   for (int d = 0; d < 3; ++d) r.sh[d] = Sx * c.cell[d] + Sy * c.cell[3 + d] + Sz * c.cell[6 + d];
   r.zero = (Sx | Sy | Sz) == 0;
   return r;
 }
 
+// Lattice shift of a neighbour bin, 8 bits per axis (|S| <= reach <= 3 in a
+// single-pass plan).
+__device__ inline int pack_shift(const int* S) {
+  return ((S[0] + 128) << 16) | ((S[1] + 128) << 8) | (S[2] + 128);
+}
+
+__device__ inline float wave_sum_f(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 template <bool BWD>
 __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
-  __shared__ float4 cpos[kCnCap];   // image position relative to the bin's first atom, rcov
-  __shared__ int cmeta[kCnCap];     // 2 * slot + (shift == 0)
-  __shared__ float cg[kCnCap];      // dE/dCN of the candidate (backward)
-  __shared__ int bcnt[kNbrMax + 1];
+  constexpr int NW = 4;                   // waves per block
+  __shared__ float4 cpos[kCnCap];         // image position relative to the bin's first atom, rcov
+  __shared__ float cg[BWD ? kCnCap : 1];  // dE/dCN of the candidate (backward)
+  __shared__ float hacc[kCnHomes][BWD ? 3 : 1];  // per home atom: CN, or dE/dr
+  __shared__ int bcnt[kNbrMax + 1];       // candidate offset of neighbour bin k
+  __shared__ int nb0[kNbrMax];            // first slot of neighbour bin k
+  __shared__ int nsh[kNbrMax];            // packed lattice shift of neighbour bin k
+  __shared__ unsigned short wl[NW][kCnStep * 64];  // per wave: hit candidates
   __shared__ double red[16 * 6];
   const int b = blockIdx.x;
   const int hs = c.bstart[b], he = c.bstart[b + 1];
@@ -285,6 +309,7 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
     if (BWD && threadIdx.x < 8) a.partials[(size_t)blockIdx.x * 8 + threadIdx.x] = 0.0;
     return;
   }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nbr = (2 * c.sr[0] + 1) * (2 * c.sr[1] + 1) * (2 * c.sr[2] + 1);
   // origin of the block's float coordinates: differences stay within a few
   // bin widths, so float keeps ~1e-7 relative precision on pair vectors
@@ -292,8 +317,11 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
   bool single = false;
   if (nbr <= kNbrMax) {
     for (int k = threadIdx.x; k < nbr; k += blockDim.x) {
-      const int wb = nbr_bin(c, b, k).wb;
-      bcnt[k + 1] = c.bstart[wb + 1] - c.bstart[wb];
+      const NbrBin nb = nbr_bin(c, b, k);
+      const int lo = c.bstart[nb.wb];
+      nb0[k] = lo;
+      nsh[k] = pack_shift(nb.S);
+      bcnt[k + 1] = c.bstart[nb.wb + 1] - lo;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -303,89 +331,111 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
     __syncthreads();
     single = bcnt[nbr] <= kCnCap;
   }
+  const float rc2 = (float)c.rc2;
   float vir[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  // passes: the whole candidate list at once, or one neighbour bin (chunk) at a time
-  const int npass_bins = single ? 1 : nbr;
-  for (int kb = 0; kb < npass_bins; ++kb) {
-    int count, off0 = 0;
-    NbrBin fixed;
-    if (single) {
-      count = bcnt[nbr];
-    } else {
-      fixed = nbr_bin(c, b, kb);
-      count = c.bstart[fixed.wb + 1] - c.bstart[fixed.wb];
-    }
-    for (off0 = 0; off0 < count; off0 += kCnCap) {
-      const int nc = min(kCnCap, count - off0);
-      __syncthreads();
-      for (int e = threadIdx.x; e < nc; e += blockDim.x) {
-        NbrBin nbx;
-        int t;
-        if (single) {
-          int k = 0;
-          while (e >= bcnt[k + 1]) ++k;
-          nbx = nbr_bin(c, b, k);
-          t = c.bstart[nbx.wb] + (e - bcnt[k]);
-        } else {
-          nbx = fixed;
-          t = c.bstart[nbx.wb] + off0 + e;
-        }
-        const double4 p = c.pos4[t];
-        cpos[e] = make_float4((float)(p.x + nbx.sh[0] - org.x), (float)(p.y + nbx.sh[1] - org.y),
-                              (float)(p.z + nbx.sh[2] - org.z), a.rcov[c.spec_s[t]]);
-        cmeta[e] = 2 * t + (nbx.zero ? 1 : 0);
-        if (BWD) cg[e] = a.dEdcn[t];
+  for (int h0 = hs; h0 < he; h0 += kCnHomes) {
+    const int nh = min(kCnHomes, he - h0);
+    for (int h = threadIdx.x; h < nh; h += blockDim.x)
+      for (int d = 0; d < (BWD ? 3 : 1); ++d) hacc[h][d] = 0.f;
+    // candidate passes: the whole list at once, or one neighbour bin (chunk) at a time
+    const int npass_bins = single ? 1 : nbr;
+    for (int kb = 0; kb < npass_bins; ++kb) {
+      int count;
+      NbrBin fixed;
+      if (single) {
+        count = bcnt[nbr];
+      } else {
+        fixed = nbr_bin(c, b, kb);
+        count = c.bstart[fixed.wb + 1] - c.bstart[fixed.wb];
       }
-      __syncthreads();
-      for (int h0 = hs; h0 < he; h0 += 256 / kCnLanes) {
-        const int s = h0 + threadIdx.x / kCnLanes, lane = threadIdx.x % kCnLanes;
-        float acc = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
-        if (s < he) {
-          const double4 pd = c.pos4[s];
-          const float px = (float)(pd.x - org.x), py = (float)(pd.y - org.y), pz = (float)(pd.z - org.z);
-          const float ri = a.rcov[c.spec_s[s]];
-          const float gs = BWD ? a.dEdcn[s] : 0.f;
-          const float rc2 = (float)c.rc2;
-          for (int e = lane; e < nc; e += kCnLanes) {
-            const int m = cmeta[e];
-            if ((m & 1) && (m >> 1) == s) continue;  // the atom itself at zero shift
-            const float4 q = cpos[e];
-            const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
-            const float r2 = vx * vx + vy * vy + vz * vz;
-            if (!(r2 < rc2)) continue;
-            float f, fp;
-            const float r = sqrtf(r2);
-            cn_term(r, ri + q.w, a.rcut, &f, &fp);
-            if (!BWD) {
-              acc += f;
-            } else {
-              const float w = fp * __frcp_rn(r);
-              const float wt = w * (gs + cg[e]);
-              fx -= wt * vx; fy -= wt * vy; fz -= wt * vz;
-              const float wv = w * gs;
-              vir[0] += wv * vx * vx; vir[1] += wv * vy * vy; vir[2] += wv * vz * vz;
-              vir[3] += wv * vx * vy; vir[4] += wv * vx * vz; vir[5] += wv * vy * vz;
+      for (int off0 = 0; off0 < count; off0 += kCnCap) {
+        const int nc = min(kCnCap, count - off0);
+        __syncthreads();  // hacc cleared; the previous pass is done with cpos
+        for (int e = threadIdx.x; e < nc; e += blockDim.x) {
+          double sh[3];
+          int t;
+          if (single) {
+            int lo = 0, hi = nbr - 1;  // the neighbour bin k with bcnt[k] <= e < bcnt[k + 1]
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (bcnt[mid] <= e) lo = mid; else hi = mid - 1;
             }
-          }
-        }
-#pragma unroll
-        for (int o = kCnLanes / 2; o > 0; o >>= 1) {
-          acc += __shfl_xor(acc, o, kCnLanes);
-          fx += __shfl_xor(fx, o, kCnLanes);
-          fy += __shfl_xor(fy, o, kCnLanes);
-          fz += __shfl_xor(fz, o, kCnLanes);
-        }
-        if (s < he && lane == 0) {
-          if (!BWD) {
-            atomicAdd(a.cn + s, acc);
+            const int ps = nsh[lo];
+            const int Sx = (ps >> 16) - 128, Sy = ((ps >> 8) & 255) - 128, Sz = (ps & 255) - 128;
+            for (int d = 0; d < 3; ++d) sh[d] = Sx * c.cell[d] + Sy * c.cell[3 + d] + Sz * c.cell[6 + d];
+            t = nb0[lo] + (e - bcnt[lo]);
           } else {
-            atomicAdd(a.gpos + 3 * s + 0, (double)fx);
-            atomicAdd(a.gpos + 3 * s + 1, (double)fy);
-            atomicAdd(a.gpos + 3 * s + 2, (double)fz);
+            for (int d = 0; d < 3; ++d) sh[d] = fixed.sh[d];
+            t = c.bstart[fixed.wb] + off0 + e;
+          }
+          const double4 p = c.pos4[t];
+          cpos[e] = make_float4((float)(p.x + sh[0] - org.x), (float)(p.y + sh[1] - org.y),
+                                (float)(p.z + sh[2] - org.z), a.rcov[c.spec_s[t]]);
+          if (BWD) cg[e] = a.dEdcn[t];
+        }
+        __syncthreads();
+        for (int h = wv; h < nh; h += NW) {  // home atom h0 + h: wave-uniform
+          const int sidx = h0 + h;
+          const double4 pd = c.pos4[sidx];
+          const float px = (float)(pd.x - org.x), py = (float)(pd.y - org.y), pz = (float)(pd.z - org.z);
+          const float ri = a.rcov[c.spec_s[sidx]];
+          const float gs = BWD ? a.dEdcn[sidx] : 0.f;
+          float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+          for (int e0 = 0; e0 < nc; e0 += kCnStep * 64) {
+            // in-range candidates of [e0, e0 + 512) -> wl[wv][0 .. cnt)
+            int cnt = 0;
+#pragma unroll
+            for (int it = 0; it < kCnStep; ++it) {
+              const int e = e0 + it * 64 + lane;
+              bool hit = false;
+              if (e < nc) {
+                const float4 q = cpos[e];
+                const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
+                hit = vx * vx + vy * vy + vz * vz < rc2;  // includes the atom itself at r = 0
+              }
+              const unsigned long long bal = __ballot(hit);
+              if (hit) wl[wv][cnt + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)e;
+              cnt += __popcll(bal);
+            }
+            __builtin_amdgcn_wave_barrier();  // the list is written and read by lanes of one wave
+            for (int k = lane; k < cnt; k += 64) {
+              const int e = wl[wv][k];
+              const float4 q = cpos[e];
+              const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
+              const float r = sqrtf(vx * vx + vy * vy + vz * vz);
+              float f, fp;
+              cn_term(r, ri + q.w, a.rcut, &f, &fp);  // 0, 0 at r = 0
+              if (!BWD) {
+                s0 += f;
+              } else {
+                const float w = r > 1e-6f ? fp * __frcp_rn(r) : 0.f;
+                const float wt = w * (gs + cg[e]);
+                s0 -= wt * vx; s1 -= wt * vy; s2 -= wt * vz;
+                const float wvv = w * gs;
+                vir[0] += wvv * vx * vx; vir[1] += wvv * vy * vy; vir[2] += wvv * vz * vz;
+                vir[3] += wvv * vx * vy; vir[4] += wvv * vx * vz; vir[5] += wvv * vy * vz;
+              }
+            }
+            __builtin_amdgcn_wave_barrier();  // the next step overwrites the list
+          }
+          s0 = wave_sum_f(s0);
+          if (BWD) { s1 = wave_sum_f(s1); s2 = wave_sum_f(s2); }
+          if (lane == 0) {  // one wave per home atom and pass: plain updates
+            hacc[h][0] += s0;
+            if (BWD) { hacc[h][1] += s1; hacc[h][2] += s2; }
           }
         }
       }
     }
+    __syncthreads();
+    for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+      if (!BWD) {
+        a.cn[h0 + h] = hacc[h][0];  // each atom is the home of exactly one block
+      } else {
+        for (int d = 0; d < 3; ++d) atomicAdd(a.gpos + 3 * (h0 + h) + d, (double)hacc[h][BWD ? d : 0]);
+      }
+    }
+    __syncthreads();  // the next home pass clears hacc
   }
   if (BWD) {
     double v[6];
@@ -622,8 +672,6 @@ int cn_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, 256)); }
 int launch_cn_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   if (c.n <= 0) return FD3_OK;
   ProfScope _ps("k_cn_cells", s);
-  int rc;
-  if ((rc = launch_fill_f32(a.cn, c.n, s))) return rc;
   hipLaunchKernelGGL(k_cn_bins<false>, dim3(nbins), dim3(256), 0, s, c, a);
   FD3_LAUNCH_CHECK("k_cn_cells");
   return FD3_OK;

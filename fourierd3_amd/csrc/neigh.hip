@@ -21,31 +21,35 @@
 
 namespace fd3 {
 
-// Smooth-cutoff CN term of one pair and its derivative wrt r (Bohr):
+// Smooth-cutoff CN term of one pair at squared distance r2 (Bohr^2) and its
+// derivative wrt r divided by r:
 //   r1 = (Rcov_ij + r_cut)/2, r' = max(r, r1)
 //   k  = 16 + (r' - r1)^2 / ((r_cut - r')^2 + 1e-6)
 //   f  = sigmoid(k (4/3 Rcov_ij / r - 1)),  f = 0 for r <= 1e-6
-__device__ inline void cn_term(float r, float rcs, float rcut, float* f, float* fp) {
-  if (!(r > 1e-6f)) {
+// Hardware rsq/rcp/exp (about 1 ulp): the sums over ~100 pairs carry float
+// rounding at that level anyway.
+__device__ inline void cn_term(float r2, float rcs, float rcut, float* f, float* fpr) {
+  if (!(r2 > 1e-12f)) {
     *f = 0.f;
-    *fp = 0.f;
+    *fpr = 0.f;
     return;
   }
+  const float ir = __builtin_amdgcn_rsqf(r2);
+  const float r = r2 * ir;
   const float r1 = 0.5f * rcs + 0.5f * rcut;
   const bool far = r > r1;
   const float rc = far ? r : r1;
   const float dr = rc - r1, dc = rcut - rc;
-  const float iden = __frcp_rn(dc * dc + 1e-6f);
+  const float iden = __builtin_amdgcn_rcpf(dc * dc + 1e-6f);
   const float k = 16.0f + dr * dr * iden;
   const float ratio = (4.0f / 3.0f) * rcs;
-  const float ir = __frcp_rn(r);
   const float arg = ratio * ir - 1.0f;
   const float s = k * arg;
-  const float sig = __frcp_rn(1.0f + __expf(-s));
+  const float sig = __builtin_amdgcn_rcpf(1.0f + __expf(-s));
   *f = sig;
   const float dk = far ? 2.0f * dr * iden * (1.0f + dr * dc * iden) : 0.0f;
   const float ds = dk * arg - k * ratio * ir * ir;
-  *fp = sig * (1.0f - sig) * ds;
+  *fpr = sig * (1.0f - sig) * ds * ir;
 }
 
 __device__ inline int floordiv(int a, int b) {
@@ -402,13 +406,11 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
               const int e = wl[wv][k];
               const float4 q = cpos[e];
               const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
-              const float r = sqrtf(vx * vx + vy * vy + vz * vz);
-              float f, fp;
-              cn_term(r, ri + q.w, a.rcut, &f, &fp);  // 0, 0 at r = 0
+              float f, w;
+              cn_term(vx * vx + vy * vy + vz * vz, ri + q.w, a.rcut, &f, &w);  // 0, 0 at r = 0
               if (!BWD) {
                 s0 += f;
               } else {
-                const float w = r > 1e-6f ? fp * __frcp_rn(r) : 0.f;
                 const float wt = w * (gs + cg[e]);
                 s0 -= wt * vx; s1 -= wt * vy; s2 -= wt * vz;
                 const float wvv = w * gs;
@@ -461,10 +463,10 @@ __global__ void __launch_bounds__(256) k_cn_edges(EdgeArgs e, CnArgs a, const do
   if (k >= e.n_edges) return;
   double v[3];
   edge_vec(e, k, pos4, v);
-  const float r = (float)sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const float r2 = (float)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
   const int64_t i = e.src[k], j = e.dst[k];
-  float f, fp;
-  cn_term(r, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fp);
+  float f, fpr;
+  cn_term(r2, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fpr);
   if (f != 0.f) atomicAdd(a.cn + i, f);
 }
 
@@ -476,11 +478,11 @@ __global__ void __launch_bounds__(256) k_cn_bwd_edges(EdgeArgs e, CnArgs a, cons
   if (k < e.n_edges) {
     double v[3];
     edge_vec(e, k, pos4, v);
-    const float r = (float)sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const float r2 = (float)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     const int64_t i = e.src[k], j = e.dst[k];
-    float f, fp;
-    cn_term(r, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fp);
-    const double w = r > 0.f ? (double)(a.dEdcn[i] * fp / r) : 0.0;
+    float f, fpr;
+    cn_term(r2, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fpr);
+    const double w = (double)(a.dEdcn[i] * fpr);
     const double g0 = w * v[0], g1 = w * v[1], g2 = w * v[2];  // dE/dv, Hartree/Bohr
     if (w != 0.0) {
       atomicAdd(a.gpos + 3 * j + 0, g0);

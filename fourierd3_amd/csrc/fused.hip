@@ -159,6 +159,28 @@ __device__ inline float pick(const float* w, int j, int p) {
 }
 
 // ------------------------------------------------------ spread + z pass ---
+// line[z[j]] += v[j], j < p.  Float atomic adds to LDS run at about a third
+// of a lane per clock per CU on gfx950, integer compare-and-swap at the rate
+// of plain LDS traffic, so the adds are compare-and-swaps of the bit
+// patterns, repeated for the lanes whose value another lane changed in
+// between (the same results as atomic adds, in some order).
+template <int P>
+__device__ __forceinline__ void lds_add_f32_cas(float* line, const int* z, const float* v, int p) {
+  float cur[kMaxOrder];
+  for (int j = 0; j < p; ++j) cur[j] = line[z[j]];
+  unsigned pend = (1u << p) - 1u;
+  while (pend) {
+    for (int j = 0; j < p; ++j) {
+      if ((pend >> j) & 1u) {
+        const unsigned as = __float_as_uint(cur[j]);
+        const unsigned old = atomicCAS((unsigned*)(line + z[j]), as, __float_as_uint(cur[j] + v[j]));
+        if (old == as) pend &= ~(1u << j);
+        else cur[j] = __uint_as_float(old);
+      }
+    }
+  }
+}
+
 // fused z passes: 512-thread blocks for the spread (measured faster), 256 for
 // the gather (its 100 VGPRs leave no room for more waves)
 constexpr int kZLogSpread = 9, kZLogGather = 9;
@@ -198,11 +220,15 @@ __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* 
       if (t < 0 || t >= TY) continue;
       float* line = ldsf + 2 * t * ls;
       const float v = wx * wy[jy];
+      int zz[kMaxOrder];
+      float add[kMaxOrder];
       for (int jz = 0; jz < p; ++jz) {
         int z = bi.z + jz;
         if (z >= nz) z -= nz;
-        atomicAdd(line + z, v * wz[jz]);
+        zz[jz] = z;
+        add[jz] = v * wz[jz];
       }
+      lds_add_f32_cas<P>(line, zz, add, p);
     }
   }
   __syncthreads();

@@ -340,7 +340,7 @@ static void carve_common(Arena& ar, Common& c, int64_t n, int r, int cn_rows) {
   c.k0 = ar.take<float2>(c.plan.C);
   c.ew_perm = ar.take<int32_t>(std::max<int64_t>(n, 1));
   c.ew_seg = ar.take<int32_t>(c.plan.nS + 1);
-  col_carve(c.plan, n, ar, &c.cols);
+  col_carve(c.plan, n, c.geom.order, ar, &c.cols);
   c.cn_raw = ar.take<float>(std::max<int64_t>(n, 1));
   c.charges = ar.take<float>(std::max<int64_t>(n * r, 1));
   c.dEdcn = ar.take<float>(std::max<int64_t>(n, 1));

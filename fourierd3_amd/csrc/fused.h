@@ -21,9 +21,10 @@ struct ColBuffers {
   float* scharge;     // (n, r) column order
   float* accA;        // (n, r) slot order: sum_nodes W dE/drho
   float* accG;        // (n, 3) slot order: sum_q c_q sum_nodes dW/du dE/drho
+  float4* part;       // (n, order, 2, r) column order: the gather's per-block sums (accA_q, accG_q)
 };
 
-void col_carve(const FftPlan& p, int64_t n, Arena& ar, ColBuffers* b);
+void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b);
 
 struct FusedArgs {
   int64_t n;
@@ -40,7 +41,7 @@ struct FusedArgs {
 int col_sort(const FusedArgs& a, hipStream_t s);
 // spreading + forward z pass: A[c][x][kz][y]
 int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s);
-// inverse z pass + gathering into cb.accA / cb.accG (zeroed here)
+// inverse z pass + gathering into cb.accA / cb.accG
 int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const float2* A, hipStream_t s);
 
 struct C6BwdArgs {

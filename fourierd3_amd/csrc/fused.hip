@@ -8,14 +8,15 @@
 // node 0, y node 0), so the contributing atoms are the columns with x node 0
 // in {x - p + 1 .. x} and y node 0 in {y0 - p + 1 .. y0 + TY - 1}: at most two
 // contiguous runs per x node.  Each atom adds c_q Wx Wy Wz into the LDS lines
-// with LDS atomics; the block then runs the in-LDS z FFT and writes the half
+// (compare-and-swap adds); the block then runs the in-LDS z FFT and writes the half
 // spectrum, so the real mesh is never written to or read from HBM.
 //
 // The inverse z pass mirrors it: after the C2R transform the block holds the
 // real lines of dE/drho in LDS and the same atoms accumulate
 //   sum_nodes W dE/drho            (-> dE/dc, rank q)
 //   c_q sum_nodes dW/du_d dE/drho  (-> dE/du)
-// over the nodes in the block, added atomically to per-atom accumulators.
+// over the nodes in the block, stored as the atom's partial sums for this
+// (x node, y tile, rank) and summed per atom by k_gather_sum in a fixed order.
 // (points_to_mesh / mesh_to_points: src/fourierd3/mesh_interpolator_d3.py:99-189.)
 
 #include "fd3_internal.h"
@@ -26,7 +27,7 @@
 
 namespace fd3 {
 
-void col_carve(const FftPlan& p, int64_t n, Arena& ar, ColBuffers* b) {
+void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b) {
   const size_t ncol = (size_t)p.nS * p.nx * p.ny;
   const int64_t nn = std::max<int64_t>(n, 1);
   b->key = ar.take<int32_t>(nn);
@@ -41,6 +42,7 @@ void col_carve(const FftPlan& p, int64_t n, Arena& ar, ColBuffers* b) {
   b->scharge = ar.take<float>(nn * p.r);
   b->accA = ar.take<float>(nn * p.r);
   b->accG = ar.take<float>(nn * 3);
+  b->part = ar.take<float4>((size_t)nn * std::max(order, 1) * 2 * p.r);
 }
 
 // ------------------------------------------------------------ column sort ---
@@ -294,11 +296,13 @@ __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const fl
     }
     const float cq = a.cb.scharge[(size_t)ai * a.r + q];
     g0 = dwx * acc;
-    const int s = bi.w;
-    atomicAdd(a.cb.accA + (size_t)s * a.r + q, wx * acc);
-    atomicAdd(a.cb.accG + 3 * (size_t)s + 0, cq * g0);
-    atomicAdd(a.cb.accG + 3 * (size_t)s + 1, cq * wx * g1);
-    atomicAdd(a.cb.accG + 3 * (size_t)s + 2, cq * wx * g2);
+    // slot h = 0: the tile holding the atom's first y row, h = 1: the next
+    // tile (rows that run past the first tile); the h = 0 block zeroes the
+    // h = 1 slot when the footprint stays in its tile
+    const bool first = bi.y >= y0 && bi.y < y0 + TY;
+    float4* pp = a.cb.part + (((size_t)ai * p + sg.jx[k]) * 2 + (first ? 0 : 1)) * a.r + q;
+    *pp = make_float4(wx * acc, cq * g0, cq * wx * g1, cq * wx * g2);
+    if (first && (TY == a.ny || bi.y - y0 + p - 1 < TY)) pp[a.r] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -312,16 +316,39 @@ int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float
   return FD3_OK;
 }
 
+// accA / accG of each atom from the gather's partial sums: p x nodes, two
+// y slots, r channels (column order in, slot order out).  A fixed order of
+// additions, no atomics.
+__global__ void __launch_bounds__(256) k_gather_sum(FusedArgs a) {
+  const int64_t ai = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ai >= a.n) return;
+  const int s = a.cb.rec_i[ai].w, r = a.r, nk = 2 * a.order;
+  const float4* pp = a.cb.part + (size_t)ai * nk * r;
+  float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+  for (int q = 0; q < r; ++q) {
+    float A = 0.f;
+    for (int k = 0; k < nk; ++k) {
+      const float4 v = pp[k * r + q];
+      A += v.x; g0 += v.y; g1 += v.z; g2 += v.w;
+    }
+    a.cb.accA[(size_t)s * r + q] = A;
+  }
+  a.cb.accG[3 * (size_t)s + 0] = g0;
+  a.cb.accG[3 * (size_t)s + 1] = g1;
+  a.cb.accG[3 * (size_t)s + 2] = g2;
+}
+
 int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const float2* A, hipStream_t s) {
-  int rc;
-  if ((rc = launch_fill_f32(a.cb.accA, std::max<int64_t>(a.n, 1) * a.r, s))) return rc;
-  if ((rc = launch_fill_f32(a.cb.accG, std::max<int64_t>(a.n, 1) * 3, s))) return rc;
   const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_c2r_z_gather", s);
   const dim3 grid(p.ny / p.TY, p.nx, p.C), block(1 << kZLogGather);
   if (a.order == 5) hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 5>), grid, block, sh, s, a, A, p.TY, t.twz);
   else hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 0>), grid, block, sh, s, a, A, p.TY, t.twz);
   FD3_LAUNCH_CHECK("k_c2r_z_gather");
+  if (a.n > 0) {
+    hipLaunchKernelGGL(k_gather_sum, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
+    FD3_LAUNCH_CHECK("k_gather_sum");
+  }
   return FD3_OK;
 }
 

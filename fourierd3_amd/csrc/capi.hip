@@ -224,42 +224,64 @@ __device__ inline double block_sum(double v, double* sh) {
   return r;
 }
 
-// First stage of the partial-sum reduction: rows [r0, r1) of a (rows, stride)
-// double array, columns 0..ncols-1, summed per block into out[block][col].
-// 256 threads = 16 row lanes x 16 columns; fixed order, so deterministic.
-__global__ void __launch_bounds__(256) k_reduce_rows(const double* __restrict__ src, int rows, int stride, int ncols,
-                                                     int rows_per_block, double* __restrict__ out) {
+// First stage of the partial-sum reduction, for up to four (rows, stride)
+// double arrays in one launch: job j owns blocks [b0[j], b0[j+1]); its block
+// lb sums rows [lb rpb, (lb+1) rpb), columns 0..ncols-1, into
+// dst[lb * stride + col].  256 threads = 16 row lanes x 16 columns; fixed
+// order, so deterministic.
+struct ReduceJobs {
+  const double* src[4];
+  double* dst[4];
+  int rows[4], stride[4], ncols[4], rpb[4];
+  int b0[5];
+  int n;
+};
+
+__global__ void __launch_bounds__(256) k_reduce_rows(ReduceJobs jb) {
   __shared__ double sh[16][17];
+  int j = 0;
+  while (j + 1 < jb.n && (int)blockIdx.x >= jb.b0[j + 1]) ++j;
+  const int lb = blockIdx.x - jb.b0[j];
+  const int stride = jb.stride[j], ncols = jb.ncols[j];
   const int col = threadIdx.x & 15, lane = threadIdx.x >> 4;
-  const int r0 = blockIdx.x * rows_per_block, r1 = min(rows, r0 + rows_per_block);
+  const int r0 = lb * jb.rpb[j], r1 = min(jb.rows[j], r0 + jb.rpb[j]);
   double s = 0.0;
   if (col < ncols)
-    for (int r = r0 + lane; r < r1; r += 16) s += src[(size_t)r * stride + col];
+    for (int r = r0 + lane; r < r1; r += 16) s += jb.src[j][(size_t)r * stride + col];
   sh[lane][col] = s;
   __syncthreads();
   if (threadIdx.x < 16) {
     double t = 0.0;
     for (int l = 0; l < 16; ++l) t += sh[l][threadIdx.x];
-    if (threadIdx.x < ncols) out[(size_t)blockIdx.x * stride + threadIdx.x] = t;
+    if (threadIdx.x < ncols) jb.dst[j][(size_t)lb * stride + threadIdx.x] = t;
   }
 }
 
-// Reduce (rows, stride) partials to at most 256 rows written to dst (room for
-// 256 * stride doubles); *src and *rows are redirected to the result.
-static int reduce_partials(const double** src, int* rows, int stride, int ncols, double* dst, hipStream_t s) {
-  if (*rows <= 256) return FD3_OK;
+// Queue the reduction of (rows, stride) partials to at most 256 rows written
+// to dst (room for 256 * stride doubles); *src and *rows are redirected to
+// the result.  Partials with <= 256 rows are left as they are.
+static void reduce_partials_add(ReduceJobs& jb, const double** src, int* rows, int stride, int ncols, double* dst) {
+  if (*rows <= 256) return;
   const int rpb = (*rows + 255) / 256;
   const int nb = (*rows + rpb - 1) / rpb;
-  ProfScope _ps("k_reduce_rows", s);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(nb), dim3(256), 0, s, *src, *rows, stride, ncols, rpb, dst);
-  FD3_LAUNCH_CHECK("k_reduce_rows");
+  const int j = jb.n++;
+  jb.src[j] = *src; jb.dst[j] = dst;
+  jb.rows[j] = *rows; jb.stride[j] = stride; jb.ncols[j] = ncols; jb.rpb[j] = rpb;
+  jb.b0[j + 1] = jb.b0[j] + nb;
   *src = dst;
   *rows = nb;
+}
+
+static int reduce_partials_run(const ReduceJobs& jb, hipStream_t s) {
+  if (jb.n == 0) return FD3_OK;
+  ProfScope _ps("k_reduce_rows", s);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(jb.b0[jb.n]), dim3(256), 0, s, jb);
+  FD3_LAUNCH_CHECK("k_reduce_rows");
   return FD3_OK;
 }
 
 // One block of 1024 threads: wave w sums columns w and w + 16 of the 31
-// partial columns (rows <= 256 after reduce_partials), thread 0 assembles.
+// partial columns (rows <= 256 after k_reduce_rows), thread 0 assembles.
 __global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
   __shared__ double tot[32];
   // ks 0..10 -> tot[0..10]; c6 0..4 -> tot[11..15]; ga 0..8 -> tot[16..24]; cnv 0..5 -> tot[25..30]
@@ -442,10 +464,12 @@ static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, 
   fa.ga = grad ? c.p_ga : nullptr; fa.nga = c.n_ga;
   fa.cnv = (grad && have_cnv) ? c.p_cn : nullptr; fa.ncnv = c.n_cn;
   int rc;
-  if ((rc = reduce_partials(&fa.ks, &fa.nks, 16, 11, c.p_red, s))) return rc;
-  if ((rc = reduce_partials(&fa.c6, &fa.nc6, 8, 5, c.p_red + 256 * 16, s))) return rc;
-  if (fa.ga && (rc = reduce_partials(&fa.ga, &fa.nga, 16, 9, c.p_red + 256 * 24, s))) return rc;
-  if (fa.cnv && (rc = reduce_partials(&fa.cnv, &fa.ncnv, 8, 6, c.p_red + 256 * 40, s))) return rc;
+  ReduceJobs jb{};
+  reduce_partials_add(jb, &fa.ks, &fa.nks, 16, 11, c.p_red);
+  reduce_partials_add(jb, &fa.c6, &fa.nc6, 8, 5, c.p_red + 256 * 16);
+  if (fa.ga) reduce_partials_add(jb, &fa.ga, &fa.nga, 16, 9, c.p_red + 256 * 24);
+  if (fa.cnv) reduce_partials_add(jb, &fa.cnv, &fa.ncnv, 8, 6, c.p_red + 256 * 40);
+  if ((rc = reduce_partials_run(jb, s))) return rc;
   fa.volume = c.geom.volume;
   for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
   fa.energy = out->energy;

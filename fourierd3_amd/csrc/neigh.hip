@@ -434,7 +434,8 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
       if (!BWD) {
         a.cn[h0 + h] = hacc[h][0];  // each atom is the home of exactly one block
       } else {
-        for (int d = 0; d < 3; ++d) atomicAdd(a.gpos + 3 * (h0 + h) + d, (double)hacc[h][BWD ? d : 0]);
+        // each atom is the home of exactly one block: a plain update
+        for (int d = 0; d < 3; ++d) a.gpos[3 * (h0 + h) + d] += (double)hacc[h][BWD ? d : 0];
       }
     }
     __syncthreads();  // the next home pass clears hacc

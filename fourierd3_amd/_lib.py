@@ -36,7 +36,7 @@ EXPORTS = (
     "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
     "fd3_spread", "fd3_gather",
     "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
-    "fd3_green_host", "fd3_interp_weights_host",
+    "fd3_green_host", "fd3_green_table_host", "fd3_interp_weights_host",
 )
 
 
@@ -95,6 +95,8 @@ _SIG = {
     "fd3_fft_r2c": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, _P, _P, _P, C.c_size_t, _P]),
     "fd3_fft_c2r": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, _P, _P, _P, C.c_size_t, _P]),
     "fd3_green_host": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, _P, _P, _P]),
+    "fd3_green_table_host": (C.c_int, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                       _P, _P, _P]),
     "fd3_interp_weights_host": (C.c_int, [C.c_int32, C.c_int32, C.c_int64, _P, _P, _P]),
 }
 

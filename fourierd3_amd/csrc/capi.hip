@@ -396,6 +396,7 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   PairInit pi;
   pi.qq = m->qq; pi.nS = nS;
   pi.s6 = (float)m->s6; pi.s8 = (float)m->s8; pi.a1 = (float)m->a1; pi.a2 = (float)m->a2;
+  pi.gdk = c.geom.interp == FD3_INTERP_EWALD ? 0.0 : ks_gtab_dk(c.plan, c.geom);
   if ((rc = fft_tables_init(c.plan, c.tab, c.geom.order, c.geom.interp == FD3_INTERP_P3M, s, &pi))) return rc;
   C6Args ca;
   ca.n = n; ca.nS = nS; ca.r = r; ca.cn_raw = c.cn_raw; ca.spec_s = spec_s; ca.slot_atom = slot_atom;
@@ -861,6 +862,27 @@ int fd3_green_host(double s6, double s8, double r0, double qq, int64_t n, const 
   for (int64_t i = 0; i < n; ++i) {
     float h6, d6, h8, d8;
     green_eval_tab(pk, tab, (float)k[i], g + i, dg + i, &h6, &d6, &h8, &d8);
+  }
+  return FD3_OK;
+}
+
+int fd3_green_table_host(double s6, double s8, double r0, double qq, double kmax, int64_t n, const double* k,
+                         float* g, float* dg) {
+  if (!(kmax > 0.0)) {
+    set_error("kmax must be positive");
+    return FD3_ERR_ARG;
+  }
+  const KsPair pk = ks_pair((float)qq, (float)s6, (float)s8, 0.0f, (float)r0);
+  const double dk = kmax * (1.0 + 1e-6) / kGtabN;
+  const float kinv = (float)(1.0 / dk);
+  float4 tab[2 * kGtabN];  // the table of this pair, built by the kernels' code
+  for (int i = 0; i < kGtabN; ++i) gtab_interval(dk, i, (double)pk.Rs * kHtabDx, pk.p6, pk.p8, tab + 2 * i);
+  for (int64_t j = 0; j < n; ++j) {
+    const float fk = (float)k[j] * kinv;
+    const int i = std::min((int)fk, kGtabN - 1);
+    float dgdw;
+    gtab_eval(tab[2 * i], tab[2 * i + 1], fk - (float)i, g + j, &dgdw);
+    dg[j] = dgdw * kinv;
   }
   return FD3_OK;
 }

@@ -201,12 +201,19 @@ __host__ __device__ inline void htab_interval(int i, float4* out) {
   double a6[6], a8[6];
   for (int f = 0; f < 2; ++f) {
     const double* v = f ? f8 : f6;
-    double c[6];  // Chebyshev coefficients
-    for (int k = 0; k < 6; ++k) {
-      double s = 0.0;
-      for (int j = 0; j < 6; ++j) s += v[j] * cos(k * (2 * j + 1) * kPi / 12.0);
-      c[k] = s * (k ? 2.0 / 6.0 : 1.0 / 6.0);
+    double c[6] = {0, 0, 0, 0, 0, 0};  // Chebyshev coefficients, T_k(u_j) by the recurrence
+    for (int j = 0; j < 6; ++j) {
+      double tm = 1.0, tk = u[j];
+      c[0] += v[j];
+      c[1] += v[j] * u[j];
+      for (int k = 2; k < 6; ++k) {
+        const double tn = 2.0 * u[j] * tk - tm;
+        c[k] += v[j] * tn;
+        tm = tk;
+        tk = tn;
+      }
     }
+    for (int k = 0; k < 6; ++k) c[k] *= k ? 2.0 / 6.0 : 1.0 / 6.0;
     // T0..T5 in monomials of u = 2w - 1
     double m[6];
     m[0] = c[0] - c[2] + c[4];
@@ -278,6 +285,86 @@ __host__ __device__ inline KsPair ks_pair(float qq, float s6, float s8, float a1
   k.q8 = k.p8 * R * kHtabInvDx;
   k.dR = sq;
   return k;
+}
+
+// Per-pair tables of G itself over k in [0, kmax] (the register k-space
+// kernel without parameter gradients): kGtabN intervals of dk = kmax/kGtabN,
+// on each the degree-7 polynomial interpolating G_XY(k) = p6 h6(kR) +
+// p8 h8(kR) at its eight Chebyshev nodes, as monomial coefficients in the
+// local coordinate w = k/dk - i in [0, 1]: {a0..a3}, {a4..a7}.  One index and
+// w per k point serve every pair, and each pair costs one Horner pass.  At
+// the finest mesh spacing the references use (kmax R ~ 60) an interval spans
+// dx <= 1 in x = kR, where the degree-7 interpolation error is < 1e-9 of G(0).
+constexpr int kGtabN = 56;
+constexpr int kGtabDeg = 7;
+
+// Monomial coefficients a[0..n) in w in [0, 1] of the polynomial interpolating
+// f at the n Chebyshev nodes u_j = cos((2j+1) pi / 2n), w_j = (u_j + 1) / 2.
+__host__ __device__ inline void cheb_fit_w(const double* f, int n, double* a) {
+  double c[kGtabDeg + 1], m[kGtabDeg + 1], t0[kGtabDeg + 1], t1[kGtabDeg + 1], t2[kGtabDeg + 1];
+  // c_k = (2 - [k = 0]) / n sum_j f_j T_k(u_j), T_k(u_j) by the recurrence
+  for (int k = 0; k < n; ++k) c[k] = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double u = cos((2 * j + 1) * kPi / (2.0 * n));
+    double tm = 1.0, tk = u;
+    c[0] += f[j];
+    if (n > 1) c[1] += f[j] * u;
+    for (int k = 2; k < n; ++k) {
+      const double tn = 2.0 * u * tk - tm;
+      c[k] += f[j] * tn;
+      tm = tk;
+      tk = tn;
+    }
+  }
+  for (int k = 0; k < n; ++k) c[k] *= (k ? 2.0 : 1.0) / n;
+  // sum_k c_k T_k(u) in monomials of u (T_{k+1} = 2u T_k - T_{k-1})
+  for (int i = 0; i < n; ++i) { m[i] = 0.0; t0[i] = 0.0; t1[i] = 0.0; }
+  t0[0] = 1.0;
+  if (n > 1) t1[1] = 1.0;
+  m[0] = c[0];
+  for (int i = 0; n > 1 && i < n; ++i) m[i] += c[1] * t1[i];
+  for (int k = 2; k < n; ++k) {
+    for (int i = 0; i < n; ++i) t2[i] = (i ? 2.0 * t1[i - 1] : 0.0) - t0[i];
+    for (int i = 0; i < n; ++i) { m[i] += c[k] * t2[i]; t0[i] = t1[i]; t1[i] = t2[i]; }
+  }
+  // u = 2w - 1: a_j = 2^j sum_{k >= j} m_k C(k, j) (-1)^(k-j)
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0, binom = 1.0;  // C(k, j), k = j, j+1, ...
+    for (int k = j; k < n; ++k) {
+      s += m[k] * binom * ((k - j) & 1 ? -1.0 : 1.0);
+      binom = binom * (k + 1) / (k + 1 - j);
+    }
+    a[j] = s * (double)(1 << j);
+  }
+}
+
+// Interval i of the per-pair G table with interval width dk (Bohr^-1).
+__host__ __device__ inline void gtab_interval(double dk, int i, double R, double p6, double p8, float4* out) {
+  constexpr int n = kGtabDeg + 1;
+  double f[n], a[n];
+  for (int j = 0; j < n; ++j) {
+    const double u = cos((2 * j + 1) * kPi / (2.0 * n));
+    double h6, h8;
+    h_exact(dk * (i + 0.5 * (u + 1.0)) * R, &h6, &h8);
+    f[j] = p6 * h6 + p8 * h8;
+  }
+  cheb_fit_w(f, n, a);
+  out[0] = make_float4((float)a[0], (float)a[1], (float)a[2], (float)a[3]);
+  out[1] = make_float4((float)a[4], (float)a[5], (float)a[6], (float)a[7]);
+}
+
+// G and dG/dw at w from one interval of a per-pair G table (Horner's scheme
+// carrying the derivative).
+__host__ __device__ __forceinline__ void gtab_eval(float4 A, float4 B, float w, float* g, float* dgdw) {
+  float p = B.w * w + B.z, d = B.w;
+  d = d * w + p; p = p * w + B.y;
+  d = d * w + p; p = p * w + B.x;
+  d = d * w + p; p = p * w + A.w;
+  d = d * w + p; p = p * w + A.z;
+  d = d * w + p; p = p * w + A.y;
+  d = d * w + p; p = p * w + A.x;
+  *g = p;
+  *dgdw = d;
 }
 
 // G and dG/dk at |k| = k from the table; h6, h8 and their w-derivatives

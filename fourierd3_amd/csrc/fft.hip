@@ -24,13 +24,21 @@ namespace fd3 {
 // ------------------------------------------------------------ tables ---
 // tw_axis[k] = exp(-2 pi i k / n) for each axis; sinc_axis[m] = sinc(m/n)^(2p)
 // with m the fftfreq integer of index m (rfftfreq for z).
-// Also the h table and, when pi.qq is set, the per-pair constants.
+// Also the h table and, when pi.qq is set, the per-pair constants and (with
+// pi.gdk > 0) the per-pair G tables.
 __global__ void k_init_tables(float2* twx, float2* twy, float2* twz, float* scx, float* scy, float* scz,
-                              float4* htab, KsPair* pc, PairInit pi, int nx, int ny, int nz, int order,
+                              float4* htab, float4* gtab, KsPair* pc, PairInit pi, int nx, int ny, int nz, int order,
                               int use_sinc) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t <= kHtabN) htab_interval(t, htab + 3 * t);
   if (pi.qq && t < pi.nS * pi.nS) pc[t] = ks_pair(pi.qq[t], pi.s6, pi.s8, pi.a1, pi.a2);
+  if (pi.qq && pi.gdk > 0.0 && pi.nS <= kKsRegMaxS && t < pi.nS * (pi.nS + 1) / 2 * kGtabN) {
+    const int pt = t / kGtabN, i = t - pt * kGtabN;  // pair pt = (X, Y >= X) in row order
+    int X = 0, r0 = 0;
+    while (pt >= r0 + pi.nS - X) { r0 += pi.nS - X; ++X; }
+    const KsPair pk = ks_pair(pi.qq[X * pi.nS + X + (pt - r0)], pi.s6, pi.s8, pi.a1, pi.a2);
+    gtab_interval(pi.gdk, i, (double)pk.Rs * kHtabDx, pk.p6, pk.p8, gtab + 2 * t);
+  }
   auto tw = [](float2* dst, int k, int n) {
     double s, c;
     sincospi(-2.0 * (double)k / (double)n, &s, &c);
@@ -161,6 +169,8 @@ struct KspaceArgs {
   const KsPair* pc;     // (nS, nS)
   const float2* twx;
   const float4* htab;
+  const float4* gtab;   // per-pair G tables (register kernel, no parameter gradients)
+  float gkinv;          // 1 / (interval width) of gtab, Bohr
   const float* scx; const float* scy; const float* scz;
   double* partials;     // (gridDim, 16)
   // dE/drho's k = 0 coefficient per channel.  It is a constant on the mesh:
@@ -256,29 +266,37 @@ __device__ inline void kspace_reduce(double (&acc)[11], double* red, const Kspac
 // nS = NS <= kKsRegMaxS species and RT ranks per tile: each thread keeps the
 // NS*RT spectra of its k point in registers and writes phi back over them,
 // so the block needs a single LDS buffer.  Lines l = (X*RT + qi)*NK + kyi.
+// Without parameter gradients G comes from the per-pair tables (one index
+// per k point, one Horner pass per pair); with them from the h table, which
+// also gives h6 and h8 separately.
+// LDS: [tables | x twiddles | lines | line bases]; the block sums reuse the
+// lines at the end.
 template <int NS, int RT, bool PARAMS>
 __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
   constexpr int MAXB = ks_maxb(NS * RT);
   constexpr int NP = NS * (NS + 1) / 2;  // unordered species pairs
   extern __shared__ float4 lds4[];
-  __shared__ double red[16 * 16];
-  __shared__ KsPair pcl[NP];
+  __shared__ KsPair pcl[PARAMS ? NP : 1];
   const int nx = a.nx, NK = a.NK;
   const int kz = blockIdx.y, q0 = blockIdx.z * RT;
   const int rt = min(RT, a.r - q0);
   const int nlines = NS * RT * NK;
   const int ls = nx + 1;
   const int lnx = ilog2(nx);  // nx a power of two
-  float4* htab = lds4;
-  float2* tws = (float2*)(htab + 3 * (kHtabN + 1));
+  float4* tab = lds4;
+  const int ntab = PARAMS ? 3 * (kHtabN + 1) : 2 * NP * kGtabN;
+  float2* tws = (float2*)(tab + ks_table_f4(NS));
   float2* S = tws + nx;
   long long* lbase = (long long*)(S + nlines * ls);
-  kspace_stage_consts(a, htab, tws);
-  for (int t = threadIdx.x; t < NP; t += blockDim.x) {  // pair t = (X, Y >= X) in row order
-    int X = 0, r0 = 0;
-    while (t >= r0 + NS - X) { r0 += NS - X; ++X; }
-    pcl[t] = a.pc[X * NS + X + (t - r0)];
-  }
+  const float4* src_tab = PARAMS ? a.htab : a.gtab;
+  for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[i] = src_tab[i];
+  stage_table(tws, a.twx, a.nx);
+  if (PARAMS)
+    for (int t = threadIdx.x; t < NP; t += blockDim.x) {  // pair t = (X, Y >= X) in row order
+      int X = 0, r0 = 0;
+      while (t >= r0 + NS - X) { r0 += NS - X; ++X; }
+      pcl[t] = a.pc[X * NS + X + (t - r0)];
+    }
 
   float lam[RT];
 #pragma unroll
@@ -296,20 +314,30 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
-      // G~ and dG~/dk of every pair first, then the spectra
+      // G and dG/d|k| (or dG/dw) of every pair first, then the spectra
       float gp[NP], dgp[NP], p_acc[4] = {0.f, 0.f, 0.f, 0.f};
       float h6v[PARAMS ? NP : 1], d6v[PARAMS ? NP : 1], h8v[PARAMS ? NP : 1], d8v[PARAMS ? NP : 1];
+      if constexpr (PARAMS) {
 #pragma unroll
-      for (int pi = 0; pi < NP; ++pi) {
-        float h6, d6, h8, d8;
-        green_eval_tab(pcl[pi], htab, kp.kn, &gp[pi], &dgp[pi], &h6, &d6, &h8, &d8);
-        if (PARAMS) {
+        for (int pi = 0; pi < NP; ++pi) {
+          float h6, d6, h8, d8;
+          green_eval_tab(pcl[pi], tab, kp.kn, &gp[pi], &dgp[pi], &h6, &d6, &h8, &d8);
           h6v[pi] = h6; d6v[pi] = d6; h8v[pi] = h8; d8v[pi] = d8;
           asm volatile("" : "+v"(h6v[pi]), "+v"(d6v[pi]), "+v"(h8v[pi]), "+v"(d8v[pi]));
+          // pin g and dG/dk here: left to itself the compiler defers the
+          // derivative and keeps every pair's Horner intermediates live
+          asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
         }
-        // pin g and dG/dk here: left to itself the compiler defers the
-        // derivative and keeps every pair's Horner intermediates live
-        asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
+      } else {
+        const float fk = kp.kn * a.gkinv;
+        const int i = min((int)fk, kGtabN - 1);
+        const float wk = fk - (float)i;
+        const float4* e = tab + 2 * i;
+#pragma unroll
+        for (int pi = 0; pi < NP; ++pi) {
+          gtab_eval(e[2 * kGtabN * pi], e[2 * kGtabN * pi + 1], wk, &gp[pi], &dgp[pi]);
+          asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
+        }
       }
       float2 sv[NS][RT], ph[NS][RT];
 #pragma unroll
@@ -342,6 +370,7 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
           if (PARAMS) kspace_param_terms(pcl[pi], kp.kn, h6v[pi], d6v[pi], h8v[pi], d8v[pi], tre, p_acc);
         }
       }
+      if (!PARAMS) d_acc *= a.gkinv;  // dG/dw -> dG/d|k|
       // ph holds sum_Y G S_Y; the 1/sinc^2p of G~ goes into f and the accumulation
       float e_acc = 0.f, f[RT];
 #pragma unroll
@@ -367,7 +396,8 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
     __syncthreads();
     lds_fft<1, MAXB>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
-  kspace_reduce(acc, red, a, PARAMS);
+  __syncthreads();  // the lines are free: reuse them for the block sums
+  kspace_reduce(acc, (double*)S, a, PARAMS);
 }
 
 // Any number of species: phi accumulates in a second LDS buffer P.
@@ -486,9 +516,12 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   const int nbuf = p->ks_reg ? 1 : 2;
   // the generic variant may take most of gfx950's 160 KB of LDS per workgroup
   const size_t ks_budget = p->ks_reg ? (size_t)62 * 1024 : (size_t)kKsGenericLds;
+  // (the register variant also reuses its lines for the block sums: >= 2 KB)
   auto ks_bytes = [&](int rt_, int nk_) {
-    return (size_t)3 * (kHtabN + 1) * sizeof(float4) + (size_t)p->nx * sizeof(float2) +
-           (size_t)nbuf * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * rt_ * nk_ * sizeof(long long);
+    const size_t tab = p->ks_reg ? (size_t)ks_table_f4(nS) * sizeof(float4) : (size_t)3 * (kHtabN + 1) * sizeof(float4);
+    const size_t lines =
+        (size_t)nbuf * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * rt_ * nk_ * sizeof(long long);
+    return tab + (size_t)p->nx * sizeof(float2) + std::max(lines, (size_t)16 * 16 * sizeof(double));
   };
   // RT: ranks per tile; the register variant takes any RT <= 4, the generic
   // one a power of two <= 8 (a last partial tile is padded with zero lines)
@@ -524,6 +557,21 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   return FD3_OK;
 }
 
+double ks_gtab_dk(const FftPlan& p, const Geom& g) {
+  double kmax = 0.0;
+  for (int sx = -1; sx <= 1; sx += 2)
+    for (int sy = -1; sy <= 1; sy += 2) {
+      const double m[3] = {0.5 * sx * p.nx, 0.5 * sy * p.ny, 0.5 * p.nz};
+      double k2 = 0.0;
+      for (int c = 0; c < 3; ++c) {
+        const double kc = m[0] * g.recip[c] + m[1] * g.recip[3 + c] + m[2] * g.recip[6 + c];
+        k2 += kc * kc;
+      }
+      kmax = std::max(kmax, std::sqrt(k2));
+    }
+  return std::max(kmax, 1e-30) * (1.0 + 1e-6) / kGtabN;
+}
+
 void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t) {
   t->twx = ar.take<float2>(p.nx);
   t->twy = ar.take<float2>(p.ny);
@@ -532,6 +580,7 @@ void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t) {
   t->scy = ar.take<float>(p.ny);
   t->scz = ar.take<float>(p.M1);
   t->htab = ar.take<float4>(3 * (kHtabN + 1));
+  t->gtab = ar.take<float4>((size_t)ks_table_f4(kKsRegMaxS));
   t->pc = ar.take<KsPair>((size_t)p.nS * p.nS);
 }
 
@@ -539,10 +588,11 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
                     const PairInit* pairs) {
   PairInit pi{};
   if (pairs) pi = *pairs;
-  const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz), std::max(kHtabN + 1, pi.nS * pi.nS));
+  const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz),
+                         std::max(std::max(kHtabN + 1, pi.nS * pi.nS), pi.nS * (pi.nS + 1) / 2 * kGtabN));
   ProfScope _ps("k_init_tables", s);
   hipLaunchKernelGGL(k_init_tables, dim3((n + 255) / 256), dim3(256), 0, s, t.twx, t.twy, t.twz, t.scx, t.scy, t.scz,
-                     t.htab, t.pc, pi, p.nx, p.ny, p.nz, order, use_sinc);
+                     t.htab, t.gtab, t.pc, pi, p.nx, p.ny, p.nz, order, use_sinc);
   FD3_LAUNCH_CHECK("k_init_tables");
   return FD3_OK;
 }
@@ -615,7 +665,8 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
                int need_params, double* partials, float2* k0, hipStream_t s) {
   KspaceArgs a;
   a.k0 = k0;
-  a.spec = spec; a.eigs = eigs; a.pc = t.pc; a.twx = t.twx; a.htab = t.htab;
+  a.spec = spec; a.eigs = eigs; a.pc = t.pc; a.twx = t.twx; a.htab = t.htab; a.gtab = t.gtab;
+  a.gkinv = (float)(1.0 / ks_gtab_dk(p, g));
   a.scx = t.scx; a.scy = t.scy; a.scz = t.scz; a.partials = partials;
   for (int i = 0; i < 9; ++i) a.recip[i] = (float)g.recip[i];
   a.neg_inv_vol = (float)(-1.0 / g.volume);

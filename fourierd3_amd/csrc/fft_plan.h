@@ -31,6 +31,7 @@ struct FftTables {
   float2 *twx, *twy, *twz;
   float *scx, *scy, *scz;
   float4* htab;  // 3 * (kHtabN + 1) entries: polynomial table of h6, h8
+  float4* gtab;  // (pairs X <= Y, kGtabN, 2): per-pair G tables of the register k-space kernel
   KsPair* pc;    // (nS, nS) per-pair constants of the k-space kernel
 };
 
@@ -39,6 +40,7 @@ struct PairInit {
   const float* qq;
   int nS;
   float s6, s8, a1, a2;
+  double gdk;  // interval width of the per-pair G tables (Bohr^-1), 0: none
 };
 
 // Butterflies per thread held by the register k-space variant's line FFTs:
@@ -47,6 +49,12 @@ constexpr int ks_maxb(int lines_per_row) { return lines_per_row <= 8 ? 2 : 4; }
 
 // Species counts for which the k-space kernel keeps a k point's spectra in registers.
 constexpr int kKsRegMaxS = 4;
+// float4 entries of the table region of the register k-space kernel: the h
+// table or the per-pair G tables, whichever is larger (one layout serves both
+// variants).
+constexpr int ks_table_f4(int nS) {
+  return 3 * (kHtabN + 1) > nS * (nS + 1) * kGtabN ? 3 * (kHtabN + 1) : nS * (nS + 1) * kGtabN;
+}
 // The generic k-space kernel: LDS budget (of the 160 KB a gfx950 workgroup may
 // use; 2 KB of static LDS come on top) and butterflies per thread of its line
 // FFTs (tiles of nS * RT * NK * nx <= 1024 * kKsGenericMaxB values).
@@ -54,6 +62,9 @@ constexpr int kKsGenericLds = 150 * 1024;
 constexpr int kKsGenericMaxB = 8;
 
 int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p);
+// Interval width of the per-pair G tables: kmax / kGtabN, kmax the largest
+// |k| of the half spectrum (a corner of the index box).
+double ks_gtab_dk(const FftPlan& p, const Geom& g);
 void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t);
 int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s,
                     const PairInit* pairs = nullptr);

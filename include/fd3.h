@@ -177,6 +177,11 @@ int fd3_fft_c2r(const int32_t ns[3], int32_t n_channels, const float* spec, floa
  * the same code the kernels run (float32); also dG/d|k|. */
 int fd3_green_host(double s6, double s8, double r0, double qq, int64_t n, const double* k, float* g,
                    float* dg_dk);
+/* The same G_XY(|k|) and dG/d|k| from the per-pair table over [0, kmax] that
+ * the k-space kernel uses when no parameter gradients are asked for (host
+ * evaluation of the kernel's code; |k| <= kmax). */
+int fd3_green_table_host(double s6, double s8, double r0, double qq, double kmax, int64_t n, const double* k,
+                         float* g, float* dg_dk);
 
 /* 1-D interpolation weights (torch-pme MeshInterpolator._compute_1d_weights as
  * used at src/fourierd3/core.py:555), host-side evaluation of the kernel code:

@@ -70,6 +70,25 @@ def test_hip_green_host_matches_oracle():
         assert np.abs(dg[1:] - dref).max() / np.abs(dref).max() < 5e-6
 
 
+def test_hip_green_pair_table_matches_oracle():
+    """The per-pair G tables of the k-space kernel (degree-7 pieces over
+    [0, kmax]) against the closed form, at the finest and a coarse kmax."""
+    from fourierd3_amd import _lib as L
+    h = L.lib()
+    for kmax in (3.0, 9.0):
+        k = np.concatenate([[0.0], np.linspace(1e-4, kmax, 3001)])
+        for (R, qq, s6, s8) in [(5.0, 4.0, 1.0, 0.7875), (7.3, 9.0, 1.0, 0.7875), (12.0, 30.0, 0.6, 1.2)]:
+            g = np.zeros(len(k), np.float32)
+            dg = np.zeros(len(k), np.float32)
+            L.check(h.fd3_green_table_host(s6, s8, R, qq, kmax, len(k), k.ctypes.data, g.ctypes.data, dg.ctypes.data))
+            ref = green_pair(torch.tensor(k, dtype=torch.float64), R, qq, s6, s8).numpy()
+            assert np.abs(g - ref).max() / np.abs(ref).max() < 1e-6
+            kk = torch.tensor(k[1:], requires_grad=True)
+            green_pair(kk, R, qq, s6, s8).sum().backward()
+            dref = kk.grad.numpy()
+            assert np.abs(dg[1:] - dref).max() / np.abs(dref).max() < 5e-6
+
+
 # ------------------------------------------------------------------ interpolation weights
 def test_cardinal_bspline_closed_forms():
     x = torch.linspace(-0.5, 0.5, 101, dtype=torch.float64)

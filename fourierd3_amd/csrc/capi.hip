@@ -555,7 +555,7 @@ int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n, const void*
   const bool grad = flags & FD3_NEED_GRAD;
   if ((rc = launch_prep_pos(positions, pos_dtype, species, n, pos4, spec_s, s))) return rc;
   if ((rc = launch_fill_f32(c.cn_raw, std::max<int64_t>(n, 1), s))) return rc;
-  CnArgs cna;
+  CnArgs cna{};
   cna.rcov = m->rcov; cna.rcut = (float)(m->r_cut * kA2B); cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
   cna.gpos = c.gpos; cna.partials = c.p_cn;
   EdgeArgs ea;
@@ -621,14 +621,15 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const vo
   const bool grad = flags & FD3_NEED_GRAD;
   if ((rc = cell_build(cg, cb, n, positions, pos_dtype, species, s))) return rc;
   const CellArgs ca = cell_args(cg, cb, n);
-  CnArgs cna;
+  CnArgs cna{};
   cna.rcov = m->rcov; cna.rcut = (float)(m->r_cut * kA2B); cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
   cna.gpos = c.gpos; cna.partials = c.p_cn;
   if ((rc = launch_cn_cells(ca, cna, cg.nbins, s))) return rc;
   if ((rc = run_mesh_path(m, c, n, cb.pos4, cb.spec_s, cb.slot_atom, flags, out, s))) return rc;
-  if (grad) {
-    if (n > 0 && (rc = launch_cn_bwd_cells(ca, cna, cg.nbins, s))) return rc;
-    if (out->grad_pos && (rc = launch_unpermute(c.gpos, cb.slot_atom, n, out->grad_pos, s))) return rc;
+  if (grad && n > 0) {
+    // the CN backward adds the last term of dE/dr and writes grad_pos (atom order, per Angstrom)
+    cna.gout = out->grad_pos; cna.slot_atom = cb.slot_atom;
+    if ((rc = launch_cn_bwd_cells(ca, cna, cg.nbins, s))) return rc;
   }
   fd3_outputs o2 = *out;
   o2.grad_cell = nullptr;  // the cell-list path reports the strain derivative only

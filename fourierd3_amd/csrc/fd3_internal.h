@@ -187,54 +187,83 @@ __host__ __device__ inline void h_exact(double x, double* h6, double* h8) {
   *h8 = n8 / x;
 }
 
-// The three float4 table entries of interval i (0 <= i <= kHtabN).
-__host__ __device__ inline void htab_interval(int i, float4* out) {
+// Per-pair tables of G itself over k in [0, kmax] (the register k-space
+// kernel without parameter gradients): kGtabN intervals of dk = kmax/kGtabN,
+// on each the degree-7 polynomial interpolating G_XY(k) = p6 h6(kR) +
+// p8 h8(kR) at its eight Chebyshev nodes, as monomial coefficients in the
+// local coordinate w = k/dk - i in [0, 1]: {a0..a3}, {a4..a7}.  One index and
+// w per k point serve every pair, and each pair costs one Horner pass.  An
+// interval spans dx = kmax R / kGtabN in x = kR; the degree-7 interpolation
+// error stays below 1e-8 of G(0) up to dx = 2, i.e. kmax R <= 112 (a 0.4 A
+// mesh spacing gives kmax ~ 7 / Bohr, and R < 16 Bohr).
+constexpr int kGtabN = 56;
+constexpr int kGtabDeg = 7;
+
+// Monomial coefficients a[0..n) in w in [0, 1] of the polynomial interpolating
+// f at the n Chebyshev nodes u_j = cos((2j+1) pi / 2n), w_j = (u_j + 1) / 2.
+__host__ __device__ inline void cheb_fit_w(const double* f, int n, double* a) {
+  double c[kGtabDeg + 1], m[kGtabDeg + 1], t0[kGtabDeg + 1], t1[kGtabDeg + 1], t2[kGtabDeg + 1];
+  // c_k = (2 - [k = 0]) / n sum_j f_j T_k(u_j), T_k(u_j) by the recurrence
+  for (int k = 0; k < n; ++k) c[k] = 0.0;
+  for (int j = 0; j < n; ++j) {
+    const double u = cos((2 * j + 1) * kPi / (2.0 * n));
+    double tm = 1.0, tk = u;
+    c[0] += f[j];
+    if (n > 1) c[1] += f[j] * u;
+    for (int k = 2; k < n; ++k) {
+      const double tn = 2.0 * u * tk - tm;
+      c[k] += f[j] * tn;
+      tm = tk;
+      tk = tn;
+    }
+  }
+  for (int k = 0; k < n; ++k) c[k] *= (k ? 2.0 : 1.0) / n;
+  // sum_k c_k T_k(u) in monomials of u (T_{k+1} = 2u T_k - T_{k-1})
+  for (int i = 0; i < n; ++i) { m[i] = 0.0; t0[i] = 0.0; t1[i] = 0.0; }
+  t0[0] = 1.0;
+  if (n > 1) t1[1] = 1.0;
+  m[0] = c[0];
+  for (int i = 0; n > 1 && i < n; ++i) m[i] += c[1] * t1[i];
+  for (int k = 2; k < n; ++k) {
+    for (int i = 0; i < n; ++i) t2[i] = (i ? 2.0 * t1[i - 1] : 0.0) - t0[i];
+    for (int i = 0; i < n; ++i) { m[i] += c[k] * t2[i]; t0[i] = t1[i]; t1[i] = t2[i]; }
+  }
+  // u = 2w - 1: a_j = 2^j sum_{k >= j} m_k C(k, j) (-1)^(k-j)
+  for (int j = 0; j < n; ++j) {
+    double s = 0.0, binom = 1.0;  // C(k, j), k = j, j+1, ...
+    for (int k = j; k < n; ++k) {
+      s += m[k] * binom * ((k - j) & 1 ? -1.0 : 1.0);
+      binom = binom * (k + 1) / (k + 1 - j);
+    }
+    a[j] = s * (double)(1 << j);
+  }
+}
+
+// x of node j (0..5) of h-table interval i: the Chebyshev nodes of the interval.
+__host__ __device__ inline double htab_node(int i, int j) {
+  return kHtabDx * (i + 0.5 * (cos((2 * j + 1) * kPi / 12.0) + 1.0));
+}
+
+// The three float4 table entries of interval i (0 <= i <= kHtabN) from h6, h8
+// at its nodes (unused for the zero interval i = kHtabN).
+__host__ __device__ inline void htab_fit(int i, const double* f6, const double* f8, float4* out) {
   if (i >= kHtabN) {
     out[0] = out[1] = out[2] = make_float4(0.f, 0.f, 0.f, 0.f);
     return;
   }
-  double f6[6], f8[6], u[6];
-  for (int j = 0; j < 6; ++j) {
-    u[j] = cos((2 * j + 1) * kPi / 12.0);
-    h_exact(kHtabDx * (i + 0.5 * (u[j] + 1.0)), &f6[j], &f8[j]);
-  }
-  double a6[6], a8[6];
-  for (int f = 0; f < 2; ++f) {
-    const double* v = f ? f8 : f6;
-    double c[6] = {0, 0, 0, 0, 0, 0};  // Chebyshev coefficients, T_k(u_j) by the recurrence
-    for (int j = 0; j < 6; ++j) {
-      double tm = 1.0, tk = u[j];
-      c[0] += v[j];
-      c[1] += v[j] * u[j];
-      for (int k = 2; k < 6; ++k) {
-        const double tn = 2.0 * u[j] * tk - tm;
-        c[k] += v[j] * tn;
-        tm = tk;
-        tk = tn;
-      }
-    }
-    for (int k = 0; k < 6; ++k) c[k] *= k ? 2.0 / 6.0 : 1.0 / 6.0;
-    // T0..T5 in monomials of u = 2w - 1
-    double m[6];
-    m[0] = c[0] - c[2] + c[4];
-    m[1] = c[1] - 3.0 * c[3] + 5.0 * c[5];
-    m[2] = 2.0 * c[2] - 8.0 * c[4];
-    m[3] = 4.0 * c[3] - 20.0 * c[5];
-    m[4] = 8.0 * c[4];
-    m[5] = 16.0 * c[5];
-    // then in monomials of w: sum_k m_k (2w - 1)^k = sum_j w^j 2^j sum_k m_k C(k,j) (-1)^(k-j)
-    const double binom[6][6] = {{1, 0, 0, 0, 0, 0},  {1, 1, 0, 0, 0, 0},   {1, 2, 1, 0, 0, 0},
-                                {1, 3, 3, 1, 0, 0},  {1, 4, 6, 4, 1, 0},   {1, 5, 10, 10, 5, 1}};
-    double* a = f ? a8 : a6;
-    for (int j = 0; j < 6; ++j) {
-      double t = 0.0;
-      for (int k = j; k < 6; ++k) t += m[k] * binom[k][j] * ((k - j) & 1 ? -1.0 : 1.0);
-      a[j] = t * (double)(1 << j);
-    }
-  }
+  double a6[kGtabDeg + 1], a8[kGtabDeg + 1];
+  cheb_fit_w(f6, 6, a6);
+  cheb_fit_w(f8, 6, a8);
   out[0] = make_float4((float)a6[0], (float)a6[1], (float)a6[2], (float)a6[3]);
   out[1] = make_float4((float)a6[4], (float)a6[5], (float)a8[0], (float)a8[1]);
   out[2] = make_float4((float)a8[2], (float)a8[3], (float)a8[4], (float)a8[5]);
+}
+
+__host__ __device__ inline void htab_interval(int i, float4* out) {
+  double f6[6] = {0, 0, 0, 0, 0, 0}, f8[6] = {0, 0, 0, 0, 0, 0};
+  if (i < kHtabN)
+    for (int j = 0; j < 6; ++j) h_exact(htab_node(i, j), &f6[j], &f8[j]);
+  htab_fit(i, f6, f8, out);
 }
 
 // h6, h8 and their derivatives with respect to the local coordinate w from
@@ -287,70 +316,30 @@ __host__ __device__ inline KsPair ks_pair(float qq, float s6, float s8, float a1
   return k;
 }
 
-// Per-pair tables of G itself over k in [0, kmax] (the register k-space
-// kernel without parameter gradients): kGtabN intervals of dk = kmax/kGtabN,
-// on each the degree-7 polynomial interpolating G_XY(k) = p6 h6(kR) +
-// p8 h8(kR) at its eight Chebyshev nodes, as monomial coefficients in the
-// local coordinate w = k/dk - i in [0, 1]: {a0..a3}, {a4..a7}.  One index and
-// w per k point serve every pair, and each pair costs one Horner pass.  At
-// the finest mesh spacing the references use (kmax R ~ 60) an interval spans
-// dx <= 1 in x = kR, where the degree-7 interpolation error is < 1e-9 of G(0).
-constexpr int kGtabN = 56;
-constexpr int kGtabDeg = 7;
-
-// Monomial coefficients a[0..n) in w in [0, 1] of the polynomial interpolating
-// f at the n Chebyshev nodes u_j = cos((2j+1) pi / 2n), w_j = (u_j + 1) / 2.
-__host__ __device__ inline void cheb_fit_w(const double* f, int n, double* a) {
-  double c[kGtabDeg + 1], m[kGtabDeg + 1], t0[kGtabDeg + 1], t1[kGtabDeg + 1], t2[kGtabDeg + 1];
-  // c_k = (2 - [k = 0]) / n sum_j f_j T_k(u_j), T_k(u_j) by the recurrence
-  for (int k = 0; k < n; ++k) c[k] = 0.0;
-  for (int j = 0; j < n; ++j) {
-    const double u = cos((2 * j + 1) * kPi / (2.0 * n));
-    double tm = 1.0, tk = u;
-    c[0] += f[j];
-    if (n > 1) c[1] += f[j] * u;
-    for (int k = 2; k < n; ++k) {
-      const double tn = 2.0 * u * tk - tm;
-      c[k] += f[j] * tn;
-      tm = tk;
-      tk = tn;
-    }
-  }
-  for (int k = 0; k < n; ++k) c[k] *= (k ? 2.0 : 1.0) / n;
-  // sum_k c_k T_k(u) in monomials of u (T_{k+1} = 2u T_k - T_{k-1})
-  for (int i = 0; i < n; ++i) { m[i] = 0.0; t0[i] = 0.0; t1[i] = 0.0; }
-  t0[0] = 1.0;
-  if (n > 1) t1[1] = 1.0;
-  m[0] = c[0];
-  for (int i = 0; n > 1 && i < n; ++i) m[i] += c[1] * t1[i];
-  for (int k = 2; k < n; ++k) {
-    for (int i = 0; i < n; ++i) t2[i] = (i ? 2.0 * t1[i - 1] : 0.0) - t0[i];
-    for (int i = 0; i < n; ++i) { m[i] += c[k] * t2[i]; t0[i] = t1[i]; t1[i] = t2[i]; }
-  }
-  // u = 2w - 1: a_j = 2^j sum_{k >= j} m_k C(k, j) (-1)^(k-j)
-  for (int j = 0; j < n; ++j) {
-    double s = 0.0, binom = 1.0;  // C(k, j), k = j, j+1, ...
-    for (int k = j; k < n; ++k) {
-      s += m[k] * binom * ((k - j) & 1 ? -1.0 : 1.0);
-      binom = binom * (k + 1) / (k + 1 - j);
-    }
-    a[j] = s * (double)(1 << j);
-  }
+// k of node j (0..7) of interval i of a per-pair G table of interval width dk.
+__host__ __device__ inline double gtab_node(double dk, int i, int j) {
+  return dk * (i + 0.5 * (cos((2 * j + 1) * kPi / (2.0 * (kGtabDeg + 1))) + 1.0));
 }
 
-// Interval i of the per-pair G table with interval width dk (Bohr^-1).
-__host__ __device__ inline void gtab_interval(double dk, int i, double R, double p6, double p8, float4* out) {
-  constexpr int n = kGtabDeg + 1;
-  double f[n], a[n];
-  for (int j = 0; j < n; ++j) {
-    const double u = cos((2 * j + 1) * kPi / (2.0 * n));
-    double h6, h8;
-    h_exact(dk * (i + 0.5 * (u + 1.0)) * R, &h6, &h8);
-    f[j] = p6 * h6 + p8 * h8;
-  }
-  cheb_fit_w(f, n, a);
+// The two float4 entries of one interval of a per-pair G table from G at its nodes.
+__host__ __device__ inline void gtab_fit(const double* f, float4* out) {
+  double a[kGtabDeg + 1];
+  cheb_fit_w(f, kGtabDeg + 1, a);
   out[0] = make_float4((float)a[0], (float)a[1], (float)a[2], (float)a[3]);
   out[1] = make_float4((float)a[4], (float)a[5], (float)a[6], (float)a[7]);
+}
+
+// G_XY at k: p6 h6(kR) + p8 h8(kR).
+__host__ __device__ inline double g_exact(double k, double R, double p6, double p8) {
+  double h6, h8;
+  h_exact(k * R, &h6, &h8);
+  return p6 * h6 + p8 * h8;
+}
+
+__host__ __device__ inline void gtab_interval(double dk, int i, double R, double p6, double p8, float4* out) {
+  double f[kGtabDeg + 1];
+  for (int j = 0; j <= kGtabDeg; ++j) f[j] = g_exact(gtab_node(dk, i, j), R, p6, p8);
+  gtab_fit(f, out);
 }
 
 // G and dG/dw at w from one interval of a per-pair G table (Horner's scheme

@@ -23,22 +23,51 @@ namespace fd3 {
 
 // ------------------------------------------------------------ tables ---
 // tw_axis[k] = exp(-2 pi i k / n) for each axis; sinc_axis[m] = sinc(m/n)^(2p)
-// with m the fftfreq integer of index m (rfftfreq for z).
-// Also the h table and, when pi.qq is set, the per-pair constants and (with
-// pi.gdk > 0) the per-pair G tables.
-__global__ void k_init_tables(float2* twx, float2* twy, float2* twz, float* scx, float* scy, float* scz,
-                              float4* htab, float4* gtab, KsPair* pc, PairInit pi, int nx, int ny, int nz, int order,
-                              int use_sinc) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t <= kHtabN) htab_interval(t, htab + 3 * t);
-  if (pi.qq && t < pi.nS * pi.nS) pc[t] = ks_pair(pi.qq[t], pi.s6, pi.s8, pi.a1, pi.a2);
-  if (pi.qq && pi.gdk > 0.0 && pi.nS <= kKsRegMaxS && t < pi.nS * (pi.nS + 1) / 2 * kGtabN) {
-    const int pt = t / kGtabN, i = t - pt * kGtabN;  // pair pt = (X, Y >= X) in row order
-    int X = 0, r0 = 0;
-    while (pt >= r0 + pi.nS - X) { r0 += pi.nS - X; ++X; }
-    const KsPair pk = ks_pair(pi.qq[X * pi.nS + X + (pt - r0)], pi.s6, pi.s8, pi.a1, pi.a2);
-    gtab_interval(pi.gdk, i, (double)pk.Rs * kHtabDx, pk.p6, pk.p8, gtab + 2 * t);
+// with m the fftfreq integer of index m (rfftfreq for z).  Also the h table
+// and, when pi.qq is set, the per-pair constants and (pi.gdk > 0) the per-pair
+// G tables.  The tables take one thread per Chebyshev node (the double
+// exponentials and cosines dominate) and one per interval for the fit:
+// blocks [0, kInitHB) build the h table (42 intervals x 6 nodes each), the
+// next ngb blocks the G tables (32 intervals x 8 nodes), the rest the
+// twiddles, sinc factors and pair constants.
+constexpr int kInitHB = (kHtabN + 1 + 41) / 42;
+constexpr int kInitGI = 256 / (kGtabDeg + 1);  // G-table intervals per block
+
+inline int init_gtab_blocks(const PairInit& pi) {
+  if (!pi.qq || !(pi.gdk > 0.0) || pi.nS > kKsRegMaxS) return 0;
+  return (pi.nS * (pi.nS + 1) / 2 * kGtabN + kInitGI - 1) / kInitGI;
+}
+
+__global__ void __launch_bounds__(256) k_init_tables(float2* twx, float2* twy, float2* twz, float* scx, float* scy,
+                                                     float* scz, float4* htab, float4* gtab, KsPair* pc, PairInit pi,
+                                                     int ngb, int nx, int ny, int nz, int order, int use_sinc) {
+  __shared__ double f[2][256];
+  const int tid = threadIdx.x;
+  if (blockIdx.x < kInitHB) {  // h table
+    const int i = blockIdx.x * 42 + tid / 6, j = tid % 6;
+    if (tid < 252 && i < kHtabN) h_exact(htab_node(i, j), &f[0][tid], &f[1][tid]);
+    __syncthreads();
+    const int ii = blockIdx.x * 42 + tid;
+    if (tid < 42 && ii <= kHtabN) htab_fit(ii, &f[0][6 * tid], &f[1][6 * tid], htab + 3 * ii);
+    return;
   }
+  if ((int)blockIdx.x < kInitHB + ngb) {  // per-pair G tables
+    const int np = pi.nS * (pi.nS + 1) / 2;
+    const int it = (blockIdx.x - kInitHB) * kInitGI + tid / (kGtabDeg + 1), j = tid % (kGtabDeg + 1);
+    if (it < np * kGtabN) {
+      const int pt = it / kGtabN, i = it - pt * kGtabN;  // pair pt = (X, Y >= X) in row order
+      int X = 0, r0 = 0;
+      while (pt >= r0 + pi.nS - X) { r0 += pi.nS - X; ++X; }
+      const KsPair pk = ks_pair(pi.qq[X * pi.nS + X + (pt - r0)], pi.s6, pi.s8, pi.a1, pi.a2);
+      f[0][tid] = g_exact(gtab_node(pi.gdk, i, j), (double)pk.Rs * kHtabDx, pk.p6, pk.p8);
+    }
+    __syncthreads();
+    const int io = (blockIdx.x - kInitHB) * kInitGI + tid;
+    if (tid < kInitGI && io < np * kGtabN) gtab_fit(&f[0][(kGtabDeg + 1) * tid], gtab + 2 * io);
+    return;
+  }
+  const int t = (blockIdx.x - kInitHB - ngb) * blockDim.x + tid;
+  if (pi.qq && t < pi.nS * pi.nS) pc[t] = ks_pair(pi.qq[t], pi.s6, pi.s8, pi.a1, pi.a2);
   auto tw = [](float2* dst, int k, int n) {
     double s, c;
     sincospi(-2.0 * (double)k / (double)n, &s, &c);
@@ -588,11 +617,11 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
                     const PairInit* pairs) {
   PairInit pi{};
   if (pairs) pi = *pairs;
-  const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz),
-                         std::max(std::max(kHtabN + 1, pi.nS * pi.nS), pi.nS * (pi.nS + 1) / 2 * kGtabN));
+  const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz), pi.nS * pi.nS);
+  const int ngb = init_gtab_blocks(pi);
   ProfScope _ps("k_init_tables", s);
-  hipLaunchKernelGGL(k_init_tables, dim3((n + 255) / 256), dim3(256), 0, s, t.twx, t.twy, t.twz, t.scx, t.scy, t.scz,
-                     t.htab, t.gtab, t.pc, pi, p.nx, p.ny, p.nz, order, use_sinc);
+  hipLaunchKernelGGL(k_init_tables, dim3(kInitHB + ngb + (n + 255) / 256), dim3(256), 0, s, t.twx, t.twy, t.twz,
+                     t.scx, t.scy, t.scz, t.htab, t.gtab, t.pc, pi, ngb, p.nx, p.ny, p.nz, order, use_sinc);
   FD3_LAUNCH_CHECK("k_init_tables");
   return FD3_OK;
 }

@@ -21,7 +21,7 @@ struct ColBuffers {
   float* scharge;     // (n, r) column order
   float* accA;        // (n, r) slot order: sum_nodes W dE/drho
   float* accG;        // (n, 3) slot order: sum_q c_q sum_nodes dW/du dE/drho
-  float4* part;       // (n, order, 2, r) column order: the gather's per-block sums (accA_q, accG_q)
+  float4* part;       // (order, 2, r, n), n in column order: the gather's per-block sums (accA_q, accG_q)
 };
 
 void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b);

@@ -33,8 +33,8 @@ void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b)
   b->key = ar.take<int32_t>(nn);
   b->tmp_i = ar.take<int4>(nn);
   b->tmp_f = ar.take<float4>(nn);
-  b->colcount = ar.take<int32_t>(ncol);
-  b->colcur = ar.take<int32_t>(ncol);
+  b->colcount = ar.take<int32_t>(2 * ncol);  // colcount and colcur: cleared by one fill
+  b->colcur = b->colcount + ncol;
   b->colstart = ar.take<int32_t>(ncol + 1);
   b->scantmp = ar.take<int32_t>(scan_tmp_elems((int64_t)ncol));
   b->rec_i = ar.take<int4>(nn);
@@ -82,8 +82,7 @@ int col_sort(const FusedArgs& a, hipStream_t s) {
   const size_t ncol = (size_t)a.nS * a.nx * a.ny;
   int rc;
   ProfScope _ps("k_col_sort", s);
-  if ((rc = launch_fill_i32(a.cb.colcount, ncol, s))) return rc;
-  if ((rc = launch_fill_i32(a.cb.colcur, ncol, s))) return rc;
+  if ((rc = launch_fill_i32(a.cb.colcount, 2 * ncol, s))) return rc;  // and colcur
   if (a.n > 0) {
     hipLaunchKernelGGL(k_col_prep, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
     FD3_LAUNCH_CHECK("k_col_prep");
@@ -300,9 +299,10 @@ __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const fl
     // tile (rows that run past the first tile); the h = 0 block zeroes the
     // h = 1 slot when the footprint stays in its tile
     const bool first = bi.y >= y0 && bi.y < y0 + TY;
-    float4* pp = a.cb.part + (((size_t)ai * p + sg.jx[k]) * 2 + (first ? 0 : 1)) * a.r + q;
-    *pp = make_float4(wx * acc, cq * g0, cq * wx * g1, cq * wx * g2);
-    if (first && (TY == a.ny || bi.y - y0 + p - 1 < TY)) pp[a.r] = make_float4(0.f, 0.f, 0.f, 0.f);
+    const size_t slot = ((size_t)(sg.jx[k] * 2 + (first ? 0 : 1)) * a.r + q) * a.n + ai;
+    a.cb.part[slot] = make_float4(wx * acc, cq * g0, cq * wx * g1, cq * wx * g2);
+    if (first && (TY == a.ny || bi.y - y0 + p - 1 < TY))
+      a.cb.part[slot + (size_t)a.r * a.n] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -323,12 +323,12 @@ __global__ void __launch_bounds__(256) k_gather_sum(FusedArgs a) {
   const int64_t ai = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (ai >= a.n) return;
   const int s = a.cb.rec_i[ai].w, r = a.r, nk = 2 * a.order;
-  const float4* pp = a.cb.part + (size_t)ai * nk * r;
+  const float4* pp = a.cb.part + ai;
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   for (int q = 0; q < r; ++q) {
     float A = 0.f;
     for (int k = 0; k < nk; ++k) {
-      const float4 v = pp[k * r + q];
+      const float4 v = pp[((size_t)k * r + q) * a.n];
       A += v.x; g0 += v.y; g1 += v.z; g2 += v.w;
     }
     a.cb.accA[(size_t)s * r + q] = A;

@@ -434,8 +434,15 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
       if (!BWD) {
         a.cn[h0 + h] = hacc[h][0];  // each atom is the home of exactly one block
       } else {
-        // each atom is the home of exactly one block: a plain update
-        for (int d = 0; d < 3; ++d) a.gpos[3 * (h0 + h) + d] += (double)hacc[h][BWD ? d : 0];
+        // each atom is the home of exactly one block: a plain update, or
+        // (gout) the final gradient straight into the output in atom order
+        const int sl = h0 + h;
+        if (a.gout) {
+          const int64_t i = a.slot_atom ? a.slot_atom[sl] : sl;
+          for (int d = 0; d < 3; ++d) a.gout[3 * i + d] = (a.gpos[3 * sl + d] + (double)hacc[h][BWD ? d : 0]) * kA2B;
+        } else {
+          for (int d = 0; d < 3; ++d) a.gpos[3 * sl + d] += (double)hacc[h][BWD ? d : 0];
+        }
       }
     }
     __syncthreads();  // the next home pass clears hacc
@@ -616,8 +623,8 @@ void cell_carve(const CellGrid& g, int64_t n, Arena& ar, CellBuffers* b) {
   b->posw = ar.take<double4>(n);
   b->flo = ar.take<int4>(n);
   b->abin = ar.take<int32_t>(n);
-  b->bcount = ar.take<int32_t>(g.nbins);
-  b->bcur = ar.take<int32_t>(g.nbins);
+  b->bcount = ar.take<int32_t>(2 * (size_t)g.nbins);  // bcount and bcur: cleared by one fill
+  b->bcur = b->bcount + g.nbins;
   b->bstart = ar.take<int32_t>(g.nbins + 1);
   b->slot_atom = ar.take<int32_t>(n);
   b->pos4 = ar.take<double4>(n);
@@ -651,8 +658,7 @@ int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* p
                hipStream_t s) {
   if (n <= 0) return FD3_OK;
   int rc;
-  if ((rc = launch_fill_i32(b.bcount, g.nbins, s))) return rc;
-  if ((rc = launch_fill_i32(b.bcur, g.nbins, s))) return rc;
+  if ((rc = launch_fill_i32(b.bcount, 2 * (size_t)g.nbins, s))) return rc;  // and bcur
   const CellArgs c = cell_args(g, b, n);
   ProfScope _ps("k_celllist_build", s);
   hipLaunchKernelGGL(k_bin, dim3(cdiv(n, 256)), dim3(256), 0, s, c, pos, dtype, n, b.posw, b.flo, b.abin, b.bcount);

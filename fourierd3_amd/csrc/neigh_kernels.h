@@ -49,6 +49,8 @@ struct CnArgs {
   const float* dEdcn; // (n) in (bwd)
   double* gpos;       // (n, 3) in/out (bwd), Hartree/Bohr
   double* partials;   // (blocks, 8) virial (bwd)
+  double* gout;       // (n, 3) or NULL (cell kernels, bwd): the final dE/dr in atom order, Hartree/Angstrom
+  const int32_t* slot_atom;  // slot -> atom for gout
 };
 
 struct EdgeArgs {

@@ -140,22 +140,22 @@ __global__ void __launch_bounds__(1 << LOG) k_c2c_y(const float2* __restrict__ i
   const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
   float2* tws = lds + TX * ls;
   stage_table(tws, twy, ny);
-  const int lny = ilog2(ny), lTX = ilog2(TX);
+  const int lTX = ilog2(TX);
+  // the y lines (stride M1 ny apart, y contiguous) go straight between HBM and
+  // the FFT's first (forward) or last (inverse) stage; the transposed side
+  // (x contiguous) passes through LDS
+  long long* lb = (long long*)(tws + ny);  // line bases of the TX y lines
+  for (int l = threadIdx.x; l < TX; l += blockDim.x) lb[l] = ((long long)(c * nx + x0 + l) * M1 + kz) * ny;
   if (DIR < 0) {
-    const float2* src = in + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile2_to_lds<U, LOG>(lds, src, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
-  } else {
-    const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
-    tile2_to_lds<U, LOG>(lds, src, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
-  }
-  __syncthreads();
-  lds_fft<DIR, MAXB>(lds, ny, TX, ls, tws, 1);
-  if (DIR < 0) {
+    __syncthreads();
+    lds_fft<DIR, MAXB>(lds, ny, TX, ls, tws, 1, GlobalLines{const_cast<float2*>(in), lb});
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
     tile2_from_lds<U, LOG>(dst, lds, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
   } else {
-    float2* dst = out + ((size_t)(c * nx + x0) * M1 + kz) * ny;
-    tile2_from_lds<U, LOG>(dst, lds, TileShape{TX, lny, 1, ls, 1}, StridedLines{(long long)M1 * ny});
+    const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
+    tile2_to_lds<U, LOG>(lds, src, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
+    __syncthreads();
+    lds_fft<DIR, MAXB>(lds, ny, TX, ls, tws, 1, LdsLines{}, GlobalLines{out, lb});
   }
 }
 
@@ -635,7 +635,8 @@ int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float
     FD3_LAUNCH_CHECK("k_r2c_z");
   }
   {
-    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
+    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
+                      (size_t)p.TX * sizeof(long long);
     ProfScope _ps("k_c2c_y_fwd", s);
     hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, A, spec, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
@@ -659,7 +660,8 @@ int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream
 
 int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, float2* A, float* out, hipStream_t s) {
   {
-    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
+    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
+                      (size_t)p.TX * sizeof(long long);
     ProfScope _ps("k_c2c_y_inv", s);
     hipLaunchKernelGGL((k_c2c_y<1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, spec, A, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
@@ -676,7 +678,8 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
 }
 
 int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
-  const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2);
+  const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
+                    (size_t)p.TX * sizeof(long long);
   if (dir < 0) {
     ProfScope _ps("k_c2c_y_fwd", s);
     hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,

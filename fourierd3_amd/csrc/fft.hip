@@ -133,8 +133,8 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
 template <int DIR, int LOG>
 __global__ void __launch_bounds__(1 << LOG) k_c2c_y(const float2* __restrict__ in, float2* __restrict__ out, int nx,
                                                     int ny, int M1, int TX, const float2* __restrict__ twy) {
-  constexpr int U = kFftLines >> LOG;     // a whole tile in one batch of loads
-  constexpr int MAXB = kFftLines >> (LOG + 2);
+  constexpr int U = kYLines >> LOG;     // a whole tile in one batch of loads
+  constexpr int MAXB = kYLines >> (LOG + 2);
   extern __shared__ float2 lds[];
   const int ls = ny + 1;
   const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
@@ -518,7 +518,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
 }
 
 // --------------------------------------------------------------- host ---
-constexpr int kYLog = 9;  // y passes: 512-thread blocks
+constexpr int kYLog = 10;  // y passes: 1024-thread blocks
 int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   for (int i = 0; i < 3; ++i) {
     const int n = ns[i];
@@ -535,7 +535,7 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   // lines per block: the register budget caps lines * length at kFftLines
   p->TY = std::max(1, std::min(p->ny, budget / std::max(p->M, 1)));
   p->TY = 1 << (31 - __builtin_clz(p->TY));
-  p->TX = std::max(1, std::min(p->nx, budget / p->ny));
+  p->TX = std::max(1, std::min(p->nx, kYLines / p->ny));
   p->TX = 1 << (31 - __builtin_clz(p->TX));
   p->LPB = std::max(1, budget / (p->nx + 1));
   // k-space: the h table, the x twiddles, one (nS <= kKsRegMaxS) or two LDS
@@ -579,7 +579,7 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   p->ks_blocks = (p->ny / NK / KT) * p->M1 * zb;
   // register budget of lds_fft<.., kFftMaxB>: lines * n <= kFftLines
   const size_t lim = kFftLines;
-  if ((size_t)p->TY * p->M > lim || (size_t)p->TX * p->ny > lim || (size_t)p->LPB * p->nx > lim) {
+  if ((size_t)p->TY * p->M > lim || (size_t)p->TX * p->ny > (size_t)kYLines || (size_t)p->LPB * p->nx > lim) {
     set_error("internal FFT tiling exceeds the per-thread register budget");
     return FD3_ERR_UNSUPPORTED;
   }
@@ -626,7 +626,16 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
   return FD3_OK;
 }
 
+// The y passes may use more than 64 KB of dynamic LDS, which must be allowed per kernel.
+static int y_lds_attr(size_t sh) {
+  if (sh <= (size_t)64 * 1024) return FD3_OK;
+  FD3_HIP(hipFuncSetAttribute((const void*)k_c2c_y<-1, kYLog>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  FD3_HIP(hipFuncSetAttribute((const void*)k_c2c_y<1, kYLog>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  return FD3_OK;
+}
+
 int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float2* A, float2* spec, hipStream_t s) {
+  int rc;
   {
     const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
     ProfScope _ps("k_r2c_z", s);
@@ -638,6 +647,7 @@ int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float
     const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
                       (size_t)p.TX * sizeof(long long);
     ProfScope _ps("k_c2c_y_fwd", s);
+    if ((rc = y_lds_attr(sh))) return rc;
     hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, A, spec, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
     FD3_LAUNCH_CHECK("k_c2c_y fwd");
@@ -659,10 +669,12 @@ int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream
 }
 
 int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, float2* A, float* out, hipStream_t s) {
+  int rc;
   {
     const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
                       (size_t)p.TX * sizeof(long long);
     ProfScope _ps("k_c2c_y_inv", s);
+    if ((rc = y_lds_attr(sh))) return rc;
     hipLaunchKernelGGL((k_c2c_y<1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, spec, A, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
     FD3_LAUNCH_CHECK("k_c2c_y inv");
@@ -678,14 +690,17 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
 }
 
 int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
+  int rc;
   const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
                     (size_t)p.TX * sizeof(long long);
   if (dir < 0) {
     ProfScope _ps("k_c2c_y_fwd", s);
+    if ((rc = y_lds_attr(sh))) return rc;
     hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
   } else {
     ProfScope _ps("k_c2c_y_inv", s);
+    if ((rc = y_lds_attr(sh))) return rc;
     hipLaunchKernelGGL((k_c2c_y<1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
                        t.twy);
   }

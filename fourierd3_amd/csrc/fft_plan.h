@@ -11,6 +11,9 @@ namespace fd3 {
 // the resulting cap on complex values per 256-thread block (32 KB of LDS).
 constexpr int kFftMaxB = 4;
 constexpr int kFftLines = kFftMaxB * 4 * 256;
+// Complex values per block of the y passes (1024 threads, 64 KB of LDS):
+// 32 x lines of 256, so the transposed side moves 256-byte rows.
+constexpr int kYLines = 2 * kFftLines;
 
 struct FftPlan {
   int nx, ny, nz, M, M1;  // M = nz/2, M1 = M + 1 (half-spectrum length)

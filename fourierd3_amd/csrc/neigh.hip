@@ -388,8 +388,10 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
           for (int e0 = 0; e0 < nc; e0 += kCnStep * 64) {
             // in-range candidates of [e0, e0 + 512) -> wl[wv][0 .. cnt)
             int cnt = 0;
+            const int nit = min(kCnStep, (nc - e0 + 63) >> 6);  // steps holding candidates
 #pragma unroll
             for (int it = 0; it < kCnStep; ++it) {
+              if (it >= nit) break;  // uniform
               const int e = e0 + it * 64 + lane;
               bool hit = false;
               if (e < nc) {

@@ -29,7 +29,7 @@ FD3_NEED_PARAMS = 2
 
 # Every symbol include/fd3.h declares (checked by the CPU test suite).
 EXPORTS = (
-    "fd3_last_error", "fd3_version", "fd3_profile_enable", "fd3_profile_report",
+    "fd3_last_error", "fd3_version", "fd3_profile_enable", "fd3_profile_only", "fd3_profile_report",
     "fd3_eval_edges_workspace", "fd3_eval_edges",
     "fd3_eval_celllist_workspace", "fd3_eval_celllist",
     "fd3_eval_celllist_batch_workspace", "fd3_eval_celllist_batch",
@@ -71,6 +71,7 @@ _SIG = {
     "fd3_last_error": (C.c_char_p, []),
     "fd3_version": (C.c_int, []),
     "fd3_profile_enable": (C.c_int, [C.c_int]),
+    "fd3_profile_only": (C.c_int, [C.c_char_p]),
     "fd3_profile_report": (C.c_int, [C.c_int, C.c_char_p, _P, _P, C.POINTER(C.c_int)]),
     "fd3_eval_edges_workspace": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, C.c_int64, C.c_int,
                                            C.POINTER(C.c_size_t)]),
@@ -168,6 +169,11 @@ def cell_array(cell_A) -> "C.Array":
 
 def profile_enable(on: bool = True) -> None:
     check(lib().fd3_profile_enable(1 if on else 0))
+
+
+def profile_only(kernel: str | None) -> None:
+    """Record only launches of `kernel` (None: every kernel)."""
+    check(lib().fd3_profile_only(kernel.encode() if kernel else None))
 
 
 def profile_report(max_entries: int = 64) -> dict:

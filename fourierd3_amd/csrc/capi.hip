@@ -39,12 +39,14 @@ struct ProfRec {
   hipEvent_t start, stop;
 };
 static bool g_prof = false;
+static std::string g_prof_only;  // record only this kernel name when non-empty
 static std::vector<ProfRec> g_prof_recs;
 static std::vector<std::pair<const char*, hipEvent_t>> g_prof_open;
 
 bool prof_enabled() { return g_prof; }
 
 void prof_record(const char* name, hipStream_t s, bool start) {
+  if (!g_prof_only.empty() && g_prof_only != name) return;
   hipEvent_t ev;
   if (hipEventCreate(&ev) != hipSuccess) return;
   if (hipEventRecord(ev, s) != hipSuccess) {
@@ -75,6 +77,8 @@ void prof_set(bool on) {
   prof_clear();
   g_prof = on;
 }
+
+void prof_only(const char* name) { g_prof_only = name ? name : ""; }
 
 int prof_report(int max_entries, char* names, double* total_ms, int64_t* counts, int* n_entries) {
   std::vector<std::string> keys;
@@ -494,6 +498,11 @@ int fd3_version(void) { return 1; }
 
 int fd3_profile_enable(int on) {
   fd3::prof_set(on != 0);
+  return FD3_OK;
+}
+
+int fd3_profile_only(const char* kernel) {
+  fd3::prof_only(kernel);
   return FD3_OK;
 }
 

@@ -34,6 +34,7 @@ int hip_fail(hipError_t e, const char* what);
 // their launch in a ProfScope, which records HIP events on the launch stream.
 bool prof_enabled();
 void prof_set(bool on);
+void prof_only(const char* name);
 int prof_report(int max_entries, char* names, double* total_ms, int64_t* counts, int* n_entries);
 void prof_record(const char* name, hipStream_t s, bool start);
 struct ProfScope {

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_r2c_z(const float* __restrict__ rho, fl
   const int lM = ilog2(M), lT = ilog2(TY);
   tile2_to_lds<kCopyU>(lds, src, TileShape{TY, lM, 1, ls, 1}, StridedLines{M});
   __syncthreads();
-  lds_fft<-1, kFftMaxB>(lds, M, TY, ls, tws, 2);
+  lds_fft<-1, kFftMaxB, 256>(lds, M, TY, ls, tws, 2);
   z_pairs<true>(lds, TY, ls, M, tws);  // X[k] = E[k] + W_nz^k O[k]
   __syncthreads();
   float2* dst = A + (size_t)(c * nx + x) * (M + 1) * ny + y0;
@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
   __syncthreads();
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
-  lds_fft<1, kFftMaxB>(lds, M, TY, ls, tws, 2);
+  lds_fft<1, kFftMaxB, 256>(lds, M, TY, ls, tws, 2);
   float2* dst = (float2*)out + ((size_t)(c * nx + x) * ny + y0) * M;
   tile2_from_lds<kCopyU>(dst, lds, TileShape{TY, lM, 1, ls, 1}, StridedLines{M});
 }
@@ -148,14 +148,14 @@ __global__ void __launch_bounds__(1 << LOG) k_c2c_y(const float2* __restrict__ i
   for (int l = threadIdx.x; l < TX; l += blockDim.x) lb[l] = ((long long)(c * nx + x0 + l) * M1 + kz) * ny;
   if (DIR < 0) {
     __syncthreads();
-    lds_fft<DIR, MAXB>(lds, ny, TX, ls, tws, 1, GlobalLines{const_cast<float2*>(in), lb});
+    lds_fft<DIR, MAXB, (1 << LOG)>(lds, ny, TX, ls, tws, 1, GlobalLines{const_cast<float2*>(in), lb});
     float2* dst = out + ((size_t)(c * M1 + kz) * ny) * nx + x0;
     tile2_from_lds<U, LOG>(dst, lds, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
   } else {
     const float2* src = in + ((size_t)(c * M1 + kz) * ny) * nx + x0;
     tile2_to_lds<U, LOG>(lds, src, TileShape{ny, lTX, 1, 1, ls}, StridedLines{nx});
     __syncthreads();
-    lds_fft<DIR, MAXB>(lds, ny, TX, ls, tws, 1, LdsLines{}, GlobalLines{out, lb});
+    lds_fft<DIR, MAXB, (1 << LOG)>(lds, ny, TX, ls, tws, 1, LdsLines{}, GlobalLines{out, lb});
   }
 }
 
@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(256) k_c2c_x(float2* __restrict__ spec, int nx
   const TileShape ts{nl, lnx, 1, ls, 1};
   tile2_to_lds<kCopyU>(lds, base, ts, StridedLines{nx});
   __syncthreads();
-  lds_fft<DIR, kFftMaxB>(lds, nx, nl, ls, tws, 1);
+  lds_fft<DIR, kFftMaxB, 256>(lds, nx, nl, ls, tws, 1);
   tile2_from_lds<kCopyU>(base, lds, ts, StridedLines{nx});
 }
 
@@ -310,7 +310,9 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
   const int kz = blockIdx.y, q0 = blockIdx.z * RT;
   const int rt = min(RT, a.r - q0);
   const int nlines = NS * RT * NK;
-  const int ls = nx + 1;
+  // padded lines when lds_fft takes its unrolled variant (same rule on the host)
+  const bool pad = fft_fixed_tile(nx, nlines, MAXB, 256);
+  const int ls = pad ? padded_stride(nx) : nx + 1;
   const int lnx = ilog2(nx);  // nx a power of two
   float4* tab = lds4;
   const int ntab = PARAMS ? 3 * (kHtabN + 1) : 2 * NP * kGtabN;
@@ -339,9 +341,10 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
     if (it) __syncthreads();  // the previous tile is done with S and lbase
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
     __syncthreads();
-    lds_fft<-1, MAXB>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
+    lds_fft<-1, MAXB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
+      const int kxe = pad ? lpad(kx) : kx;  // element kx of a line
       const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
       // G and dG/d|k| (or dG/dw) of every pair first, then the spectra
       float gp[NP], dgp[NP], p_acc[4] = {0.f, 0.f, 0.f, 0.f};
@@ -373,7 +376,7 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
       for (int X = 0; X < NS; ++X)
 #pragma unroll
         for (int q = 0; q < RT; ++q) {
-          sv[X][q] = S[((X * RT + q) * NK + kyi) * ls + kx];
+          sv[X][q] = S[((X * RT + q) * NK + kyi) * ls + kxe];
           ph[X][q] = make_float2(0.f, 0.f);
         }
       float d_acc = 0.f;
@@ -409,7 +412,7 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
 #pragma unroll
         for (int q = 0; q < RT; ++q) {
           e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
-          S[((X * RT + q) * NK + kyi) * ls + kx] = make_float2(f[q] * ph[X][q].x, f[q] * ph[X][q].y);
+          S[((X * RT + q) * NK + kyi) * ls + kxe] = make_float2(f[q] * ph[X][q].x, f[q] * ph[X][q].y);
         }
       kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, PARAMS);
     }
@@ -423,7 +426,7 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
     }
     if (!a.need_grad) continue;
     __syncthreads();
-    lds_fft<1, MAXB>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
+    lds_fft<1, MAXB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
   __syncthreads();  // the lines are free: reuse them for the block sums
   kspace_reduce(acc, (double*)S, a, PARAMS);
@@ -456,7 +459,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     if (it) __syncthreads();
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
     __syncthreads();
-    lds_fft<-1, kKsGenericMaxB>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
+    lds_fft<-1, kKsGenericMaxB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
@@ -512,7 +515,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     }
     if (!a.need_grad) continue;
     __syncthreads();
-    lds_fft<1, kKsGenericMaxB>(P, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
+    lds_fft<1, kKsGenericMaxB, 256>(P, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
   kspace_reduce(acc, red, a, params);
 }
@@ -548,8 +551,11 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   // (the register variant also reuses its lines for the block sums: >= 2 KB)
   auto ks_bytes = [&](int rt_, int nk_) {
     const size_t tab = p->ks_reg ? (size_t)ks_table_f4(nS) * sizeof(float4) : (size_t)3 * (kHtabN + 1) * sizeof(float4);
+    // the register variant pads its lines when lds_fft unrolls (fft_fixed_tile)
+    const bool pad = p->ks_reg && fft_fixed_tile(p->nx, nS * rt_ * nk_, ks_maxb(nS * rt_), 256);
+    const int ls = pad ? padded_stride(p->nx) : p->nx + 1;
     const size_t lines =
-        (size_t)nbuf * nS * rt_ * nk_ * (p->nx + 1) * sizeof(float2) + (size_t)nS * rt_ * nk_ * sizeof(long long);
+        (size_t)nbuf * nS * rt_ * nk_ * ls * sizeof(float2) + (size_t)nS * rt_ * nk_ * sizeof(long long);
     return tab + (size_t)p->nx * sizeof(float2) + std::max(lines, (size_t)16 * 16 * sizeof(double));
   };
   // RT: ranks per tile; the register variant takes any RT <= 4, the generic

@@ -180,6 +180,152 @@ struct GlobalLines : LineIO {
   __device__ GlobalLines(float2* g_, const long long* base_) : LineIO{g_, base_} {}
 };
 
+// Padded line layout: element e of a line sits at lpad(e) = e + e/32, lines
+// at stride padded_stride(n).  The radix-4 stages' scattered stores (element
+// 4j + m, 16 (j/4) + j%4 + 4m, ...) then spread over the 64 LDS banks instead
+// of piling 8 lanes on a bank.
+__host__ __device__ constexpr int lpad(int e) { return e + (e >> 5); }
+__host__ __device__ constexpr int padded_stride(int n) { return n + (n >> 5); }
+
+// Whether lds_fft has an unrolled variant for n, nlines, MAXB and a block of
+// nt threads (a full tile of n = 128 or 256).  Callers that lay their lines
+// out padded (stride padded_stride(n), element e at lpad(e)) do so exactly
+// when this holds.
+__host__ __device__ constexpr bool fft_fixed_tile(int n, int nlines, int maxb, int nt) {
+  return (n == 256 && nt >= 64 && nlines == maxb * (nt / 64)) || (n == 128 && nt >= 32 && nlines == maxb * (nt / 32));
+}
+
+// lds_fft for a compile-time length n = 2^LOGN, block size NT and a full
+// tile: nlines = MAXB * NT / (n/4) lines at stride n + 1 (PAD = false) or in
+// the padded layout (PAD = true).  The same Stockham stages as the general
+// version below, unrolled, so that every LDS offset (line c * stride,
+// element m * n/4 or m * Ns) is an immediate and no stage tests its span at
+// run time.
+template <int DIR, int MAXB, int LOGN, int NT, bool PAD, class IN, class OUT>
+__device__ __forceinline__ void lds_fft_fixed(float2* buf, const float2* __restrict__ tw, int tws, IN in, OUT out) {
+  constexpr bool kGin = std::is_same<IN, GlobalLines>::value;
+  constexpr bool kGout = std::is_same<OUT, GlobalLines>::value;
+  constexpr int N = 1 << LOGN, NQ = N >> 2, LNQ = LOGN - 2, LS = PAD ? padded_stride(N) : N + 1;
+  static_assert(!PAD || NQ >= 32, "padded FFT: n >= 128");
+  constexpr int LSTEP = NT / NQ, DL = LSTEP * LS, S4 = LOGN >> 1;
+  static_assert(LOGN >= 2 && NT % NQ == 0, "fixed FFT: n/4 must divide the block");
+  constexpr bool kGoLast = kGout && !(LOGN & 1);  // the last radix-4 stage writes global
+  // the thread index is made opaque here so that the per-thread addresses
+  // below are formed in each call, not hoisted out of a caller's loop and
+  // kept live (in VGPRs) across the caller's own work
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int j = tid & (NQ - 1);
+  const int l0 = tid >> LNQ;
+  float2* const base = buf + l0 * LS;
+  // element j + m n/4 of a line (n/4 >= 32 when padded: lpad adds m n/128)
+  const int jr = PAD ? lpad(j) : j;
+  constexpr int RQ = PAD ? NQ + (NQ >> 5) : NQ;
+  auto stage = [&](auto s_, auto gi, auto go) __attribute__((always_inline)) {
+    constexpr int S = decltype(s_)::value;
+    constexpr bool GI = decltype(gi)::value, GO = decltype(go)::value;
+    constexpr int Ns = 1 << (2 * S);
+    const int k = j & (Ns - 1);
+    float2 w1 = make_float2(1.f, 0.f), w2 = w1, w3 = w1;
+    if constexpr (Ns > 1) {
+      const int t = (k << (LNQ - 2 * S)) * tws;  // k * n / (4 Ns)
+      w1 = tw[t]; w2 = tw[2 * t]; w3 = tw[3 * t];
+      if (DIR > 0) { w1.y = -w1.y; w2.y = -w2.y; w3.y = -w3.y; }
+    }
+    float2 a[MAXB][4];
+    bool live[MAXB];
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+      if constexpr (GI) {
+        const long long b = in.base[l0 + c * LSTEP];
+        live[c] = b >= 0;
+        const float2* src = in.g + (live[c] ? b : 0) + j;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[c][m] = src[m * NQ];
+      } else {
+        live[c] = true;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) a[c][m] = base[c * DL + jr + m * RQ];
+      }
+    }
+    float2 v[MAXB][4];
+#pragma unroll
+    for (int c = 0; c < MAXB; ++c) {
+      float2 a0 = a[c][0], a1 = a[c][1], a2 = a[c][2], a3 = a[c][3];
+      if (GI && !live[c]) a0 = a1 = a2 = a3 = make_float2(0.f, 0.f);
+      if constexpr (Ns > 1) { a1 = cmul(a1, w1); a2 = cmul(a2, w2); a3 = cmul(a3, w3); }
+      const float2 b0 = cadd(a0, a2), b1 = csub(a0, a2), b2 = cadd(a1, a3);
+      const float2 d = csub(a1, a3);
+      const float2 b3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+      v[c][0] = cadd(b0, b2); v[c][2] = csub(b0, b2);
+      v[c][1] = cadd(b1, b3); v[c][3] = csub(b1, b3);
+    }
+    if constexpr (GO) {
+#pragma unroll
+      for (int c = 0; c < MAXB; ++c) {
+        const long long b = out.base[l0 + c * LSTEP];
+        if (b >= 0) {
+          float2* dst = out.g + b + j;
+#pragma unroll
+          for (int m = 0; m < 4; ++m) dst[m * NQ] = v[c][m];
+        }
+      }
+    } else {
+      if (!GI) __syncthreads();  // every LDS read of this stage is done
+      // element dst0 + m Ns; padded: lpad(dst0 + m Ns) = lpad(dst0) + m Ns +
+      // an offset that depends on m only (dst0 % 32 + 3 Ns < 32 for Ns <= 8,
+      // dst0 % 32 < 16 for Ns = 16, dst0 % 32 = k % 32 for Ns >= 32)
+      const int dst0 = ((j - k) << 2) + k;
+      float2* D = base + (PAD ? lpad(dst0) : dst0);
+#pragma unroll
+      for (int c = 0; c < MAXB; ++c)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          const int mo = !PAD ? 0 : Ns >= 32 ? m * (Ns >> 5) : Ns == 16 ? (m >> 1) : 0;
+          D[c * DL + m * Ns + mo] = v[c][m];
+        }
+    }
+    __syncthreads();
+  };
+  using T = std::true_type;
+  using F = std::false_type;
+  using I = std::integral_constant<bool, kGin>;
+  stage(std::integral_constant<int, 0>{}, I{}, std::integral_constant<bool, kGoLast && S4 == 1>{});
+  if constexpr (S4 > 1) stage(std::integral_constant<int, 1>{}, F{}, std::integral_constant<bool, kGoLast && S4 == 2>{});
+  if constexpr (S4 > 2) stage(std::integral_constant<int, 2>{}, F{}, std::integral_constant<bool, kGoLast && S4 == 3>{});
+  if constexpr (S4 > 3) stage(std::integral_constant<int, 3>{}, F{}, std::integral_constant<bool, kGoLast && S4 == 4>{});
+  if constexpr (S4 > 4) stage(std::integral_constant<int, 4>{}, F{}, std::integral_constant<bool, kGoLast && S4 == 5>{});
+  static_assert(S4 <= 5, "fixed FFT: n <= 1024");
+  if constexpr ((LOGN & 1) != 0) {  // final radix-2 stage, Ns = n/2, in place per butterfly
+    constexpr int NH = N >> 1, LH = LOGN - 1;
+    constexpr int NL = MAXB * LSTEP;           // lines in the tile
+    constexpr int BPT = NL * NH / NT;          // butterflies per thread
+    constexpr int LPI = NT / NH;               // lines per round of NT butterflies
+    static_assert(NT % NH == 0 && (NL * NH) % NT == 0, "fixed FFT: radix-2 stage shape");
+    const int jj = tid & (NH - 1);
+    const int line0 = tid >> LH;
+    float2 w = tw[jj * tws];
+    if (DIR > 0) w.y = -w.y;
+    float2* const L = buf + line0 * LS + (PAD ? lpad(jj) : jj);
+    constexpr int RH = PAD ? NH + (NH >> 5) : NH;  // lpad(jj + n/2) - lpad(jj), n/2 >= 32
+#pragma unroll
+    for (int i = 0; i < BPT; ++i) {
+      const float2 a0 = L[i * LPI * LS], a1 = cmul(L[i * LPI * LS + RH], w);
+      if constexpr (kGout) {
+        const long long gb = out.base[line0 + i * LPI];
+        if (gb >= 0) {
+          out.g[gb + jj] = cadd(a0, a1);
+          out.g[gb + jj + NH] = csub(a0, a1);
+        }
+      } else {
+        L[i * LPI * LS] = cadd(a0, a1);
+        L[i * LPI * LS + RH] = csub(a0, a1);
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // In-place FFT of `nlines` lines of length n (power of two, n <= 4 blockDim.x)
 // held in LDS at buf[l * ls + e].  tw[k * tws] = exp(-2 pi i k / n).  DIR = -1
 // forward (exp(-i...)), +1 inverse (exp(+i...), unnormalised).  Stockham
@@ -189,12 +335,24 @@ struct GlobalLines : LineIO {
 // Requires nlines * n / 4 <= MAXB * blockDim.x.  With a GlobalLines input the
 // caller need not fill buf; with a GlobalLines output buf is left holding
 // intermediate data.  Ends with __syncthreads().
-template <int DIR, int MAXB, class IN = LdsLines, class OUT = LdsLines>
+//
+// NT > 0 names the block size at compile time: full tiles of n = 128 or 256
+// at stride n + 1, or in the padded layout, then take lds_fft_fixed (the
+// padded layout is only valid there: a caller pads exactly when
+// fft_fixed_tile holds).
+template <int DIR, int MAXB, int NT = 0, class IN = LdsLines, class OUT = LdsLines>
 __device__ __forceinline__ void lds_fft(float2* buf, int n, int nlines, int ls, const float2* __restrict__ tw, int tws,
-                                        IN in = {},
-                        OUT out = {}) {
+                                        IN in = {}, OUT out = {}) {
   constexpr bool kGin = std::is_same<IN, GlobalLines>::value;
   constexpr bool kGout = std::is_same<OUT, GlobalLines>::value;
+  if constexpr (NT >= 64) {
+    if (fft_fixed_tile(n, nlines, MAXB, NT)) {
+      if (n == 256 && ls == 257) return lds_fft_fixed<DIR, MAXB, 8, NT, false>(buf, tw, tws, in, out);
+      if (n == 256 && ls == padded_stride(256)) return lds_fft_fixed<DIR, MAXB, 8, NT, true>(buf, tw, tws, in, out);
+      if (n == 128 && ls == 129) return lds_fft_fixed<DIR, MAXB, 7, NT, false>(buf, tw, tws, in, out);
+      if (n == 128 && ls == padded_stride(128)) return lds_fft_fixed<DIR, MAXB, 7, NT, true>(buf, tw, tws, in, out);
+    }
+  }
   const int log2n = __ffs(n) - 1;
   if (kGin && log2n < 2) {  // no radix-4 stage to read the input: copy it in
     for (int e = threadIdx.x; e < (nlines << log2n); e += blockDim.x) {

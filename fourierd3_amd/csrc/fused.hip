@@ -233,7 +233,7 @@ __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* 
     }
   }
   __syncthreads();
-  lds_fft<-1, (kFftLines >> (LOG + 2))>(lds, M, TY, ls, tws, 2);
+  lds_fft<-1, (kFftLines >> (LOG + 2)), (1 << LOG)>(lds, M, TY, ls, tws, 2);
   z_pairs<true>(lds, TY, ls, M, tws);
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const fl
   segs_compact(&sg, p);
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
-  lds_fft<1, (kFftLines >> (LOG + 2))>(lds, M, TY, ls, tws, 2);
+  lds_fft<1, (kFftLines >> (LOG + 2)), (1 << LOG)>(lds, M, TY, ls, tws, 2);
   const float* ldsf = (const float*)lds;
   const int total = sg.off[sg.n];
   for (int i = threadIdx.x; i < total; i += blockDim.x) {

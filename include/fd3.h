@@ -88,8 +88,12 @@ int fd3_version(void);
  * starts recording a HIP event pair around every kernel launch (do not enable
  * it during graph capture); fd3_profile_report synchronises on the recorded
  * events, writes up to max_entries kernel names (64 chars each into `names`),
- * total milliseconds and launch counts, and clears the records. */
+ * total milliseconds and launch counts, and clears the records.
+ * fd3_profile_only(name) limits the recording to launches of that kernel
+ * (NULL or "" records every kernel again), so that a timed loop carries one
+ * event pair per launch of the kernel of interest and nothing else. */
 int fd3_profile_enable(int on);
+int fd3_profile_only(const char* kernel);
 int fd3_profile_report(int max_entries, char* names, double* total_ms, int64_t* counts, int* n_entries);
 
 /* ---- whole-path entry points ------------------------------------------------

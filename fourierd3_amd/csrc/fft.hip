@@ -536,7 +536,8 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   p->nS = nS; p->r = r; p->C = nS * r;
   const int budget = kFftLines;  // complex values per block for the pass kernels
   // lines per block: the register budget caps lines * length at kFftLines
-  p->TY = std::max(1, std::min(p->ny, budget / std::max(p->M, 1)));
+  // (kZLines for the z passes)
+  p->TY = std::max(1, std::min(p->ny, kZLines / std::max(p->M, 1)));
   p->TY = 1 << (31 - __builtin_clz(p->TY));
   p->TX = std::max(1, std::min(p->nx, kYLines / p->ny));
   p->TX = 1 << (31 - __builtin_clz(p->TX));

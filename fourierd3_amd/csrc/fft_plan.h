@@ -14,6 +14,10 @@ constexpr int kFftLines = kFftMaxB * 4 * 256;
 // Complex values per block of the y passes (1024 threads, 64 KB of LDS):
 // 32 x lines of 256, so the transposed side moves 256-byte rows.
 constexpr int kYLines = 2 * kFftLines;
+// Complex values per block of the z passes (256 threads, 16 y lines of 128
+// at nz = 256): small tiles keep eight blocks in flight per CU, so one
+// block's atom loads and barriers overlap the others' FFT work.
+constexpr int kZLines = kFftLines / 2;
 
 struct FftPlan {
   int nx, ny, nz, M, M1;  // M = nz/2, M1 = M + 1 (half-spectrum length)

@@ -182,11 +182,14 @@ __device__ __forceinline__ void lds_add_f32_cas(float* line, const int* z, const
   }
 }
 
-// fused z passes: 512-thread blocks for the spread (measured faster), 256 for
-// the gather (its 100 VGPRs leave no room for more waves)
-constexpr int kZLogSpread = 9, kZLogGather = 9;
+// fused z passes: 256-thread blocks of kZLines complex values (measured
+// against 512 threads x 32 lines and 128 x 8); z_tile relies on the two
+// being equal
+constexpr int kZLogSpread = 8, kZLogGather = 8;
+static_assert(kZLogSpread == kZLogGather, "one z tile for both passes");
 // grid (ny/TY, nx, C); LDS: TY lines of M+1 complex (= nz + 2 floats).
-template <int LOG, int P>  // P > 0: compile-time interpolation order
+// MAXB: butterflies per thread of lds_fft (TY * M <= 4 MAXB 2^LOG).
+template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
 __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* __restrict__ A, int TY,
                                                       const float2* __restrict__ twz) {
   extern __shared__ float2 lds[];
@@ -233,16 +236,16 @@ __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* 
     }
   }
   __syncthreads();
-  lds_fft<-1, (kFftLines >> (LOG + 2)), (1 << LOG)>(lds, M, TY, ls, tws, 2);
+  lds_fft<-1, MAXB, (1 << LOG)>(lds, M, TY, ls, tws, 2);
   z_pairs<true>(lds, TY, ls, M, tws);
   __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
-  tile2_from_lds<(kFftLines >> LOG) + 1, LOG>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
+  tile2_from_lds<4 * MAXB + 1, LOG>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
 }
 
 // ------------------------------------------------------ z pass + gather ---
-template <int LOG, int P>  // P > 0: compile-time interpolation order
+template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
 __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const float2* __restrict__ A, int TY,
                                                       const float2* __restrict__ twz) {
   extern __shared__ float2 lds[];
@@ -255,12 +258,12 @@ __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const fl
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
   segs_lookup(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
-  tile2_to_lds<(kFftLines >> LOG) + 1, LOG>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
+  tile2_to_lds<4 * MAXB + 1, LOG>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
   __syncthreads();
   segs_compact(&sg, p);
   z_pairs<false>(lds, TY, ls, M, tws);
   __syncthreads();
-  lds_fft<1, (kFftLines >> (LOG + 2)), (1 << LOG)>(lds, M, TY, ls, tws, 2);
+  lds_fft<1, MAXB, (1 << LOG)>(lds, M, TY, ls, tws, 2);
   const float* ldsf = (const float*)lds;
   const int total = sg.off[sg.n];
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
@@ -306,12 +309,31 @@ __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const fl
   }
 }
 
+// y lines per block of the fused z passes: the plan's TY, raised (within the
+// register budget of MAXB = 4) until an atom's p y nodes span at most two
+// tiles, which the gather's two partial-sum slots per x node assume.
+static int z_tile(const FusedArgs& a, const FftPlan& p, int* maxb) {
+  int TY = p.TY;
+  while (TY + 1 < a.order && TY < p.ny) TY <<= 1;
+  const size_t vals = (size_t)TY * p.M;
+  *maxb = vals <= (size_t)kZLines ? 2 : 4;
+  if (vals > (size_t)4 * 4 * (1 << kZLogSpread)) {
+    set_error("z tile of %d lines of %d exceeds the FFT register budget", TY, p.M);
+    return -1;
+  }
+  return TY;
+}
+
 int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s) {
-  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
+  int maxb;
+  const int TY = z_tile(a, p, &maxb);
+  if (TY < 0) return FD3_ERR_UNSUPPORTED;
+  const size_t sh = (size_t)TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_spread_r2c_z", s);
-  const dim3 grid(p.ny / p.TY, p.nx, p.C), block(1 << kZLogSpread);
-  if (a.order == 5) hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 5>), grid, block, sh, s, a, A, p.TY, t.twz);
-  else hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 0>), grid, block, sh, s, a, A, p.TY, t.twz);
+  const dim3 grid(p.ny / TY, p.nx, p.C), block(1 << kZLogSpread);
+  if (maxb == 4) hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 0, 4>), grid, block, sh, s, a, A, TY, t.twz);
+  else if (a.order == 5) hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 5, 2>), grid, block, sh, s, a, A, TY, t.twz);
+  else hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 0, 2>), grid, block, sh, s, a, A, TY, t.twz);
   FD3_LAUNCH_CHECK("k_spread_r2c_z");
   return FD3_OK;
 }
@@ -339,11 +361,15 @@ __global__ void __launch_bounds__(256) k_gather_sum(FusedArgs a) {
 }
 
 int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const float2* A, hipStream_t s) {
-  const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
+  int maxb;
+  const int TY = z_tile(a, p, &maxb);
+  if (TY < 0) return FD3_ERR_UNSUPPORTED;
+  const size_t sh = (size_t)TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_c2r_z_gather", s);
-  const dim3 grid(p.ny / p.TY, p.nx, p.C), block(1 << kZLogGather);
-  if (a.order == 5) hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 5>), grid, block, sh, s, a, A, p.TY, t.twz);
-  else hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 0>), grid, block, sh, s, a, A, p.TY, t.twz);
+  const dim3 grid(p.ny / TY, p.nx, p.C), block(1 << kZLogGather);
+  if (maxb == 4) hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 0, 4>), grid, block, sh, s, a, A, TY, t.twz);
+  else if (a.order == 5) hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 5, 2>), grid, block, sh, s, a, A, TY, t.twz);
+  else hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 0, 2>), grid, block, sh, s, a, A, TY, t.twz);
   FD3_LAUNCH_CHECK("k_c2r_z_gather");
   if (a.n > 0) {
     hipLaunchKernelGGL(k_gather_sum, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);

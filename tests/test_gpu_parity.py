@@ -236,6 +236,26 @@ def test_pme_lagrange_matches_oracle(order):
     assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
+@pytest.mark.parametrize("method,order", [("spme", 5), ("pme", 7)])
+def test_tall_box_z_tiles_match_oracle(method, order):
+    """A box stretched 32x along z gets a 1024-point z mesh: the fused z passes
+    then hold few y lines per block, and at p = 7 (Lagrange) they widen their
+    tiles so that an atom's y nodes span at most two of them."""
+    _say("tall box", method, order)
+    b = W.sio2_box(120, seed=13)
+    stretch = np.diag([1.0, 1.0, 32.0])
+    box = W.Box(numbers=b.numbers, positions=b.positions @ stretch, cell=b.cell @ stretch)
+    calc = FourierD3ASECalculator(device=_dev(), method=method, interpolation_nodes=order)
+    calc.calculate(_atoms(box))
+    o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order)
+    assert list(o.ns) == list(calc._model.ns_mesh) and int(o.ns[2]) == 1024
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"], e_ref, f_ref,
+                         s_ref, box.n_atoms)
+    print(f"[errors] tall box {method} p={order} mesh {list(o.ns)}: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
 def test_param_gradients_match_oracle_autograd():
     _say("param gradients")
     box = W.sio2_box(96, seed=13)

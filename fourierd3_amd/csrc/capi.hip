@@ -390,18 +390,43 @@ static int setup_common(const fd3_model* m, const fd3_geom* g, Common& c) {
   return FD3_OK;
 }
 
-// Everything after the CN: C6 chain, spread, FFTs, k-space, adjoint.
-// pos4/spec_s in slot order; slot_atom null for the identity.
+static FusedArgs fused_args(const fd3_model* m, const Common& c, int64_t n, const double4* pos4,
+                            const int32_t* spec_s) {
+  FusedArgs fa;
+  fa.n = n; fa.nS = m->n_species; fa.r = m->n_rank;
+  fa.nx = c.geom.nx; fa.ny = c.geom.ny; fa.nz = c.geom.nz; fa.order = c.geom.order; fa.interp = c.geom.interp;
+  for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
+  fa.pos4 = pos4; fa.spec_s = spec_s; fa.charges = c.charges; fa.cb = c.cols;
+  return fa;
+}
+
+// The part of the mesh path that needs only the model, the cell, positions
+// and species: FFT and Green's function tables, and the column sort of the
+// atoms (mesh methods).
+static int mesh_prep(const fd3_model* m, Common& c, int64_t n, const double4* pos4, const int32_t* spec_s,
+                     hipStream_t s) {
+  int rc;
+  PairInit pi;
+  pi.qq = m->qq; pi.nS = m->n_species;
+  pi.s6 = (float)m->s6; pi.s8 = (float)m->s8; pi.a1 = (float)m->a1; pi.a2 = (float)m->a2;
+  pi.gdk = c.geom.interp == FD3_INTERP_EWALD ? 0.0 : ks_gtab_dk(c.plan, c.geom);
+  if ((rc = fft_tables_init(c.plan, c.tab, c.geom.order, c.geom.interp == FD3_INTERP_P3M, s, &pi))) return rc;
+  if (c.geom.interp != FD3_INTERP_EWALD) {
+    const FusedArgs fa = fused_args(m, c, n, pos4, spec_s);
+    if ((rc = col_sort(fa, s))) return rc;
+  }
+  return FD3_OK;
+}
+
+// Everything after the CN: tables and column sort, C6 chain, spread, FFTs,
+// k-space, adjoint.  pos4/spec_s in slot order; slot_atom null for the
+// identity.
 static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4* pos4, const int32_t* spec_s,
                          const int32_t* slot_atom, int flags, const fd3_outputs* out, hipStream_t s) {
   int rc;
   const int r = m->n_rank, nS = m->n_species;
   const bool grad = flags & FD3_NEED_GRAD;
-  PairInit pi;
-  pi.qq = m->qq; pi.nS = nS;
-  pi.s6 = (float)m->s6; pi.s8 = (float)m->s8; pi.a1 = (float)m->a1; pi.a2 = (float)m->a2;
-  pi.gdk = c.geom.interp == FD3_INTERP_EWALD ? 0.0 : ks_gtab_dk(c.plan, c.geom);
-  if ((rc = fft_tables_init(c.plan, c.tab, c.geom.order, c.geom.interp == FD3_INTERP_P3M, s, &pi))) return rc;
+  if ((rc = mesh_prep(m, c, n, pos4, spec_s, s))) return rc;
   C6Args ca;
   ca.n = n; ca.nS = nS; ca.r = r; ca.cn_raw = c.cn_raw; ca.spec_s = spec_s; ca.slot_atom = slot_atom;
   ca.cnref = m->cnref; ca.vq = m->vq; ca.eigs = m->eigs; ca.qq = m->qq; ca.cncorr = m->cncorr;
@@ -432,13 +457,8 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
     }
     return FD3_OK;
   }
-  // atoms into mesh columns, then spreading fused into the forward z pass
-  FusedArgs fa;
-  fa.n = n; fa.nS = nS; fa.r = r;
-  fa.nx = c.geom.nx; fa.ny = c.geom.ny; fa.nz = c.geom.nz; fa.order = c.geom.order; fa.interp = c.geom.interp;
-  for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
-  fa.pos4 = pos4; fa.spec_s = spec_s; fa.charges = c.charges; fa.cb = c.cols;
-  if ((rc = col_sort(fa, s))) return rc;
+  // spreading fused into the forward z pass (atoms sorted into columns by mesh_prep)
+  const FusedArgs fa = fused_args(m, c, n, pos4, spec_s);
   float2* spec = (float2*)c.buf1;
   if ((rc = spread_r2c_z(fa, c.plan, c.tab, c.buf2, s))) return rc;
   if ((rc = fft_y(c.plan, c.tab, c.buf2, spec, -1, s))) return rc;

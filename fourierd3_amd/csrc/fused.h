@@ -18,7 +18,6 @@ struct ColBuffers {
   int32_t* scantmp;   // scratch of the exclusive scan
   int4* rec_i;        // (n) column order
   float4* rec_f;      // (n) column order
-  float* scharge;     // (n, r) column order
   float* accA;        // (n, r) slot order: sum_nodes W dE/drho
   float* accG;        // (n, 3) slot order: sum_q c_q sum_nodes dW/du dE/drho
   float4* part;       // (order, 2, r, n), n in column order: the gather's per-block sums (accA_q, accG_q)
@@ -37,7 +36,9 @@ struct FusedArgs {
   ColBuffers cb;
 };
 
-// sort atoms into mesh columns (after the charges are known)
+// sort atoms into mesh columns (positions and species only: the z passes
+// read each atom's charges through its slot index, so this may run before
+// or beside the CN and C6 kernels)
 int col_sort(const FusedArgs& a, hipStream_t s);
 // spreading + forward z pass: A[c][x][kz][y]
 int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float2* A, hipStream_t s);

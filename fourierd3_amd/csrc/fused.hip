@@ -39,7 +39,6 @@ void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b)
   b->scantmp = ar.take<int32_t>(scan_tmp_elems((int64_t)ncol));
   b->rec_i = ar.take<int4>(nn);
   b->rec_f = ar.take<float4>(nn);
-  b->scharge = ar.take<float>(nn * p.r);
   b->accA = ar.take<float>(nn * p.r);
   b->accG = ar.take<float>(nn * 3);
   b->part = ar.take<float4>((size_t)nn * std::max(order, 1) * 2 * p.r);
@@ -74,7 +73,6 @@ __global__ void k_col_scatter(FusedArgs a) {
   const int pos = a.cb.colstart[key] + atomicAdd(a.cb.colcur + key, 1);
   a.cb.rec_i[pos] = a.cb.tmp_i[s];
   a.cb.rec_f[pos] = a.cb.tmp_f[s];
-  for (int q = 0; q < a.r; ++q) a.cb.scharge[(size_t)pos * a.r + q] = a.charges[s * a.r + q];
 }
 
 
@@ -214,7 +212,7 @@ __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* 
     const float4 xf = a.cb.rec_f[ai];
     float w[kMaxOrder], dw[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder];
     interp_weights<float>(p, a.interp, xf.x, w, dw);
-    const float wx = pick(w, sg.jx[k], p) * a.cb.scharge[(size_t)ai * a.r + q];
+    const float wx = pick(w, sg.jx[k], p) * a.charges[(size_t)bi.w * a.r + q];
     interp_weights<float>(p, a.interp, xf.y, wy, dw);
     interp_weights<float>(p, a.interp, xf.z, wz, dw);
     for (int jy = 0; jy < p; ++jy) {
@@ -296,7 +294,7 @@ __global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const fl
       g1 += dwy[jy] * sz;
       g2 += wy[jy] * sdz;
     }
-    const float cq = a.cb.scharge[(size_t)ai * a.r + q];
+    const float cq = a.charges[(size_t)bi.w * a.r + q];
     g0 = dwx * acc;
     // slot h = 0: the tile holding the atom's first y row, h = 1: the next
     // tile (rows that run past the first tile); the h = 0 block zeroes the

@@ -498,15 +498,14 @@ __device__ inline void r2c_split(float2* L, int M, int k, const float2* __restri
   } else {
     const int km = M - k;
     const float2 zk = L[k], zm = L[km];
-    // E_k = (zk + conj zm)/2, O_k = -i (zk - conj zm)/2, X_k = E_k + W^k O_k
+    // E_k = (zk + conj zm)/2, O_k = -i (zk - conj zm)/2, X_k = E_k + W^k O_k.
+    // E_{M-k} = conj E_k, O_{M-k} = conj O_k and W^{M-k} = -conj W^k, so
+    // X_{M-k} = conj(E_k - W^k O_k).
     const float2 Ek = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
     const float2 Ok = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
-    const float2 Em = make_float2(0.5f * (zm.x + zk.x), 0.5f * (zm.y - zk.y));
-    const float2 Om = make_float2(0.5f * (zm.y + zk.y), -0.5f * (zm.x - zk.x));
-    const float2 Xk = cadd(Ek, cmul(twz[k], Ok));
-    const float2 Xm = cadd(Em, cmul(twz[km], Om));
-    L[k] = Xk;
-    L[km] = Xm;
+    const float2 T = cmul(twz[k], Ok);
+    L[k] = cadd(Ek, T);
+    L[km] = make_float2(Ek.x - T.x, T.y - Ek.y);
   }
 }
 
@@ -519,12 +518,11 @@ __device__ inline void c2r_merge(float2* L, int M, int k, const float2* __restri
   } else {
     const int km = M - k;
     const float2 xk = L[k], xm = L[km];
+    // A_{M-k} = conj A_k; with w^{M-k} = -conj w^k, B_{M-k} = conj B_k
     const float2 Ak = make_float2(xk.x + xm.x, xk.y - xm.y);
     const float2 Bk = cmulc(make_float2(xk.x - xm.x, xk.y + xm.y), twz[k]);
-    const float2 Am = make_float2(xm.x + xk.x, xm.y - xk.y);
-    const float2 Bm = cmulc(make_float2(xm.x - xk.x, xm.y + xk.y), twz[km]);
     L[k] = make_float2(Ak.x - Bk.y, Ak.y + Bk.x);
-    L[km] = make_float2(Am.x - Bm.y, Am.y + Bm.x);
+    L[km] = make_float2(Ak.x + Bk.y, Bk.x - Ak.y);
   }
 }
 

@@ -146,8 +146,8 @@ __global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, 
   }
 }
 
-// Device-wide exclusive scan in three steps: per-block scans of 1024
-// elements, a scan of the block totals (k_scan1), and the offset add.
+// Device-wide exclusive scan in two steps: per-block scans of 1024
+// elements, then the offset add (k_scan_add).
 __global__ void __launch_bounds__(256) k_scan_blocks(const int32_t* __restrict__ in, int64_t n,
                                                      int32_t* __restrict__ out, int32_t* __restrict__ bsum) {
   __shared__ int32_t sh[256];
@@ -174,30 +174,42 @@ __global__ void __launch_bounds__(256) k_scan_blocks(const int32_t* __restrict__
   if (threadIdx.x == 255) bsum[blockIdx.x] = sh[255];
 }
 
-__global__ void k_scan_add(int32_t* __restrict__ out, int64_t n, const int32_t* __restrict__ boff,
-                           int64_t nblocks, int64_t* __restrict__ total64) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += boff[i / 1024];
-  if (i == 0) {
-    out[n] = boff[nblocks];
-    if (total64) total64[0] = boff[nblocks];
+// Block b of 1024 elements adds the sum of the block totals before it (each
+// block reduces that prefix itself, so no separate scan of the totals is
+// launched); the last block also writes out[n] = total.
+__global__ void __launch_bounds__(1024) k_scan_add(int32_t* __restrict__ out, int64_t n,
+                                                   const int32_t* __restrict__ bsum, int64_t* __restrict__ total64) {
+  __shared__ int64_t red[16];
+  const int64_t b = blockIdx.x;
+  int64_t v = 0;
+  for (int64_t i = threadIdx.x; i < b; i += 1024) v += bsum[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int64_t off = 0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w) off += red[w];
+  const int64_t i = b * 1024 + threadIdx.x;
+  if (i < n) out[i] += (int32_t)off;
+  if (b == (int64_t)gridDim.x - 1 && threadIdx.x == 0) {
+    const int64_t tot = off + bsum[b];
+    out[n] = (int32_t)tot;
+    if (total64) total64[0] = tot;
   }
 }
 
 int scan_exclusive(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, int64_t* total, hipStream_t s) {
   const int64_t nb = std::max<int64_t>(1, cdiv(n, 1024));
   int32_t* bsum = tmp;
-  int32_t* boff = tmp + nb;
   hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, in, n, out, bsum);
   FD3_LAUNCH_CHECK("k_scan_blocks");
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, bsum, nb, boff, (int64_t*)nullptr);
-  FD3_LAUNCH_CHECK("k_scan1");
-  hipLaunchKernelGGL(k_scan_add, dim3(cdiv(std::max<int64_t>(n, 1), 256)), dim3(256), 0, s, out, n, boff, nb, total);
+  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, out, n, bsum, total);
   FD3_LAUNCH_CHECK("k_scan_add");
   return FD3_OK;
 }
 
-size_t scan_tmp_elems(int64_t n) { return 2 * (size_t)std::max<int64_t>(1, cdiv(n, 1024)) + 1; }
+size_t scan_tmp_elems(int64_t n) { return (size_t)std::max<int64_t>(1, cdiv(n, 1024)); }
 
 __global__ void k_scatter(const int32_t* __restrict__ abin, int64_t n, const int32_t* __restrict__ bstart,
                           int32_t* __restrict__ bcur, int32_t* __restrict__ slot_atom) {

@@ -227,15 +227,17 @@ class FourierD3ASECalculator(_Base):
         (``src/fourierd3/calculator.py:125-192``)."""
         super().calculate(atoms, properties, system_changes)
         atoms = atoms if atoms is not None else self.atoms
-        if self.cnfunc != "smooth_cut":
-            raise ValueError(f"Unknown cnfunc '{self.cnfunc}'. Expected 'smooth_cut' or 'd4'.") \
-                if self.cnfunc != "d4" else NotImplementedError("cnfunc='d4' is not built")
+        if self.cnfunc not in ("smooth_cut", "d4"):
+            raise ValueError(f"Unknown cnfunc '{self.cnfunc}'. Expected 'smooth_cut' or 'd4'.")
         if self._model is None:
             self._build_model(atoms)
         cell = _cell_array(atoms)
         self._model._update_cell(torch.tensor(cell, dtype=torch.float64))
         pos = torch.tensor(np.asarray(atoms.positions, dtype=np.float64), device=self.device)
-        res = self.evaluate(pos, cell)
+        if self.cnfunc == "d4":   # calculator.py:169-171: positions only, CN in k-space
+            res = self._model._run_d4(pos, torch.tensor(cell, dtype=torch.float64), True, False)
+        else:
+            res = self.evaluate(pos, cell)
         e_ha = res["energy"].item()
         volume_b = abs(np.linalg.det(cell)) * self.angstrom_to_bohr ** 3
         self.results["energy"] = e_ha * self.HARTREE_TO_EV

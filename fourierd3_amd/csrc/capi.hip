@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "fd3_internal.h"
+#include "cn_d4.h"
 #include "ewald.h"
 #include "fft_plan.h"
 #include "fused.h"
@@ -422,11 +423,12 @@ static int mesh_prep(const fd3_model* m, Common& c, int64_t n, const double4* po
 // k-space, adjoint.  pos4/spec_s in slot order; slot_atom null for the
 // identity.
 static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4* pos4, const int32_t* spec_s,
-                         const int32_t* slot_atom, int flags, const fd3_outputs* out, hipStream_t s) {
+                         const int32_t* slot_atom, int flags, const fd3_outputs* out, hipStream_t s,
+                         bool prepped = false) {
   int rc;
   const int r = m->n_rank, nS = m->n_species;
   const bool grad = flags & FD3_NEED_GRAD;
-  if ((rc = mesh_prep(m, c, n, pos4, spec_s, s))) return rc;
+  if (!prepped && (rc = mesh_prep(m, c, n, pos4, spec_s, s))) return rc;
   C6Args ca;
   ca.n = n; ca.nS = nS; ca.r = r; ca.cn_raw = c.cn_raw; ca.spec_s = spec_s; ca.slot_atom = slot_atom;
   ca.cnref = m->cnref; ca.vq = m->vq; ca.eigs = m->eigs; ca.qq = m->qq; ca.cncorr = m->cncorr;
@@ -507,11 +509,142 @@ static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, 
   return FD3_OK;
 }
 
+// ------------------------------------------------------------- D4 CN path ---
+struct D4Bufs {
+  FftPlan pcn;      // the mesh plan with one rank: channels = species
+  CnKsPlan kp;
+  float2* rho_hat;  // (nS, M1, ny, nx) x-transformed species density spectrum
+  float* ones;      // (n) unit charges of the one-hot density
+  float* g1;        // (n, 3) forward gather's sum_nodes dW/du phi
+};
+
+static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n, Common& c, Arena& ar,
+                     double4** pos4, int32_t** spec_s, D4Bufs& db) {
+  int rc;
+  if ((rc = setup_common(m, g, c))) return rc;
+  if (!d4) {
+    set_error("D4 CN parameters are NULL");
+    return FD3_ERR_ARG;
+  }
+  if (g->interp == FD3_INTERP_EWALD) {
+    set_error("cnfunc='d4' with method='ewald' is not built");
+    return FD3_ERR_UNSUPPORTED;
+  }
+  if (n < 0) {
+    set_error("negative atom count");
+    return FD3_ERR_ARG;
+  }
+  const int nS = m->n_species;
+  db.pcn = c.plan;
+  db.pcn.r = 1;
+  db.pcn.C = nS;
+  if ((rc = cn_ks_plan(db.pcn, nS, &db.kp))) return rc;
+  carve_common(ar, c, n, m->n_rank, db.kp.blocks + (int)cdiv(std::max<int64_t>(n, 1), 256));
+  *pos4 = ar.take<double4>(std::max<int64_t>(n, 1));
+  *spec_s = ar.take<int32_t>(std::max<int64_t>(n, 1));
+  db.rho_hat = ar.take<float2>(db.pcn.mesh_complex());
+  db.ones = ar.take<float>(std::max<int64_t>(n, 1));
+  db.g1 = ar.take<float>(std::max<int64_t>(3 * n, 3));
+  return FD3_OK;
+}
+
+// The spread / y / k-space / y / gather passes of the D4 CN over the species
+// channels: forward (charges 1, the CN and dW phi per atom) or backward
+// (charges dE/dCN, the force and the strain derivative).
+static int d4_passes(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs& db, int64_t n,
+                     const double4* pos4, const int32_t* spec_s, bool bwd, hipStream_t s) {
+  int rc;
+  FusedArgs fa = fused_args(m, c, n, pos4, spec_s);
+  fa.r = 1;
+  fa.charges = db.ones;
+  FusedArgs fs = fa;
+  if (bwd) fs.charges = c.dEdcn;
+  float2* spec = (float2*)c.buf1;
+  if ((rc = spread_r2c_z(fs, db.pcn, c.tab, c.buf2, s))) return rc;
+  if ((rc = fft_y(db.pcn, c.tab, c.buf2, spec, -1, s))) return rc;
+  CnKsArgs ka;
+  ka.spec = spec; ka.rho_hat = db.rho_hat; ka.twx = c.tab.twx;
+  ka.scx = c.tab.scx; ka.scy = c.tab.scy; ka.scz = c.tab.scz;
+  ka.r0 = d4->r0; ka.delta = d4->delta; ka.k0 = (float)d4->k0;
+  ka.partials = c.p_cn; ka.inv_vol = 1.0 / c.geom.volume;
+  for (int i = 0; i < 9; ++i) ka.recip[i] = (float)c.geom.recip[i];
+  ka.nx = c.geom.nx; ka.ny = c.geom.ny; ka.nz = c.geom.nz; ka.M1 = c.plan.M1; ka.nS = m->n_species;
+  ka.NK = db.kp.NK; ka.KT = db.kp.KT; ka.use_sinc = c.geom.interp == FD3_INTERP_P3M;
+  if ((rc = cn_ks_run(ka, db.kp, bwd, s))) return rc;
+  if ((rc = fft_y(db.pcn, c.tab, spec, c.buf2, 1, s))) return rc;
+  if ((rc = c2r_z_gather(fa, db.pcn, c.tab, c.buf2, s))) return rc;  // and the per-atom sums
+  CnAtomArgs aa;
+  aa.n = n; aa.nx = c.geom.nx; aa.ny = c.geom.ny; aa.nz = c.geom.nz;
+  for (int i = 0; i < 9; ++i) aa.inv[i] = c.geom.inv[i];
+  aa.inv_vol = 1.0 / c.geom.volume;
+  aa.spec_s = spec_s; aa.selfcont = d4->selfcont;
+  aa.accA = c.cols.accA; aa.accG = c.cols.accG;
+  aa.cn = c.cn_raw; aa.g1 = db.g1; aa.dEdcn = c.dEdcn; aa.gpos = c.gpos;
+  aa.partials = c.p_cn + (size_t)db.kp.blocks * 8;
+  return bwd ? cn_d4_force(aa, s) : cn_d4_fin(aa, s);
+}
+
 }  // namespace fd3
 
 using namespace fd3;
 
 extern "C" {
+
+int fd3_eval_d4_workspace(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n_atoms, int flags,
+                          size_t* bytes) {
+  (void)flags;
+  Common c;
+  Arena ar;
+  double4* p4;
+  int32_t* sp;
+  D4Bufs db;
+  int rc = d4_layout(m, d4, g, n_atoms, c, ar, &p4, &sp, db);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+int fd3_eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n, const void* positions,
+                int pos_dtype, const int32_t* species, int flags, void* workspace, size_t ws_bytes,
+                const fd3_outputs* out, void* stream) {
+  hipStream_t s = (hipStream_t)stream;
+  Common c;
+  Arena ar(workspace, ws_bytes);
+  double4* pos4;
+  int32_t* spec_s;
+  D4Bufs db;
+  int rc = d4_layout(m, d4, g, n, c, ar, &pos4, &spec_s, db);
+  if (rc) return rc;
+  if (!ar.ok() || !workspace) {
+    set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
+    return FD3_ERR_WORKSPACE;
+  }
+  if (!out) {
+    set_error("outputs are NULL");
+    return FD3_ERR_ARG;
+  }
+  if (n > 0 && (!positions || !species)) {
+    set_error("input pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  if (!d4->r0 || !d4->delta || !d4->selfcont) {
+    set_error("D4 CN tables are NULL");
+    return FD3_ERR_ARG;
+  }
+  const bool grad = flags & FD3_NEED_GRAD;
+  if ((rc = launch_prep_pos(positions, pos_dtype, species, n, pos4, spec_s, s))) return rc;
+  if ((rc = mesh_prep(m, c, n, pos4, spec_s, s))) return rc;
+  if ((rc = fill_value(db.ones, n, 1.0f, s))) return rc;
+  if ((rc = d4_passes(m, d4, c, db, n, pos4, spec_s, false, s))) return rc;
+  if ((rc = run_mesh_path(m, c, n, pos4, spec_s, nullptr, flags, out, s, true))) return rc;
+  if (grad && n > 0) {
+    if ((rc = d4_passes(m, d4, c, db, n, pos4, spec_s, true, s))) return rc;
+    if (out->grad_pos && (rc = launch_unpermute(c.gpos, nullptr, n, out->grad_pos, s))) return rc;
+  }
+  fd3_outputs o2 = *out;
+  o2.grad_cell = nullptr;  // the D4 path reports the strain derivative only
+  return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, n > 0, &o2, s);
+}
 
 const char* fd3_last_error(void) { return g_err.c_str(); }
 int fd3_version(void) { return 1; }

@@ -1,0 +1,56 @@
+// D4 coordination numbers in reciprocal space (cn_d4.hip).
+#pragma once
+
+#include "fd3_internal.h"
+#include "fft_plan.h"
+
+namespace fd3 {
+
+// Arguments of the D4 CN k-space kernel.  spec: the species spectra after the
+// z and y passes (channel X = species, r = 1), transformed in x and
+// overwritten with the convolution; rho_hat: the x-transformed density
+// spectrum (written by the forward variant, read by the backward one).
+struct CnKsArgs {
+  float2* spec;
+  float2* rho_hat;
+  const float2* twx;
+  const float* scx; const float* scy; const float* scz;
+  const float* r0;     // (nS, nS) Bohr
+  const float* delta;  // (nS, nS)
+  float k0;
+  double* partials;    // (blocks, 8): strain-derivative sums of the backward variant
+  double inv_vol;      // 1 / Omega (Bohr^-3)
+  float recip[9];      // rows b_i of 2 pi cell^-T (Bohr^-1)
+  int nx, ny, nz, M1, nS, NK, KT, use_sinc;
+};
+
+struct CnKsPlan {
+  int NK, KT, blocks;
+  size_t lds;
+};
+
+int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp);
+int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s);
+
+// Per-atom steps (slot order).
+struct CnAtomArgs {
+  int64_t n;
+  int nx, ny, nz;
+  double inv[9];      // cell^-1 (Bohr^-1)
+  double inv_vol;     // 1 / Omega (Bohr^-3)
+  const int32_t* spec_s;
+  const float* selfcont;  // (nS) f_XX(0)
+  const float* accA;  // (n) sum_nodes W phi (forward gather)
+  const float* accG;  // (n, 3) sum_nodes dW/du of the gathered field
+  float* cn;          // (n) D4 CN (written by cn_d4_fin)
+  float* g1;          // (n, 3) forward gather's sum_nodes dW/du phi, kept for the force
+  const float* dEdcn; // (n) dE/dCN (backward)
+  double* gpos;       // (n, 3) dE/dr (Bohr), added to
+  double* partials;   // (blocks, 8) strain-derivative sums
+};
+
+int cn_d4_fin(const CnAtomArgs& a, hipStream_t s);
+int cn_d4_force(const CnAtomArgs& a, hipStream_t s);
+int fill_value(float* p, int64_t n, float v, hipStream_t s);
+
+}  // namespace fd3

@@ -1,0 +1,248 @@
+// D4 coordination numbers in reciprocal space (cnfunc = 'd4'):
+//   CN_i = (1/Omega) sum_nodes W_i phi_{X_i} - f_{X_i X_i}(0),
+//   phi_X = C2R( sum_Y G^CN_XY(|k|) / sinc^2p(k) * R2C(rho_Y) ),
+// rho_Y the one-hot density of species Y spread with the same weights as the
+// D3 path (src/fourierd3/core.py:389-447, src/fourierd3/kspace_filter_d3.py:101-158).
+// G^CN is the Fourier transform of f_XY(r) = (delta_XY / 2) erfc(k0 (r - r0_XY) / r0_XY)
+// (src/fourierd3/pair_pot.py:235-402).
+//
+// The mesh passes reuse the fused spread / z FFT, y FFT and z FFT / gather
+// kernels with one rank (channels = species).  The k-space step is its own
+// kernel: x FFT, the species-pair convolution, inverse x FFT.  Its backward
+// variant applies G^CN to the dE/dCN-weighted density and accumulates the
+// strain derivative of sum_j dE/dCN_j CN_j, for which the forward variant
+// stores the x-transformed density spectrum.
+
+#include "cn_d4.h"
+#include "fft_lds.h"
+
+namespace fd3 {
+
+// G^CN(k) and dG^CN/dk of one pair, float.  x = k r0:
+//   G = 4 pi delta r0^3 exp(-x^2 / (4 k0^2)) H(x),  H = (sin x - x cos x)/x^3 + sin x / (2 k0^2 x),
+// the closed form of pair_pot.py:341-370; below x = 1 P(x) = (sin x - x cos
+// x)/x^3 and sin x / x come from their Taylor series (the closed form loses
+// digits to cancellation there; the reference switches to a k^2 expansion
+// below x = 1e-3).
+__device__ inline void gcn_eval(float k, float r0, float delta, float k0, float* g, float* dg) {
+  const float x = k * r0;
+  const float a = 0.25f / (k0 * k0), b = 0.5f / (k0 * k0);
+  float P, Pd, Q, Qd;  // P, Q and their x-derivatives
+  if (x < 1.0f) {
+    const float x2 = x * x;
+    P = 1.0f / 3.0f + x2 * (-1.0f / 30.0f + x2 * (1.0f / 840.0f + x2 * (-1.0f / 45360.0f + x2 * (1.0f / 3991680.0f))));
+    Pd = x * (-1.0f / 15.0f + x2 * (1.0f / 210.0f + x2 * (-1.0f / 7560.0f + x2 * (1.0f / 498960.0f))));
+    Q = 1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f + x2 * (1.0f / 362880.0f))));
+    Qd = x * (-1.0f / 3.0f + x2 * (1.0f / 30.0f + x2 * (-1.0f / 840.0f + x2 * (1.0f / 45360.0f))));
+  } else {
+    float s, c;
+    sincosf(x, &s, &c);
+    const float ix = 1.0f / x, ix2 = ix * ix;
+    P = (s - x * c) * ix2 * ix;
+    Pd = s * ix2 - 3.0f * P * ix;
+    Q = s * ix;
+    Qd = (c - Q) * ix;
+  }
+  const float H = P + b * Q, Hd = Pd + b * Qd;
+  const float E = expf(-a * x * x);
+  const float pre = 4.0f * 3.14159265358979f * delta * r0 * r0 * r0 * E;
+  *g = pre * H;
+  *dg = pre * r0 * (Hd - 2.0f * a * x * H);
+}
+
+__device__ inline float sinc_prod(const CnKsArgs& a, int kx, int ky, int kz) {
+  if (!a.use_sinc) return 1.0f;
+  return 1.0f / fmaxf(a.scx[kx] * a.scy[ky] * a.scz[kz], 1e-10f);
+}
+
+// One block: kz plane blockIdx.y, ky rows [ky0, ky0 + NK) for KT tiles.
+// LDS: [x twiddles | S lines | P lines | line bases]; S holds the spectra of
+// the nS species (line l = X * NK + kyi), P the convolution being formed.
+template <bool BWD>
+__global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
+  extern __shared__ float2 lds[];
+  __shared__ double red[16 * 6];
+  const int nx = a.nx, nS = a.nS, NK = a.NK;
+  const int kz = blockIdx.y;
+  const int nlines = nS * NK, ls = nx + 1, lnx = ilog2(nx);
+  float2* tws = lds;
+  float2* S = tws + nx;
+  float2* P = S + nlines * ls;
+  long long* lbase = (long long*)(P + nlines * ls);
+  stage_table(tws, a.twx, nx);
+  const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
+  double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int it = 0; it < a.KT; ++it) {
+    const int ky0 = (blockIdx.x * a.KT + it) * NK;
+    if (it) __syncthreads();
+    for (int l = threadIdx.x; l < nlines; l += blockDim.x) {
+      const int kyi = l % NK, X = l / NK;
+      lbase[l] = (((long long)X * a.M1 + kz) * a.ny + ky0 + kyi) * nx;
+    }
+    __syncthreads();
+    lds_fft<-1, kKsGenericMaxB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
+    if (!BWD) {  // keep the x-transformed density for the backward pass
+      tile2_from_lds<kCopyU>(a.rho_hat, S, TileShape{nlines, lnx, 1, ls, 1}, LineTable{lbase});
+      __syncthreads();  // the copy reads other threads' columns before they are overwritten
+    }
+    for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
+      const int kyi = t >> lnx, kx = t & (nx - 1), ky = ky0 + kyi;
+      const float mx = (float)(kx < (nx + 1) / 2 ? kx : kx - nx);
+      const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
+      const float mz = (float)kz;
+      const float k0v = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
+      const float k1v = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
+      const float k2v = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
+      const float kn = sqrtf(k0v * k0v + k1v * k1v + k2v * k2v);
+      const float isc = sinc_prod(a, kx, ky, kz);
+      for (int X = 0; X < nS; ++X) P[(X * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
+      float vs = 0.f;  // sum_XY dG/dk Re(conj(S_X) rho_Y)
+      for (int X = 0; X < nS; ++X) {
+        const int ix = (X * NK + kyi) * ls + kx;
+        const float2 sx = S[ix];
+        float2 rx = make_float2(0.f, 0.f);
+        if (BWD) rx = a.rho_hat[lbase[X * NK + kyi] + kx];
+        for (int Y = X; Y < nS; ++Y) {
+          const int iy = (Y * NK + kyi) * ls + kx;
+          float g, dg;
+          gcn_eval(kn, a.r0[X * nS + Y], a.delta[X * nS + Y], a.k0, &g, &dg);
+          g *= isc;
+          const float2 sy = S[iy];
+          float2 px = P[ix];
+          px.x += g * sy.x; px.y += g * sy.y;
+          P[ix] = px;
+          if (Y != X) {
+            float2 py = P[iy];
+            py.x += g * sx.x; py.y += g * sx.y;
+            P[iy] = py;
+          }
+          if (BWD) {
+            const float2 ry = a.rho_hat[lbase[Y * NK + kyi] + kx];
+            float t2 = sx.x * ry.x + sx.y * ry.y;
+            if (Y != X) t2 += sy.x * rx.x + sy.y * rx.y;
+            vs += dg * isc * t2;
+          }
+        }
+      }
+      for (int X = 0; X < nS; ++X) S[(X * NK + kyi) * ls + kx] = P[(X * NK + kyi) * ls + kx];
+      if (BWD && kn > 0.f) {
+        // d|k|/d eps_ab = -k_a k_b / |k| under the strain cell' = (1 + eps) cell
+        const double f = -(double)w * vs / kn;
+        vir[0] += f * k0v * k0v; vir[1] += f * k1v * k1v; vir[2] += f * k2v * k2v;
+        vir[3] += f * k0v * k1v; vir[4] += f * k0v * k2v; vir[5] += f * k1v * k2v;
+      }
+    }
+    __syncthreads();
+    lds_fft<1, kKsGenericMaxB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
+  }
+  if (BWD) {
+    const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) vir[i] *= a.inv_vol;
+    block_sum_store<6>(vir, red, a.partials + (size_t)bid * 8);
+  }
+}
+
+int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
+  int NK = std::max(1, std::min(p.ny, 256 / p.nx));
+  while (NK > 1 && (size_t)nS * NK * p.nx > (size_t)1024 * kKsGenericMaxB) NK >>= 1;
+  if ((size_t)nS * NK * p.nx > (size_t)1024 * kKsGenericMaxB) {
+    set_error("D4 CN k-space tile of %d species x %d points exceeds the kernel's FFT capacity", nS, p.nx);
+    return FD3_ERR_UNSUPPORTED;
+  }
+  kp->NK = NK;
+  kp->KT = 1;
+  while (kp->KT < 4 && (p.ny / NK) % (2 * kp->KT) == 0 && (size_t)(p.ny / NK / (2 * kp->KT)) * p.M1 >= 4096)
+    kp->KT <<= 1;
+  kp->blocks = (p.ny / NK / kp->KT) * p.M1;
+  kp->lds = (size_t)p.nx * sizeof(float2) + 2 * (size_t)nS * NK * (p.nx + 1) * sizeof(float2) +
+            (size_t)nS * NK * sizeof(long long);
+  if (kp->lds > (size_t)kKsGenericLds) {
+    set_error("D4 CN k-space tile of %d species x %d points exceeds the LDS of a workgroup", nS, p.nx);
+    return FD3_ERR_UNSUPPORTED;
+  }
+  return FD3_OK;
+}
+
+int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s) {
+  if (kp.lds > (size_t)64 * 1024) {
+    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kp.lds));
+    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)kp.lds));
+  }
+  const dim3 grid(a.ny / kp.NK / kp.KT, a.M1);
+  CnKsArgs b = a;
+  b.NK = kp.NK;
+  b.KT = kp.KT;
+  ProfScope _ps(bwd ? "k_kspace_cn_bwd" : "k_kspace_cn", s);
+  if (bwd) hipLaunchKernelGGL(k_kspace_cn<true>, grid, dim3(256), kp.lds, s, b);
+  else hipLaunchKernelGGL(k_kspace_cn<false>, grid, dim3(256), kp.lds, s, b);
+  FD3_LAUNCH_CHECK("k_kspace_cn");
+  return FD3_OK;
+}
+
+// CN from the forward gather (accA = sum W phi, accG = sum dW/du phi): CN =
+// accA / Omega - f_XX(0); accG is kept for the force.
+__global__ void k_cn_d4_fin(CnAtomArgs a) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  a.cn[s] = (float)((double)a.accA[s] * a.inv_vol) - a.selfcont[a.spec_s[s]];
+  for (int d = 0; d < 3; ++d) a.g1[3 * s + d] = a.accG[3 * s + d];
+}
+
+int cn_d4_fin(const CnAtomArgs& a, hipStream_t s) {
+  if (a.n <= 0) return FD3_OK;
+  ProfScope _ps("k_cn_d4_fin", s);
+  hipLaunchKernelGGL(k_cn_d4_fin, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
+  FD3_LAUNCH_CHECK("k_cn_d4_fin");
+  return FD3_OK;
+}
+
+// Force and the per-atom part of the strain derivative from the backward
+// gather (accG = sum dW/du psi):
+//   dE/du_i = (g_i g1_i + accG_i) / Omega  (mesh coordinates u = n f),
+//   dE/dr_b += sum_d dE/du_d n_d cell^-1[b][d],
+//   virial_ab -= delta_ab sum_i g_i (CN_i + f_XX(0))   (the 1/Omega of every CN).
+__global__ void __launch_bounds__(256) k_cn_d4_force(CnAtomArgs a) {
+  __shared__ double red[16 * 6];
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (s < a.n) {
+    const double g = a.dEdcn[s];
+    const double ns[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
+    double du[3];
+    for (int d = 0; d < 3; ++d) du[d] = (g * a.g1[3 * s + d] + (double)a.accG[3 * s + d]) * a.inv_vol;
+    for (int b = 0; b < 3; ++b) {
+      double t = 0.0;
+      for (int d = 0; d < 3; ++d) t += du[d] * ns[d] * a.inv[b * 3 + d];
+      a.gpos[3 * s + b] += t;
+    }
+    const double e = -g * ((double)a.cn[s] + (double)a.selfcont[a.spec_s[s]]);
+    v[0] = v[1] = v[2] = e;
+  }
+  block_sum_store<6>(v, red, a.partials + (size_t)blockIdx.x * 8);
+}
+
+int cn_d4_force(const CnAtomArgs& a, hipStream_t s) {
+  if (a.n <= 0) return FD3_OK;
+  ProfScope _ps("k_cn_d4_force", s);
+  hipLaunchKernelGGL(k_cn_d4_force, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
+  FD3_LAUNCH_CHECK("k_cn_d4_force");
+  return FD3_OK;
+}
+
+// A float buffer set to one value.
+__global__ void k_fill_value(float* p, int64_t n, float v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+int fill_value(float* p, int64_t n, float v, hipStream_t s) {
+  if (n <= 0) return FD3_OK;
+  hipLaunchKernelGGL(k_fill_value, dim3(cdiv(n, 256)), dim3(256), 0, s, p, n, v);
+  FD3_LAUNCH_CHECK("k_fill_value");
+  return FD3_OK;
+}
+
+}  // namespace fd3

@@ -199,6 +199,24 @@ def get_ns_mesh(cell_bohr: np.ndarray, mesh_spacing_bohr: float) -> np.ndarray:
     return np.array([1 << int(math.ceil(math.log2(a))) for a in approx], dtype=np.int64)
 
 
+# D4 CN pair-function constants (src/fourierd3/pair_pot.py:275-279)
+CN_D4_K0, CN_D4_K1, CN_D4_K2, CN_D4_K3 = 7.5, 4.10451, 19.08857, 2 * 11.28174 ** 2
+
+
+def cn_d4_tables(types: Sequence[int]):
+    """Pair constants of ``CNPotential`` (``src/fourierd3/pair_pot.py:296-322``)
+    in float64: r0 = rcov_cn_X + rcov_cn_Y (Bohr), delta = k1 exp(-(|EN_X -
+    EN_Y| + k2)^2 / k3), and the self term f_XX(0) = delta_XX / 2 erfc(-k0)."""
+    import math
+    t = np.asarray(types, dtype=np.int64)
+    rc = load_rcov_cn()[t].astype(np.float64)
+    en = load_en()[t].astype(np.float64)
+    r0 = rc[:, None] + rc[None, :]
+    delta = CN_D4_K1 * np.exp(-(np.abs(en[:, None] - en[None, :]) + CN_D4_K2) ** 2 / CN_D4_K3)
+    selfcont = np.diagonal(delta) / 2.0 * math.erfc(-CN_D4_K0)
+    return {"r0": r0, "delta": delta, "selfcont": selfcont.copy(), "k0": CN_D4_K0}
+
+
 def species_tables(types: Sequence[int], params=PBE_PARAMS):
     """Per-species constants of ``D3Potential`` (``src/fourierd3/pair_pot.py:49-105``)
     and of the CN model (``src/fourierd3/core.py:152-161``), in float64.

@@ -124,6 +124,29 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, co
                       int pos_dtype, const int32_t* species, int flags, void* workspace,
                       size_t workspace_bytes, const fd3_outputs* out, void* stream);
 
+/* fd3_eval_d4: FourierD3ASECalculator.calculate / FourierD3.forward(positions)
+ *   with cnfunc='d4' and method='spme'/'pme' (src/fourierd3/core.py:389-447,
+ *   480-583; src/fourierd3/calculator.py:169-171): the coordination numbers
+ *   come from a k-space convolution of the species densities
+ *   (src/fourierd3/kspace_filter_d3.py:101-158) with the transform of the D4
+ *   pair function (src/fourierd3/pair_pot.py:235-402) on the same mesh, so no
+ *   neighbour list is built.  Energy, and with FD3_NEED_GRAD grad_pos and
+ *   virial (grad_shifts and grad_cell are not produced); cn receives the D4
+ *   CN.  method='ewald' returns FD3_ERR_UNSUPPORTED.
+ */
+typedef struct {
+  const float* r0;        /* (nS, nS) r_cov_X + r_cov_Y of the D4 CN radii, Bohr [device] */
+  const float* delta;     /* (nS, nS) k1 exp(-(|EN_X - EN_Y| + k2)^2 / k3) [device] */
+  const float* selfcont;  /* (nS) f_XX(0) = delta_XX / 2 erfc(-k0) [device] */
+  double k0;              /* erfc steepness (7.5 in the reference) */
+} fd3_cn_d4;
+
+int fd3_eval_d4_workspace(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n_atoms,
+                          int flags, size_t* bytes);
+int fd3_eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n_atoms,
+                const void* positions, int pos_dtype, const int32_t* species, int flags, void* workspace,
+                size_t workspace_bytes, const fd3_outputs* out, void* stream);
+
 /* fd3_eval_celllist_batch: n_boxes independent periodic boxes that share one
  *   model (one species set): box b owns atoms [atom_offsets[b], atom_offsets[b+1])
  *   of the packed positions (N_total, 3) and species (N_total) [device], has its

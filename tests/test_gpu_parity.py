@@ -222,6 +222,59 @@ def test_module_autograd_matches_reference_calculator(dtype):
     assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
+# ---------------------------------------------------------------- D4 CN ---
+@pytest.mark.parametrize("case", ["sio2_spme", "sio2_pme6", "hcno_triclinic_spme"])
+def test_d4_calculator_matches_oracle(case):
+    """cnfunc='d4': CN from the k-space convolution of the species densities
+    (src/fourierd3/core.py:389-447), no neighbour list; E, F and stress against
+    the FP64 restatement on the same mesh."""
+    _say("d4 calculator", case)
+    if case.startswith("sio2"):
+        box = W.strained(W.sio2_box(96, seed=21), 0.1, seed=21)
+    else:
+        box = W.strained(W.hcno_box(500, seed=22), 0.1, seed=22)
+    method, order = ("pme", 6) if case == "sio2_pme6" else ("spme", 5)
+    calc = FourierD3ASECalculator(device=_dev(), cnfunc="d4", method=method, interpolation_nodes=order)
+    calc.calculate(_atoms(box))
+    o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order, cnfunc="d4")
+    assert list(o.ns) == list(calc._model.ns_mesh)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"], e_ref, f_ref,
+                         s_ref, box.n_atoms)
+    print(f"[errors] d4 {case}: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
+def test_d4_cn_and_module_autograd_match_oracle():
+    """compute_cn_d4 against the restatement, and FourierD3(cnfunc='d4').forward
+    + autograd through a strain tensor as the reference calculator drives it
+    (src/fourierd3/calculator.py:146-192): positions and cell gradients."""
+    _say("d4 module")
+    box = W.strained(W.sio2_box(120, seed=23), 0.08, seed=23)
+    dev = _dev()
+    model = FourierD3(box.numbers, box.cell, device=dev, cnfunc="d4", verbose=False)
+    o = OracleFourierD3(box.numbers, box.cell, cnfunc="d4")
+    pos_b = torch.tensor(box.positions, dtype=torch.float64) * ANGSTROM_TO_BOHR
+    cn = model.compute_cn_d4(pos_b.to(dev)).double().cpu().numpy()
+    cn_ref = o.cn_d4(pos_b).numpy()
+    print(f"[errors] d4 cn max {np.abs(cn - cn_ref).max():.3e} (cn up to {np.abs(cn_ref).max():.2f})", flush=True)
+    assert np.abs(cn - cn_ref).max() < 1e-5 * max(1.0, np.abs(cn_ref).max())
+    cell = torch.tensor(box.cell, dtype=torch.float32, device=dev)
+    strain = torch.zeros(3, 3, dtype=torch.float32, device=dev, requires_grad=True)
+    scell = cell + torch.einsum("ab,Ab->Aa", strain, cell)
+    model._update_cell(scell)
+    pos = torch.tensor(box.positions, dtype=torch.float32, device=dev, requires_grad=True)
+    spos = pos + torch.einsum("ab,ib->ia", strain, pos)
+    e = model(spos) * HARTREE_TO_EV
+    e.backward()
+    forces = (-pos.grad).double().cpu().numpy()
+    stress = (strain.grad / model.volume * ANGSTROM_TO_BOHR ** 3).double().detach().cpu().numpy()
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    de, df, ds = _errors(float(e), forces, stress, e_ref, f_ref, s_ref, box.n_atoms)
+    print(f"[errors] d4 module: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
 @pytest.mark.parametrize("order", [4, 6])
 def test_pme_lagrange_matches_oracle(order):
     _say("pme", order)

@@ -71,7 +71,14 @@ def test_module_constructor_checks():
     with pytest.raises(ValueError):
         FourierD3(box.numbers, box.cell, device="cpu", method="p3m", verbose=False)
     with pytest.raises(NotImplementedError):
-        FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d4", verbose=False)
+        FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d4", method="ewald", verbose=False)
+    with pytest.raises(ValueError):
+        FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d3", verbose=False)
+    md4 = FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d4", verbose=False)
+    d4 = T.cn_d4_tables(md4.species_unique)
+    # f_XX(0) = delta_XX / 2 erfc(-k0) ~ delta_XX (pair_pot.py:324-327); r0 symmetric, in Bohr
+    assert np.allclose(d4["selfcont"], np.diagonal(d4["delta"]), rtol=1e-12)
+    assert np.allclose(d4["r0"], d4["r0"].T) and float(md4.cn_selfcont[0]) == pytest.approx(d4["selfcont"][0])
     # Ewald: the k grid is ceil(k_cutoff |a_i| / 2 pi) with the cell in Bohr (core.py:228-230)
     me = FourierD3(box.numbers, box.cell, device="cpu", method="ewald", k_cutoff=4.0, verbose=False)
     la = np.linalg.norm(box.cell * T.ANGSTROM_TO_BOHR, axis=1)

@@ -17,8 +17,8 @@ from scipy.integrate import quad
 from fourierd3_amd import tables as T
 from fourierd3_amd import workloads as W
 from oracle import direct_sum
-from oracle.fd3_oracle import (ANGSTROM_TO_BOHR, OracleFourierD3, cardinal_bspline, green_pair,
-                               interpolation_weights, ns_mesh_rule, oracle_decomp)
+from oracle.fd3_oracle import (ANGSTROM_TO_BOHR, CN_K0, OracleFourierD3, cardinal_bspline, green_cn_pair,
+                               green_pair, interpolation_weights, ns_mesh_rule, oracle_decomp)
 from oracle.neighbours import neighbour_list
 
 
@@ -392,6 +392,69 @@ def test_oracle_forces_and_stress_match_finite_differences():
         e1 = o.calculate(box.positions @ m1.T, box.cell @ m1.T)[0]
         e2 = o.calculate(box.positions @ m2.T, box.cell @ m2.T)[0]
         assert (e1 - e2) / (2 * h) / vol == pytest.approx(s[a, b], abs=2e-9)
+
+
+# ---------------------------------------------------------------- D4 CN ---
+@pytest.mark.parametrize("r0,delta", [(1.6, 1.2), (3.4, 0.9), (7.9, 0.4)])
+def test_green_cn_matches_hankel_transform(r0, delta):
+    """G^CN(k) = 4 pi / k int_0^inf r f(r) sin(kr) dr with f = delta/2 erfc(k0 (r - r0)/r0)
+    (the closed form drops a term of order exp(-k0^2), pair_pot.py:251-256)."""
+    from scipy.special import erfc
+    g0 = float(green_cn_pair(torch.tensor(0.0, dtype=torch.float64), r0, delta))
+    v0, _ = quad(lambda r: r * r * 0.5 * delta * erfc(CN_K0 * (r - r0) / r0), 0, 3 * r0, limit=400)
+    assert g0 == pytest.approx(4 * math.pi * v0, rel=1e-10)
+    for x in [0.3, 1.0, 2.7, 6.0, 13.0, 25.0]:
+        k = x / r0
+        v, _ = quad(lambda r: r * 0.5 * delta * erfc(CN_K0 * (r - r0) / r0) * math.sin(k * r), 0, 3 * r0,
+                    limit=800, epsabs=1e-14, epsrel=1e-13)
+        g = float(green_cn_pair(torch.tensor(k, dtype=torch.float64), r0, delta))
+        assert abs(g - 4 * math.pi / k * v) < 1e-9 * abs(g0)
+
+
+def test_cn_d4_ewald_converges_to_direct_sum():
+    """Ewald D4 CN with a large k cutoff against the real-space sum of the pair
+    function over images, self pair excluded (what f_XX(0) removes)."""
+    from scipy.special import erfc
+    rng = np.random.default_rng(1)
+    cell = np.diag([6.0, 6.6, 5.7]) + 0.3 * rng.standard_normal((3, 3))
+    pos = rng.random((4, 3)) @ cell
+    Z = np.array([8, 14, 1, 6])
+    o = OracleFourierD3(Z, cell, method="ewald", cnfunc="d4", k_cutoff=45.0)
+    cn = o.cn_d4(torch.tensor(pos) * ANGSTROM_TO_BOHR).numpy()
+    pb, cb = pos * ANGSTROM_TO_BOHR, cell * ANGSTROM_TO_BOHR
+    r0, dl, sp = o.cn_r0.numpy(), o.cn_delta.numpy(), o.species.numpy()
+    img = np.array(np.meshgrid(*[np.arange(-6, 7)] * 3, indexing="ij")).reshape(3, -1).T @ cb
+    ref = np.zeros(4)
+    for i in range(4):
+        for j in range(4):
+            d = np.linalg.norm(pb[j] + img - pb[i], axis=1)
+            if i == j:
+                d = d[d > 1e-9]
+            x, y = sp[i], sp[j]
+            ref[i] += (0.5 * dl[x, y] * erfc(CN_K0 * (d - r0[x, y]) / r0[x, y])).sum()
+    assert np.abs(cn - ref).max() < 5e-6
+    assert ref[0] > 0.1 and ref[1] > 0.1  # the O and Si atoms are bonded to something
+
+
+def test_d4_oracle_forces_and_stress_match_finite_differences():
+    box = W.strained(W.sio2_box(48, seed=6), 0.1, seed=6)
+    o = OracleFourierD3(box.numbers, box.cell, ns_mesh=(16, 16, 16), cnfunc="d4")
+    e0, f, s = o.calculate(box.positions, box.cell)
+    h = 1e-4
+    for a in [0, 9, 30]:
+        for d in range(3):
+            p1 = box.positions.copy(); p1[a, d] += h
+            p2 = box.positions.copy(); p2[a, d] -= h
+            fd = -(o.calculate(p1, box.cell)[0] - o.calculate(p2, box.cell)[0]) / (2 * h)
+            assert fd == pytest.approx(f[a, d], abs=5e-7)
+    vol = abs(np.linalg.det(box.cell))
+    h = 1e-5
+    for a, b in [(0, 0), (1, 2), (2, 1)]:
+        eps = np.zeros((3, 3)); eps[a, b] = h
+        m1, m2 = np.eye(3) + eps, np.eye(3) - eps
+        e1 = o.calculate(box.positions @ m1.T, box.cell @ m1.T)[0]
+        e2 = o.calculate(box.positions @ m2.T, box.cell @ m2.T)[0]
+        assert (e1 - e2) / (2 * h) / vol == pytest.approx(s[a, b], abs=5e-9)
 
 
 def test_param_gradient_matches_finite_differences():

@@ -38,6 +38,8 @@ def _args():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--atoms", type=int, default=100_000)
     ap.add_argument("--c6tol", type=float, default=1.0)
+    ap.add_argument("--cnfunc", default="smooth_cut", choices=["smooth_cut", "d4"],
+                    help="CN model (d4: k-space D4 CN, no neighbour list)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain-gpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP events")
@@ -75,7 +77,7 @@ def run_gpu(args):
 
     dev = torch.device("cuda:0")
     box = W.hcno_box(args.atoms, seed=0)
-    calc = FourierD3ASECalculator(device=dev, c6tol=args.c6tol)
+    calc = FourierD3ASECalculator(device=dev, c6tol=args.c6tol, cnfunc=args.cnfunc)
     from types import SimpleNamespace
     calc._build_model(SimpleNamespace(numbers=box.numbers, cell=box.cell, pbc=np.array([True] * 3)))
     model = calc._model
@@ -350,7 +352,7 @@ def main():
         "config": {"workload": f"H/C/N/O cubic box {box.n_atoms} atoms, SPME E+F+stress",
                    "n_atoms": box.n_atoms, "box_A": float(box.cell[0, 0]), "mesh": [int(v) for v in model.ns_mesh],
                    "species": int(model.n_species), "rank": int(model.n_rank), "c6tol": args.c6tol,
-                   "r_cut_A": 6.0, "interpolation_nodes": 5, "mesh_spacing_A": 1.2},
+                   "r_cut_A": 6.0, "interpolation_nodes": 5, "mesh_spacing_A": 1.2, "cnfunc": args.cnfunc},
         "energy_eV": info["energy_ev"],
         "roofline": info["roofline"],
         "kernels": info["kernels"],

@@ -121,6 +121,9 @@ class FourierD3ASECalculator(_Base):
         (3, 3) Hartree (dE/d strain) and, if asked, grad_params (4,).  No host
         synchronisation happens here; ``out`` may hold preallocated outputs."""
         model = self._model
+        if self.cnfunc == "d4":
+            cell_t = torch.as_tensor(np.asarray(cell_A, dtype=np.float64))
+            return model._run_d4(positions, cell_t, need_grad or need_params, need_params)
         n = positions.shape[0]
         g = model._geom(np.asarray(cell_A, dtype=np.float64))
         m = model._model_struct()

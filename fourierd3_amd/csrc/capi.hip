@@ -516,6 +516,7 @@ struct D4Bufs {
   float2* rho_hat;  // (nS, M1, ny, nx) x-transformed species density spectrum
   float* ones;      // (n) unit charges of the one-hot density
   float* g1;        // (n, 3) forward gather's sum_nodes dW/du phi
+  float4* tab;      // per-pair G^CN tables
 };
 
 static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n, Common& c, Arena& ar,
@@ -545,12 +546,26 @@ static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g,
   db.rho_hat = ar.take<float2>(db.pcn.mesh_complex());
   db.ones = ar.take<float>(std::max<int64_t>(n, 1));
   db.g1 = ar.take<float>(std::max<int64_t>(3 * n, 3));
+  db.tab = ar.take<float4>((size_t)nS * (nS + 1) * kGtabN);
   return FD3_OK;
 }
 
 // The spread / y / k-space / y / gather passes of the D4 CN over the species
 // channels: forward (charges 1, the CN and dW phi per atom) or backward
 // (charges dE/dCN, the force and the strain derivative).
+static CnKsArgs d4_ks_args(const fd3_model* m, const fd3_cn_d4* d4, const Common& c, const D4Bufs& db) {
+  CnKsArgs ka;
+  ka.spec = (float2*)c.buf1; ka.rho_hat = db.rho_hat; ka.twx = c.tab.twx;
+  ka.scx = c.tab.scx; ka.scy = c.tab.scy; ka.scz = c.tab.scz;
+  ka.r0 = d4->r0; ka.delta = d4->delta; ka.k0 = (float)d4->k0;
+  ka.tab = db.tab; ka.tkinv = (float)(1.0 / ks_gtab_dk(c.plan, c.geom));
+  ka.partials = c.p_cn; ka.inv_vol = 1.0 / c.geom.volume;
+  for (int i = 0; i < 9; ++i) ka.recip[i] = (float)c.geom.recip[i];
+  ka.nx = c.geom.nx; ka.ny = c.geom.ny; ka.nz = c.geom.nz; ka.M1 = c.plan.M1; ka.nS = m->n_species;
+  ka.NK = db.kp.NK; ka.KT = db.kp.KT; ka.use_sinc = c.geom.interp == FD3_INTERP_P3M;
+  return ka;
+}
+
 static int d4_passes(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs& db, int64_t n,
                      const double4* pos4, const int32_t* spec_s, bool bwd, hipStream_t s) {
   int rc;
@@ -562,14 +577,8 @@ static int d4_passes(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs&
   float2* spec = (float2*)c.buf1;
   if ((rc = spread_r2c_z(fs, db.pcn, c.tab, c.buf2, s))) return rc;
   if ((rc = fft_y(db.pcn, c.tab, c.buf2, spec, -1, s))) return rc;
-  CnKsArgs ka;
-  ka.spec = spec; ka.rho_hat = db.rho_hat; ka.twx = c.tab.twx;
-  ka.scx = c.tab.scx; ka.scy = c.tab.scy; ka.scz = c.tab.scz;
-  ka.r0 = d4->r0; ka.delta = d4->delta; ka.k0 = (float)d4->k0;
-  ka.partials = c.p_cn; ka.inv_vol = 1.0 / c.geom.volume;
-  for (int i = 0; i < 9; ++i) ka.recip[i] = (float)c.geom.recip[i];
-  ka.nx = c.geom.nx; ka.ny = c.geom.ny; ka.nz = c.geom.nz; ka.M1 = c.plan.M1; ka.nS = m->n_species;
-  ka.NK = db.kp.NK; ka.KT = db.kp.KT; ka.use_sinc = c.geom.interp == FD3_INTERP_P3M;
+  const CnKsArgs ka = d4_ks_args(m, d4, c, db);
+  if (!bwd && (rc = cn_tab_init(ka, ks_gtab_dk(c.plan, c.geom), s))) return rc;
   if ((rc = cn_ks_run(ka, db.kp, bwd, s))) return rc;
   if ((rc = fft_y(db.pcn, c.tab, spec, c.buf2, 1, s))) return rc;
   if ((rc = c2r_z_gather(fa, db.pcn, c.tab, c.buf2, s))) return rc;  // and the per-atom sums

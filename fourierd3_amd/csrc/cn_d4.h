@@ -18,6 +18,8 @@ struct CnKsArgs {
   const float* r0;     // (nS, nS) Bohr
   const float* delta;  // (nS, nS)
   float k0;
+  float4* tab;         // (pairs X <= Y, kGtabN, 2) per-pair tables of G^CN
+  float tkinv;         // 1 / (table interval width), Bohr
   double* partials;    // (blocks, 8): strain-derivative sums of the backward variant
   double inv_vol;      // 1 / Omega (Bohr^-3)
   float recip[9];      // rows b_i of 2 pi cell^-T (Bohr^-1)
@@ -25,11 +27,13 @@ struct CnKsArgs {
 };
 
 struct CnKsPlan {
-  int NK, KT, blocks;
-  size_t lds;
+  int NK, KT, blocks, maxb;
+  size_t lds, lds_bwd;  // dynamic LDS of the forward and backward variants
 };
 
 int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp);
+// Builds a.tab (interval width dk) from a.r0, a.delta, a.k0.
+int cn_tab_init(const CnKsArgs& a, double dk, hipStream_t s);
 int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s);
 
 // Per-atom steps (slot order).

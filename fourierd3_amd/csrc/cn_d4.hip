@@ -8,7 +8,9 @@
 //
 // The mesh passes reuse the fused spread / z FFT, y FFT and z FFT / gather
 // kernels with one rank (channels = species).  The k-space step is its own
-// kernel: x FFT, the species-pair convolution, inverse x FFT.  Its backward
+// kernel: x FFT, the species-pair convolution with G^CN from per-pair tables
+// (degree-7 pieces on [0, kmax], as for the D3 Green's function), inverse x
+// FFT.  Its backward
 // variant applies G^CN to the dE/dCN-weighted density and accumulates the
 // strain derivative of sum_j dE/dCN_j CN_j, for which the forward variant
 // stores the x-transformed density spectrum.
@@ -18,36 +20,51 @@
 
 namespace fd3 {
 
-// G^CN(k) and dG^CN/dk of one pair, float.  x = k r0:
-//   G = 4 pi delta r0^3 exp(-x^2 / (4 k0^2)) H(x),  H = (sin x - x cos x)/x^3 + sin x / (2 k0^2 x),
-// the closed form of pair_pot.py:341-370; below x = 1 P(x) = (sin x - x cos
-// x)/x^3 and sin x / x come from their Taylor series (the closed form loses
-// digits to cancellation there; the reference switches to a k^2 expansion
-// below x = 1e-3).
-__device__ inline void gcn_eval(float k, float r0, float delta, float k0, float* g, float* dg) {
-  const float x = k * r0;
-  const float a = 0.25f / (k0 * k0), b = 0.5f / (k0 * k0);
-  float P, Pd, Q, Qd;  // P, Q and their x-derivatives
-  if (x < 1.0f) {
-    const float x2 = x * x;
-    P = 1.0f / 3.0f + x2 * (-1.0f / 30.0f + x2 * (1.0f / 840.0f + x2 * (-1.0f / 45360.0f + x2 * (1.0f / 3991680.0f))));
-    Pd = x * (-1.0f / 15.0f + x2 * (1.0f / 210.0f + x2 * (-1.0f / 7560.0f + x2 * (1.0f / 498960.0f))));
-    Q = 1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f + x2 * (1.0f / 362880.0f))));
-    Qd = x * (-1.0f / 3.0f + x2 * (1.0f / 30.0f + x2 * (-1.0f / 840.0f + x2 * (1.0f / 45360.0f))));
+// G^CN in double (the table's samples): the same closed form and series.
+__host__ __device__ inline double gcn_exact(double k, double r0, double delta, double k0) {
+  const double x = k * r0, a = 0.25 / (k0 * k0), b = 0.5 / (k0 * k0);
+  double P, Q;
+  if (x < 0.5) {
+    const double x2 = x * x;
+    P = 1.0 / 3.0 + x2 * (-1.0 / 30.0 + x2 * (1.0 / 840.0 + x2 * (-1.0 / 45360.0 + x2 * (1.0 / 3991680.0))));
+    Q = 1.0 + x2 * (-1.0 / 6.0 + x2 * (1.0 / 120.0 + x2 * (-1.0 / 5040.0 + x2 * (1.0 / 362880.0))));
   } else {
-    float s, c;
-    sincosf(x, &s, &c);
-    const float ix = 1.0f / x, ix2 = ix * ix;
-    P = (s - x * c) * ix2 * ix;
-    Pd = s * ix2 - 3.0f * P * ix;
-    Q = s * ix;
-    Qd = (c - Q) * ix;
+    const double s = sin(x), c = cos(x);
+    P = (s - x * c) / (x * x * x);
+    Q = s / x;
   }
-  const float H = P + b * Q, Hd = Pd + b * Qd;
-  const float E = expf(-a * x * x);
-  const float pre = 4.0f * 3.14159265358979f * delta * r0 * r0 * r0 * E;
-  *g = pre * H;
-  *dg = pre * r0 * (Hd - 2.0f * a * x * H);
+  return 4.0 * kPi * delta * r0 * r0 * r0 * exp(-a * x * x) * (P + b * Q);
+}
+
+// Per-pair tables of G^CN on [0, kGtabN dk): degree-7 pieces, the layout of
+// the D3 per-pair tables (fit on Chebyshev nodes in double, one thread per
+// node; 32 intervals per block).
+__global__ void __launch_bounds__(256) k_cn_tab_init(const float* r0, const float* delta, float k0, int nS, double dk,
+                                                     float4* tab) {
+  __shared__ double f[256];
+  constexpr int GI = 256 / (kGtabDeg + 1);
+  const int tid = threadIdx.x, np = nS * (nS + 1) / 2;
+  const int it = blockIdx.x * GI + tid / (kGtabDeg + 1), j = tid % (kGtabDeg + 1);
+  if (it < np * kGtabN) {
+    const int pt = it / kGtabN, i = it - pt * kGtabN;  // pair pt = (X, Y >= X) in row order
+    int X = 0, rr = 0;
+    while (pt >= rr + nS - X) { rr += nS - X; ++X; }
+    const int Y = X + (pt - rr);
+    f[tid] = gcn_exact(gtab_node(dk, i, j), r0[X * nS + Y], delta[X * nS + Y], k0);
+  }
+  __syncthreads();
+  const int io = blockIdx.x * GI + tid;
+  if (tid < GI && io < np * kGtabN) gtab_fit(&f[(kGtabDeg + 1) * tid], tab + 2 * io);
+}
+
+int cn_tab_init(const CnKsArgs& a, double dk, hipStream_t s) {
+  const int np = a.nS * (a.nS + 1) / 2;
+  const int GI = 256 / (kGtabDeg + 1);
+  ProfScope _ps("k_cn_tab_init", s);
+  hipLaunchKernelGGL(k_cn_tab_init, dim3(cdiv(np * kGtabN, GI)), dim3(256), 0, s, a.r0, a.delta, a.k0, a.nS, dk,
+                     a.tab);
+  FD3_LAUNCH_CHECK("k_cn_tab_init");
+  return FD3_OK;
 }
 
 __device__ inline float sinc_prod(const CnKsArgs& a, int kx, int ky, int kz) {
@@ -56,9 +73,10 @@ __device__ inline float sinc_prod(const CnKsArgs& a, int kx, int ky, int kz) {
 }
 
 // One block: kz plane blockIdx.y, ky rows [ky0, ky0 + NK) for KT tiles.
-// LDS: [x twiddles | S lines | P lines | line bases]; S holds the spectra of
-// the nS species (line l = X * NK + kyi), P the convolution being formed.
-template <bool BWD>
+// LDS: [x twiddles | S lines | P lines | line bases | R lines (backward)]; S
+// holds the spectra of the nS species (line l = X * NK + kyi), P the
+// convolution being formed, R the stored density spectrum of the same lines.
+template <bool BWD, int MAXB>
 __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
   extern __shared__ float2 lds[];
   __shared__ double red[16 * 6];
@@ -69,6 +87,7 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
   float2* S = tws + nx;
   float2* P = S + nlines * ls;
   long long* lbase = (long long*)(P + nlines * ls);
+  float2* R = (float2*)(lbase + nlines);  // backward only: the stored density spectrum of the tile
   stage_table(tws, a.twx, nx);
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
   double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -80,10 +99,13 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
       lbase[l] = (((long long)X * a.M1 + kz) * a.ny + ky0 + kyi) * nx;
     }
     __syncthreads();
-    lds_fft<-1, kKsGenericMaxB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
+    lds_fft<-1, MAXB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
     if (!BWD) {  // keep the x-transformed density for the backward pass
-      tile2_from_lds<kCopyU>(a.rho_hat, S, TileShape{nlines, lnx, 1, ls, 1}, LineTable{lbase});
+      tile2_from_lds<4>(a.rho_hat, S, TileShape{nlines, lnx, 1, ls, 1}, LineTable{lbase});
       __syncthreads();  // the copy reads other threads' columns before they are overwritten
+    } else {
+      tile2_to_lds<4>(R, a.rho_hat, TileShape{nlines, lnx, 1, ls, 1}, LineTable{lbase});
+      __syncthreads();
     }
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1), ky = ky0 + kyi;
@@ -95,17 +117,22 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
       const float k2v = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
       const float kn = sqrtf(k0v * k0v + k1v * k1v + k2v * k2v);
       const float isc = sinc_prod(a, kx, ky, kz);
+      const float fk = kn * a.tkinv;
+      const int ti = min((int)fk, kGtabN - 1);
+      const float tw = fk - (float)ti;
+      const float4* te = a.tab + 2 * ti;
       for (int X = 0; X < nS; ++X) P[(X * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
       float vs = 0.f;  // sum_XY dG/dk Re(conj(S_X) rho_Y)
-      for (int X = 0; X < nS; ++X) {
+      for (int X = 0, pt = 0; X < nS; ++X) {
         const int ix = (X * NK + kyi) * ls + kx;
         const float2 sx = S[ix];
         float2 rx = make_float2(0.f, 0.f);
-        if (BWD) rx = a.rho_hat[lbase[X * NK + kyi] + kx];
-        for (int Y = X; Y < nS; ++Y) {
+        if (BWD) rx = R[ix];
+        for (int Y = X; Y < nS; ++Y, ++pt) {
           const int iy = (Y * NK + kyi) * ls + kx;
           float g, dg;
-          gcn_eval(kn, a.r0[X * nS + Y], a.delta[X * nS + Y], a.k0, &g, &dg);
+          gtab_eval(te[2 * kGtabN * pt], te[2 * kGtabN * pt + 1], tw, &g, &dg);
+          dg *= a.tkinv;  // dG/dw -> dG/dk
           g *= isc;
           const float2 sy = S[iy];
           float2 px = P[ix];
@@ -117,7 +144,7 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
             P[iy] = py;
           }
           if (BWD) {
-            const float2 ry = a.rho_hat[lbase[Y * NK + kyi] + kx];
+            const float2 ry = R[iy];
             float t2 = sx.x * ry.x + sx.y * ry.y;
             if (Y != X) t2 += sy.x * rx.x + sy.y * rx.y;
             vs += dg * isc * t2;
@@ -133,7 +160,7 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
       }
     }
     __syncthreads();
-    lds_fft<1, kKsGenericMaxB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
+    lds_fft<1, MAXB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
   if (BWD) {
     const int bid = blockIdx.y * gridDim.x + blockIdx.x;
@@ -144,9 +171,14 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
 }
 
 int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
-  int NK = std::max(1, std::min(p.ny, 256 / p.nx));
-  while (NK > 1 && (size_t)nS * NK * p.nx > (size_t)1024 * kKsGenericMaxB) NK >>= 1;
-  if ((size_t)nS * NK * p.nx > (size_t)1024 * kKsGenericMaxB) {
+  // two butterflies per thread (tiles of 2048 values: with 4 species and
+  // 256-point lines a full tile of the unrolled FFT) where the species fit,
+  // else up to eight
+  kp->maxb = (size_t)nS * p.nx <= 2048 ? 2 : kKsGenericMaxB;
+  const size_t cap = (size_t)1024 * kp->maxb;
+  int NK = 1;
+  while (NK * 2 <= p.ny && (size_t)nS * NK * 2 * p.nx <= cap) NK <<= 1;
+  if ((size_t)nS * NK * p.nx > cap) {
     set_error("D4 CN k-space tile of %d species x %d points exceeds the kernel's FFT capacity", nS, p.nx);
     return FD3_ERR_UNSUPPORTED;
   }
@@ -157,7 +189,8 @@ int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
   kp->blocks = (p.ny / NK / kp->KT) * p.M1;
   kp->lds = (size_t)p.nx * sizeof(float2) + 2 * (size_t)nS * NK * (p.nx + 1) * sizeof(float2) +
             (size_t)nS * NK * sizeof(long long);
-  if (kp->lds > (size_t)kKsGenericLds) {
+  kp->lds_bwd = kp->lds + (size_t)nS * NK * (p.nx + 1) * sizeof(float2);
+  if (kp->lds_bwd > (size_t)kKsGenericLds) {
     set_error("D4 CN k-space tile of %d species x %d points exceeds the LDS of a workgroup", nS, p.nx);
     return FD3_ERR_UNSUPPORTED;
   }
@@ -165,19 +198,25 @@ int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
 }
 
 int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s) {
-  if (kp.lds > (size_t)64 * 1024) {
-    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kp.lds));
-    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)kp.lds));
+  if (kp.lds_bwd > (size_t)64 * 1024) {
+    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<false, kKsGenericMaxB>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kp.lds_bwd));
+    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<true, kKsGenericMaxB>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kp.lds_bwd));
   }
+  const size_t sh = bwd ? kp.lds_bwd : kp.lds;
   const dim3 grid(a.ny / kp.NK / kp.KT, a.M1);
   CnKsArgs b = a;
   b.NK = kp.NK;
   b.KT = kp.KT;
   ProfScope _ps(bwd ? "k_kspace_cn_bwd" : "k_kspace_cn", s);
-  if (bwd) hipLaunchKernelGGL(k_kspace_cn<true>, grid, dim3(256), kp.lds, s, b);
-  else hipLaunchKernelGGL(k_kspace_cn<false>, grid, dim3(256), kp.lds, s, b);
+  if (kp.maxb == 2) {
+    if (bwd) hipLaunchKernelGGL((k_kspace_cn<true, 2>), grid, dim3(256), sh, s, b);
+    else hipLaunchKernelGGL((k_kspace_cn<false, 2>), grid, dim3(256), sh, s, b);
+  } else {
+    if (bwd) hipLaunchKernelGGL((k_kspace_cn<true, kKsGenericMaxB>), grid, dim3(256), sh, s, b);
+    else hipLaunchKernelGGL((k_kspace_cn<false, kKsGenericMaxB>), grid, dim3(256), sh, s, b);
+  }
   FD3_LAUNCH_CHECK("k_kspace_cn");
   return FD3_OK;
 }

@@ -395,6 +395,53 @@ def test_edge_list_and_cell_list_paths_agree_at_headline_size():
     print("[info] 100k: sum F", f.sum(0), "max |F|", np.abs(f).max(), flush=True)
 
 
+def test_graph_capture_replays_match_eager():
+    """The whole-path entry point allocates nothing and never synchronises the
+    host, so an evaluation can be captured in a HIP graph.  Replayed three
+    times with the positions moved in place in between (eager work and
+    synchronisations between the replays), every replay must equal an eager
+    evaluation of the same positions."""
+    _say("graph capture")
+    from types import SimpleNamespace
+    dev = _dev()
+    box = W.hcno_box(4000, seed=31)
+    calc = FourierD3ASECalculator(device=dev)
+    calc._build_model(SimpleNamespace(numbers=box.numbers, cell=box.cell, pbc=np.array([True] * 3)))
+    f64 = dict(dtype=torch.float64, device=dev)
+    n = box.n_atoms
+    pos = torch.tensor(box.positions, **f64)
+    out = {"energy": torch.zeros(1, **f64), "grad_pos": torch.zeros((n, 3), **f64),
+           "virial": torch.zeros((3, 3), **f64), "grad_params": torch.zeros(4, **f64)}
+    calc.evaluate(pos, box.cell, out=out)      # sizes and allocates the workspace
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        calc.evaluate(pos, box.cell, out=out)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(5)
+    for rep in range(3):
+        pos.add_(torch.tensor(rng.normal(0.0, 0.02, (n, 3)), **f64))
+        eager = {k: torch.zeros_like(v) for k, v in out.items()}
+        calc.evaluate(pos, box.cell, out=eager)
+        torch.cuda.synchronize()
+        for v in out.values():
+            v.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        e_g, e_e = float(out["energy"][0]), float(eager["energy"][0])
+        dg = (out["grad_pos"] - eager["grad_pos"]).abs().max().item()
+        dv = (out["virial"] - eager["virial"]).abs().max().item()
+        print(f"[errors] graph replay {rep}: dE {abs(e_g - e_e):.3e} Ha, dgrad {dg:.3e}, dvir {dv:.3e}", flush=True)
+        gscale = eager["grad_pos"].abs().max().item()
+        # the spreading's float adds land in any order, so replays agree with
+        # eager runs to float rounding, not bitwise; moving the atoms changes
+        # the energy by far more than this
+        assert abs(e_g - e_e) <= 1e-7 * abs(e_e)
+        assert dg <= 1e-4 * gscale and dv <= 1e-5 * eager["virial"].abs().max().item()
+
+
 def test_batched_boxes_match_single_box_calls():
     """fd3_eval_celllist_batch on boxes of different sizes, cells and meshes
     gives each box exactly what its own fd3_eval_celllist call gives."""

@@ -100,7 +100,7 @@ class FourierD3(torch.nn.Module):
 
     Args mirror ``src/fourierd3/core.py:50-68``.  ``method`` is 'spme'
     (P3M B-splines, 1..5 nodes), 'pme' (Lagrange, 2..7 nodes) or 'ewald';
-    ``cnfunc`` is 'smooth_cut' or 'd4' (k-space D4 CN, mesh methods).  The C6 reference table is the synthetic stand-in of
+    ``cnfunc`` is 'smooth_cut' or 'd4' (D4 CN in reciprocal space).  The C6 reference table is the synthetic stand-in of
     ``fourierd3_amd.tables`` unless ``FD3_C6REF`` points at a real one.
     ``ns_mesh`` (extension) overrides the mesh rule.
     """
@@ -129,8 +129,6 @@ class FourierD3(torch.nn.Module):
             raise ValueError(f"unknown method '{method}'; expected 'spme', 'pme' or 'ewald'")
         if cnfunc not in ("smooth_cut", "d4"):
             raise ValueError(f"Unknown cnfunc '{cnfunc}'. Expected 'smooth_cut' or 'd4'.")
-        if cnfunc == "d4" and method == "ewald":
-            raise NotImplementedError("cnfunc='d4' with method='ewald' is not built")
 
         z = np.asarray(species, dtype=np.int64)
         uniq = np.unique(z)                                         # core.py:104

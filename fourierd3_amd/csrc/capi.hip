@@ -527,10 +527,6 @@ static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g,
     set_error("D4 CN parameters are NULL");
     return FD3_ERR_ARG;
   }
-  if (g->interp == FD3_INTERP_EWALD) {
-    set_error("cnfunc='d4' with method='ewald' is not built");
-    return FD3_ERR_UNSUPPORTED;
-  }
   if (n < 0) {
     set_error("negative atom count");
     return FD3_ERR_ARG;
@@ -539,11 +535,23 @@ static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g,
   db.pcn = c.plan;
   db.pcn.r = 1;
   db.pcn.C = nS;
-  if ((rc = cn_ks_plan(db.pcn, nS, &db.kp))) return rc;
+  size_t nrho;
+  if (g->interp == FD3_INTERP_EWALD) {  // k-space rows: one per 256 k points
+    if (nS > 64) {
+      set_error("the Ewald D4 CN supports up to 64 species");
+      return FD3_ERR_UNSUPPORTED;
+    }
+    db.kp = CnKsPlan{};
+    db.kp.blocks = (int)cdiv((int64_t)c.plan.nx * c.plan.ny * c.plan.nz, 256);
+    nrho = (size_t)c.plan.nx * c.plan.ny * c.plan.nz * nS;
+  } else {
+    if ((rc = cn_ks_plan(db.pcn, nS, &db.kp))) return rc;
+    nrho = db.pcn.mesh_complex();
+  }
   carve_common(ar, c, n, m->n_rank, db.kp.blocks + (int)cdiv(std::max<int64_t>(n, 1), 256));
   *pos4 = ar.take<double4>(std::max<int64_t>(n, 1));
   *spec_s = ar.take<int32_t>(std::max<int64_t>(n, 1));
-  db.rho_hat = ar.take<float2>(db.pcn.mesh_complex());
+  db.rho_hat = ar.take<float2>(nrho);
   db.ones = ar.take<float>(std::max<int64_t>(n, 1));
   db.g1 = ar.take<float>(std::max<int64_t>(3 * n, 3));
   db.tab = ar.take<float4>((size_t)nS * (nS + 1) * kGtabN);
@@ -566,8 +574,50 @@ static CnKsArgs d4_ks_args(const fd3_model* m, const fd3_cn_d4* d4, const Common
   return ka;
 }
 
+static CnAtomArgs d4_atom_args(const fd3_cn_d4* d4, const Common& c, const D4Bufs& db, int64_t n,
+                               const int32_t* spec_s) {
+  CnAtomArgs aa;
+  aa.n = n;
+  // Ewald: the per-atom sums are derivatives in fractional coordinates
+  const bool ew = c.geom.interp == FD3_INTERP_EWALD;
+  aa.nx = ew ? 1 : c.geom.nx; aa.ny = ew ? 1 : c.geom.ny; aa.nz = ew ? 1 : c.geom.nz;
+  for (int i = 0; i < 9; ++i) aa.inv[i] = c.geom.inv[i];
+  aa.inv_vol = 1.0 / c.geom.volume;
+  aa.spec_s = spec_s; aa.selfcont = d4->selfcont;
+  aa.accA = c.cols.accA; aa.accG = c.cols.accG;
+  aa.cn = c.cn_raw; aa.g1 = db.g1; aa.dEdcn = c.dEdcn; aa.gpos = c.gpos;
+  aa.partials = c.p_cn + (size_t)db.kp.blocks * 8;
+  return aa;
+}
+
+// The Ewald form: structure factors of the species weights on the full k
+// grid, G^CN, per-atom sums.
+static int d4_passes_ewald(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs& db, int64_t n,
+                           const double4* pos4, const int32_t* spec_s, bool bwd, hipStream_t s) {
+  int rc;
+  EwCnArgs ea;
+  ea.n = n; ea.nS = m->n_species; ea.nx = c.plan.nx; ea.ny = c.plan.ny; ea.nz = c.plan.nz;
+  for (int i = 0; i < 9; ++i) { ea.inv[i] = c.geom.inv[i]; ea.recip[i] = c.geom.recip[i]; }
+  ea.volume = c.geom.volume;
+  ea.pos4 = pos4; ea.spec_s = spec_s; ea.w = bwd ? c.dEdcn : db.ones;
+  ea.perm = c.ew_perm; ea.seg = c.ew_seg;
+  ea.tab = db.tab; ea.tkinv = (float)(1.0 / ks_gtab_dk(c.plan, c.geom));
+  ea.S = (float2*)c.buf1; ea.Srho = db.rho_hat; ea.phi = c.buf2;
+  ea.partials = c.p_cn; ea.accA = c.cols.accA; ea.accG = c.cols.accG;
+  if (!bwd) {
+    if ((rc = ewald_partition(spec_s, n, m->n_species, c.ew_perm, c.ew_seg, s))) return rc;
+    CnKsArgs ka{};
+    ka.r0 = d4->r0; ka.delta = d4->delta; ka.k0 = (float)d4->k0; ka.tab = db.tab; ka.nS = m->n_species;
+    if ((rc = cn_tab_init(ka, ks_gtab_dk(c.plan, c.geom), s))) return rc;
+  }
+  if ((rc = ewcn_pass(ea, bwd, s))) return rc;
+  const CnAtomArgs aa = d4_atom_args(d4, c, db, n, spec_s);
+  return bwd ? cn_d4_force(aa, s) : cn_d4_fin(aa, s);
+}
+
 static int d4_passes(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs& db, int64_t n,
                      const double4* pos4, const int32_t* spec_s, bool bwd, hipStream_t s) {
+  if (c.geom.interp == FD3_INTERP_EWALD) return d4_passes_ewald(m, d4, c, db, n, pos4, spec_s, bwd, s);
   int rc;
   FusedArgs fa = fused_args(m, c, n, pos4, spec_s);
   fa.r = 1;
@@ -582,14 +632,7 @@ static int d4_passes(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs&
   if ((rc = cn_ks_run(ka, db.kp, bwd, s))) return rc;
   if ((rc = fft_y(db.pcn, c.tab, spec, c.buf2, 1, s))) return rc;
   if ((rc = c2r_z_gather(fa, db.pcn, c.tab, c.buf2, s))) return rc;  // and the per-atom sums
-  CnAtomArgs aa;
-  aa.n = n; aa.nx = c.geom.nx; aa.ny = c.geom.ny; aa.nz = c.geom.nz;
-  for (int i = 0; i < 9; ++i) aa.inv[i] = c.geom.inv[i];
-  aa.inv_vol = 1.0 / c.geom.volume;
-  aa.spec_s = spec_s; aa.selfcont = d4->selfcont;
-  aa.accA = c.cols.accA; aa.accG = c.cols.accG;
-  aa.cn = c.cn_raw; aa.g1 = db.g1; aa.dEdcn = c.dEdcn; aa.gpos = c.gpos;
-  aa.partials = c.p_cn + (size_t)db.kp.blocks * 8;
+  const CnAtomArgs aa = d4_atom_args(d4, c, db, n, spec_s);
   return bwd ? cn_d4_force(aa, s) : cn_d4_fin(aa, s);
 }
 

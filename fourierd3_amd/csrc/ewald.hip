@@ -237,6 +237,158 @@ __global__ void __launch_bounds__(256) k_ewald_adj(EwaldArgs a, int64_t kper) {
   for (int d = 0; d < 3; ++d) atomicAdd(a.accG + 3 * s + d, (float)(2.0 * kPi * io * B[d]));
 }
 
+// ----------------------------------------------------------- D4 CN ---
+template <bool BWD>
+__global__ void __launch_bounds__(256) k_ewcn_sf(EwCnArgs a) {
+  __shared__ double fr[256][3];
+  __shared__ float wt[256];
+  __shared__ double red[16 * 6];
+  const int nS = a.nS;
+  const int64_t nk = (int64_t)a.nx * a.ny * a.nz;
+  const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = kk < nk;
+  const int64_t kc = live ? kk : 0;
+  const int iz = (int)(kc % a.nz), iy = (int)((kc / a.nz) % a.ny), ix = (int)(kc / ((int64_t)a.nz * a.ny));
+  const double m0 = fftfreq_m(ix, a.nx), m1 = fftfreq_m(iy, a.ny), m2 = fftfreq_m(iz, a.nz);
+  for (int X = 0; X < nS; ++X) {
+    float2 acc = make_float2(0.f, 0.f);
+    const int s0 = a.seg[X], s1 = a.seg[X + 1];
+    for (int c0 = s0; c0 < s1; c0 += 256) {
+      __syncthreads();
+      const int j = c0 + threadIdx.x;
+      if (j < s1) {
+        const int i = a.perm[j];
+        double f[3];
+        frac_of(a.pos4, i, a.inv, f);
+        fr[threadIdx.x][0] = f[0]; fr[threadIdx.x][1] = f[1]; fr[threadIdx.x][2] = f[2];
+        wt[threadIdx.x] = a.w[i];
+      }
+      __syncthreads();
+      const int nc = min(256, s1 - c0);
+      for (int jj = 0; jj < nc; ++jj) {
+        const float2 e = phase_exp(m0 * fr[jj][0] + m1 * fr[jj][1] + m2 * fr[jj][2]);
+        acc.x += wt[jj] * e.x;
+        acc.y += wt[jj] * e.y;
+      }
+    }
+    if (live) {
+      a.S[kk * nS + X] = acc;
+      if (!BWD) a.Srho[kk * nS + X] = acc;
+    }
+  }
+  double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (live) {
+    const float kv0 = (float)(m0 * a.recip[0] + m1 * a.recip[3] + m2 * a.recip[6]);
+    const float kv1 = (float)(m0 * a.recip[1] + m1 * a.recip[4] + m2 * a.recip[7]);
+    const float kv2 = (float)(m0 * a.recip[2] + m1 * a.recip[5] + m2 * a.recip[8]);
+    const float kn = sqrtf(kv0 * kv0 + kv1 * kv1 + kv2 * kv2);
+    const float fk = kn * a.tkinv;
+    const int ti = min((int)fk, kGtabN - 1);
+    const float tw = fk - (float)ti;
+    const float4* te = a.tab + 2 * ti;
+    const float2* Sk = a.S + kk * nS;
+    const float2* Rk = a.Srho + kk * nS;
+    float vs = 0.f;
+    for (int X = 0; X < nS; ++X) {
+      float2 ph = make_float2(0.f, 0.f);
+      const float2 sx = Sk[X];
+      for (int Y = 0; Y < nS; ++Y) {
+        const int lo = min(X, Y), hi = max(X, Y);
+        const int pt = lo * nS - lo * (lo - 1) / 2 + (hi - lo);  // pair (lo, hi) in row order
+        float g, dg;
+        gtab_eval(te[2 * kGtabN * pt], te[2 * kGtabN * pt + 1], tw, &g, &dg);
+        const float2 sy = Sk[Y];
+        ph.x += g * sy.x;
+        ph.y += g * sy.y;
+        if (BWD) {
+          const float2 ry = Rk[Y];
+          vs += dg * a.tkinv * (sx.x * ry.x + sx.y * ry.y);
+        }
+      }
+      a.phi[kk * nS + X] = ph;
+    }
+    if (BWD && kn > 0.f) {
+      const double f = -(double)vs / kn / a.volume;
+      vir[0] = f * kv0 * kv0; vir[1] = f * kv1 * kv1; vir[2] = f * kv2 * kv2;
+      vir[3] = f * kv0 * kv1; vir[4] = f * kv0 * kv2; vir[5] = f * kv1 * kv2;
+    }
+  }
+  if (BWD) block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
+}
+
+// grid (ceil(n / 256), k ranges): per atom, this block's k range.
+template <bool BWD>
+__global__ void __launch_bounds__(256) k_ewcn_atom(EwCnArgs a, int64_t kper) {
+  __shared__ float2 phs[kEwKChunk * 64];  // kEwKChunk k points x nS <= 64 species
+  const int nS = a.nS;
+  const int64_t nk = (int64_t)a.nx * a.ny * a.nz;
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s < a.n;
+  const int64_t k0 = (int64_t)blockIdx.y * kper, k1 = min(nk, k0 + kper);
+  int X = 0;
+  double f[3] = {0.0, 0.0, 0.0};
+  if (live) {
+    X = a.spec_s[s];
+    frac_of(a.pos4, s, a.inv, f);
+  }
+  double A = 0.0, B[3] = {0.0, 0.0, 0.0};
+  for (int64_t kc = k0; kc < k1; kc += kEwKChunk) {
+    const int nc = (int)min((int64_t)kEwKChunk, k1 - kc);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nc * nS; e += blockDim.x) phs[e] = a.phi[kc * nS + e];
+    __syncthreads();
+    if (!live) continue;
+    for (int j = 0; j < nc; ++j) {
+      const int64_t kk = kc + j;
+      const int iz = (int)(kk % a.nz), iy = (int)((kk / a.nz) % a.ny), ix = (int)(kk / ((int64_t)a.nz * a.ny));
+      const double m0 = fftfreq_m(ix, a.nx), m1 = fftfreq_m(iy, a.ny), m2 = fftfreq_m(iz, a.nz);
+      const float2 e = phase_exp(m0 * f[0] + m1 * f[1] + m2 * f[2]);
+      const float2 p = phs[j * nS + X];
+      if (!BWD) A += (double)(e.x * p.x + e.y * p.y);
+      const float im = e.x * p.y - e.y * p.x;  // Im(conj(e) phi)
+      B[0] += m0 * im; B[1] += m1 * im; B[2] += m2 * im;
+    }
+  }
+  if (!live) return;
+  if (!BWD) atomicAdd(a.accA + s, (float)A);
+  for (int d = 0; d < 3; ++d) atomicAdd(a.accG + 3 * s + d, (float)(2.0 * kPi * B[d]));
+}
+
+int ewald_partition(const int32_t* spec_s, int64_t n, int nS, int32_t* perm, int32_t* seg, hipStream_t s) {
+  ProfScope _ps("k_ewald_partition", s);
+  hipLaunchKernelGGL(k_ewald_partition, dim3(1), dim3(1024), 0, s, spec_s, n, nS, perm, seg);
+  FD3_LAUNCH_CHECK("k_ewald_partition");
+  return FD3_OK;
+}
+
+int ewcn_pass(const EwCnArgs& a, bool bwd, hipStream_t s) {
+  if (a.nS > 64) {
+    set_error("the Ewald D4 CN supports up to 64 species");
+    return FD3_ERR_UNSUPPORTED;
+  }
+  const int64_t nk = (int64_t)a.nx * a.ny * a.nz;
+  {
+    ProfScope _ps(bwd ? "k_ewcn_sf_bwd" : "k_ewcn_sf", s);
+    if (bwd) hipLaunchKernelGGL(k_ewcn_sf<true>, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_ewcn_sf<false>, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, a);
+    FD3_LAUNCH_CHECK("k_ewcn_sf");
+  }
+  if (a.n == 0) return FD3_OK;
+  int rc;
+  if ((rc = launch_fill_f32(a.accA, (size_t)a.n, s))) return rc;
+  if ((rc = launch_fill_f32(a.accG, (size_t)a.n * 3, s))) return rc;
+  const unsigned nbx = (unsigned)((a.n + 255) / 256);
+  int64_t ksplit = std::max<int64_t>(1, std::min<int64_t>((nk + kEwKChunk - 1) / kEwKChunk, 1024 / nbx + 1));
+  int64_t kper = (nk + ksplit - 1) / ksplit;
+  kper = (kper + kEwKChunk - 1) / kEwKChunk * kEwKChunk;
+  ksplit = (nk + kper - 1) / kper;
+  ProfScope _ps(bwd ? "k_ewcn_atom_bwd" : "k_ewcn_atom", s);
+  if (bwd) hipLaunchKernelGGL(k_ewcn_atom<true>, dim3(nbx, (unsigned)ksplit), dim3(256), 0, s, a, kper);
+  else hipLaunchKernelGGL(k_ewcn_atom<false>, dim3(nbx, (unsigned)ksplit), dim3(256), 0, s, a, kper);
+  FD3_LAUNCH_CHECK("k_ewcn_atom");
+  return FD3_OK;
+}
+
 // ------------------------------------------------------------------ host ---
 int ewald_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   for (int i = 0; i < 3; ++i)

@@ -132,7 +132,8 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, co
  *   pair function (src/fourierd3/pair_pot.py:235-402) on the same mesh, so no
  *   neighbour list is built.  Energy, and with FD3_NEED_GRAD grad_pos and
  *   virial (grad_shifts and grad_cell are not produced); cn receives the D4
- *   CN.  method='ewald' returns FD3_ERR_UNSUPPORTED.
+ *   CN.  With method='ewald' (geom.interp = FD3_INTERP_EWALD) the CN comes
+ *   from structure factors on the Ewald k grid (src/fourierd3/core.py:430-441).
  */
 typedef struct {
   const float* r0;        /* (nS, nS) r_cov_X + r_cov_Y of the D4 CN radii, Bohr [device] */

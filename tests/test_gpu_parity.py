@@ -223,7 +223,7 @@ def test_module_autograd_matches_reference_calculator(dtype):
 
 
 # ---------------------------------------------------------------- D4 CN ---
-@pytest.mark.parametrize("case", ["sio2_spme", "sio2_pme6", "hcno_triclinic_spme"])
+@pytest.mark.parametrize("case", ["sio2_spme", "sio2_pme6", "hcno_triclinic_spme", "sio2_ewald", "hcno_ewald"])
 def test_d4_calculator_matches_oracle(case):
     """cnfunc='d4': CN from the k-space convolution of the species densities
     (src/fourierd3/core.py:389-447), no neighbour list; E, F and stress against
@@ -232,8 +232,9 @@ def test_d4_calculator_matches_oracle(case):
     if case.startswith("sio2"):
         box = W.strained(W.sio2_box(96, seed=21), 0.1, seed=21)
     else:
-        box = W.strained(W.hcno_box(500, seed=22), 0.1, seed=22)
-    method, order = ("pme", 6) if case == "sio2_pme6" else ("spme", 5)
+        box = W.strained(W.hcno_box(500 if "ewald" not in case else 200, seed=22), 0.1, seed=22)
+    method, order = {"sio2_pme6": ("pme", 6), "sio2_ewald": ("ewald", 5), "hcno_ewald": ("ewald", 5)}.get(
+        case, ("spme", 5))
     calc = FourierD3ASECalculator(device=_dev(), cnfunc="d4", method=method, interpolation_nodes=order)
     calc.calculate(_atoms(box))
     o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order, cnfunc="d4")

@@ -70,8 +70,6 @@ def test_module_constructor_checks():
     box = W.sio2_box(30, seed=1)
     with pytest.raises(ValueError):
         FourierD3(box.numbers, box.cell, device="cpu", method="p3m", verbose=False)
-    with pytest.raises(NotImplementedError):
-        FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d4", method="ewald", verbose=False)
     with pytest.raises(ValueError):
         FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d3", verbose=False)
     md4 = FourierD3(box.numbers, box.cell, device="cpu", cnfunc="d4", verbose=False)

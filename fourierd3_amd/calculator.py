@@ -99,6 +99,38 @@ class FourierD3ASECalculator(_Base):
     def _update_cell(self, cell):
         self._model._update_cell(cell=cell)
 
+    def _build_graph(self, atoms, rcut=None):
+        """Directed neighbour pairs within ``rcut`` (Angstrom, default r_cut)
+        as ``edge_index`` (2, E) long and integer ``unit_shifts`` (E, 3) in
+        this calculator's dtype, the matscipy ``neighbour_list("ijS")``
+        semantics of ``src/fourierd3/calculator.py:94-123``, built on the GPU
+        (``fd3_neighbours_count/fill``; pairs sorted by (i, j, S)).  The
+        evaluation paths do not need it: they enumerate pairs from a cell list."""
+        if rcut is None:
+            rcut = self.r_cut
+        if not bool(np.asarray(atoms.pbc).all()):
+            raise AssertionError("particle-mesh only supports 3d pbc")
+        dev = self.device
+        pos = torch.tensor(np.asarray(atoms.positions, dtype=np.float64), device=dev)
+        n = pos.shape[0]
+        arr = L.cell_array(_cell_array(atoms))
+        h = L.lib()
+        nb = C.c_size_t()
+        L.check(h.fd3_neighbours_workspace(n, arr, float(rcut), C.byref(nb)))
+        ws = torch.empty(max(nb.value, 256), dtype=torch.uint8, device=dev)
+        cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        L.check(h.fd3_neighbours_count(n, L.ptr(pos), L.FD3_DTYPE_F64, arr, float(rcut), L.ptr(ws), ws.numel(),
+                                       L.ptr(cnt), stream))
+        e = int(cnt.item())
+        src = torch.empty(max(e, 1), dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        sh = torch.empty((max(e, 1), 3), dtype=torch.int32, device=dev)
+        L.check(h.fd3_neighbours_fill(n, L.ptr(pos), L.FD3_DTYPE_F64, arr, float(rcut), L.ptr(ws), ws.numel(), e,
+                                      L.ptr(src), L.ptr(dst), L.ptr(sh), stream))
+        edge_index = torch.stack((src[:e], dst[:e]))
+        return edge_index, sh[:e].to(self.dtype)
+
     def _build_model(self, atoms):
         """Instantiate the model for the species and cell of ``atoms``
         (``src/fourierd3/calculator.py:74-92``)."""

@@ -396,6 +396,30 @@ def test_supercell_at_headline_size():
     assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
+def test_supercell_at_one_million_atoms():
+    """The large configuration (1M SiO2 atoms, mesh 512^3 by the reference's
+    rule) through the same exact property: a 2x2x2 replica of a 125k-atom box
+    (mesh 256^3) has 8x its energy and the same forces and stress."""
+    _say("supercell 125k -> 1M")
+    small = W.sio2_box(125_000, seed=24)
+    calc1 = FourierD3ASECalculator(device=_dev())
+    calc1.calculate(_atoms(small))
+    r1 = dict(calc1.results)
+    reps = np.array(list(itertools.product(range(2), repeat=3)))
+    big = W.Box(numbers=np.tile(small.numbers, 8),
+                positions=(small.positions[None] + (reps @ small.cell)[:, None]).reshape(-1, 3),
+                cell=2 * small.cell)
+    calc2 = FourierD3ASECalculator(device=_dev())
+    calc2.calculate(_atoms(big))
+    r2 = calc2.results
+    assert list(calc2._model.ns_mesh) == [512] * 3 and list(calc1._model.ns_mesh) == [256] * 3
+    de = abs(r2["energy"] - 8 * r1["energy"]) / big.n_atoms
+    df = np.abs(r2["forces"] - np.tile(r1["forces"], (8, 1))).max()
+    ds = np.abs(r2["stress"] - r1["stress"]).max()
+    print(f"[errors] 1M supercell: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
 def test_edge_list_and_cell_list_paths_agree_at_headline_size():
     _say("edges vs cells, 100k")
     box = W.hcno_box(100_000, seed=0)

@@ -181,6 +181,52 @@ class FourierD3ASECalculator(_Base):
             L.ptr(model._species_i32), flags, L.ptr(self._ws), self._ws.numel(), C.byref(o), stream))
         return out
 
+    def evaluate_npt(self, positions: torch.Tensor, cell: torch.Tensor, ref_cell_A: np.ndarray,
+                     reach: float = 1.1, need_grad: bool = True, need_params: bool = False,
+                     out: dict | None = None) -> dict:
+        """``evaluate`` for NPT steps (``fd3_eval_celllist_npt``), after the
+        reference's ``_update_cell`` (src/fourierd3/core.py:244-267): the cell
+        is the device tensor ``cell`` (3, 3) float64 Angstrom, read on the
+        stream, so a barostat can change it in place between replays of a
+        captured graph.  ``ref_cell_A`` (host) fixes the mesh, the cell list
+        and the |k| tables, which cover cells whose plane spacings shrink, or
+        largest |k| grows, by up to ``reach``.  ``out["status"]`` (int32, on
+        the device) reads 0, or a bit mask when the cell left that range; the
+        energy and virial are then NaN."""
+        model = self._model
+        if self.cnfunc == "d4":
+            raise NotImplementedError("the device-cell path covers cnfunc='smooth_cut'")
+        if cell.dtype != torch.float64 or cell.shape != (3, 3) or cell.device != positions.device:
+            raise ValueError("cell must be a (3, 3) float64 tensor on the positions' device")
+        cell = cell.contiguous()
+        n = positions.shape[0]
+        g = model._geom(np.asarray(ref_cell_A, dtype=np.float64))
+        m = model._model_struct()
+        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        nbytes = C.c_size_t()
+        L.check(L.lib().fd3_eval_celllist_npt_workspace(C.byref(m), C.byref(g), reach, n, flags, C.byref(nbytes)))
+        if self._ws is None or self._ws.numel() < nbytes.value:
+            self._ws = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        if out is None:
+            out = {"energy": torch.empty(1, **f64), "grad_pos": torch.empty((n, 3), **f64),
+                   "virial": torch.empty((3, 3), **f64), "grad_params": torch.empty(4, **f64)}
+        if "status" not in out:
+            out["status"] = torch.zeros(1, dtype=torch.int32, device=self.device)
+        o = L.Outputs()
+        o.energy = L.ptr(out["energy"])
+        if need_grad:
+            o.grad_pos = L.ptr(out["grad_pos"])
+            o.virial = L.ptr(out["virial"])
+        if need_params:
+            o.grad_params = L.ptr(out["grad_params"])
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        L.check(L.lib().fd3_eval_celllist_npt(
+            C.byref(m), C.byref(g), reach, L.ptr(cell), L.ptr(out["status"]), n, L.ptr(positions),
+            L.dtype_code(positions), L.ptr(model._species_i32), flags, L.ptr(self._ws), self._ws.numel(),
+            C.byref(o), stream))
+        return out
+
     def evaluate_batch(self, positions: torch.Tensor, species: torch.Tensor, offsets, cells,
                        need_grad: bool = True, need_params: bool = False) -> dict:
         """E+F+stress of many boxes in one call (``fd3_eval_celllist_batch``).

@@ -147,6 +147,7 @@ int make_geom(const fd3_geom* g, Geom* out) {
   for (int i = 0; i < 3; ++i)
     for (int d = 0; d < 3; ++d) out->recip[i * 3 + d] = 2.0 * kPi * inv[d * 3 + i];
   out->volume = fabs(det);
+  out->dyn = nullptr;
   out->nx = g->ns[0];
   out->ny = g->ns[1];
   out->nz = g->ns[2];
@@ -211,6 +212,8 @@ struct FinalArgs {
   const double* cnv; int ncnv;   // (ncnv, 8) or null
   double volume;
   double inv[9];
+  const CellDyn* dyn = nullptr;     // when set, volume and inv are read from it
+  const int32_t* status = nullptr;  // device cell check: nonzero poisons the outputs with NaN
   double* energy;
   double* virial;
   double* grad_cell;
@@ -309,6 +312,12 @@ __global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  if (a.dyn) {
+    a.volume = a.dyn->volume;
+    for (int i = 0; i < 9; ++i) a.inv[i] = a.dyn->inv[i];
+  }
+  if (a.status && *a.status != 0)  // the cell left the range the plan covers
+    for (int k = 0; k < 31; ++k) tot[k] = __builtin_nan("");
   const double om = a.volume;
   const double ekp = -tot[0] / (2.0 * om);  // reciprocal-space energy
   if (a.energy) a.energy[0] = ekp + 0.5 * tot[11];
@@ -356,6 +365,7 @@ struct Common {
   double* gpos;
   double *p_ks, *p_c6, *p_ga, *p_cn, *p_red;
   int n_c6, n_ga, n_cn;
+  const int32_t* status = nullptr;  // device cell check (NPT entry point)
 };
 
 static void carve_common(Arena& ar, Common& c, int64_t n, int r, int cn_rows) {
@@ -397,6 +407,7 @@ static FusedArgs fused_args(const fd3_model* m, const Common& c, int64_t n, cons
   fa.n = n; fa.nS = m->n_species; fa.r = m->n_rank;
   fa.nx = c.geom.nx; fa.ny = c.geom.ny; fa.nz = c.geom.nz; fa.order = c.geom.order; fa.interp = c.geom.interp;
   for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
+  fa.dyn = c.geom.dyn;
   fa.pos4 = pos4; fa.spec_s = spec_s; fa.charges = c.charges; fa.cb = c.cols;
   return fa;
 }
@@ -473,6 +484,7 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
     C6BwdArgs ba;
     ba.n = n; ba.nS = nS; ba.r = r; ba.nx = c.geom.nx; ba.ny = c.geom.ny; ba.nz = c.geom.nz;
     for (int i = 0; i < 9; ++i) ba.inv[i] = c.geom.inv[i];
+    ba.dyn = c.geom.dyn;
     ba.pos4 = pos4; ba.spec_s = spec_s; ba.charges = c.charges; ba.accA = c.cols.accA; ba.accG = c.cols.accG;
     ba.k0 = c.k0;
     ba.cn_raw = c.cn_raw; ba.cnref = m->cnref; ba.vq = m->vq; ba.eigs = m->eigs; ba.qq = m->qq;
@@ -499,6 +511,8 @@ static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, 
   if ((rc = reduce_partials_run(jb, s))) return rc;
   fa.volume = c.geom.volume;
   for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
+  fa.dyn = c.geom.dyn;
+  fa.status = c.status;
   fa.energy = out->energy;
   fa.virial = grad ? out->virial : nullptr;
   fa.grad_cell = grad ? out->grad_cell : nullptr;
@@ -784,18 +798,91 @@ int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n, const void*
 }
 
 // --------------------------------------------------------- cell-list path ---
-static int cells_layout(const fd3_model* m, const fd3_geom* g, int64_t n, Common& c, CellGrid& cg, CellBuffers& cb,
-                        Arena& ar) {
+static int cells_layout(const fd3_model* m, const fd3_geom* g, int64_t n, double reach, bool dyn, Common& c,
+                        CellGrid& cg, CellBuffers& cb, Arena& ar, CellDyn** cd, int32_t** st) {
   int rc;
   if ((rc = setup_common(m, g, c))) return rc;
   if (n < 0) {
     set_error("negative atom count");
     return FD3_ERR_ARG;
   }
-  if ((rc = cell_grid_make(g->cell, m->r_cut, &cg))) return rc;
+  if ((rc = cell_grid_make(g->cell, m->r_cut, &cg, reach))) return rc;
+  c.geom.kreach = reach;
   carve_common(ar, c, n, m->n_rank, cg.nbins);
   cell_carve(cg, std::max<int64_t>(n, 1), ar, &cb);
+  if (dyn) {
+    *cd = ar.take<CellDyn>(1);
+    *st = ar.take<int32_t>(1);
+  }
   return FD3_OK;
+}
+
+// Device cell update of the NPT entry point: the cell (Angstrom, row i =
+// a_i) is read from device memory, converted to Bohr, inverted, and checked
+// against what the plan made from the reference cell covers: every cell-list
+// axis still reaches r_cut (sr_a d_a / nb_a >= r_cut with d_a the plane
+// spacing) and, for the per-pair G tables, the mesh's largest |k| stays in
+// the tabulated range.  status: 0, or 1 (singular or not finite) | 2
+// (cell-list reach) | 4 (|k| range).
+struct CellCheck {
+  int nb[3], sr[3];
+  double rc;        // Bohr
+  double kmax_tab;  // largest tabulated |k| (Bohr^-1), 0 = no table limit
+  int nx, ny, nz;
+};
+
+__global__ void k_cell_update(const double* __restrict__ cell_A, CellCheck ck, CellDyn* __restrict__ out,
+                              int32_t* __restrict__ status, int32_t* __restrict__ status_user) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double c[9];
+  for (int i = 0; i < 9; ++i) c[i] = cell_A[i] * kA2B;
+  const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
+                     c[2] * (c[3] * c[7] - c[4] * c[6]);
+  int st = 0;
+  double inv[9];
+  if (!(fabs(det) > 1e-12) || !isfinite(det)) {
+    st |= 1;
+    for (int i = 0; i < 9; ++i) inv[i] = (i % 4 == 0) ? 1.0 : 0.0;  // keeps later kernels finite
+  } else {
+    inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
+    inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
+    inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
+    inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
+    inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
+    inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
+    inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
+    inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
+    inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double nrm = sqrt(inv[a] * inv[a] + inv[3 + a] * inv[3 + a] + inv[6 + a] * inv[6 + a]);
+    if (!(ck.sr[a] / (nrm * ck.nb[a]) >= ck.rc)) st |= 2;
+  }
+  double recip[9];
+  for (int i = 0; i < 3; ++i)
+    for (int d = 0; d < 3; ++d) recip[i * 3 + d] = 2.0 * kPi * inv[d * 3 + i];
+  if (ck.kmax_tab > 0.0) {
+    for (int sx = -1; sx <= 1; sx += 2)
+      for (int sy = -1; sy <= 1; sy += 2) {
+        const double mm[3] = {0.5 * sx * ck.nx, 0.5 * sy * ck.ny, 0.5 * ck.nz};
+        double k2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          const double kc = mm[0] * recip[d] + mm[1] * recip[3 + d] + mm[2] * recip[6 + d];
+          k2 += kc * kc;
+        }
+        if (!(sqrt(k2) <= ck.kmax_tab)) st |= 4;
+      }
+  }
+  for (int i = 0; i < 9; ++i) {
+    out->cell[i] = c[i];
+    out->inv[i] = inv[i];
+    out->recip[i] = recip[i];
+    out->recip_f[i] = (float)recip[i];
+  }
+  out->volume = (st & 1) ? 1.0 : fabs(det);
+  out->neg_inv_vol_f = (float)(-1.0 / out->volume);
+  *status = st;
+  if (status_user) *status_user = st;
 }
 
 int fd3_eval_celllist_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, int flags, size_t* bytes) {
@@ -804,21 +891,42 @@ int fd3_eval_celllist_workspace(const fd3_model* m, const fd3_geom* g, int64_t n
   CellGrid cg;
   CellBuffers cb;
   Arena ar;
-  int rc = cells_layout(m, g, n_atoms, c, cg, cb, ar);
+  CellDyn* cd = nullptr;
+  int32_t* st = nullptr;
+  int rc = cells_layout(m, g, n_atoms, 1.0, false, c, cg, cb, ar, &cd, &st);
   if (rc) return rc;
   *bytes = align_up(ar.off);
   return FD3_OK;
 }
 
-int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const void* positions, int pos_dtype,
-                      const int32_t* species, int flags, void* workspace, size_t ws_bytes, const fd3_outputs* out,
-                      void* stream) {
-  hipStream_t s = (hipStream_t)stream;
+int fd3_eval_celllist_npt_workspace(const fd3_model* m, const fd3_geom* g, double reach, int64_t n_atoms, int flags,
+                                    size_t* bytes) {
+  (void)flags;
+  Common c;
+  CellGrid cg;
+  CellBuffers cb;
+  Arena ar;
+  CellDyn* cd = nullptr;
+  int32_t* st = nullptr;
+  int rc = cells_layout(m, g, n_atoms, reach, true, c, cg, cb, ar, &cd, &st);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+// The cell-list pipeline; with cell_dev the cell comes from device memory
+// (bins, search ranges and tables planned from g->cell with the reach factor).
+static int eval_celllist(const fd3_model* m, const fd3_geom* g, double reach, const double* cell_dev,
+                         int32_t* status_user, int64_t n, const void* positions, int pos_dtype,
+                         const int32_t* species, int flags, void* workspace, size_t ws_bytes,
+                         const fd3_outputs* out, hipStream_t s) {
   Common c;
   CellGrid cg;
   CellBuffers cb;
   Arena ar(workspace, ws_bytes);
-  int rc = cells_layout(m, g, n, c, cg, cb, ar);
+  CellDyn* cd = nullptr;
+  int32_t* st = nullptr;
+  int rc = cells_layout(m, g, n, reach, cell_dev != nullptr, c, cg, cb, ar, &cd, &st);
   if (rc) return rc;
   if (!ar.ok() || !workspace) {
     set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
@@ -833,6 +941,21 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const vo
     return FD3_ERR_ARG;
   }
   const bool grad = flags & FD3_NEED_GRAD;
+  if (cell_dev) {
+    CellCheck ck;
+    for (int a = 0; a < 3; ++a) { ck.nb[a] = cg.nb[a]; ck.sr[a] = cg.sr[a]; }
+    ck.rc = cg.rc;
+    ck.kmax_tab = c.plan.ks_reg ? ks_gtab_dk(c.plan, c.geom) * kGtabN : 0.0;
+    ck.nx = c.geom.nx; ck.ny = c.geom.ny; ck.nz = c.geom.nz;
+    {
+      ProfScope _ps("k_cell_update", s);
+      hipLaunchKernelGGL(k_cell_update, dim3(1), dim3(64), 0, s, cell_dev, ck, cd, st, status_user);
+      FD3_LAUNCH_CHECK("k_cell_update");
+    }
+    c.geom.dyn = cd;
+    cg.dyn = cd;
+    c.status = st;
+  }
   if ((rc = cell_build(cg, cb, n, positions, pos_dtype, species, s))) return rc;
   const CellArgs ca = cell_args(cg, cb, n);
   CnArgs cna{};
@@ -848,6 +971,28 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const vo
   fd3_outputs o2 = *out;
   o2.grad_cell = nullptr;  // the cell-list path reports the strain derivative only
   return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, n > 0, &o2, s);
+}
+
+int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n, const void* positions, int pos_dtype,
+                      const int32_t* species, int flags, void* workspace, size_t ws_bytes, const fd3_outputs* out,
+                      void* stream) {
+  return eval_celllist(m, g, 1.0, nullptr, nullptr, n, positions, pos_dtype, species, flags, workspace, ws_bytes,
+                       out, (hipStream_t)stream);
+}
+
+int fd3_eval_celllist_npt(const fd3_model* m, const fd3_geom* g, double reach, const double* cell, int32_t* status,
+                          int64_t n, const void* positions, int pos_dtype, const int32_t* species, int flags,
+                          void* workspace, size_t ws_bytes, const fd3_outputs* out, void* stream) {
+  if (!cell) {
+    set_error("device cell pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  if (g && g->interp == FD3_INTERP_EWALD) {
+    set_error("the device-cell entry point supports the mesh methods (spme, pme)");
+    return FD3_ERR_UNSUPPORTED;
+  }
+  return eval_celllist(m, g, reach, cell, status, n, positions, pos_dtype, species, flags, workspace, ws_bytes, out,
+                       (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------- batched path ---

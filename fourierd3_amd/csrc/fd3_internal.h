@@ -50,6 +50,18 @@ struct ProfScope {
 };
 
 // -------------------------------------------------------------- geometry ---
+// Cell in device memory, written on the stream by k_cell_update from a
+// device-resident cell (NPT steps: the cell changes without a host round
+// trip, so a captured graph replays with the current cell).  Kernels whose
+// arguments carry a non-null CellDyn pointer read the cell-dependent values
+// from it instead of their by-value copies.
+struct CellDyn {
+  double cell[9], inv[9], recip[9];  // Bohr, as in Geom
+  double volume;
+  float recip_f[9];    // recip in float, as the k-space kernels take it
+  float neg_inv_vol_f; // -1 / volume in float
+};
+
 // Box geometry in Bohr, passed to kernels by value.
 struct Geom {
   double cell[9];   // rows = lattice vectors
@@ -58,6 +70,8 @@ struct Geom {
   double volume;
   int nx, ny, nz;
   int order, interp;
+  const CellDyn* dyn = nullptr;  // device cell, or nullptr: the values above are used
+  double kreach = 1.0;           // |k| tables cover kreach times this cell's largest |k|
 };
 
 int make_geom(const fd3_geom* g, Geom* out);  // validates, converts to Bohr

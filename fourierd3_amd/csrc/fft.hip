@@ -210,8 +210,21 @@ struct KspaceArgs {
   float2* k0;           // (C)
   float recip[9];       // rows b_i of 2 pi cell^-T (Bohr^-1)
   float neg_inv_vol;    // -1/Omega
+  const CellDyn* dyn = nullptr;  // when set, recip and neg_inv_vol are read from it
   int nx, ny, nz, M1, nS, r, RT, NK, KT, use_sinc, need_grad, need_params;
 };
+
+// Reciprocal vectors and volume from the device cell when there is one
+// (wave-uniform: kept in scalar registers like the by-value arguments).
+__device__ inline float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+__device__ inline void ks_refresh(KspaceArgs& a) {
+  if (!a.dyn) return;
+  for (int i = 0; i < 9; ++i) a.recip[i] = uniform_f(a.dyn->recip_f[i]);
+  a.neg_inv_vol = uniform_f(a.dyn->neg_inv_vol_f);
+}
 
 // Dynamic LDS of both k-space kernels: [h table | x twiddles | lines ... |
 // line bases], the table first so that its float4 entries are 16-byte aligned.
@@ -302,6 +315,7 @@ __device__ inline void kspace_reduce(double (&acc)[11], double* red, const Kspac
 // lines at the end.
 template <int NS, int RT, bool PARAMS>
 __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
+  ks_refresh(a);
   constexpr int MAXB = ks_maxb(NS * RT);
   constexpr int NP = NS * (NS + 1) / 2;  // unordered species pairs
   extern __shared__ float4 lds4[];
@@ -434,6 +448,7 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
 
 // Any number of species: phi accumulates in a second LDS buffer P.
 __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
+  ks_refresh(a);
   extern __shared__ float4 lds4[];
   __shared__ double red[16 * 16];
   const int nx = a.nx, nS = a.nS, RT = a.RT, NK = a.NK;
@@ -605,7 +620,7 @@ double ks_gtab_dk(const FftPlan& p, const Geom& g) {
       }
       kmax = std::max(kmax, std::sqrt(k2));
     }
-  return std::max(kmax, 1e-30) * (1.0 + 1e-6) / kGtabN;
+  return std::max(kmax, 1e-30) * (1.0 + 1e-6) * g.kreach / kGtabN;
 }
 
 void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t) {
@@ -724,6 +739,7 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
   a.scx = t.scx; a.scy = t.scy; a.scz = t.scz; a.partials = partials;
   for (int i = 0; i < 9; ++i) a.recip[i] = (float)g.recip[i];
   a.neg_inv_vol = (float)(-1.0 / g.volume);
+  a.dyn = g.dyn;
   a.nx = p.nx; a.ny = p.ny; a.nz = p.nz; a.M1 = p.M1; a.nS = p.nS; a.r = p.r; a.RT = p.RT; a.NK = p.NK; a.KT = p.KT;
   a.use_sinc = g.interp == FD3_INTERP_P3M;
   a.need_grad = need_grad; a.need_params = need_params;

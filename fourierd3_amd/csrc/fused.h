@@ -30,6 +30,7 @@ struct FusedArgs {
   int nS, r;
   int nx, ny, nz, order, interp;
   double inv[9];
+  const CellDyn* dyn = nullptr;  // when set, inv is read from it
   const double4* pos4;
   const int32_t* spec_s;
   const float* charges;  // (n, r) slot order
@@ -49,6 +50,7 @@ struct C6BwdArgs {
   int64_t n;
   int nS, r, nx, ny, nz;
   double inv[9];
+  const CellDyn* dyn = nullptr;  // when set, inv is read from it
   const double4* pos4;
   const int32_t* spec_s;
   const float* charges;

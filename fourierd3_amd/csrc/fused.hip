@@ -48,6 +48,8 @@ void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b)
 __global__ void k_col_prep(FusedArgs a) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
+  if (a.dyn)
+    for (int i = 0; i < 9; ++i) a.inv[i] = a.dyn->inv[i];
   const double4 r = a.pos4[s];
   const int X = a.spec_s[s];
   const int ns[3] = {a.nx, a.ny, a.nz};
@@ -385,6 +387,8 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
 // Block partials (16 doubles): sum_i f_i (x) dE/dr_i, f = r cell^-1.
 __global__ void __launch_bounds__(256) k_c6_bwd(C6BwdArgs a) {
   __shared__ double red[16 * 9];
+  if (a.dyn)
+    for (int i = 0; i < 9; ++i) a.inv[i] = a.dyn->inv[i];
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (s < a.n) {

@@ -52,6 +52,15 @@ __device__ inline void cn_term(float r2, float rcs, float rcut, float* f, float*
   *fpr = sig * (1.0f - sig) * ds * ir;
 }
 
+// Take the cell and its inverse from the device cell when there is one.
+__device__ inline void cell_refresh(CellArgs& c) {
+  if (!c.dyn) return;
+  for (int i = 0; i < 9; ++i) {
+    c.cell[i] = c.dyn->cell[i];
+    c.inv[i] = c.dyn->inv[i];
+  }
+}
+
 __device__ inline int floordiv(int a, int b) {
   int q = a / b;
   if ((a % b) != 0 && ((a < 0) != (b < 0))) --q;
@@ -93,6 +102,7 @@ __global__ void k_bin(CellArgs c, const void* pos, int dtype, int64_t n, double4
                       int4* __restrict__ flo, int32_t* __restrict__ abin, int32_t* __restrict__ bcount) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  cell_refresh(c);
   double r[3];
   load_pos(pos, dtype, i, r);
   double fw[3];
@@ -310,6 +320,7 @@ __device__ inline float wave_sum_f(float v) {
 
 template <bool BWD>
 __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
+  cell_refresh(c);
   constexpr int NW = 4;                   // waves per block
   __shared__ float4 cpos[kCnCap];         // image position relative to the bin's first atom, rcov
   __shared__ float cg[BWD ? kCnCap : 1];  // dE/dCN of the candidate (backward)
@@ -582,7 +593,7 @@ __global__ void k_nb_sort(const int32_t* __restrict__ offset, int64_t n, int64_t
 }
 
 // ------------------------------------------------------------------ host ---
-int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g) {
+int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g, double reach) {
   double c[9];
   for (int i = 0; i < 9; ++i) c[i] = cell_A[i] * kA2B;
   const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
@@ -606,16 +617,21 @@ int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g) {
   inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
   inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
   const double rc = r_cut_A * kA2B;
+  if (!(reach >= 1.0) || !std::isfinite(reach)) {
+    set_error("cell-list reach factor must be >= 1, got %g", reach);
+    return FD3_ERR_ARG;
+  }
+  const double rs = rc * reach;  // bins and search ranges cover this radius
   long long total = 1;
   for (int a = 0; a < 3; ++a) {
     // plane spacing normal to lattice vector a: 1 / |column a of cell^-1|
     const double nrm = sqrt(inv[0 * 3 + a] * inv[0 * 3 + a] + inv[1 * 3 + a] * inv[1 * 3 + a] +
                             inv[2 * 3 + a] * inv[2 * 3 + a]);
     const double d = 1.0 / nrm;
-    int nb = (int)std::max(1.0, floor(d / rc));
+    int nb = (int)std::max(1.0, floor(d / rs));
     nb = std::min(nb, 256);
     g->nb[a] = nb;
-    g->sr[a] = (int)ceil(rc * nb / d - 1e-12);
+    g->sr[a] = (int)ceil(rs * nb / d - 1e-12);
     if (g->sr[a] < 1) g->sr[a] = 1;
     if (g->sr[a] > 64) {
       set_error("box is too thin for r_cut along axis %d (%d images)", a, g->sr[a]);
@@ -630,6 +646,7 @@ int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g) {
   g->nbins = (int)total;
   g->rc = rc;
   g->rc2 = rc * rc;
+  g->dyn = nullptr;
   return FD3_OK;
 }
 
@@ -658,6 +675,7 @@ CellArgs cell_args(const CellGrid& g, const CellBuffers& b, int64_t n) {
     c.sr[a] = g.sr[a];
   }
   c.rc2 = g.rc2;
+  c.dyn = g.dyn;
   c.n = n;
   c.pos4 = b.pos4;
   c.sbin = b.sbin;

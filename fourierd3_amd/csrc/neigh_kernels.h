@@ -13,6 +13,7 @@ struct CellGrid {
   int nb[3], sr[3];        // bins and search reach per lattice direction
   int nbins;
   double rc, rc2;          // cutoff (Bohr)
+  const CellDyn* dyn = nullptr;  // device cell or nullptr
 };
 
 struct CellBuffers {
@@ -31,6 +32,7 @@ struct CellBuffers {
 
 struct CellArgs {
   double cell[9], inv[9];
+  const CellDyn* dyn = nullptr;  // when set, cell and inv are read from it
   int nb[3], sr[3];
   double rc2;
   int64_t n;
@@ -61,7 +63,10 @@ struct EdgeArgs {
   int shifts_dtype;
 };
 
-int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g);
+// Bins of plane spacing >= reach * r_cut and search ranges that cover
+// reach * r_cut: a cell whose plane spacings shrink by up to the factor
+// reach still has every pair within r_cut in range.
+int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g, double reach = 1.0);
 void cell_carve(const CellGrid& g, int64_t n, Arena& ar, CellBuffers* b);
 CellArgs cell_args(const CellGrid& g, const CellBuffers& b, int64_t n);
 int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* pos, int dtype, const int32_t* species,

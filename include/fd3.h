@@ -124,6 +124,25 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, co
                       int pos_dtype, const int32_t* species, int flags, void* workspace,
                       size_t workspace_bytes, const fd3_outputs* out, void* stream);
 
+/* fd3_eval_celllist_npt: fd3_eval_celllist for NPT steps, after
+ *   FourierD3._update_cell (src/fourierd3/core.py:244-267), with the cell read
+ *   from device memory on the stream, so that no host round trip sits between
+ *   a barostat's cell update and the evaluation and a captured graph replays
+ *   with the current cell.  cell: (3,3) Angstrom, row i = a_i, float64
+ *   [device].  As in the reference the mesh (g->ns) stays fixed; g->cell is a
+ *   reference cell from which the cell list and the |k| tables are planned to
+ *   cover cells whose plane spacings shrink, and largest |k| grows, by up to
+ *   the factor reach (>= 1).  status: int32 [device] or NULL, written on the
+ *   stream: 0, or 1 (cell singular or not finite) | 2 (plane spacing below
+ *   the planned reach) | 4 (|k| beyond the tables); when nonzero the energy,
+ *   virial and grad_params are NaN.  Mesh methods only (not Ewald). */
+int fd3_eval_celllist_npt_workspace(const fd3_model* m, const fd3_geom* g, double reach, int64_t n_atoms,
+                                    int flags, size_t* bytes);
+int fd3_eval_celllist_npt(const fd3_model* m, const fd3_geom* g, double reach, const double* cell,
+                          int32_t* status, int64_t n_atoms, const void* positions, int pos_dtype,
+                          const int32_t* species, int flags, void* workspace, size_t workspace_bytes,
+                          const fd3_outputs* out, void* stream);
+
 /* fd3_eval_d4: FourierD3ASECalculator.calculate / FourierD3.forward(positions)
  *   with cnfunc='d4' and method='spme'/'pme' (src/fourierd3/core.py:389-447,
  *   480-583; src/fourierd3/calculator.py:169-171): the coordination numbers

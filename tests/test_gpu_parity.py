@@ -487,6 +487,89 @@ def test_graph_capture_replays_match_eager():
         assert dg <= 1e-4 * gscale and dv <= 1e-5 * eager["virial"].abs().max().item()
 
 
+def test_npt_device_cell_matches_oracle_and_replays():
+    """NPT steps with the cell in device memory (fd3_eval_celllist_npt): the
+    plan (mesh, cell list, |k| tables) comes from the starting cell, as the
+    reference keeps ns_mesh fixed across _update_cell.  Each strained cell
+    must match the oracle at that cell on the fixed mesh and the host-cell
+    path; a captured graph replayed after in-place updates of positions and
+    cell must equal the eager call; a cell that shrinks past the planned
+    reach sets the status and makes the energy NaN."""
+    _say("npt device cell")
+    dev = _dev()
+    box0 = W.hcno_box(1500, seed=41)
+    calc = FourierD3ASECalculator(device=dev)
+    calc._build_model(SimpleNamespace(numbers=box0.numbers, cell=box0.cell, pbc=np.array([True] * 3)))
+    f64 = dict(dtype=torch.float64, device=dev)
+    n = box0.n_atoms
+    steps = [box0] + [W.strained(box0, 0.03, seed=41 + k) for k in range(3)]
+    eager = []
+    for k, b in enumerate(steps):
+        pos = torch.tensor(b.positions, **f64)
+        cell = torch.tensor(b.cell, **f64)
+        r = calc.evaluate_npt(pos, cell, box0.cell, reach=1.1)
+        h = calc.evaluate(pos, b.cell)
+        torch.cuda.synchronize()
+        assert int(r["status"][0]) == 0
+        vol = abs(np.linalg.det(b.cell))
+        e = float(r["energy"][0]) * T.HARTREE_TO_EV
+        f = -r["grad_pos"].cpu().numpy() * T.HARTREE_TO_EV
+        st = r["virial"].cpu().numpy() * T.HARTREE_TO_EV / vol
+        o = OracleFourierD3(box0.numbers, box0.cell)
+        e_ref, f_ref, s_ref = o.calculate(b.positions, b.cell)
+        de, df, ds = _errors(e, f, st, e_ref, f_ref, s_ref, n)
+        dh = abs(float(h["energy"][0]) - float(r["energy"][0]))
+        dgh = (h["grad_pos"] - r["grad_pos"]).abs().max().item()
+        print(f"[errors] npt step {k}: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3; "
+              f"vs host cell dE {dh:.3e} Ha dgrad {dgh:.3e}", flush=True)
+        assert de < E_TOL and df < F_TOL and ds < S_TOL
+        assert dh <= 1e-7 * abs(float(h["energy"][0])) and dgh <= 1e-4 * h["grad_pos"].abs().max().item()
+        eager.append({kk: v.clone() for kk, v in r.items()})
+    # one captured evaluation, replayed with positions and cell changed in place
+    pos = torch.tensor(box0.positions, **f64)
+    cell = torch.tensor(box0.cell, **f64)
+    out = {"energy": torch.zeros(1, **f64), "grad_pos": torch.zeros((n, 3), **f64),
+           "virial": torch.zeros((3, 3), **f64), "grad_params": torch.zeros(4, **f64),
+           "status": torch.zeros(1, dtype=torch.int32, device=dev)}
+    calc.evaluate_npt(pos, cell, box0.cell, reach=1.1, out=out)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        calc.evaluate_npt(pos, cell, box0.cell, reach=1.1, out=out)
+    torch.cuda.synchronize()
+    for k in [2, 1, 3, 0]:
+        pos.copy_(torch.tensor(steps[k].positions, **f64))
+        cell.copy_(torch.tensor(steps[k].cell, **f64))
+        for v in out.values():
+            v.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        ref = eager[k]
+        de = abs(float(out["energy"][0]) - float(ref["energy"][0]))
+        dg = (out["grad_pos"] - ref["grad_pos"]).abs().max().item()
+        dv = (out["virial"] - ref["virial"]).abs().max().item()
+        print(f"[errors] npt graph replay step {k}: dE {de:.3e} Ha dgrad {dg:.3e} dvir {dv:.3e}", flush=True)
+        assert int(out["status"][0]) == 0
+        assert de <= 1e-7 * abs(float(ref["energy"][0]))
+        assert dg <= 1e-4 * ref["grad_pos"].abs().max().item()
+        assert dv <= 1e-5 * ref["virial"].abs().max().item()
+    # compressed by 15 %, the largest |k| leaves the tables planned with
+    # reach 1.1 (status 4); by 40 %, the cell-list plane spacings (bins of
+    # 1.45 r_cut here) fall short of r_cut as well (status 2)
+    for scale, bit in [(0.85, 4), (0.6, 2)]:
+        pos.copy_(torch.tensor(box0.positions * scale, **f64))
+        cell.copy_(torch.tensor(box0.cell * scale, **f64))
+        graph.replay()
+        torch.cuda.synchronize()
+        st = int(out["status"][0])
+        print(f"[errors] npt cell scaled by {scale}: status {st}", flush=True)
+        assert st & bit
+        assert np.isnan(float(out["energy"][0])) and torch.isnan(out["virial"]).all()
+
+
 def test_batched_boxes_match_single_box_calls():
     """fd3_eval_celllist_batch on boxes of different sizes, cells and meshes
     gives each box exactly what its own fd3_eval_celllist call gives."""

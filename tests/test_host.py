@@ -43,6 +43,23 @@ def test_status_codes_map_to_reference_exceptions():
     arr = L.cell_array(np.eye(3) * 5.0)
     with pytest.raises(ValueError, match="r_cut"):
         L.check(h.fd3_neighbours_workspace(10, arr, -1.0, C.byref(nbytes)))
+    # NPT entry point: reach below 1, no device cell, Ewald (mesh methods only)
+    m = L.Model()
+    m.n_species, m.n_rank = 1, 1
+    for f in ("rcov", "cnref", "vq", "eigs", "qq"):
+        setattr(m, f, 8)  # never dereferenced: the checks fail first
+    m.r_cut = 6.0
+    g16 = L.make_geom(np.eye(3) * 20.0, [16] * 3, 5, L.FD3_INTERP_P3M)
+    with pytest.raises(ValueError, match="reach"):
+        L.check(h.fd3_eval_celllist_npt_workspace(C.byref(m), C.byref(g16), 0.9, 10, 1, C.byref(nbytes)))
+    L.check(h.fd3_eval_celllist_npt_workspace(C.byref(m), C.byref(g16), 1.1, 10, 1, C.byref(nbytes)))
+    with pytest.raises(ValueError, match="device cell"):
+        L.check(h.fd3_eval_celllist_npt(C.byref(m), C.byref(g16), 1.1, None, None, 10, None, 1, None, 1,
+                                        None, 0, None, None))
+    gew = L.make_geom(np.eye(3) * 20.0, [4] * 3, 5, L.FD3_INTERP_EWALD)
+    with pytest.raises(NotImplementedError, match="mesh methods"):
+        L.check(h.fd3_eval_celllist_npt(C.byref(m), C.byref(gew), 1.1, 8, None, 10, None, 1, None, 1,
+                                        None, 0, None, None))
 
 
 def test_workspace_sizes_scale_with_mesh_and_atoms():

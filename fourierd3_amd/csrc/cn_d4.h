@@ -28,6 +28,7 @@ struct CnKsArgs {
 
 struct CnKsPlan {
   int NK, KT, blocks, maxb;
+  bool reg;             // species count <= 4: the register-resident forward kernel
   size_t lds, lds_bwd;  // dynamic LDS of the forward and backward variants
 };
 

@@ -76,17 +76,21 @@ __device__ inline float sinc_prod(const CnKsArgs& a, int kx, int ky, int kz) {
 // LDS: [x twiddles | S lines | P lines | line bases | R lines (backward)]; S
 // holds the spectra of the nS species (line l = X * NK + kyi), P the
 // convolution being formed, R the stored density spectrum of the same lines.
-template <bool BWD, int MAXB>
+// NS > 0 (species count known at compile time, forward only): each thread
+// keeps the nS values of its k point in registers and writes the convolution
+// straight back into S, so there is no P region.
+template <bool BWD, int MAXB, int NS>
 __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
+  static_assert(!(BWD && NS > 0), "the register form is the forward pass only");
   extern __shared__ float2 lds[];
   __shared__ double red[16 * 6];
-  const int nx = a.nx, nS = a.nS, NK = a.NK;
+  const int nx = a.nx, nS = NS > 0 ? NS : a.nS, NK = a.NK;
   const int kz = blockIdx.y;
   const int nlines = nS * NK, ls = nx + 1, lnx = ilog2(nx);
   float2* tws = lds;
   float2* S = tws + nx;
   float2* P = S + nlines * ls;
-  long long* lbase = (long long*)(P + nlines * ls);
+  long long* lbase = (long long*)(P + (NS > 0 ? 0 : nlines * ls));
   float2* R = (float2*)(lbase + nlines);  // backward only: the stored density spectrum of the tile
   stage_table(tws, a.twx, nx);
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
@@ -121,6 +125,28 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
       const int ti = min((int)fk, kGtabN - 1);
       const float tw = fk - (float)ti;
       const float4* te = a.tab + 2 * ti;
+      if constexpr (NS > 0) {
+        float2 sx[NS], px[NS];
+#pragma unroll
+        for (int X = 0; X < NS; ++X) {
+          sx[X] = S[(X * NK + kyi) * ls + kx];
+          px[X] = make_float2(0.f, 0.f);
+        }
+#pragma unroll
+        for (int X = 0; X < NS; ++X)
+#pragma unroll
+          for (int Y = X; Y < NS; ++Y) {
+            const int pt = X * NS - X * (X - 1) / 2 + (Y - X);
+            float g, dg;
+            gtab_eval(te[2 * kGtabN * pt], te[2 * kGtabN * pt + 1], tw, &g, &dg);
+            g *= isc;
+            px[X].x += g * sx[Y].x; px[X].y += g * sx[Y].y;
+            if (Y != X) { px[Y].x += g * sx[X].x; px[Y].y += g * sx[X].y; }
+          }
+#pragma unroll
+        for (int X = 0; X < NS; ++X) S[(X * NK + kyi) * ls + kx] = px[X];
+        continue;
+      }
       for (int X = 0; X < nS; ++X) P[(X * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
       float vs = 0.f;  // sum_XY dG/dk Re(conj(S_X) rho_Y)
       for (int X = 0, pt = 0; X < nS; ++X) {
@@ -175,6 +201,7 @@ int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
   // 256-point lines a full tile of the unrolled FFT) where the species fit,
   // else up to eight
   kp->maxb = (size_t)nS * p.nx <= 2048 ? 2 : kKsGenericMaxB;
+  kp->reg = nS <= 4;
   const size_t cap = (size_t)1024 * kp->maxb;
   int NK = 1;
   while (NK * 2 <= p.ny && (size_t)nS * NK * 2 * p.nx <= cap) NK <<= 1;
@@ -187,9 +214,11 @@ int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
   while (kp->KT < 4 && (p.ny / NK) % (2 * kp->KT) == 0 && (size_t)(p.ny / NK / (2 * kp->KT)) * p.M1 >= 4096)
     kp->KT <<= 1;
   kp->blocks = (p.ny / NK / kp->KT) * p.M1;
-  kp->lds = (size_t)p.nx * sizeof(float2) + 2 * (size_t)nS * NK * (p.nx + 1) * sizeof(float2) +
-            (size_t)nS * NK * sizeof(long long);
-  kp->lds_bwd = kp->lds + (size_t)nS * NK * (p.nx + 1) * sizeof(float2);
+  const size_t tile = (size_t)nS * NK * (p.nx + 1) * sizeof(float2);
+  // the backward variant stays generic: holding the stored density as well
+  // takes the register form to about 170 VGPRs, and it measured slower
+  kp->lds = (size_t)p.nx * sizeof(float2) + (kp->reg ? 1 : 2) * tile + (size_t)nS * NK * sizeof(long long);
+  kp->lds_bwd = (size_t)p.nx * sizeof(float2) + 3 * tile + (size_t)nS * NK * sizeof(long long);
   if (kp->lds_bwd > (size_t)kKsGenericLds) {
     set_error("D4 CN k-space tile of %d species x %d points exceeds the LDS of a workgroup", nS, p.nx);
     return FD3_ERR_UNSUPPORTED;
@@ -197,26 +226,44 @@ int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
   return FD3_OK;
 }
 
-int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s) {
-  if (kp.lds_bwd > (size_t)64 * 1024) {
-    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<false, kKsGenericMaxB>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kp.lds_bwd));
-    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<true, kKsGenericMaxB>,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kp.lds_bwd));
+template <bool BWD, int MAXB, int NS>
+static int cn_ks_launch(const CnKsArgs& b, const CnKsPlan& kp, dim3 grid, size_t sh, hipStream_t s) {
+  if (sh > (size_t)64 * 1024)
+    FD3_HIP(hipFuncSetAttribute((const void*)k_kspace_cn<BWD, MAXB, NS>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  hipLaunchKernelGGL((k_kspace_cn<BWD, MAXB, NS>), grid, dim3(256), sh, s, b);
+  return FD3_OK;
+}
+
+template <bool BWD, int MAXB>
+static int cn_ks_species(const CnKsArgs& b, const CnKsPlan& kp, dim3 grid, size_t sh, hipStream_t s) {
+  if constexpr (BWD) {
+    return cn_ks_launch<true, MAXB, 0>(b, kp, grid, sh, s);
+  } else {
+    if (!kp.reg) return cn_ks_launch<false, MAXB, 0>(b, kp, grid, sh, s);
+    switch (b.nS) {
+      case 1: return cn_ks_launch<false, MAXB, 1>(b, kp, grid, sh, s);
+      case 2: return cn_ks_launch<false, MAXB, 2>(b, kp, grid, sh, s);
+      case 3: return cn_ks_launch<false, MAXB, 3>(b, kp, grid, sh, s);
+      default: return cn_ks_launch<false, MAXB, 4>(b, kp, grid, sh, s);
+    }
   }
+}
+
+int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s) {
   const size_t sh = bwd ? kp.lds_bwd : kp.lds;
   const dim3 grid(a.ny / kp.NK / kp.KT, a.M1);
   CnKsArgs b = a;
   b.NK = kp.NK;
   b.KT = kp.KT;
   ProfScope _ps(bwd ? "k_kspace_cn_bwd" : "k_kspace_cn", s);
-  if (kp.maxb == 2) {
-    if (bwd) hipLaunchKernelGGL((k_kspace_cn<true, 2>), grid, dim3(256), sh, s, b);
-    else hipLaunchKernelGGL((k_kspace_cn<false, 2>), grid, dim3(256), sh, s, b);
-  } else {
-    if (bwd) hipLaunchKernelGGL((k_kspace_cn<true, kKsGenericMaxB>), grid, dim3(256), sh, s, b);
-    else hipLaunchKernelGGL((k_kspace_cn<false, kKsGenericMaxB>), grid, dim3(256), sh, s, b);
-  }
+  int rc;
+  if (kp.maxb == 2)
+    rc = bwd ? cn_ks_species<true, 2>(b, kp, grid, sh, s) : cn_ks_species<false, 2>(b, kp, grid, sh, s);
+  else
+    rc = bwd ? cn_ks_species<true, kKsGenericMaxB>(b, kp, grid, sh, s)
+             : cn_ks_species<false, kKsGenericMaxB>(b, kp, grid, sh, s);
+  if (rc) return rc;
   FD3_LAUNCH_CHECK("k_kspace_cn");
   return FD3_OK;
 }

@@ -243,14 +243,22 @@ def test_build_graph_matches_oracle_neighbour_list():
 
 
 # ---------------------------------------------------------------- D4 CN ---
-@pytest.mark.parametrize("case", ["sio2_spme", "sio2_pme6", "hcno_triclinic_spme", "sio2_ewald", "hcno_ewald"])
+@pytest.mark.parametrize("case", ["sio2_spme", "sio2_pme6", "hcno_triclinic_spme", "sio2_ewald", "hcno_ewald",
+                                  "carbon_spme", "hco_spme", "mixed_z6_spme"])
 def test_d4_calculator_matches_oracle(case):
     """cnfunc='d4': CN from the k-space convolution of the species densities
     (src/fourierd3/core.py:389-447), no neighbour list; E, F and stress against
-    the FP64 restatement on the same mesh."""
+    the FP64 restatement on the same mesh.  Species counts 1 to 4 run the
+    register form of the forward CN k-space kernel, 6 the generic one."""
     _say("d4 calculator", case)
     if case.startswith("sio2"):
         box = W.strained(W.sio2_box(96, seed=21), 0.1, seed=21)
+    elif case in ("carbon_spme", "hco_spme"):
+        b = W.hcno_box(300, seed=23)
+        z = np.full(b.n_atoms, 6) if case == "carbon_spme" else np.where(b.numbers == 7, 8, b.numbers)
+        box = W.strained(W.Box(numbers=z, positions=b.positions, cell=b.cell), 0.1, seed=23)
+    elif case == "mixed_z6_spme":
+        box = W.strained(W.mixed_z_box(200, zmax=6, seed=24), 0.1, seed=24)
     else:
         box = W.strained(W.hcno_box(500 if "ewald" not in case else 200, seed=22), 0.1, seed=22)
     method, order = {"sio2_pme6": ("pme", 6), "sio2_ewald": ("ewald", 5), "hcno_ewald": ("ewald", 5)}.get(

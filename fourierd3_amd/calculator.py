@@ -184,7 +184,8 @@ class FourierD3ASECalculator(_Base):
     def evaluate_npt(self, positions: torch.Tensor, cell: torch.Tensor, ref_cell_A: np.ndarray,
                      reach: float = 1.1, need_grad: bool = True, need_params: bool = False,
                      out: dict | None = None) -> dict:
-        """``evaluate`` for NPT steps (``fd3_eval_celllist_npt``), after the
+        """``evaluate`` for NPT steps (``fd3_eval_celllist_npt``, or
+        ``fd3_eval_d4_npt`` with cnfunc='d4'), after the
         reference's ``_update_cell`` (src/fourierd3/core.py:244-267): the cell
         is the device tensor ``cell`` (3, 3) float64 Angstrom, read on the
         stream, so a barostat can change it in place between replays of a
@@ -194,8 +195,7 @@ class FourierD3ASECalculator(_Base):
         the device) reads 0, or a bit mask when the cell left that range; the
         energy and virial are then NaN."""
         model = self._model
-        if self.cnfunc == "d4":
-            raise NotImplementedError("the device-cell path covers cnfunc='smooth_cut'")
+        d4 = self.cnfunc == "d4"
         if cell.dtype != torch.float64 or cell.shape != (3, 3) or cell.device != positions.device:
             raise ValueError("cell must be a (3, 3) float64 tensor on the positions' device")
         cell = cell.contiguous()
@@ -204,7 +204,13 @@ class FourierD3ASECalculator(_Base):
         m = model._model_struct()
         flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
         nbytes = C.c_size_t()
-        L.check(L.lib().fd3_eval_celllist_npt_workspace(C.byref(m), C.byref(g), reach, n, flags, C.byref(nbytes)))
+        if d4:
+            d4s = model._d4_struct()
+            L.check(L.lib().fd3_eval_d4_npt_workspace(C.byref(m), C.byref(d4s), C.byref(g), reach, n, flags,
+                                                      C.byref(nbytes)))
+        else:
+            L.check(L.lib().fd3_eval_celllist_npt_workspace(C.byref(m), C.byref(g), reach, n, flags,
+                                                            C.byref(nbytes)))
         if self._ws is None or self._ws.numel() < nbytes.value:
             self._ws = torch.empty(nbytes.value, dtype=torch.uint8, device=self.device)
         f64 = dict(dtype=torch.float64, device=self.device)
@@ -221,10 +227,12 @@ class FourierD3ASECalculator(_Base):
         if need_params:
             o.grad_params = L.ptr(out["grad_params"])
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        L.check(L.lib().fd3_eval_celllist_npt(
-            C.byref(m), C.byref(g), reach, L.ptr(cell), L.ptr(out["status"]), n, L.ptr(positions),
-            L.dtype_code(positions), L.ptr(model._species_i32), flags, L.ptr(self._ws), self._ws.numel(),
-            C.byref(o), stream))
+        args = (reach, L.ptr(cell), L.ptr(out["status"]), n, L.ptr(positions), L.dtype_code(positions),
+                L.ptr(model._species_i32), flags, L.ptr(self._ws), self._ws.numel(), C.byref(o), stream)
+        if d4:
+            L.check(L.lib().fd3_eval_d4_npt(C.byref(m), C.byref(d4s), C.byref(g), *args))
+        else:
+            L.check(L.lib().fd3_eval_celllist_npt(C.byref(m), C.byref(g), *args))
         return out
 
     def evaluate_batch(self, positions: torch.Tensor, species: torch.Tensor, offsets, cells,

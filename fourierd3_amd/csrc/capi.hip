@@ -523,6 +523,87 @@ static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, 
   return FD3_OK;
 }
 
+// Device cell update of the NPT entry point: the cell (Angstrom, row i =
+// a_i) is read from device memory, converted to Bohr, inverted, and checked
+// against what the plan made from the reference cell covers: every cell-list
+// axis still reaches r_cut (sr_a d_a / nb_a >= r_cut with d_a the plane
+// spacing) and, for the per-pair G tables, the mesh's largest |k| stays in
+// the tabulated range.  status: 0, or 1 (singular or not finite) | 2
+// (cell-list reach) | 4 (|k| range).
+struct CellCheck {
+  int nb[3], sr[3];
+  double rc;        // Bohr
+  double kmax_tab;  // largest tabulated |k| (Bohr^-1), 0 = no table limit
+  int nx, ny, nz;
+};
+
+__global__ void k_cell_update(const double* __restrict__ cell_A, CellCheck ck, CellDyn* __restrict__ out,
+                              int32_t* __restrict__ status, int32_t* __restrict__ status_user) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double c[9];
+  for (int i = 0; i < 9; ++i) c[i] = cell_A[i] * kA2B;
+  const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
+                     c[2] * (c[3] * c[7] - c[4] * c[6]);
+  int st = 0;
+  double inv[9];
+  if (!(fabs(det) > 1e-12) || !isfinite(det)) {
+    st |= 1;
+    for (int i = 0; i < 9; ++i) inv[i] = (i % 4 == 0) ? 1.0 : 0.0;  // keeps later kernels finite
+  } else {
+    inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
+    inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
+    inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
+    inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
+    inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
+    inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
+    inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
+    inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
+    inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
+  }
+  for (int a = 0; a < 3; ++a) {
+    const double nrm = sqrt(inv[a] * inv[a] + inv[3 + a] * inv[3 + a] + inv[6 + a] * inv[6 + a]);
+    if (!(ck.sr[a] / (nrm * ck.nb[a]) >= ck.rc)) st |= 2;
+  }
+  double recip[9];
+  for (int i = 0; i < 3; ++i)
+    for (int d = 0; d < 3; ++d) recip[i * 3 + d] = 2.0 * kPi * inv[d * 3 + i];
+  if (ck.kmax_tab > 0.0) {
+    for (int sx = -1; sx <= 1; sx += 2)
+      for (int sy = -1; sy <= 1; sy += 2) {
+        const double mm[3] = {0.5 * sx * ck.nx, 0.5 * sy * ck.ny, 0.5 * ck.nz};
+        double k2 = 0.0;
+        for (int d = 0; d < 3; ++d) {
+          const double kc = mm[0] * recip[d] + mm[1] * recip[3 + d] + mm[2] * recip[6 + d];
+          k2 += kc * kc;
+        }
+        if (!(sqrt(k2) <= ck.kmax_tab)) st |= 4;
+      }
+  }
+  for (int i = 0; i < 9; ++i) {
+    out->cell[i] = c[i];
+    out->inv[i] = inv[i];
+    out->recip[i] = recip[i];
+    out->recip_f[i] = (float)recip[i];
+  }
+  out->volume = (st & 1) ? 1.0 : fabs(det);
+  out->neg_inv_vol_f = (float)(-1.0 / out->volume);
+  out->inv_volume = 1.0 / out->volume;
+  *status = st;
+  if (status_user) *status_user = st;
+}
+
+// Queue k_cell_update and point the pipeline at the device cell.
+static int launch_cell_update(Common& c, CellCheck ck, const double* cell_dev, CellDyn* cd, int32_t* st,
+                              int32_t* status_user, hipStream_t s) {
+  ck.nx = c.geom.nx; ck.ny = c.geom.ny; ck.nz = c.geom.nz;
+  ProfScope _ps("k_cell_update", s);
+  hipLaunchKernelGGL(k_cell_update, dim3(1), dim3(64), 0, s, cell_dev, ck, cd, st, status_user);
+  FD3_LAUNCH_CHECK("k_cell_update");
+  c.geom.dyn = cd;
+  c.status = st;
+  return FD3_OK;
+}
+
 // ------------------------------------------------------------- D4 CN path ---
 struct D4Bufs {
   FftPlan pcn;      // the mesh plan with one rank: channels = species
@@ -534,9 +615,15 @@ struct D4Bufs {
 };
 
 static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n, Common& c, Arena& ar,
-                     double4** pos4, int32_t** spec_s, D4Bufs& db) {
+                     double4** pos4, int32_t** spec_s, D4Bufs& db, double reach = 1.0, CellDyn** cd = nullptr,
+                     int32_t** st = nullptr) {
   int rc;
   if ((rc = setup_common(m, g, c))) return rc;
+  if (!(reach >= 1.0) || !std::isfinite(reach)) {
+    set_error("reach factor must be >= 1, got %g", reach);
+    return FD3_ERR_ARG;
+  }
+  c.geom.kreach = reach;
   if (!d4) {
     set_error("D4 CN parameters are NULL");
     return FD3_ERR_ARG;
@@ -569,6 +656,10 @@ static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g,
   db.ones = ar.take<float>(std::max<int64_t>(n, 1));
   db.g1 = ar.take<float>(std::max<int64_t>(3 * n, 3));
   db.tab = ar.take<float4>((size_t)nS * (nS + 1) * kGtabN);
+  if (cd) {
+    *cd = ar.take<CellDyn>(1);
+    *st = ar.take<int32_t>(1);
+  }
   return FD3_OK;
 }
 
@@ -585,6 +676,7 @@ static CnKsArgs d4_ks_args(const fd3_model* m, const fd3_cn_d4* d4, const Common
   for (int i = 0; i < 9; ++i) ka.recip[i] = (float)c.geom.recip[i];
   ka.nx = c.geom.nx; ka.ny = c.geom.ny; ka.nz = c.geom.nz; ka.M1 = c.plan.M1; ka.nS = m->n_species;
   ka.NK = db.kp.NK; ka.KT = db.kp.KT; ka.use_sinc = c.geom.interp == FD3_INTERP_P3M;
+  ka.dyn = c.geom.dyn;
   return ka;
 }
 
@@ -601,6 +693,7 @@ static CnAtomArgs d4_atom_args(const fd3_cn_d4* d4, const Common& c, const D4Buf
   aa.accA = c.cols.accA; aa.accG = c.cols.accG;
   aa.cn = c.cn_raw; aa.g1 = db.g1; aa.dEdcn = c.dEdcn; aa.gpos = c.gpos;
   aa.partials = c.p_cn + (size_t)db.kp.blocks * 8;
+  aa.dyn = c.geom.dyn;
   return aa;
 }
 
@@ -670,16 +763,21 @@ int fd3_eval_d4_workspace(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geo
   return FD3_OK;
 }
 
-int fd3_eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n, const void* positions,
-                int pos_dtype, const int32_t* species, int flags, void* workspace, size_t ws_bytes,
-                const fd3_outputs* out, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+// The D4 pipeline; with cell_dev the cell comes from device memory (tables
+// planned from g->cell with the reach factor).
+static int eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, double reach, const double* cell_dev,
+                   int32_t* status_user, int64_t n, const void* positions, int pos_dtype, const int32_t* species,
+                   int flags, void* workspace, size_t ws_bytes, const fd3_outputs* out, hipStream_t s) {
   Common c;
   Arena ar(workspace, ws_bytes);
   double4* pos4;
   int32_t* spec_s;
   D4Bufs db;
-  int rc = d4_layout(m, d4, g, n, c, ar, &pos4, &spec_s, db);
+  CellDyn* cd = nullptr;
+  int32_t* st = nullptr;
+  int rc = d4_layout(m, d4, g, n, c, ar, &pos4, &spec_s, db, reach, cell_dev ? &cd : nullptr, &st);
   if (rc) return rc;
   if (!ar.ok() || !workspace) {
     set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
@@ -698,6 +796,13 @@ int fd3_eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int6
     return FD3_ERR_ARG;
   }
   const bool grad = flags & FD3_NEED_GRAD;
+  if (cell_dev) {
+    CellCheck ck;  // no cell list; the CN tables always cover |k|
+    for (int a = 0; a < 3; ++a) { ck.nb[a] = 1; ck.sr[a] = 1; }
+    ck.rc = 0.0;
+    ck.kmax_tab = ks_gtab_dk(c.plan, c.geom) * kGtabN;
+    if ((rc = launch_cell_update(c, ck, cell_dev, cd, st, status_user, s))) return rc;
+  }
   if ((rc = launch_prep_pos(positions, pos_dtype, species, n, pos4, spec_s, s))) return rc;
   if ((rc = mesh_prep(m, c, n, pos4, spec_s, s))) return rc;
   if ((rc = fill_value(db.ones, n, 1.0f, s))) return rc;
@@ -710,6 +815,46 @@ int fd3_eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int6
   fd3_outputs o2 = *out;
   o2.grad_cell = nullptr;  // the D4 path reports the strain derivative only
   return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, n > 0, &o2, s);
+}
+
+extern "C" {
+
+int fd3_eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int64_t n, const void* positions,
+                int pos_dtype, const int32_t* species, int flags, void* workspace, size_t ws_bytes,
+                const fd3_outputs* out, void* stream) {
+  return eval_d4(m, d4, g, 1.0, nullptr, nullptr, n, positions, pos_dtype, species, flags, workspace, ws_bytes, out,
+                 (hipStream_t)stream);
+}
+
+int fd3_eval_d4_npt_workspace(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, double reach,
+                              int64_t n_atoms, int flags, size_t* bytes) {
+  (void)flags;
+  Common c;
+  Arena ar;
+  double4* p4;
+  int32_t* sp;
+  D4Bufs db;
+  CellDyn* cd = nullptr;
+  int32_t* st = nullptr;
+  int rc = d4_layout(m, d4, g, n_atoms, c, ar, &p4, &sp, db, reach, &cd, &st);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+int fd3_eval_d4_npt(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, double reach, const double* cell,
+                    int32_t* status, int64_t n, const void* positions, int pos_dtype, const int32_t* species,
+                    int flags, void* workspace, size_t ws_bytes, const fd3_outputs* out, void* stream) {
+  if (!cell) {
+    set_error("device cell pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  if (g && g->interp == FD3_INTERP_EWALD) {
+    set_error("the device-cell entry point supports the mesh methods (spme, pme)");
+    return FD3_ERR_UNSUPPORTED;
+  }
+  return eval_d4(m, d4, g, reach, cell, status, n, positions, pos_dtype, species, flags, workspace, ws_bytes, out,
+                 (hipStream_t)stream);
 }
 
 const char* fd3_last_error(void) { return g_err.c_str(); }
@@ -817,74 +962,6 @@ static int cells_layout(const fd3_model* m, const fd3_geom* g, int64_t n, double
   return FD3_OK;
 }
 
-// Device cell update of the NPT entry point: the cell (Angstrom, row i =
-// a_i) is read from device memory, converted to Bohr, inverted, and checked
-// against what the plan made from the reference cell covers: every cell-list
-// axis still reaches r_cut (sr_a d_a / nb_a >= r_cut with d_a the plane
-// spacing) and, for the per-pair G tables, the mesh's largest |k| stays in
-// the tabulated range.  status: 0, or 1 (singular or not finite) | 2
-// (cell-list reach) | 4 (|k| range).
-struct CellCheck {
-  int nb[3], sr[3];
-  double rc;        // Bohr
-  double kmax_tab;  // largest tabulated |k| (Bohr^-1), 0 = no table limit
-  int nx, ny, nz;
-};
-
-__global__ void k_cell_update(const double* __restrict__ cell_A, CellCheck ck, CellDyn* __restrict__ out,
-                              int32_t* __restrict__ status, int32_t* __restrict__ status_user) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double c[9];
-  for (int i = 0; i < 9; ++i) c[i] = cell_A[i] * kA2B;
-  const double det = c[0] * (c[4] * c[8] - c[5] * c[7]) - c[1] * (c[3] * c[8] - c[5] * c[6]) +
-                     c[2] * (c[3] * c[7] - c[4] * c[6]);
-  int st = 0;
-  double inv[9];
-  if (!(fabs(det) > 1e-12) || !isfinite(det)) {
-    st |= 1;
-    for (int i = 0; i < 9; ++i) inv[i] = (i % 4 == 0) ? 1.0 : 0.0;  // keeps later kernels finite
-  } else {
-    inv[0] = (c[4] * c[8] - c[5] * c[7]) / det;
-    inv[1] = (c[2] * c[7] - c[1] * c[8]) / det;
-    inv[2] = (c[1] * c[5] - c[2] * c[4]) / det;
-    inv[3] = (c[5] * c[6] - c[3] * c[8]) / det;
-    inv[4] = (c[0] * c[8] - c[2] * c[6]) / det;
-    inv[5] = (c[2] * c[3] - c[0] * c[5]) / det;
-    inv[6] = (c[3] * c[7] - c[4] * c[6]) / det;
-    inv[7] = (c[1] * c[6] - c[0] * c[7]) / det;
-    inv[8] = (c[0] * c[4] - c[1] * c[3]) / det;
-  }
-  for (int a = 0; a < 3; ++a) {
-    const double nrm = sqrt(inv[a] * inv[a] + inv[3 + a] * inv[3 + a] + inv[6 + a] * inv[6 + a]);
-    if (!(ck.sr[a] / (nrm * ck.nb[a]) >= ck.rc)) st |= 2;
-  }
-  double recip[9];
-  for (int i = 0; i < 3; ++i)
-    for (int d = 0; d < 3; ++d) recip[i * 3 + d] = 2.0 * kPi * inv[d * 3 + i];
-  if (ck.kmax_tab > 0.0) {
-    for (int sx = -1; sx <= 1; sx += 2)
-      for (int sy = -1; sy <= 1; sy += 2) {
-        const double mm[3] = {0.5 * sx * ck.nx, 0.5 * sy * ck.ny, 0.5 * ck.nz};
-        double k2 = 0.0;
-        for (int d = 0; d < 3; ++d) {
-          const double kc = mm[0] * recip[d] + mm[1] * recip[3 + d] + mm[2] * recip[6 + d];
-          k2 += kc * kc;
-        }
-        if (!(sqrt(k2) <= ck.kmax_tab)) st |= 4;
-      }
-  }
-  for (int i = 0; i < 9; ++i) {
-    out->cell[i] = c[i];
-    out->inv[i] = inv[i];
-    out->recip[i] = recip[i];
-    out->recip_f[i] = (float)recip[i];
-  }
-  out->volume = (st & 1) ? 1.0 : fabs(det);
-  out->neg_inv_vol_f = (float)(-1.0 / out->volume);
-  *status = st;
-  if (status_user) *status_user = st;
-}
-
 int fd3_eval_celllist_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, int flags, size_t* bytes) {
   (void)flags;
   Common c;
@@ -946,15 +1023,8 @@ static int eval_celllist(const fd3_model* m, const fd3_geom* g, double reach, co
     for (int a = 0; a < 3; ++a) { ck.nb[a] = cg.nb[a]; ck.sr[a] = cg.sr[a]; }
     ck.rc = cg.rc;
     ck.kmax_tab = c.plan.ks_reg ? ks_gtab_dk(c.plan, c.geom) * kGtabN : 0.0;
-    ck.nx = c.geom.nx; ck.ny = c.geom.ny; ck.nz = c.geom.nz;
-    {
-      ProfScope _ps("k_cell_update", s);
-      hipLaunchKernelGGL(k_cell_update, dim3(1), dim3(64), 0, s, cell_dev, ck, cd, st, status_user);
-      FD3_LAUNCH_CHECK("k_cell_update");
-    }
-    c.geom.dyn = cd;
+    if ((rc = launch_cell_update(c, ck, cell_dev, cd, st, status_user, s))) return rc;
     cg.dyn = cd;
-    c.status = st;
   }
   if ((rc = cell_build(cg, cb, n, positions, pos_dtype, species, s))) return rc;
   const CellArgs ca = cell_args(cg, cb, n);

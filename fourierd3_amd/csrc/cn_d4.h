@@ -24,6 +24,7 @@ struct CnKsArgs {
   double inv_vol;      // 1 / Omega (Bohr^-3)
   float recip[9];      // rows b_i of 2 pi cell^-T (Bohr^-1)
   int nx, ny, nz, M1, nS, NK, KT, use_sinc;
+  const CellDyn* dyn = nullptr;  // when set, recip and inv_vol are read from it
 };
 
 struct CnKsPlan {
@@ -52,6 +53,7 @@ struct CnAtomArgs {
   const float* dEdcn; // (n) dE/dCN (backward)
   double* gpos;       // (n, 3) dE/dr (Bohr), added to
   double* partials;   // (blocks, 8) strain-derivative sums
+  const CellDyn* dyn = nullptr;  // when set, inv and inv_vol are read from it
 };
 
 int cn_d4_fin(const CnAtomArgs& a, hipStream_t s);

@@ -82,6 +82,11 @@ __device__ inline float sinc_prod(const CnKsArgs& a, int kx, int ky, int kz) {
 template <bool BWD, int MAXB, int NS>
 __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
   static_assert(!(BWD && NS > 0), "the register form is the forward pass only");
+  if (a.dyn) {
+    for (int i = 0; i < 9; ++i)
+      a.recip[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.dyn->recip_f[i])));
+    a.inv_vol = a.dyn->inv_volume;
+  }
   extern __shared__ float2 lds[];
   __shared__ double red[16 * 6];
   const int nx = a.nx, nS = NS > 0 ? NS : a.nS, NK = a.NK;
@@ -270,9 +275,16 @@ int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s) {
 
 // CN from the forward gather (accA = sum W phi, accG = sum dW/du phi): CN =
 // accA / Omega - f_XX(0); accG is kept for the force.
+__device__ inline void atom_refresh(CnAtomArgs& a) {
+  if (!a.dyn) return;
+  for (int i = 0; i < 9; ++i) a.inv[i] = a.dyn->inv[i];
+  a.inv_vol = a.dyn->inv_volume;
+}
+
 __global__ void k_cn_d4_fin(CnAtomArgs a) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
+  atom_refresh(a);
   a.cn[s] = (float)((double)a.accA[s] * a.inv_vol) - a.selfcont[a.spec_s[s]];
   for (int d = 0; d < 3; ++d) a.g1[3 * s + d] = a.accG[3 * s + d];
 }
@@ -292,6 +304,7 @@ int cn_d4_fin(const CnAtomArgs& a, hipStream_t s) {
 //   virial_ab -= delta_ab sum_i g_i (CN_i + f_XX(0))   (the 1/Omega of every CN).
 __global__ void __launch_bounds__(256) k_cn_d4_force(CnAtomArgs a) {
   __shared__ double red[16 * 6];
+  atom_refresh(a);
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (s < a.n) {

@@ -57,7 +57,7 @@ struct ProfScope {
 // from it instead of their by-value copies.
 struct CellDyn {
   double cell[9], inv[9], recip[9];  // Bohr, as in Geom
-  double volume;
+  double volume, inv_volume;
   float recip_f[9];    // recip in float, as the k-space kernels take it
   float neg_inv_vol_f; // -1 / volume in float
 };

@@ -167,6 +167,19 @@ int fd3_eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, int6
                 const void* positions, int pos_dtype, const int32_t* species, int flags, void* workspace,
                 size_t workspace_bytes, const fd3_outputs* out, void* stream);
 
+/* fd3_eval_d4_npt: fd3_eval_d4 with the cell read from device memory, as
+ *   fd3_eval_celllist_npt (src/fourierd3/core.py:244-267, which also updates
+ *   kspace_filter_cn): cell (3,3) Angstrom float64 [device], g->cell the
+ *   reference cell from which the G and G^CN tables are planned for |k| up to
+ *   reach times its largest; status [device] or NULL as there (bit 4: |k|
+ *   beyond the tables).  Mesh methods only. */
+int fd3_eval_d4_npt_workspace(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, double reach,
+                              int64_t n_atoms, int flags, size_t* bytes);
+int fd3_eval_d4_npt(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, double reach, const double* cell,
+                    int32_t* status, int64_t n_atoms, const void* positions, int pos_dtype,
+                    const int32_t* species, int flags, void* workspace, size_t workspace_bytes,
+                    const fd3_outputs* out, void* stream);
+
 /* fd3_eval_celllist_batch: n_boxes independent periodic boxes that share one
  *   model (one species set): box b owns atoms [atom_offsets[b], atom_offsets[b+1])
  *   of the packed positions (N_total, 3) and species (N_total) [device], has its

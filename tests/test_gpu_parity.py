@@ -495,7 +495,8 @@ def test_graph_capture_replays_match_eager():
         assert dg <= 1e-4 * gscale and dv <= 1e-5 * eager["virial"].abs().max().item()
 
 
-def test_npt_device_cell_matches_oracle_and_replays():
+@pytest.mark.parametrize("cnfunc", ["smooth_cut", "d4"])
+def test_npt_device_cell_matches_oracle_and_replays(cnfunc):
     """NPT steps with the cell in device memory (fd3_eval_celllist_npt): the
     plan (mesh, cell list, |k| tables) comes from the starting cell, as the
     reference keeps ns_mesh fixed across _update_cell.  Each strained cell
@@ -503,10 +504,10 @@ def test_npt_device_cell_matches_oracle_and_replays():
     path; a captured graph replayed after in-place updates of positions and
     cell must equal the eager call; a cell that shrinks past the planned
     reach sets the status and makes the energy NaN."""
-    _say("npt device cell")
+    _say("npt device cell", cnfunc)
     dev = _dev()
     box0 = W.hcno_box(1500, seed=41)
-    calc = FourierD3ASECalculator(device=dev)
+    calc = FourierD3ASECalculator(device=dev, cnfunc=cnfunc)
     calc._build_model(SimpleNamespace(numbers=box0.numbers, cell=box0.cell, pbc=np.array([True] * 3)))
     f64 = dict(dtype=torch.float64, device=dev)
     n = box0.n_atoms
@@ -523,7 +524,7 @@ def test_npt_device_cell_matches_oracle_and_replays():
         e = float(r["energy"][0]) * T.HARTREE_TO_EV
         f = -r["grad_pos"].cpu().numpy() * T.HARTREE_TO_EV
         st = r["virial"].cpu().numpy() * T.HARTREE_TO_EV / vol
-        o = OracleFourierD3(box0.numbers, box0.cell)
+        o = OracleFourierD3(box0.numbers, box0.cell, cnfunc=cnfunc)
         e_ref, f_ref, s_ref = o.calculate(b.positions, b.cell)
         de, df, ds = _errors(e, f, st, e_ref, f_ref, s_ref, n)
         dh = abs(float(h["energy"][0]) - float(r["energy"][0]))
@@ -566,8 +567,8 @@ def test_npt_device_cell_matches_oracle_and_replays():
         assert dv <= 1e-5 * ref["virial"].abs().max().item()
     # compressed by 15 %, the largest |k| leaves the tables planned with
     # reach 1.1 (status 4); by 40 %, the cell-list plane spacings (bins of
-    # 1.45 r_cut here) fall short of r_cut as well (status 2)
-    for scale, bit in [(0.85, 4), (0.6, 2)]:
+    # 1.45 r_cut here) fall short of r_cut as well (status 2; D4 has no cell list)
+    for scale, bit in [(0.85, 4), (0.6, 2)] if cnfunc == "smooth_cut" else [(0.85, 4)]:
         pos.copy_(torch.tensor(box0.positions * scale, **f64))
         cell.copy_(torch.tensor(box0.cell * scale, **f64))
         graph.replay()

@@ -34,6 +34,7 @@ EXPORTS = (
     "fd3_eval_celllist_workspace", "fd3_eval_celllist",
     "fd3_eval_celllist_npt_workspace", "fd3_eval_celllist_npt",
     "fd3_eval_d4_npt_workspace", "fd3_eval_d4_npt",
+    "fd3_eval_edges_npt_workspace", "fd3_eval_edges_npt",
     "fd3_eval_d4_workspace", "fd3_eval_d4",
     "fd3_eval_celllist_batch_workspace", "fd3_eval_celllist_batch",
     "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
@@ -96,6 +97,11 @@ _SIG = {
                                                   C.c_int, C.POINTER(C.c_size_t)]),
     "fd3_eval_celllist_npt": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_double, _P, _P, C.c_int64, _P,
                                         C.c_int, _P, C.c_int, _P, C.c_size_t, C.POINTER(Outputs), _P]),
+    "fd3_eval_edges_npt_workspace": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_double, C.c_int64,
+                                               C.c_int64, C.c_int, C.POINTER(C.c_size_t)]),
+    "fd3_eval_edges_npt": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_double, _P, _P, C.c_int64, _P,
+                                     C.c_int, _P, C.c_int64, _P, _P, _P, C.c_int, C.c_int, _P, C.c_size_t,
+                                     C.POINTER(Outputs), _P]),
     "fd3_eval_d4_npt_workspace": (C.c_int, [C.POINTER(Model), C.POINTER(CnD4), C.POINTER(Geom), C.c_double,
                                             C.c_int64, C.c_int, C.POINTER(C.c_size_t)]),
     "fd3_eval_d4_npt": (C.c_int, [C.POINTER(Model), C.POINTER(CnD4), C.POINTER(Geom), C.c_double, _P, _P,

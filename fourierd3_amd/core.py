@@ -110,7 +110,8 @@ class FourierD3(torch.nn.Module):
                  device=torch.device("cuda" if torch.cuda.is_available() else "cpu"),
                  dtype=torch.float32, method: str = "spme", params=None, cnfunc="smooth_cut",
                  interpolation_nodes: int = 5, k_cutoff: float = 10.0, r_cut: float = 6.0,
-                 cncorr: Optional[torch.Tensor] = None, verbose=True, ns_mesh=None) -> None:
+                 cncorr: Optional[torch.Tensor] = None, verbose=True, ns_mesh=None,
+                 npt_reach: Optional[float] = None) -> None:
         super().__init__()
         device = torch.device(device)
         self.device = device
@@ -199,6 +200,14 @@ class FourierD3(torch.nn.Module):
             print("Using mesh size", self.ns_mesh)
         self._interp = {"spme": L.FD3_INTERP_P3M, "pme": L.FD3_INTERP_LAGRANGE, "ewald": L.FD3_INTERP_EWALD}[method]
         self._ws = None
+        # NPT: with npt_reach set (mesh methods), forward reads the cell from
+        # the device (no host copy in _update_cell, so a step can be captured
+        # in a graph); the mesh and |k| tables are planned from this starting
+        # cell for cells whose largest |k| grows by up to npt_reach.
+        # npt_status (device int32) is nonzero once a cell left that range.
+        self._npt_reach = None if npt_reach is None or method == "ewald" else float(npt_reach)
+        self._npt_ref_cell = cell_np.copy()
+        self.npt_status = torch.zeros(1, dtype=torch.int32, device=device)
         self._update_cell(cell_A)
 
     # ------------------------------------------------------------------
@@ -234,7 +243,8 @@ class FourierD3(torch.nn.Module):
         self._cell_A = cell
         self.cell = cell * A2B
         self.volume = torch.abs(_det3(self.cell))
-        self._cell_host = cell.detach().to(torch.float64).cpu().numpy().copy()
+        if self._npt_reach is None:
+            self._cell_host = cell.detach().to(torch.float64).cpu().numpy().copy()
 
     # ------------------------------------------------------------------
     def _run_edges(self, positions, edge_index, shifts, cell_A, need_grad: bool, need_params: bool):
@@ -259,12 +269,20 @@ class FourierD3(torch.nn.Module):
             pos = pos.double()
         if sh.dtype not in (torch.float32, torch.float64):
             sh = sh.double()
-        cell_host = cell_A.detach().to(torch.float64).cpu().numpy()
+        npt = self._npt_reach is not None
+        if npt:
+            cell_dev = cell_A.detach().to(device=dev, dtype=torch.float64).contiguous()
+            g = self._geom(self._npt_ref_cell)
+        else:
+            g = self._geom(cell_A.detach().to(torch.float64).cpu().numpy())
         m = self._model_struct()
-        g = self._geom(cell_host)
         flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
         nbytes = C.c_size_t()
-        L.check(L.lib().fd3_eval_edges_workspace(C.byref(m), C.byref(g), n, ne, flags, C.byref(nbytes)))
+        if npt:
+            L.check(L.lib().fd3_eval_edges_npt_workspace(C.byref(m), C.byref(g), self._npt_reach, n, ne, flags,
+                                                         C.byref(nbytes)))
+        else:
+            L.check(L.lib().fd3_eval_edges_workspace(C.byref(m), C.byref(g), n, ne, flags, C.byref(nbytes)))
         ws = self._workspace(nbytes.value)
         f64 = dict(dtype=torch.float64, device=dev)
         res = {
@@ -287,10 +305,13 @@ class FourierD3(torch.nn.Module):
         if need_params:
             out.grad_params = L.ptr(res["grad_params"])
         stream = torch.cuda.current_stream(dev).cuda_stream
-        L.check(L.lib().fd3_eval_edges(
-            C.byref(m), C.byref(g), n, L.ptr(pos), L.dtype_code(pos), L.ptr(self._species_i32), ne,
-            L.ptr(ei[0]), L.ptr(ei[1]), L.ptr(sh), L.dtype_code(sh), flags,
-            L.ptr(ws), ws.numel(), C.byref(out), stream))
+        rest = (n, L.ptr(pos), L.dtype_code(pos), L.ptr(self._species_i32), ne, L.ptr(ei[0]), L.ptr(ei[1]),
+                L.ptr(sh), L.dtype_code(sh), flags, L.ptr(ws), ws.numel(), C.byref(out), stream)
+        if npt:
+            L.check(L.lib().fd3_eval_edges_npt(C.byref(m), C.byref(g), self._npt_reach, L.ptr(cell_dev),
+                                               L.ptr(self.npt_status), *rest))
+        else:
+            L.check(L.lib().fd3_eval_edges(C.byref(m), C.byref(g), *rest))
         return res
 
     def _d4_struct(self) -> L.CnD4:

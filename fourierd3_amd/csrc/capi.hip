@@ -876,16 +876,26 @@ int fd3_profile_report(int max_entries, char* names, double* total_ms, int64_t* 
 
 // ------------------------------------------------------------ edges path ---
 static int edges_layout(const fd3_model* m, const fd3_geom* g, int64_t n, int64_t ne, Common& c, Arena& ar,
-                        double4** pos4, int32_t** spec_s) {
+                        double4** pos4, int32_t** spec_s, double reach = 1.0, CellDyn** cd = nullptr,
+                        int32_t** st = nullptr) {
   int rc;
   if ((rc = setup_common(m, g, c))) return rc;
   if (n < 0 || ne < 0) {
     set_error("negative atom or edge count");
     return FD3_ERR_ARG;
   }
+  if (!(reach >= 1.0) || !std::isfinite(reach)) {
+    set_error("reach factor must be >= 1, got %g", reach);
+    return FD3_ERR_ARG;
+  }
+  c.geom.kreach = reach;
   carve_common(ar, c, n, m->n_rank, cn_blocks(ne));
   *pos4 = ar.take<double4>(std::max<int64_t>(n, 1));
   *spec_s = ar.take<int32_t>(std::max<int64_t>(n, 1));
+  if (cd) {
+    *cd = ar.take<CellDyn>(1);
+    *st = ar.take<int32_t>(1);
+  }
   return FD3_OK;
 }
 
@@ -902,16 +912,23 @@ int fd3_eval_edges_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_at
   return FD3_OK;
 }
 
-int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n, const void* positions, int pos_dtype,
-                   const int32_t* species, int64_t ne, const int64_t* edge_src, const int64_t* edge_dst,
-                   const void* shifts, int shifts_dtype, int flags, void* workspace, size_t ws_bytes,
-                   const fd3_outputs* out, void* stream) {
-  hipStream_t s = (hipStream_t)stream;
+}  // extern "C"
+
+// The edge-list pipeline; with cell_dev the cell comes from device memory
+// (shifts are Cartesian, so the CN kernels need no cell; the mesh kernels
+// and the cell gradient read the device cell).
+static int eval_edges(const fd3_model* m, const fd3_geom* g, double reach, const double* cell_dev,
+                      int32_t* status_user, int64_t n, const void* positions, int pos_dtype,
+                      const int32_t* species, int64_t ne, const int64_t* edge_src, const int64_t* edge_dst,
+                      const void* shifts, int shifts_dtype, int flags, void* workspace, size_t ws_bytes,
+                      const fd3_outputs* out, hipStream_t s) {
   Common c;
   Arena ar(workspace, ws_bytes);
   double4* pos4;
   int32_t* spec_s;
-  int rc = edges_layout(m, g, n, ne, c, ar, &pos4, &spec_s);
+  CellDyn* cd = nullptr;
+  int32_t* st = nullptr;
+  int rc = edges_layout(m, g, n, ne, c, ar, &pos4, &spec_s, reach, cell_dev ? &cd : nullptr, &st);
   if (rc) return rc;
   if (!ar.ok() || !workspace) {
     set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
@@ -926,6 +943,13 @@ int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n, const void*
     return FD3_ERR_ARG;
   }
   const bool grad = flags & FD3_NEED_GRAD;
+  if (cell_dev) {
+    CellCheck ck;  // no cell list
+    for (int a = 0; a < 3; ++a) { ck.nb[a] = 1; ck.sr[a] = 1; }
+    ck.rc = 0.0;
+    ck.kmax_tab = c.plan.ks_reg ? ks_gtab_dk(c.plan, c.geom) * kGtabN : 0.0;
+    if ((rc = launch_cell_update(c, ck, cell_dev, cd, st, status_user, s))) return rc;
+  }
   if ((rc = launch_prep_pos(positions, pos_dtype, species, n, pos4, spec_s, s))) return rc;
   if ((rc = launch_fill_f32(c.cn_raw, std::max<int64_t>(n, 1), s))) return rc;
   CnArgs cna{};
@@ -940,6 +964,47 @@ int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n, const void*
     if (out->grad_pos && (rc = launch_unpermute(c.gpos, nullptr, n, out->grad_pos, s))) return rc;
   }
   return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, true, out, s);
+}
+
+extern "C" {
+
+int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n, const void* positions, int pos_dtype,
+                   const int32_t* species, int64_t ne, const int64_t* edge_src, const int64_t* edge_dst,
+                   const void* shifts, int shifts_dtype, int flags, void* workspace, size_t ws_bytes,
+                   const fd3_outputs* out, void* stream) {
+  return eval_edges(m, g, 1.0, nullptr, nullptr, n, positions, pos_dtype, species, ne, edge_src, edge_dst, shifts,
+                    shifts_dtype, flags, workspace, ws_bytes, out, (hipStream_t)stream);
+}
+
+int fd3_eval_edges_npt_workspace(const fd3_model* m, const fd3_geom* g, double reach, int64_t n_atoms,
+                                 int64_t n_edges, int flags, size_t* bytes) {
+  (void)flags;
+  Common c;
+  Arena ar;
+  double4* p4;
+  int32_t* sp;
+  CellDyn* cd = nullptr;
+  int32_t* st = nullptr;
+  int rc = edges_layout(m, g, n_atoms, n_edges, c, ar, &p4, &sp, reach, &cd, &st);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+int fd3_eval_edges_npt(const fd3_model* m, const fd3_geom* g, double reach, const double* cell, int32_t* status,
+                       int64_t n, const void* positions, int pos_dtype, const int32_t* species, int64_t ne,
+                       const int64_t* edge_src, const int64_t* edge_dst, const void* shifts, int shifts_dtype,
+                       int flags, void* workspace, size_t ws_bytes, const fd3_outputs* out, void* stream) {
+  if (!cell) {
+    set_error("device cell pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  if (g && g->interp == FD3_INTERP_EWALD) {
+    set_error("the device-cell entry point supports the mesh methods (spme, pme)");
+    return FD3_ERR_UNSUPPORTED;
+  }
+  return eval_edges(m, g, reach, cell, status, n, positions, pos_dtype, species, ne, edge_src, edge_dst, shifts,
+                    shifts_dtype, flags, workspace, ws_bytes, out, (hipStream_t)stream);
 }
 
 // --------------------------------------------------------- cell-list path ---

@@ -118,6 +118,21 @@ int fd3_eval_edges(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, const
                    const int64_t* edge_dst, const void* shifts, int shifts_dtype, int flags,
                    void* workspace, size_t workspace_bytes, const fd3_outputs* out, void* stream);
 
+/* fd3_eval_edges_npt: fd3_eval_edges with the cell read from device memory
+ *   (FourierD3._update_cell then forward, src/fourierd3/core.py:244-267,
+ *   480-583; the MLFF-in-NPT caller shape): cell (3,3) Angstrom float64
+ *   [device], g->cell the reference cell for the |k| tables (reach as in
+ *   fd3_eval_celllist_npt), status [device] or NULL (bits 1 and 4).  The
+ *   shifts stay Cartesian, as the caller computes them from its current
+ *   cell; grad_cell is dE/dcell at the device cell.  Mesh methods only. */
+int fd3_eval_edges_npt_workspace(const fd3_model* m, const fd3_geom* g, double reach, int64_t n_atoms,
+                                 int64_t n_edges, int flags, size_t* bytes);
+int fd3_eval_edges_npt(const fd3_model* m, const fd3_geom* g, double reach, const double* cell,
+                       int32_t* status, int64_t n_atoms, const void* positions, int pos_dtype,
+                       const int32_t* species, int64_t n_edges, const int64_t* edge_src,
+                       const int64_t* edge_dst, const void* shifts, int shifts_dtype, int flags,
+                       void* workspace, size_t workspace_bytes, const fd3_outputs* out, void* stream);
+
 int fd3_eval_celllist_workspace(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, int flags,
                                 size_t* bytes);
 int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, const void* positions,

@@ -579,6 +579,39 @@ def test_npt_device_cell_matches_oracle_and_replays(cnfunc):
         assert np.isnan(float(out["energy"][0])) and torch.isnan(out["virial"]).all()
 
 
+def test_module_npt_forward_reads_device_cell():
+    """FourierD3(npt_reach=...): _update_cell keeps the cell on the device and
+    forward (fd3_eval_edges_npt) reads it there, the caller shape of an MLFF
+    in NPT.  Autograd through a strain, as the reference calculator drives
+    it, at three strained cells on the starting cell's mesh, against the
+    oracle on that mesh."""
+    _say("module npt")
+    dev = _dev()
+    box0 = W.hcno_box(600, seed=51)
+    model = FourierD3(box0.numbers, box0.cell, device=dev, dtype=torch.float64, verbose=False, npt_reach=1.1)
+    o = OracleFourierD3(box0.numbers, box0.cell)
+    for k in range(3):
+        box = W.strained(box0, 0.03, seed=51 + k)
+        i, j, S = neighbour_list(box.positions, box.cell, 6.0)
+        cell = torch.tensor(box.cell, dtype=torch.float64, device=dev)
+        strain = torch.zeros(3, 3, dtype=torch.float64, device=dev, requires_grad=True)
+        scell = cell + torch.einsum("ab,Ab->Aa", strain, cell)
+        model._update_cell(scell)
+        pos = torch.tensor(box.positions, dtype=torch.float64, device=dev, requires_grad=True)
+        spos = pos + torch.einsum("ab,ib->ia", strain, pos)
+        ei = torch.tensor(np.stack([i, j]), dtype=torch.long, device=dev)
+        shifts = torch.tensor(S, dtype=torch.float64, device=dev) @ scell
+        e = model(spos, ei, shifts) * HARTREE_TO_EV
+        e.backward()
+        assert int(model.npt_status[0]) == 0
+        forces = (-pos.grad).cpu().numpy()
+        stress = (strain.grad / model.volume * ANGSTROM_TO_BOHR ** 3).detach().cpu().numpy()
+        e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell, edges=(i, j, S))
+        de, df, ds = _errors(float(e), forces, stress, e_ref, f_ref, s_ref, box.n_atoms)
+        print(f"[errors] module npt step {k}: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
+        assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
 def test_batched_boxes_match_single_box_calls():
     """fd3_eval_celllist_batch on boxes of different sizes, cells and meshes
     gives each box exactly what its own fd3_eval_celllist call gives."""

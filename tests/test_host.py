@@ -60,6 +60,21 @@ def test_status_codes_map_to_reference_exceptions():
     with pytest.raises(NotImplementedError, match="mesh methods"):
         L.check(h.fd3_eval_celllist_npt(C.byref(m), C.byref(gew), 1.1, 8, None, 10, None, 1, None, 1,
                                         None, 0, None, None))
+    with pytest.raises(ValueError, match="reach"):
+        L.check(h.fd3_eval_edges_npt_workspace(C.byref(m), C.byref(g16), 0.5, 10, 20, 1, C.byref(nbytes)))
+    L.check(h.fd3_eval_edges_npt_workspace(C.byref(m), C.byref(g16), 1.1, 10, 20, 1, C.byref(nbytes)))
+    with pytest.raises(ValueError, match="device cell"):
+        L.check(h.fd3_eval_edges_npt(C.byref(m), C.byref(g16), 1.1, None, None, 10, None, 1, None, 20, None,
+                                     None, None, 1, 1, None, 0, None, None))
+    with pytest.raises(NotImplementedError, match="mesh methods"):
+        L.check(h.fd3_eval_edges_npt(C.byref(m), C.byref(gew), 1.1, 8, None, 10, None, 1, None, 20, None,
+                                     None, None, 1, 1, None, 0, None, None))
+    d4 = L.CnD4()
+    with pytest.raises(ValueError, match="reach"):
+        L.check(h.fd3_eval_d4_npt_workspace(C.byref(m), C.byref(d4), C.byref(g16), 0.5, 10, 1, C.byref(nbytes)))
+    with pytest.raises(NotImplementedError, match="mesh methods"):
+        L.check(h.fd3_eval_d4_npt(C.byref(m), C.byref(d4), C.byref(gew), 1.1, 8, None, 10, None, 1, None, 1,
+                                  None, 0, None, None))
 
 
 def test_workspace_sizes_scale_with_mesh_and_atoms():

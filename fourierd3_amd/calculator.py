@@ -24,7 +24,7 @@ import torch
 
 from . import _lib as L
 from . import tables as T
-from .core import FourierD3
+from .core import FourierD3, check_dtype
 
 try:  # pragma: no cover - ASE is optional
     from ase.calculators.calculator import Calculator as _Base, all_changes
@@ -77,6 +77,7 @@ class FourierD3ASECalculator(_Base):
                  k_cutoff: float = 10.0, mesh_spacing: float = 1.2, interpolation_nodes: int = 5,
                  dtype=torch.float32, **kwargs):
         super().__init__(**kwargs)
+        check_dtype(dtype)  # float32 kernels; dtype=torch.float64 raises NotImplementedError
         self.device = torch.device(device) if device is not None else torch.device(
             "cuda" if torch.cuda.is_available() else "cpu")
         self.r_cut = float(r_cut)

@@ -12,6 +12,14 @@ any input requires grad, its exact gradients with respect to positions,
 shifts, the cell passed to ``_update_cell`` and (optionally) the BJ
 parameters, from hand-written adjoint kernels.  ``torch.autograd`` sees one
 ``Function`` whose backward scales those gradients by the incoming gradient.
+That backward is first-order only: a second derivative such as a
+force-matching loss built with ``create_graph=True`` raises instead of
+silently dropping the Hessian terms.
+
+Precision follows the reference's default ``dtype=torch.float32``: mesh and
+pair arithmetic in float32, energy / virial / gradient reductions in float64.
+``dtype=torch.float64`` (the reference's all-FP64 mode) is not built and
+raises ``NotImplementedError``.
 """
 from __future__ import annotations
 
@@ -36,6 +44,38 @@ def _det3(m: torch.Tensor) -> torch.Tensor:
     return m[0, 0] * c00 + m[0, 1] * c10 + m[0, 2] * c20
 
 
+def check_dtype(dtype) -> None:
+    """The kernels compute in float32 with float64 reductions, the
+    reference's default precision (``src/fourierd3/core.py:60,71-80``); its
+    all-float64 mode has no kernels here, so it is refused rather than
+    silently computed in float32."""
+    if dtype == torch.float64:
+        raise NotImplementedError(
+            "dtype=torch.float64 is not built: the HIP kernels compute the mesh and pair terms in float32 "
+            "with float64 reductions (the reference's default dtype=torch.float32); positions and shifts "
+            "may still be passed as float64")
+    if dtype != torch.float32:
+        raise ValueError(f"unsupported dtype {dtype}; expected torch.float32")
+
+
+def _first_order(backward):
+    """Backward of a Function whose gradients come ready-made from the adjoint
+    kernels (no graph): first-order only.  ``create_graph=True`` (grad mode on
+    inside backward) raises at once.  ``torch.autograd.function.
+    once_differentiable`` alone raises only when the incoming gradient
+    requires grad, so a force loss built with ``create_graph=True`` from a
+    plain ``grad(E, pos)`` would silently get constant forces."""
+    def wrapper(ctx, *grads):
+        if torch.is_grad_enabled():
+            raise RuntimeError(
+                "FourierD3 gradients come from hand-written first-order adjoint kernels: second derivatives "
+                "(create_graph=True, e.g. a force-matching loss) are not supported")
+        with torch.no_grad():
+            return backward(ctx, *grads)
+    wrapper.__doc__ = backward.__doc__
+    return wrapper
+
+
 class _EdgesEnergy(torch.autograd.Function):
     """E(positions, shifts, cell, params) with gradients from the adjoint kernels."""
 
@@ -50,6 +90,7 @@ class _EdgesEnergy(torch.autograd.Function):
         return res["energy"][0].to(model.dtype)
 
     @staticmethod
+    @_first_order
     def backward(ctx, g):
         gpos, gsh, gcell, gpar = ctx.saved_tensors
         dp, ds, dc, dpr = ctx.dtypes
@@ -77,7 +118,7 @@ class _D4Energy(torch.autograd.Function):
             pos = positions.detach().to(torch.float64)
             c = cell_A.detach().to(device=pos.device, dtype=torch.float64)
             rhs = res["virial"] - res["grad_pos"].T @ pos
-            gcell = torch.linalg.solve(c.T, rhs.T)
+            gcell = torch.linalg.solve_ex(c.T, rhs.T)[0]  # no info check: no host sync
         ctx.save_for_backward(res["grad_pos"], gcell, res["grad_params"])
         ctx.dtypes = (positions.dtype, cell_A.dtype, params.dtype)
         ctx.param_device = params.device
@@ -85,6 +126,7 @@ class _D4Energy(torch.autograd.Function):
         return res["energy"][0].to(model.dtype)
 
     @staticmethod
+    @_first_order
     def backward(ctx, g):
         gpos, gcell, gpar = ctx.saved_tensors
         dp, dc, dpr = ctx.dtypes
@@ -113,6 +155,7 @@ class FourierD3(torch.nn.Module):
                  cncorr: Optional[torch.Tensor] = None, verbose=True, ns_mesh=None,
                  npt_reach: Optional[float] = None) -> None:
         super().__init__()
+        check_dtype(dtype)
         device = torch.device(device)
         self.device = device
         self.dtype = dtype
@@ -337,10 +380,21 @@ class FourierD3(torch.nn.Module):
             pos = pos.double()
         m = self._model_struct()
         d4 = self._d4_struct()
-        g = self._geom(cell_A.detach().to(torch.float64).cpu().numpy())
+        # with npt_reach the cell stays on the device (fd3_eval_d4_npt): no
+        # host copy, so the step can be captured in a graph
+        npt = self._npt_reach is not None
+        if npt:
+            cell_dev = cell_A.detach().to(device=dev, dtype=torch.float64).contiguous()
+            g = self._geom(self._npt_ref_cell)
+        else:
+            g = self._geom(cell_A.detach().to(torch.float64).cpu().numpy())
         flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
         nbytes = C.c_size_t()
-        L.check(L.lib().fd3_eval_d4_workspace(C.byref(m), C.byref(d4), C.byref(g), n, flags, C.byref(nbytes)))
+        if npt:
+            L.check(L.lib().fd3_eval_d4_npt_workspace(C.byref(m), C.byref(d4), C.byref(g), self._npt_reach, n,
+                                                      flags, C.byref(nbytes)))
+        else:
+            L.check(L.lib().fd3_eval_d4_workspace(C.byref(m), C.byref(d4), C.byref(g), n, flags, C.byref(nbytes)))
         ws = self._workspace(nbytes.value)
         f64 = dict(dtype=torch.float64, device=dev)
         res = {
@@ -359,9 +413,13 @@ class FourierD3(torch.nn.Module):
         if need_params:
             out.grad_params = L.ptr(res["grad_params"])
         stream = torch.cuda.current_stream(dev).cuda_stream
-        L.check(L.lib().fd3_eval_d4(
-            C.byref(m), C.byref(d4), C.byref(g), n, L.ptr(pos), L.dtype_code(pos), L.ptr(self._species_i32),
-            flags, L.ptr(ws), ws.numel(), C.byref(out), stream))
+        rest = (n, L.ptr(pos), L.dtype_code(pos), L.ptr(self._species_i32), flags, L.ptr(ws), ws.numel(),
+                C.byref(out), stream)
+        if npt:
+            L.check(L.lib().fd3_eval_d4_npt(C.byref(m), C.byref(d4), C.byref(g), self._npt_reach, L.ptr(cell_dev),
+                                            L.ptr(self.npt_status), *rest))
+        else:
+            L.check(L.lib().fd3_eval_d4(C.byref(m), C.byref(d4), C.byref(g), *rest))
         return res
 
     def forward(self, positions: torch.Tensor, edge_index: torch.Tensor = None,

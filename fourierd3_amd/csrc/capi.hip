@@ -592,6 +592,30 @@ __global__ void k_cell_update(const double* __restrict__ cell_A, CellCheck ck, C
   if (status_user) *status_user = st;
 }
 
+// A nonzero device-cell status means pairs beyond the cell-list reach were
+// missed or G was read past its table: the per-atom gradients are as wrong as
+// the energy, so they are poisoned with NaN like it (k_finalize does the rest).
+__global__ void k_poison_grads(const int32_t* __restrict__ status, double* __restrict__ a, int64_t na,
+                               double* __restrict__ b, int64_t nb) {
+  if (*status == 0) return;
+  const double nan = __builtin_nan("");
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < na + nb; i += (int64_t)gridDim.x * blockDim.x) {
+    if (i < na) a[i] = nan;
+    else b[i - na] = nan;
+  }
+}
+
+static int launch_poison(const Common& c, double* gpos, int64_t n_pos, double* gsh, int64_t n_sh, hipStream_t s) {
+  if (!c.status) return FD3_OK;
+  const int64_t na = gpos ? 3 * n_pos : 0, nb = gsh ? 3 * n_sh : 0;
+  if (na + nb == 0) return FD3_OK;
+  ProfScope _ps("k_poison_grads", s);
+  hipLaunchKernelGGL(k_poison_grads, dim3((unsigned)std::min<int64_t>(cdiv(na + nb, 256), 1024)), dim3(256), 0, s,
+                     c.status, gpos, na, gsh, nb);
+  FD3_LAUNCH_CHECK("k_poison_grads");
+  return FD3_OK;
+}
+
 // Queue k_cell_update and point the pipeline at the device cell.
 static int launch_cell_update(Common& c, CellCheck ck, const double* cell_dev, CellDyn* cd, int32_t* st,
                               int32_t* status_user, hipStream_t s) {
@@ -811,6 +835,7 @@ static int eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, d
   if (grad && n > 0) {
     if ((rc = d4_passes(m, d4, c, db, n, pos4, spec_s, true, s))) return rc;
     if (out->grad_pos && (rc = launch_unpermute(c.gpos, nullptr, n, out->grad_pos, s))) return rc;
+    if ((rc = launch_poison(c, out->grad_pos, n, nullptr, 0, s))) return rc;
   }
   fd3_outputs o2 = *out;
   o2.grad_cell = nullptr;  // the D4 path reports the strain derivative only
@@ -962,6 +987,7 @@ static int eval_edges(const fd3_model* m, const fd3_geom* g, double reach, const
   if (grad) {
     if ((rc = launch_cn_bwd_edges(ea, cna, pos4, spec_s, out->grad_shifts, s))) return rc;
     if (out->grad_pos && (rc = launch_unpermute(c.gpos, nullptr, n, out->grad_pos, s))) return rc;
+    if ((rc = launch_poison(c, out->grad_pos, n, out->grad_shifts, ne, s))) return rc;
   }
   return run_finalize(c, grad, (flags & FD3_NEED_PARAMS) != 0, true, out, s);
 }
@@ -1102,6 +1128,7 @@ static int eval_celllist(const fd3_model* m, const fd3_geom* g, double reach, co
     // the CN backward adds the last term of dE/dr and writes grad_pos (atom order, per Angstrom)
     cna.gout = out->grad_pos; cna.slot_atom = cb.slot_atom;
     if ((rc = launch_cn_bwd_cells(ca, cna, cg.nbins, s))) return rc;
+    if ((rc = launch_poison(c, out->grad_pos, n, nullptr, 0, s))) return rc;
   }
   fd3_outputs o2 = *out;
   o2.grad_cell = nullptr;  // the cell-list path reports the strain derivative only
@@ -1186,7 +1213,7 @@ int fd3_eval_celllist_batch(const fd3_model* m, int32_t n_boxes, const fd3_geom*
 
 // ------------------------------------------------------------ neighbours ---
 static int nb_layout(int64_t n, const double cell[9], double r_cut, CellGrid& cg, CellBuffers& cb, Arena& ar,
-                     int32_t** count_atom, int32_t** offset) {
+                     int32_t** count_atom, int64_t** offset) {
   if (n < 0) {
     set_error("negative atom count");
     return FD3_ERR_ARG;
@@ -1195,7 +1222,7 @@ static int nb_layout(int64_t n, const double cell[9], double r_cut, CellGrid& cg
   if ((rc = cell_grid_make(cell, r_cut, &cg))) return rc;
   cell_carve(cg, std::max<int64_t>(n, 1), ar, &cb);
   *count_atom = ar.take<int32_t>(std::max<int64_t>(n, 1));
-  *offset = ar.take<int32_t>(n + 1);
+  *offset = ar.take<int64_t>(n + 1);
   return FD3_OK;
 }
 
@@ -1203,7 +1230,8 @@ int fd3_neighbours_workspace(int64_t n, const double cell[9], double r_cut, size
   CellGrid cg;
   CellBuffers cb;
   Arena ar;
-  int32_t *ca, *of;
+  int32_t* ca;
+  int64_t* of;
   int rc = nb_layout(n, cell, r_cut, cg, cb, ar, &ca, &of);
   if (rc) return rc;
   *bytes = align_up(ar.off);
@@ -1211,7 +1239,7 @@ int fd3_neighbours_workspace(int64_t n, const double cell[9], double r_cut, size
 }
 
 static int nb_prepare(int64_t n, const void* pos, int dtype, const double cell[9], double r_cut, void* ws,
-                      size_t ws_bytes, int64_t* total, CellGrid& cg, CellBuffers& cb, int32_t** offset,
+                      size_t ws_bytes, int64_t* total, CellGrid& cg, CellBuffers& cb, int64_t** offset,
                       hipStream_t s) {
   Arena ar(ws, ws_bytes);
   int32_t* count_atom;
@@ -1234,7 +1262,7 @@ int fd3_neighbours_count(int64_t n, const void* pos, int dtype, const double cel
                          size_t ws_bytes, int64_t* n_pairs, void* stream) {
   CellGrid cg;
   CellBuffers cb;
-  int32_t* offset;
+  int64_t* offset;
   return nb_prepare(n, pos, dtype, cell, r_cut, ws, ws_bytes, n_pairs, cg, cb, &offset, (hipStream_t)stream);
 }
 
@@ -1243,7 +1271,7 @@ int fd3_neighbours_fill(int64_t n, const void* pos, int dtype, const double cell
                         void* stream) {
   CellGrid cg;
   CellBuffers cb;
-  int32_t* offset;
+  int64_t* offset;
   hipStream_t s = (hipStream_t)stream;
   int rc = nb_prepare(n, pos, dtype, cell, r_cut, ws, ws_bytes, nullptr, cg, cb, &offset, s);
   if (rc || n == 0) return rc;

@@ -129,7 +129,9 @@ __global__ void k_bin(CellArgs c, const void* pos, int dtype, int64_t n, double4
 }
 
 // Exclusive scan of n ints into out[0..n] (out[n] = total); one block of 1024.
-__global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, int64_t n, int32_t* __restrict__ out,
+// OutT = int64_t for the neighbour-pair offsets, whose total may pass 2^31.
+template <typename OutT>
+__global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, int64_t n, OutT* __restrict__ out,
                                                 int64_t* __restrict__ total64) {
   __shared__ int64_t part[1024];
   const int t = threadIdx.x;
@@ -147,11 +149,11 @@ __global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, 
   }
   int64_t run = t ? part[t - 1] : 0;
   for (int64_t i = lo; i < hi; ++i) {
-    out[i] = (int32_t)run;
+    out[i] = (OutT)run;
     run += in[i];
   }
   if (t == 1023) {
-    out[n] = (int32_t)part[1023];
+    out[n] = (OutT)part[1023];
     if (total64) total64[0] = part[1023];
   }
 }
@@ -545,7 +547,7 @@ __global__ void k_nb_count(CellArgs c, int32_t* __restrict__ count_atom) {
   count_atom[c.slot_atom[s]] = cnt;
 }
 
-__global__ void k_nb_fill(CellArgs c, const int32_t* __restrict__ offset, int64_t capacity, int64_t* __restrict__ src,
+__global__ void k_nb_fill(CellArgs c, const int64_t* __restrict__ offset, int64_t capacity, int64_t* __restrict__ src,
                           int64_t* __restrict__ dst, int32_t* __restrict__ sh) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= c.n) return;
@@ -573,11 +575,11 @@ __device__ inline bool nb_less(int64_t j1, const int32_t* s1, int64_t j2, const 
 }
 
 // Sort each atom's pairs by (j, S).
-__global__ void k_nb_sort(const int32_t* __restrict__ offset, int64_t n, int64_t capacity, int64_t* __restrict__ dst,
+__global__ void k_nb_sort(const int64_t* __restrict__ offset, int64_t n, int64_t capacity, int64_t* __restrict__ dst,
                           int32_t* __restrict__ sh) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int64_t lo = offset[i], hi = min((int64_t)offset[i + 1], capacity);
+  const int64_t lo = offset[i], hi = min(offset[i + 1], capacity);
   for (int64_t a = lo + 1; a < hi; ++a) {
     const int64_t vj = dst[a];
     const int32_t vs[3] = {sh[3 * a], sh[3 * a + 1], sh[3 * a + 2]};
@@ -695,7 +697,8 @@ int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* p
   ProfScope _ps("k_celllist_build", s);
   hipLaunchKernelGGL(k_bin, dim3(cdiv(n, 256)), dim3(256), 0, s, c, pos, dtype, n, b.posw, b.flo, b.abin, b.bcount);
   FD3_LAUNCH_CHECK("k_bin");
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, b.bcount, (int64_t)g.nbins, b.bstart, (int64_t*)nullptr);
+  hipLaunchKernelGGL(k_scan1<int32_t>, dim3(1), dim3(1024), 0, s, b.bcount, (int64_t)g.nbins, b.bstart,
+                     (int64_t*)nullptr);
   FD3_LAUNCH_CHECK("k_scan1");
   // scatter into sbin (free until k_fill_sorted), then sort each bin into slot_atom
   hipLaunchKernelGGL(k_scatter, dim3(cdiv(n, 256)), dim3(256), 0, s, b.abin, n, b.bstart, b.bcur, b.sbin);
@@ -741,18 +744,18 @@ int launch_cn_bwd_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4,
   return FD3_OK;
 }
 
-int nb_count(const CellArgs& c, int32_t* count_atom, int32_t* offset, int64_t* total, hipStream_t s) {
+int nb_count(const CellArgs& c, int32_t* count_atom, int64_t* offset, int64_t* total, hipStream_t s) {
   if (c.n <= 0) {
     return FD3_OK;
   }
   hipLaunchKernelGGL(k_nb_count, dim3(cdiv(c.n, 256)), dim3(256), 0, s, c, count_atom);
   FD3_LAUNCH_CHECK("k_nb_count");
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, count_atom, c.n, offset, total);
+  hipLaunchKernelGGL(k_scan1<int64_t>, dim3(1), dim3(1024), 0, s, count_atom, c.n, offset, total);
   FD3_LAUNCH_CHECK("k_scan1");
   return FD3_OK;
 }
 
-int nb_fill(const CellArgs& c, const int32_t* offset, int64_t capacity, int64_t* src, int64_t* dst, int32_t* sh,
+int nb_fill(const CellArgs& c, const int64_t* offset, int64_t capacity, int64_t* src, int64_t* dst, int32_t* sh,
             hipStream_t s) {
   if (c.n <= 0) return FD3_OK;
   hipLaunchKernelGGL(k_nb_fill, dim3(cdiv(c.n, 256)), dim3(256), 0, s, c, offset, capacity, src, dst, sh);

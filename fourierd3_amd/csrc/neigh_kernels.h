@@ -80,8 +80,9 @@ size_t scan_tmp_elems(int64_t n);
 int launch_cn_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, hipStream_t s);
 int launch_cn_bwd_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, double* gshift,
                         hipStream_t s);
-int nb_count(const CellArgs& c, int32_t* count_atom, int32_t* offset, int64_t* total, hipStream_t s);
-int nb_fill(const CellArgs& c, const int32_t* offset, int64_t capacity, int64_t* src, int64_t* dst, int32_t* sh,
+// pair offsets are int64: the pair total of a large box may pass 2^31
+int nb_count(const CellArgs& c, int32_t* count_atom, int64_t* offset, int64_t* total, hipStream_t s);
+int nb_fill(const CellArgs& c, const int64_t* offset, int64_t capacity, int64_t* src, int64_t* dst, int32_t* sh,
             hipStream_t s);
 
 // implemented in mesh.hip

@@ -150,7 +150,8 @@ int fd3_eval_celllist(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, co
  *   the factor reach (>= 1).  status: int32 [device] or NULL, written on the
  *   stream: 0, or 1 (cell singular or not finite) | 2 (plane spacing below
  *   the planned reach) | 4 (|k| beyond the tables); when nonzero the energy,
- *   virial and grad_params are NaN.  Mesh methods only (not Ewald). */
+ *   virial, grad_params and grad_pos (and, for the edge-list form,
+ *   grad_shifts) are NaN.  Mesh methods only (not Ewald). */
 int fd3_eval_celllist_npt_workspace(const fd3_model* m, const fd3_geom* g, double reach, int64_t n_atoms,
                                     int flags, size_t* bytes);
 int fd3_eval_celllist_npt(const fd3_model* m, const fd3_geom* g, double reach, const double* cell,

@@ -197,11 +197,12 @@ def test_calculator_matches_oracle(case):
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
 def test_module_autograd_matches_reference_calculator(dtype):
     """FourierD3.forward + autograd through a strain tensor, exactly as the
-    reference calculator drives it (src/fourierd3/calculator.py:146-188)."""
+    reference calculator drives it (src/fourierd3/calculator.py:146-188), with
+    float32 and float64 positions, cell and shifts into the float32 model."""
     _say("module autograd", dtype)
     box = W.strained(W.sio2_box(120, seed=9), 0.08, seed=9)
     i, j, S = neighbour_list(box.positions, box.cell, 6.0)
-    model = FourierD3(box.numbers, box.cell, device=_dev(), dtype=dtype, verbose=False)
+    model = FourierD3(box.numbers, box.cell, device=_dev(), verbose=False)
     dev = _dev()
     cell = torch.tensor(box.cell, dtype=dtype, device=dev)
     strain = torch.zeros(3, 3, dtype=dtype, device=dev, requires_grad=True)
@@ -577,19 +578,21 @@ def test_npt_device_cell_matches_oracle_and_replays(cnfunc):
         print(f"[errors] npt cell scaled by {scale}: status {st}", flush=True)
         assert st & bit
         assert np.isnan(float(out["energy"][0])) and torch.isnan(out["virial"]).all()
+        assert torch.isnan(out["grad_pos"]).all()  # forces are poisoned too, not silently wrong
 
 
-def test_module_npt_forward_reads_device_cell():
+@pytest.mark.parametrize("cnfunc", ["smooth_cut", "d4"])
+def test_module_npt_forward_reads_device_cell(cnfunc):
     """FourierD3(npt_reach=...): _update_cell keeps the cell on the device and
-    forward (fd3_eval_edges_npt) reads it there, the caller shape of an MLFF
-    in NPT.  Autograd through a strain, as the reference calculator drives
-    it, at three strained cells on the starting cell's mesh, against the
-    oracle on that mesh."""
-    _say("module npt")
+    forward (fd3_eval_edges_npt, or fd3_eval_d4_npt for cnfunc='d4') reads it
+    there, the caller shape of an MLFF in NPT.  Autograd through a strain, as
+    the reference calculator drives it, at three strained cells on the
+    starting cell's mesh, against the oracle on that mesh."""
+    _say("module npt", cnfunc)
     dev = _dev()
     box0 = W.hcno_box(600, seed=51)
-    model = FourierD3(box0.numbers, box0.cell, device=dev, dtype=torch.float64, verbose=False, npt_reach=1.1)
-    o = OracleFourierD3(box0.numbers, box0.cell)
+    model = FourierD3(box0.numbers, box0.cell, device=dev, verbose=False, npt_reach=1.1, cnfunc=cnfunc)
+    o = OracleFourierD3(box0.numbers, box0.cell, cnfunc=cnfunc)
     for k in range(3):
         box = W.strained(box0, 0.03, seed=51 + k)
         i, j, S = neighbour_list(box.positions, box.cell, 6.0)
@@ -599,16 +602,22 @@ def test_module_npt_forward_reads_device_cell():
         model._update_cell(scell)
         pos = torch.tensor(box.positions, dtype=torch.float64, device=dev, requires_grad=True)
         spos = pos + torch.einsum("ab,ib->ia", strain, pos)
-        ei = torch.tensor(np.stack([i, j]), dtype=torch.long, device=dev)
-        shifts = torch.tensor(S, dtype=torch.float64, device=dev) @ scell
-        e = model(spos, ei, shifts) * HARTREE_TO_EV
+        if cnfunc == "d4":
+            e = model(spos) * HARTREE_TO_EV
+        else:
+            ei = torch.tensor(np.stack([i, j]), dtype=torch.long, device=dev)
+            shifts = torch.tensor(S, dtype=torch.float64, device=dev) @ scell
+            e = model(spos, ei, shifts) * HARTREE_TO_EV
         e.backward()
         assert int(model.npt_status[0]) == 0
         forces = (-pos.grad).cpu().numpy()
         stress = (strain.grad / model.volume * ANGSTROM_TO_BOHR ** 3).detach().cpu().numpy()
-        e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell, edges=(i, j, S))
+        if cnfunc == "d4":
+            e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+        else:
+            e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell, edges=(i, j, S))
         de, df, ds = _errors(float(e), forces, stress, e_ref, f_ref, s_ref, box.n_atoms)
-        print(f"[errors] module npt step {k}: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
+        print(f"[errors] module npt {cnfunc} step {k}: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
         assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
@@ -664,7 +673,7 @@ def test_ewald_module_autograd_and_param_gradients():
     box = W.strained(W.sio2_box(48, seed=32), 0.08, seed=32)
     i, j, S = neighbour_list(box.positions, box.cell, 6.0)
     dev = _dev()
-    model = FourierD3(box.numbers, box.cell, device=dev, dtype=torch.float64, method="ewald", k_cutoff=8.0,
+    model = FourierD3(box.numbers, box.cell, device=dev, method="ewald", k_cutoff=8.0,
                       verbose=False)
     model.params.requires_grad_(True)
     cell = torch.tensor(box.cell, dtype=torch.float64, device=dev)
@@ -729,7 +738,7 @@ def test_compute_cn_smooth_and_c6_weights_match_oracle():
     box = W.strained(W.hcno_box(300, seed=40), 0.1, seed=40)
     i, j, S = neighbour_list(box.positions, box.cell, 6.0)
     dev = _dev()
-    model = FourierD3(box.numbers, box.cell, device=dev, dtype=torch.float64, verbose=False)
+    model = FourierD3(box.numbers, box.cell, device=dev, verbose=False)
     pB = torch.tensor(box.positions, dtype=torch.float64) * ANGSTROM_TO_BOHR
     shB = torch.tensor(S, dtype=torch.float64) @ torch.tensor(box.cell) * ANGSTROM_TO_BOHR
     ei = torch.tensor(np.stack([i, j]))

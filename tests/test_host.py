@@ -139,3 +139,46 @@ def test_workload_generators():
     f0 = b.positions @ np.linalg.inv(b.cell)
     f1 = t.positions @ np.linalg.inv(t.cell)
     assert np.allclose(f0, f1)
+
+
+def test_float64_dtype_is_refused_not_silently_float32():
+    """The kernels compute in float32 with float64 reductions; the reference's
+    all-float64 mode (dtype=torch.float64, src/fourierd3/core.py:60) is not
+    built and must raise rather than return float32-level answers."""
+    from fourierd3_amd import FourierD3, FourierD3ASECalculator
+    box = W.sio2_box(30, seed=1)
+    with pytest.raises(NotImplementedError, match="float64"):
+        FourierD3(box.numbers, box.cell, device="cpu", dtype=torch.float64, verbose=False)
+    with pytest.raises(NotImplementedError, match="float64"):
+        FourierD3ASECalculator(device="cpu", dtype=torch.float64)
+    with pytest.raises(ValueError):
+        FourierD3(box.numbers, box.cell, device="cpu", dtype=torch.float16, verbose=False)
+
+
+def test_backward_is_first_order_only():
+    """The custom backward returns gradients the adjoint kernels computed, so
+    it has no graph: a second derivative (create_graph=True, e.g. a force
+    loss) must raise instead of silently dropping the Hessian terms.  The HIP
+    call is replaced by fixed gradients, so no GPU is needed."""
+    from fourierd3_amd import FourierD3
+    box = W.sio2_box(30, seed=1)
+    n = box.n_atoms
+    for cnfunc in ("smooth_cut", "d4"):
+        m = FourierD3(box.numbers, box.cell, device="cpu", verbose=False, cnfunc=cnfunc)
+        f64 = dict(dtype=torch.float64)
+        fake = {"energy": torch.tensor([-1.5], **f64), "grad_pos": torch.full((n, 3), 0.25, **f64),
+                "grad_shifts": torch.zeros((4, 3), **f64), "grad_cell": torch.zeros((3, 3), **f64),
+                "virial": torch.zeros((3, 3), **f64), "grad_params": torch.zeros(4, **f64)}
+        m._run_edges = lambda *a, **k: fake
+        m._run_d4 = lambda *a, **k: fake
+        pos = torch.tensor(box.positions, requires_grad=True)
+        if cnfunc == "d4":
+            e = m(pos)
+        else:
+            e = m(pos, torch.zeros((2, 4), dtype=torch.long), torch.zeros((4, 3), dtype=torch.float64))
+        (g,) = torch.autograd.grad(e, pos)                      # first order: the kernels' gradient
+        assert torch.allclose(g, fake["grad_pos"])
+        e2 = m(pos) if cnfunc == "d4" else m(pos, torch.zeros((2, 4), dtype=torch.long),
+                                              torch.zeros((4, 3), dtype=torch.float64))
+        with pytest.raises(RuntimeError, match="second derivatives"):
+            torch.autograd.grad(e2, pos, create_graph=True)

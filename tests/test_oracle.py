@@ -436,6 +436,95 @@ def test_cn_d4_ewald_converges_to_direct_sum():
     assert ref[0] > 0.1 and ref[1] > 0.1  # the O and Si atoms are bonded to something
 
 
+def _d4_box():
+    rng = np.random.default_rng(1)
+    cell = np.diag([6.0, 6.6, 5.7]) + 0.3 * rng.standard_normal((3, 3))
+    return np.array([8, 14, 1, 6]), rng.random((4, 3)) @ cell, cell
+
+
+@pytest.mark.parametrize("method,order", [("spme", 5), ("spme", 3), ("pme", 6), ("pme", 4)])
+def test_cn_d4_mesh_converges_to_ewald(method, order):
+    """The mesh branch of cn_d4 (spread of the one-hot species density ->
+    rfftn -> G^CN, divided by sinc^2p for SPME -> irfftn(norm='forward') ->
+    gather, core.py:417-446, kspace_filter_d3.py:116-158) against the Ewald
+    branch, which the test above pins to the direct real-space sum: on the
+    same box the difference falls with the mesh and is below 2e-5 at 128^3.
+    A normalisation slip (a factor of the mesh size or of 2), a missing or
+    doubled sinc division, or a wrong gather would leave an O(1e-2..1)
+    difference."""
+    Z, pos, cell = _d4_box()
+    pb = torch.tensor(pos) * ANGSTROM_TO_BOHR
+    ref = OracleFourierD3(Z, cell, method="ewald", cnfunc="d4", k_cutoff=45.0).cn_d4(pb).numpy()
+    errs = []
+    for n in (32, 64, 128):
+        o = OracleFourierD3(Z, cell, method=method, interpolation_nodes=order, cnfunc="d4", ns_mesh=(n, n, n))
+        errs.append(float(np.abs(o.cn_d4(pb).numpy() - ref).max()))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 2e-5
+    assert ref[0] > 0.1 and ref[1] > 0.1
+
+
+@pytest.mark.parametrize("method,order", [("spme", 5), ("spme", 4), ("spme", 1), ("pme", 5), ("pme", 6)])
+def test_gather_matches_brute_force_interpolation(method, order):
+    """``gather`` (mesh_to_points, mesh_interpolator_d3.py:141-189) against a
+    brute-force sum over every mesh node with weights built independently:
+    SPME weights are scipy's B-spline basis element of order p on the knots
+    -p/2 .. p/2 at the periodic node distance (no node-set convention needed:
+    the spline vanishes outside), PME weights scipy's Lagrange polynomial
+    through the p nearest nodes.  Also the adjoint identity
+    <spread(c), phi> = <c, gather(phi)> that makes gather the spread's
+    transpose."""
+    from scipy.interpolate import BSpline, lagrange
+    rng = np.random.default_rng(7)
+    cell = np.diag([5.0, 6.0, 5.5]) + 0.4 * rng.standard_normal((3, 3))
+    Z = np.array([8, 14, 8, 1, 6])
+    ns = (8, 12, 10)
+    o = OracleFourierD3(Z, cell, method=method, interpolation_nodes=order, ns_mesh=ns)
+    pos = rng.random((5, 3)) @ cell
+    pb = torch.tensor(pos) * ANGSTROM_TO_BOHR
+    mesh = torch.tensor(rng.standard_normal((3,) + ns))
+    got = o.gather(pb, mesh).numpy()
+    u = (pos @ np.linalg.inv(cell)) * np.array(ns)
+    if method == "spme":
+        basis = BSpline.basis_element(np.arange(order + 1) - order / 2.0, extrapolate=False)
+
+        def w1(ud, g, n):
+            v = 0.0
+            for m in (-1, 0, 1):
+                t = ud - g - m * n
+                b = basis(t)
+                v += 0.0 if np.isnan(b) else float(b)
+            return v
+    else:
+        def w1(ud, g, n):
+            base = np.floor(ud) if order % 2 == 0 else np.round(ud)
+            nodes = base + np.arange(1 - (order + 1) // 2, 1 + order // 2)
+            for k, x in enumerate(nodes):
+                if int(x) % n == g:
+                    y = np.zeros(order)
+                    y[k] = 1.0
+                    return float(lagrange(nodes - base, y)(ud - base))
+            return 0.0
+    ref = np.zeros((5, 3))
+    for i in range(5):
+        wx = np.array([w1(u[i, 0], g, ns[0]) for g in range(ns[0])])
+        wy = np.array([w1(u[i, 1], g, ns[1]) for g in range(ns[1])])
+        wz = np.array([w1(u[i, 2], g, ns[2]) for g in range(ns[2])])
+        if method == "spme" and order > 1:
+            assert wx.sum() == pytest.approx(1.0, abs=1e-12)
+        ref[i] = np.einsum("a,b,c,kabc->k", wx, wy, wz, mesh.numpy())
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+    # adjoint: spread puts charge c_i (rank channel q, species block of atom i)
+    c = torch.tensor(rng.standard_normal((5, 2)))
+    rho = o.spread(pb, c)                                    # (nS, 2, nx, ny, nz)
+    phi = torch.tensor(rng.standard_normal(tuple(rho.shape)))
+    lhs = float((rho * phi).sum())
+    g = o.gather(pb, phi.reshape(-1, *ns))                   # (N, nS * 2)
+    sp = o.species.numpy()
+    rhs = sum(float(c[i, q]) * float(g[i, sp[i] * 2 + q]) for i in range(5) for q in range(2))
+    assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
 def test_d4_oracle_forces_and_stress_match_finite_differences():
     box = W.strained(W.sio2_box(48, seed=6), 0.1, seed=6)
     o = OracleFourierD3(box.numbers, box.cell, ns_mesh=(16, 16, 16), cnfunc="d4")

@@ -26,6 +26,7 @@ FD3_DTYPE_F32 = 0
 FD3_DTYPE_F64 = 1
 FD3_NEED_GRAD = 1
 FD3_NEED_PARAMS = 2
+FD3_NOALLOC_SCRATCH_BYTES = 256 << 20
 
 # Every symbol include/fd3.h declares (checked by the CPU test suite).
 EXPORTS = (
@@ -37,6 +38,7 @@ EXPORTS = (
     "fd3_eval_edges_npt_workspace", "fd3_eval_edges_npt",
     "fd3_eval_d4_workspace", "fd3_eval_d4",
     "fd3_eval_celllist_batch_workspace", "fd3_eval_celllist_batch",
+    "fd3_eval_celllist_noalloc",
     "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
     "fd3_spread", "fd3_gather",
     "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
@@ -110,6 +112,8 @@ _SIG = {
                                                     C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_size_t)]),
     "fd3_eval_celllist_batch": (C.c_int, [C.POINTER(Model), C.c_int32, C.POINTER(Geom), C.POINTER(C.c_int64), _P,
                                           C.c_int, _P, C.c_int, _P, C.c_size_t, C.POINTER(Outputs), _P]),
+    "fd3_eval_celllist_noalloc": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, _P, C.c_int, _P, C.c_int,
+                                            C.POINTER(Outputs), _P]),
     "fd3_neighbours_workspace": (C.c_int, [C.c_int64, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_size_t)]),
     "fd3_neighbours_count": (C.c_int, [C.c_int64, _P, C.c_int, C.POINTER(C.c_double), C.c_double, _P, C.c_size_t,
                                        _P, _P]),

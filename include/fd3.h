@@ -213,6 +213,23 @@ int fd3_eval_celllist_batch(const fd3_model* m, int32_t n_boxes, const fd3_geom*
                             const void* positions, int pos_dtype, const int32_t* species, int flags, void* workspace,
                             size_t workspace_bytes, const fd3_outputs* outs, void* stream);
 
+/* fd3_eval_celllist_noalloc: fd3_eval_celllist with no workspace argument
+ *   and no allocation of any kind.  No counterpart in the reference: it is for
+ *   callers that cannot allocate device memory (code under HIP graph capture,
+ *   tight serving loops, harnesses that forbid hipMalloc).  The workspace is a
+ *   static device array of FD3_NOALLOC_SCRATCH_BYTES (256 MiB) that every GPU
+ *   the library runs on holds until the process ends; a box whose
+ *   fd3_eval_celllist_workspace exceeds it is rejected with
+ *   FD3_ERR_WORKSPACE before any device work (e.g. the 100k-atom 256^3 box
+ *   needs ~2.3 GB: use fd3_eval_celllist).  The current device must be the
+ *   stream's device.  All calls on one GPU share its array: calls that
+ *   overlap on one GPU (two streams, two threads) race and must be
+ *   serialised by the caller. */
+#define FD3_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int fd3_eval_celllist_noalloc(const fd3_model* m, const fd3_geom* g, int64_t n_atoms, const void* positions,
+                              int pos_dtype, const int32_t* species, int flags, const fd3_outputs* out,
+                              void* stream);
+
 /* ---- component entry points (tests and callers that assemble their own path) ----
  * Neighbour list (matscipy "ijS", src/fourierd3/calculator.py:103-109):
  *   fd3_neighbours_count writes the number of directed pairs to *n_pairs [device int64];

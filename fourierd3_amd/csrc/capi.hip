@@ -359,9 +359,9 @@ struct Common {
   float2* k0;        // (C) k = 0 coefficients of dE/drho
   int32_t* ew_perm;  // Ewald: atoms by species
   int32_t* ew_seg;
-  float* cn_raw;
+  double* cn_raw;  // CN and dE/dCN in double: the CN -> C6 chain runs in double
   float* charges;
-  float* dEdcn;
+  double* dEdcn;
   double* gpos;
   double *p_ks, *p_c6, *p_ga, *p_cn, *p_red;
   int n_c6, n_ga, n_cn;
@@ -378,9 +378,9 @@ static void carve_common(Arena& ar, Common& c, int64_t n, int r, int cn_rows) {
   c.ew_perm = ar.take<int32_t>(std::max<int64_t>(n, 1));
   c.ew_seg = ar.take<int32_t>(c.plan.nS + 1);
   col_carve(c.plan, n, c.geom.order, ar, &c.cols);
-  c.cn_raw = ar.take<float>(std::max<int64_t>(n, 1));
+  c.cn_raw = ar.take<double>(std::max<int64_t>(n, 1));
   c.charges = ar.take<float>(std::max<int64_t>(n * r, 1));
-  c.dEdcn = ar.take<float>(std::max<int64_t>(n, 1));
+  c.dEdcn = ar.take<double>(std::max<int64_t>(n, 1));
   c.gpos = ar.take<double>(std::max<int64_t>(3 * n, 3));
   c.n_c6 = c6_blocks(n);
   c.n_ga = c6_blocks(n);  // partials of k_c6_bwd
@@ -635,6 +635,7 @@ struct D4Bufs {
   float2* rho_hat;  // (nS, M1, ny, nx) x-transformed species density spectrum
   float* ones;      // (n) unit charges of the one-hot density
   float* g1;        // (n, 3) forward gather's sum_nodes dW/du phi
+  float* wf;        // (n) dE/dCN as float: the charges of the backward density
   float4* tab;      // per-pair G^CN tables
 };
 
@@ -679,6 +680,7 @@ static int d4_layout(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g,
   db.rho_hat = ar.take<float2>(nrho);
   db.ones = ar.take<float>(std::max<int64_t>(n, 1));
   db.g1 = ar.take<float>(std::max<int64_t>(3 * n, 3));
+  db.wf = ar.take<float>(std::max<int64_t>(n, 1));
   db.tab = ar.take<float4>((size_t)nS * (nS + 1) * kGtabN);
   if (cd) {
     *cd = ar.take<CellDyn>(1);
@@ -730,7 +732,7 @@ static int d4_passes_ewald(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D
   ea.n = n; ea.nS = m->n_species; ea.nx = c.plan.nx; ea.ny = c.plan.ny; ea.nz = c.plan.nz;
   for (int i = 0; i < 9; ++i) { ea.inv[i] = c.geom.inv[i]; ea.recip[i] = c.geom.recip[i]; }
   ea.volume = c.geom.volume;
-  ea.pos4 = pos4; ea.spec_s = spec_s; ea.w = bwd ? c.dEdcn : db.ones;
+  ea.pos4 = pos4; ea.spec_s = spec_s; ea.w = bwd ? db.wf : db.ones;
   ea.perm = c.ew_perm; ea.seg = c.ew_seg;
   ea.tab = db.tab; ea.tkinv = (float)(1.0 / ks_gtab_dk(c.plan, c.geom));
   ea.S = (float2*)c.buf1; ea.Srho = db.rho_hat; ea.phi = c.buf2;
@@ -754,7 +756,7 @@ static int d4_passes(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs&
   fa.r = 1;
   fa.charges = db.ones;
   FusedArgs fs = fa;
-  if (bwd) fs.charges = c.dEdcn;
+  if (bwd) fs.charges = db.wf;
   float2* spec = (float2*)c.buf1;
   if ((rc = spread_r2c_z(fs, db.pcn, c.tab, c.buf2, s))) return rc;
   if ((rc = fft_y(db.pcn, c.tab, c.buf2, spec, -1, s))) return rc;
@@ -833,6 +835,7 @@ static int eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, d
   if ((rc = d4_passes(m, d4, c, db, n, pos4, spec_s, false, s))) return rc;
   if ((rc = run_mesh_path(m, c, n, pos4, spec_s, nullptr, flags, out, s, true))) return rc;
   if (grad && n > 0) {
+    if ((rc = launch_to_f32(c.dEdcn, db.wf, n, s))) return rc;
     if ((rc = d4_passes(m, d4, c, db, n, pos4, spec_s, true, s))) return rc;
     if (out->grad_pos && (rc = launch_unpermute(c.gpos, nullptr, n, out->grad_pos, s))) return rc;
     if ((rc = launch_poison(c, out->grad_pos, n, nullptr, 0, s))) return rc;
@@ -976,9 +979,9 @@ static int eval_edges(const fd3_model* m, const fd3_geom* g, double reach, const
     if ((rc = launch_cell_update(c, ck, cell_dev, cd, st, status_user, s))) return rc;
   }
   if ((rc = launch_prep_pos(positions, pos_dtype, species, n, pos4, spec_s, s))) return rc;
-  if ((rc = launch_fill_f32(c.cn_raw, std::max<int64_t>(n, 1), s))) return rc;
+  if ((rc = launch_fill_f64(c.cn_raw, std::max<int64_t>(n, 1), s))) return rc;
   CnArgs cna{};
-  cna.rcov = m->rcov; cna.rcut = (float)(m->r_cut * kA2B); cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
+  cna.rcov = m->rcov; cna.rcut = m->r_cut * kA2B; cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
   cna.gpos = c.gpos; cna.partials = c.p_cn;
   EdgeArgs ea;
   ea.n_edges = ne; ea.src = edge_src; ea.dst = edge_dst; ea.shifts = shifts; ea.shifts_dtype = shifts_dtype;
@@ -1120,7 +1123,7 @@ static int eval_celllist(const fd3_model* m, const fd3_geom* g, double reach, co
   if ((rc = cell_build(cg, cb, n, positions, pos_dtype, species, s))) return rc;
   const CellArgs ca = cell_args(cg, cb, n);
   CnArgs cna{};
-  cna.rcov = m->rcov; cna.rcut = (float)(m->r_cut * kA2B); cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
+  cna.rcov = m->rcov; cna.rcut = m->r_cut * kA2B; cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
   cna.gpos = c.gpos; cna.partials = c.p_cn;
   if ((rc = launch_cn_cells(ca, cna, cg.nbins, s))) return rc;
   if ((rc = run_mesh_path(m, c, n, cb.pos4, cb.spec_s, cb.slot_atom, flags, out, s))) return rc;

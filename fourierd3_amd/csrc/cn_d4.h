@@ -48,9 +48,9 @@ struct CnAtomArgs {
   const float* selfcont;  // (nS) f_XX(0)
   const float* accA;  // (n) sum_nodes W phi (forward gather)
   const float* accG;  // (n, 3) sum_nodes dW/du of the gathered field
-  float* cn;          // (n) D4 CN (written by cn_d4_fin)
+  double* cn;         // (n) D4 CN (written by cn_d4_fin)
   float* g1;          // (n, 3) forward gather's sum_nodes dW/du phi, kept for the force
-  const float* dEdcn; // (n) dE/dCN (backward)
+  const double* dEdcn; // (n) dE/dCN (backward)
   double* gpos;       // (n, 3) dE/dr (Bohr), added to
   double* partials;   // (blocks, 8) strain-derivative sums
   const CellDyn* dyn = nullptr;  // when set, inv and inv_vol are read from it

@@ -285,7 +285,7 @@ __global__ void k_cn_d4_fin(CnAtomArgs a) {
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   atom_refresh(a);
-  a.cn[s] = (float)((double)a.accA[s] * a.inv_vol) - a.selfcont[a.spec_s[s]];
+  a.cn[s] = (double)a.accA[s] * a.inv_vol - (double)a.selfcont[a.spec_s[s]];
   for (int d = 0; d < 3; ++d) a.g1[3 * s + d] = a.accG[3 * s + d];
 }
 
@@ -317,7 +317,7 @@ __global__ void __launch_bounds__(256) k_cn_d4_force(CnAtomArgs a) {
       for (int d = 0; d < 3; ++d) t += du[d] * ns[d] * a.inv[b * 3 + d];
       a.gpos[3 * s + b] += t;
     }
-    const double e = -g * ((double)a.cn[s] + (double)a.selfcont[a.spec_s[s]]);
+    const double e = -g * (a.cn[s] + (double)a.selfcont[a.spec_s[s]]);
     v[0] = v[1] = v[2] = e;
   }
   block_sum_store<6>(v, red, a.partials + (size_t)blockIdx.x * 8);

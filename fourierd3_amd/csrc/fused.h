@@ -57,7 +57,7 @@ struct C6BwdArgs {
   const float* accA;
   const float* accG;
   const float2* k0;  // (nS * r) or NULL: k = 0 part of dE/dc, added to accA
-  const float* cn_raw;
+  const double* cn_raw;
   const float* cnref;
   const float* vq;
   const float* eigs;
@@ -65,7 +65,7 @@ struct C6BwdArgs {
   const float* cncorr;
   double s6, s8, a1, a2;
   double* gpos;      // (n, 3) out: mesh part of dE/dr, Hartree/Bohr
-  float* dEdcn;      // (n) out
+  double* dEdcn;     // (n) out
   double* partials;  // (c6_blocks(n), 16): sum_i f_i (x) dE/dr_i
 };
 int launch_c6_bwd(const C6BwdArgs& a, hipStream_t s);

@@ -379,7 +379,7 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
 }
 
 // ------------------------------------------------------ C6 chain adjoint ---
-// Per atom, from the gathered sums:
+// Per atom, from the gathered sums (in double, like the forward chain):
 //   dE/dc_q  = accA_q + lambda_q c_q phi_XX(0)       (self term, src/fourierd3/core.py:547-550)
 //   dE/dr_b  = sum_d accG_d n_d cell^-1[b][d]
 //   dE/dCN   = sum_p L_p (g_p - sum_p' L_p' g_p') (-8)(CN' - ref_p) a_X,   g_p = sum_q dE/dc_q V[X,p,q]
@@ -405,45 +405,30 @@ __global__ void __launch_bounds__(256) k_c6_bwd(C6BwdArgs a) {
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) part[i * 3 + j] = f[i] * gr[j];
     const float qqx = a.qq[X * a.nS + X];
-    const float R = (float)(a.a1 * sqrt(3.0 * (double)qqx) + a.a2);
-    const float R2 = R * R, R6 = R2 * R2 * R2;
-    const float phi0 = (float)a.s6 / R6 + 3.0f * (float)a.s8 * qqx / (R6 * R2);
-    float cn = a.cn_raw[s];
-    float ca = 1.0f;
+    const double R = a.a1 * sqrt(3.0 * (double)qqx) + a.a2;
+    const double R2 = R * R, R6 = R2 * R2 * R2;
+    const double phi0 = a.s6 / R6 + 3.0 * a.s8 * (double)qqx / (R6 * R2);
+    double cn = a.cn_raw[s];
+    double ca = 1.0;
     if (a.cncorr) {
-      ca = a.cncorr[2 * X + 1];
-      cn = ca * cn + a.cncorr[2 * X];
+      ca = (double)a.cncorr[2 * X + 1];
+      cn = ca * cn + (double)a.cncorr[2 * X];
     }
-    float L[kNRef];
-    float mx = -INFINITY;
+    double L[kNRef];
+    c6_softmax(a.cnref, X, cn, L);
+    double gp[kNRef] = {0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < a.r; ++q) {
+      // with the charges the mesh and the self term used (float-rounded)
+      double dc = (double)a.accA[s * a.r + q] + (double)a.eigs[q] * (double)a.charges[s * a.r + q] * phi0;
+      if (a.k0) dc += (double)a.k0[X * a.r + q].x;
+      for (int pp = 0; pp < kNRef; ++pp) gp[pp] += dc * (double)a.vq[(X * kNRef + pp) * a.r + q];
+    }
+    double gbar = 0.0;
+    for (int pp = 0; pp < kNRef; ++pp) gbar += L[pp] * gp[pp];
+    double dcn = 0.0;
     for (int pp = 0; pp < kNRef; ++pp) {
       const float ref = a.cnref[X * kNRef + pp];
-      const float d = cn - ref;
-      L[pp] = ref == -1.0f ? -INFINITY : -4.0f * d * d;
-      mx = fmaxf(mx, L[pp]);
-    }
-    float sum = 0.f;
-    for (int pp = 0; pp < kNRef; ++pp) {
-      L[pp] = L[pp] == -INFINITY ? 0.f : expf(L[pp] - mx);
-      sum += L[pp];
-    }
-    float gp[kNRef];
-    float gbar = 0.f;
-    for (int pp = 0; pp < kNRef; ++pp) {
-      L[pp] /= sum;
-      float g = 0.f;
-      for (int q = 0; q < a.r; ++q) {
-        float dc = a.accA[s * a.r + q] + a.eigs[q] * a.charges[s * a.r + q] * phi0;
-        if (a.k0) dc += a.k0[X * a.r + q].x;
-        g += dc * a.vq[(X * kNRef + pp) * a.r + q];
-      }
-      gp[pp] = g;
-      gbar += L[pp] * g;
-    }
-    float dcn = 0.f;
-    for (int pp = 0; pp < kNRef; ++pp) {
-      const float ref = a.cnref[X * kNRef + pp];
-      if (ref != -1.0f) dcn += L[pp] * (gp[pp] - gbar) * (-8.0f) * (cn - ref);
+      if (ref != -1.0f) dcn += L[pp] * (gp[pp] - gbar) * (-8.0) * (cn - (double)ref);
     }
     a.dEdcn[s] = dcn * ca;
   }

@@ -24,7 +24,11 @@ __global__ void k_prep_pos(const void* pos, int dtype, const int32_t* __restrict
 
 // CN -> L_p (softmax over valid references) -> c_q, and the per-atom self
 // term.  src/fourierd3/core.py:383-384 (cncorr), 469-478 (weights),
-// 530-532 (charges), 547-550 (V_self).
+// 530-532 (charges), 547-550 (V_self).  The chain runs in double (the
+// softmax of -4 (CN - ref)^2 is steep, and its float rounding shows in the
+// forces of heavy atoms); the charges are stored as float for the mesh, and
+// the self term uses those same float-rounded charges so that it cancels the
+// mesh self-interaction of exactly the charges that were spread.
 // partials per block (8 doubles): [0] sum_i sum_q lambda_q c_iq^2 phi_XX(0),
 // [1..4] the same sum with phi_XX(0) replaced by d phi_XX(0)/d(s6, s8, a1, a2).
 __global__ void __launch_bounds__(256) k_c6_fwd(C6Args a) {
@@ -33,34 +37,22 @@ __global__ void __launch_bounds__(256) k_c6_fwd(C6Args a) {
   double acc[5] = {0, 0, 0, 0, 0};
   if (s < a.n) {
     const int X = a.spec_s[s];
-    float cn = a.cn_raw[s];
-    if (a.cncorr) cn = a.cncorr[2 * X + 1] * cn + a.cncorr[2 * X];
-    if (a.cn_out) a.cn_out[a.slot_atom ? a.slot_atom[s] : s] = cn;
-    float L[kNRef];
-    float mx = -INFINITY;
-    for (int p = 0; p < kNRef; ++p) {
-      const float ref = a.cnref[X * kNRef + p];
-      const float d = cn - ref;
-      L[p] = ref == -1.0f ? -INFINITY : -4.0f * d * d;
-      mx = fmaxf(mx, L[p]);
-    }
-    float sum = 0.f;
-    for (int p = 0; p < kNRef; ++p) {
-      L[p] = L[p] == -INFINITY ? 0.f : expf(L[p] - mx);
-      sum += L[p];
-    }
-    const float inv = 1.0f / sum;
+    double cn = a.cn_raw[s];
+    if (a.cncorr) cn = (double)a.cncorr[2 * X + 1] * cn + (double)a.cncorr[2 * X];
+    if (a.cn_out) a.cn_out[a.slot_atom ? a.slot_atom[s] : s] = (float)cn;
+    double L[kNRef];
+    c6_softmax(a.cnref, X, cn, L);
     const float qqx = a.qq[X * a.nS + X];
     const double R = a.a1 * sqrt(3.0 * (double)qqx) + a.a2;
     const double R2 = R * R, R6 = R2 * R2 * R2, R8 = R6 * R2;
     const double phi0 = a.s6 / R6 + 3.0 * a.s8 * qqx / R8;
     double t = 0.0;
     for (int q = 0; q < a.r; ++q) {
-      float c = 0.f;
-      for (int p = 0; p < kNRef; ++p) c += L[p] * a.vq[(X * kNRef + p) * a.r + q];
-      c *= inv;
-      a.charges[s * a.r + q] = c;
-      t += (double)a.eigs[q] * (double)c * (double)c;
+      double c = 0.0;
+      for (int p = 0; p < kNRef; ++p) c += L[p] * (double)a.vq[(X * kNRef + p) * a.r + q];
+      const float cf = (float)c;
+      a.charges[s * a.r + q] = cf;
+      t += (double)a.eigs[q] * (double)cf * (double)cf;
     }
     acc[0] = t * phi0;
     acc[1] = t / R6;
@@ -69,8 +61,7 @@ __global__ void __launch_bounds__(256) k_c6_fwd(C6Args a) {
     acc[3] = t * dphidR * sqrt(3.0 * (double)qqx);
     acc[4] = t * dphidR;
   }
-  double v[5] = {acc[0], acc[1], acc[2], acc[3], acc[4]};
-  block_sum_store<5>(v, red, a.partials + (size_t)blockIdx.x * 8);
+  block_sum_store<5>(acc, red, a.partials + (size_t)blockIdx.x * 8);
 }
 
 // Mesh coordinate setup of one atom: base node and 1-D weights per axis.
@@ -133,123 +124,6 @@ __device__ inline float lane_sum(float v) {
   return v;
 }
 
-// Gathering with the adjoint of the C6 chain.  With phi = dE/drho on the mesh:
-//   dE/dc_iq  = sum_nodes W phi[X q] + lambda_q c_iq phi_XX(0)        (self term)
-//   dE/du_d   = sum_nodes dW/du_d sum_q c_iq phi[X q]
-//   dE/dr_b   = sum_d dE/du_d n_d cell^-1[b][d]                       (mesh force)
-//   dE/dCN_i  = sum_p L_p (g_p - sum_q L_q g_q) (-8)(CN' - ref_p) * a_X,  g_p = sum_q dE/dc_q V[X,p,q]
-// (the adjoints of src/fourierd3/core.py:469-478, 530-550 and of points_to_mesh).
-// Per block partials (16 doubles): [0..8] sum_i f_i (x) dE/dr_i (fractional
-// coordinates f = r cell^-1, for dE/dcell).
-template <int LPA, int RMAX>
-__global__ void __launch_bounds__(256) k_gather_bwd(GatherArgs a) {
-  __shared__ double red[16 * 9];
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t s = gid / LPA;
-  const int lane = (int)(gid % LPA);
-  const int p = a.g.order;
-  const bool valid_atom = s < a.n;
-  float A[RMAX];
-#pragma unroll
-  for (int q = 0; q < RMAX; ++q) A[q] = 0.f;
-  float gu[3] = {0.f, 0.f, 0.f};
-  AtomStencil st;
-  int X = 0;
-  float c[RMAX];
-  if (valid_atom) {
-    atom_stencil(a.g, atom_pos(a.pos4, a.raw_pos, a.raw_dtype, s), &st);
-    X = a.spec_s[s];
-#pragma unroll
-    for (int q = 0; q < RMAX; ++q) c[q] = q < a.r ? a.charges[s * a.r + q] : 0.f;
-    if (lane < p * p) {
-      const int ia = lane / p, ib = lane - ia * p;
-      const int gx = wrap(st.b[0] + ia, a.g.nx), gy = wrap(st.b[1] + ib, a.g.ny);
-      const size_t M = (size_t)a.g.nx * a.g.ny * a.g.nz;
-      const float* base = a.mesh + (size_t)X * a.r * M + ((size_t)gx * a.g.ny + gy) * a.g.nz;
-      const float wx = st.w[0][ia], wy = st.w[1][ib], dwx = st.dw[0][ia], dwy = st.dw[1][ib];
-      for (int cz = 0; cz < p; ++cz) {
-        const int gz = wrap(st.b[2] + cz, a.g.nz);
-        const float wz = st.w[2][cz], dwz = st.dw[2][cz];
-        const float wxyz = wx * wy * wz;
-        float sc = 0.f;
-#pragma unroll
-        for (int q = 0; q < RMAX; ++q) {
-          if (q < a.r) {
-            const float v = base[q * M + gz];
-            A[q] += wxyz * v;
-            sc += c[q] * v;
-          }
-        }
-        gu[0] += dwx * wy * wz * sc;
-        gu[1] += wx * dwy * wz * sc;
-        gu[2] += wx * wy * dwz * sc;
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < RMAX; ++q) A[q] = lane_sum<LPA>(A[q]);
-  for (int d = 0; d < 3; ++d) gu[d] = lane_sum<LPA>(gu[d]);
-  double part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  if (valid_atom && lane == 0) {
-    const double ns[3] = {(double)a.g.nx, (double)a.g.ny, (double)a.g.nz};
-    double gr[3];
-    for (int b = 0; b < 3; ++b) {
-      gr[b] = 0.0;
-      for (int d = 0; d < 3; ++d) gr[b] += (double)gu[d] * ns[d] * a.g.inv[b * 3 + d];
-    }
-    for (int b = 0; b < 3; ++b) a.gpos[3 * s + b] = gr[b];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) part[i * 3 + j] = st.u[i] * gr[j];
-    // adjoint of the C6 chain
-    const float qqx = a.qq[X * a.nS + X];
-    const float R = (float)(a.a1 * sqrt(3.0 * (double)qqx) + a.a2);
-    const float R2 = R * R, R6 = R2 * R2 * R2;
-    const float phi0 = (float)a.s6 / R6 + 3.0f * (float)a.s8 * qqx / (R6 * R2);
-    float dc[RMAX];
-#pragma unroll
-    for (int q = 0; q < RMAX; ++q) dc[q] = q < a.r ? A[q] + a.eigs[q] * c[q] * phi0 : 0.f;
-    if (a.dEdc) {
-      for (int q = 0; q < a.r; ++q) a.dEdc[s * a.r + q] = dc[q];
-    }
-    float cn = a.cn_raw[s];
-    float ca = 1.0f;
-    if (a.cncorr) {
-      ca = a.cncorr[2 * X + 1];
-      cn = ca * cn + a.cncorr[2 * X];
-    }
-    float L[kNRef], gp[kNRef];
-    float mx = -INFINITY;
-    for (int pp = 0; pp < kNRef; ++pp) {
-      const float ref = a.cnref[X * kNRef + pp];
-      const float d = cn - ref;
-      L[pp] = ref == -1.0f ? -INFINITY : -4.0f * d * d;
-      mx = fmaxf(mx, L[pp]);
-    }
-    float sum = 0.f;
-    for (int pp = 0; pp < kNRef; ++pp) {
-      L[pp] = L[pp] == -INFINITY ? 0.f : expf(L[pp] - mx);
-      sum += L[pp];
-    }
-    float gbar = 0.f;
-    for (int pp = 0; pp < kNRef; ++pp) {
-      L[pp] /= sum;
-      float g = 0.f;
-#pragma unroll
-      for (int q = 0; q < RMAX; ++q)
-        if (q < a.r) g += dc[q] * a.vq[(X * kNRef + pp) * a.r + q];
-      gp[pp] = g;
-      gbar += L[pp] * g;
-    }
-    float dcn = 0.f;
-    for (int pp = 0; pp < kNRef; ++pp) {
-      const float ref = a.cnref[X * kNRef + pp];
-      if (ref != -1.0f) dcn += L[pp] * (gp[pp] - gbar) * (-8.0f) * (cn - ref);
-    }
-    a.dEdcn[s] = dcn * ca;
-  }
-  if (a.partials) block_sum_store<9>(part, red, a.partials + (size_t)blockIdx.x * 16);
-}
-
 // Plain gather (mesh_to_points): out[i, q] = sum_nodes W mesh[X_i r + q].
 template <int LPA, int RMAX>
 __global__ void __launch_bounds__(256) k_gather_plain(GatherArgs a, float* out) {
@@ -281,6 +155,18 @@ __global__ void __launch_bounds__(256) k_gather_plain(GatherArgs a, float* out) 
   for (int q = 0; q < RMAX; ++q) A[q] = lane_sum<LPA>(A[q]);
   if (s < a.n && lane == 0)
     for (int q = 0; q < a.r; ++q) out[s * a.r + q] = A[q];
+}
+
+__global__ void k_to_f32(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = (float)src[i];
+}
+
+int launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return FD3_OK;
+  hipLaunchKernelGGL(k_to_f32, dim3(cdiv(n, 256)), dim3(256), 0, s, src, dst, n);
+  FD3_LAUNCH_CHECK("k_to_f32");
+  return FD3_OK;
 }
 
 // grad_pos[slot_atom[s]] = gpos[s] * (Bohr per Angstrom)
@@ -360,27 +246,6 @@ int launch_spread(const SpreadArgs& a, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_spread<64>, grid, dim3(256), 0, s, a);
   FD3_LAUNCH_CHECK("k_spread");
-  return FD3_OK;
-}
-
-int launch_gather_bwd(const GatherArgs& a, hipStream_t s) {
-  const int lpa = lanes_per_atom(a.g.order);
-  const dim3 grid(gather_blocks(a.n, a.g.order));
-  if (a.r > 32) {
-    set_error("rank %d exceeds the gather kernel's limit of 32", a.r);
-    return FD3_ERR_UNSUPPORTED;
-  }
-  ProfScope _ps("k_gather_bwd", s);
-  if (lpa == 32) {
-    if (a.r <= 8) hipLaunchKernelGGL((k_gather_bwd<32, 8>), grid, dim3(256), 0, s, a);
-    else if (a.r <= 16) hipLaunchKernelGGL((k_gather_bwd<32, 16>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_gather_bwd<32, 32>), grid, dim3(256), 0, s, a);
-  } else {
-    if (a.r <= 8) hipLaunchKernelGGL((k_gather_bwd<64, 8>), grid, dim3(256), 0, s, a);
-    else if (a.r <= 16) hipLaunchKernelGGL((k_gather_bwd<64, 16>), grid, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_gather_bwd<64, 32>), grid, dim3(256), 0, s, a);
-  }
-  FD3_LAUNCH_CHECK("k_gather_bwd");
   return FD3_OK;
 }
 

@@ -22,7 +22,7 @@ inline MeshGeom mesh_geom(const Geom& g) {
 struct C6Args {
   int64_t n;
   int nS, r;
-  const float* cn_raw;     // (n) slot order
+  const double* cn_raw;    // (n) slot order
   const int32_t* spec_s;   // (n)
   const int32_t* slot_atom;  // may be null (identity)
   const float* cnref;      // (nS, 7)
@@ -56,19 +56,7 @@ struct GatherArgs {
   const void* raw_pos;
   int raw_dtype;
   const int32_t* spec_s;
-  const float* charges;
-  const float* mesh;   // dE/drho (nS*r, nx, ny, nz)
-  const float* cn_raw;
-  const float* cnref;
-  const float* vq;
-  const float* eigs;
-  const float* qq;
-  const float* cncorr;
-  double s6, s8, a1, a2;
-  double* gpos;        // (n, 3) slot order, dE/dr in Hartree/Bohr (mesh part)
-  float* dEdcn;        // (n)
-  float* dEdc;         // (n, r) or null
-  double* partials;    // (blocks, 16) or null
+  const float* mesh;   // (nS*r, nx, ny, nz)
 };
 
 int launch_fill_f32(float* p, size_t n, hipStream_t s);
@@ -81,7 +69,30 @@ int c6_blocks(int64_t n);
 int gather_blocks(int64_t n, int order);
 int launch_c6_fwd(const C6Args& a, hipStream_t s);
 int launch_spread(const SpreadArgs& a, hipStream_t s);
-int launch_gather_bwd(const GatherArgs& a, hipStream_t s);
+// double -> float copy (n values)
+int launch_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
+
+// The softmax weights L_p = softmax_p(-4 (CN - ref_p)^2) over the valid
+// references of species X (src/fourierd3/core.py:469-478), in double;
+// returns 1 / (sum of the unnormalised weights).
+__device__ inline double c6_softmax(const float* __restrict__ cnref, int X, double cn, double* L) {
+  double mx = -INFINITY;
+  for (int p = 0; p < kNRef; ++p) {
+    const float ref = cnref[X * kNRef + p];
+    const double d = cn - (double)ref;
+    L[p] = ref == -1.0f ? -INFINITY : -4.0 * d * d;
+    mx = fmax(mx, L[p]);
+  }
+  double sum = 0.0;
+  for (int p = 0; p < kNRef; ++p) {
+    L[p] = L[p] == -INFINITY ? 0.0 : exp(L[p] - mx);
+    sum += L[p];
+  }
+  const double inv = 1.0 / sum;
+  for (int p = 0; p < kNRef; ++p) L[p] *= inv;
+  return inv;
+}
+
 int launch_gather_plain(const GatherArgs& a, float* out, hipStream_t s);
 int launch_unpermute(const double* gpos, const int32_t* slot_atom, int64_t n, double* out, hipStream_t s);
 

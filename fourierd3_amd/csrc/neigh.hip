@@ -26,30 +26,51 @@ namespace fd3 {
 //   r1 = (Rcov_ij + r_cut)/2, r' = max(r, r1)
 //   k  = 16 + (r' - r1)^2 / ((r_cut - r')^2 + 1e-6)
 //   f  = sigmoid(k (4/3 Rcov_ij / r - 1)),  f = 0 for r <= 1e-6
-// Hardware rsq/rcp/exp (about 1 ulp): the sums over ~100 pairs carry float
-// rounding at that level anyway.
-__device__ inline void cn_term(float r2, float rcs, float rcut, float* f, float* fpr) {
-  if (!(r2 > 1e-12f)) {
-    *f = 0.f;
-    *fpr = 0.f;
+// (src/fourierd3/core.py:352-378), evaluated in double: with heavy elements
+// dE/dCN reaches hundreds of eV per unit CN, so an atom's CN force is a sum
+// of terms of that size that cancel to a few eV/A, and float terms (relative
+// error ~1e-6 here) leave errors of 1e-3 eV/A.  sigma (1 - sigma) is formed
+// as e sigma^2 with e = exp(-s), which does not cancel where sigma -> 1.
+// 1/sqrt(x) and 1/x for positive normal x in float range: the hardware
+// float estimate (~1 ulp of float) and one Newton step in double, which
+// squares the relative error to ~1e-14 (the CN term needs ~1e-10).
+__device__ inline double rsq_d(double x) {
+  const double y = (double)__builtin_amdgcn_rsqf((float)x);
+  return y * (1.5 - 0.5 * x * y * y);
+}
+__device__ inline double rcp_d(double x) {
+  const double y = (double)__builtin_amdgcn_rcpf((float)x);
+  return y * (2.0 - x * y);
+}
+
+__device__ inline void cn_term(double r2, double rcs, double rcut, double* f, double* fpr) {
+  if (!(r2 > 1e-12)) {
+    *f = 0.0;
+    *fpr = 0.0;
     return;
   }
-  const float ir = __builtin_amdgcn_rsqf(r2);
-  const float r = r2 * ir;
-  const float r1 = 0.5f * rcs + 0.5f * rcut;
+  const double ir = rsq_d(r2);
+  const double r = r2 * ir;
+  const double r1 = 0.5 * rcs + 0.5 * rcut;
   const bool far = r > r1;
-  const float rc = far ? r : r1;
-  const float dr = rc - r1, dc = rcut - rc;
-  const float iden = __builtin_amdgcn_rcpf(dc * dc + 1e-6f);
-  const float k = 16.0f + dr * dr * iden;
-  const float ratio = (4.0f / 3.0f) * rcs;
-  const float arg = ratio * ir - 1.0f;
-  const float s = k * arg;
-  const float sig = __builtin_amdgcn_rcpf(1.0f + __expf(-s));
+  const double rc = far ? r : r1;
+  const double dr = rc - r1, dc = rcut - rc;
+  const double iden = rcp_d(dc * dc + 1e-6);
+  const double k = 16.0 + dr * dr * iden;
+  const double ratio = (4.0 / 3.0) * rcs;
+  const double arg = ratio * ir - 1.0;
+  const double s = k * arg;
+  if (s < -80.0) {  // sigma < 2e-35 and its derivative below 1e-22: zero (and exp(-s) stays finite)
+    *f = 0.0;
+    *fpr = 0.0;
+    return;
+  }
+  const double e = exp(-s);
+  const double sig = rcp_d(1.0 + e);
   *f = sig;
-  const float dk = far ? 2.0f * dr * iden * (1.0f + dr * dc * iden) : 0.0f;
-  const float ds = dk * arg - k * ratio * ir * ir;
-  *fpr = sig * (1.0f - sig) * ds * ir;
+  const double dk = far ? 2.0 * dr * iden * (1.0 + dr * dc * iden) : 0.0;
+  const double ds = dk * arg - k * ratio * ir * ir;
+  *fpr = e * sig * sig * ds * ir;
 }
 
 // Take the cell and its inverse from the device cell when there is one.
@@ -282,7 +303,13 @@ __global__ void k_fill_sorted(const int32_t* __restrict__ slot_atom, int64_t n, 
 // Backward: dE/dr_s -= sum_t (g_s + g_t) f'(r) v / r,
 //           virial  += sum_t g_s f'(r) v v^T / r        (directed pairs s -> t)
 // with g = dE/dCN (src/fourierd3/core.py:348-386 and its adjoint).
-constexpr int kCnCap = 1024;   // candidates staged per pass
+// Candidates staged per pass: CAP is a template argument, 640 when the
+// neighbour bins of a home bin hold fewer than ~600 on average (large boxes:
+// 27 bins of ~20 atoms at 1 g/cm^3 H/C/N/O), which keeps the kernel's LDS
+// within five blocks per CU, and 1536 for small boxes with many images per
+// bin.  Home bins whose candidates overflow CAP take one neighbour bin per
+// pass instead (correct for any count, slower).
+constexpr int kCnCapSmall = 640, kCnCapLarge = 1536;
 constexpr int kNbrMax = 343;   // neighbour bins of a single-pass plan (reach <= 3)
 constexpr int kCnHomes = 128;  // home atoms per pass
 constexpr int kCnStep = 8;     // ballot steps (of 64 candidates) per hit list
@@ -314,19 +341,24 @@ __device__ inline int pack_shift(const int* S) {
   return ((S[0] + 128) << 16) | ((S[1] + 128) << 8) | (S[2] + 128);
 }
 
-__device__ inline float wave_sum_f(float v) {
+__device__ inline double wave_sum_d(double v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
-template <bool BWD>
-__global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
+// At most 128 VGPRs in the backward form: four waves per SIMD, not three.
+template <bool BWD, int kCnCap>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_cn_bins(CellArgs c, CnArgs a) {
   cell_refresh(c);
   constexpr int NW = 4;                   // waves per block
-  __shared__ float4 cpos[kCnCap];         // image position relative to the bin's first atom, rcov
-  __shared__ float cg[BWD ? kCnCap : 1];  // dE/dCN of the candidate (backward)
-  __shared__ float hacc[kCnHomes][BWD ? 3 : 1];  // per home atom: CN, or dE/dr
+  // candidate image positions relative to the bin's first atom as float
+  // pairs: hi (with rcov in w) drives the distance filter, hi + lo gives the
+  // pair vector to double precision for the term itself
+  __shared__ float4 cpos[kCnCap];
+  __shared__ float4 clo[kCnCap];
+  __shared__ double cg[BWD ? kCnCap : 1];         // dE/dCN of the candidate (backward)
+  __shared__ double hacc[kCnHomes][BWD ? 3 : 1];  // per home atom: CN, or dE/dr
   __shared__ int bcnt[kNbrMax + 1];       // candidate offset of neighbour bin k
   __shared__ int nb0[kNbrMax];            // first slot of neighbour bin k
   __shared__ int nsh[kNbrMax];            // packed lattice shift of neighbour bin k
@@ -361,11 +393,11 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
     single = bcnt[nbr] <= kCnCap;
   }
   const float rc2 = (float)c.rc2;
-  float vir[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int h0 = hs; h0 < he; h0 += kCnHomes) {
     const int nh = min(kCnHomes, he - h0);
     for (int h = threadIdx.x; h < nh; h += blockDim.x)
-      for (int d = 0; d < (BWD ? 3 : 1); ++d) hacc[h][d] = 0.f;
+      for (int d = 0; d < (BWD ? 3 : 1); ++d) hacc[h][d] = 0.0;
     // candidate passes: the whole list at once, or one neighbour bin (chunk) at a time
     const int npass_bins = single ? 1 : nbr;
     for (int kb = 0; kb < npass_bins; ++kb) {
@@ -398,18 +430,22 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
             t = c.bstart[fixed.wb] + off0 + e;
           }
           const double4 p = c.pos4[t];
-          cpos[e] = make_float4((float)(p.x + sh[0] - org.x), (float)(p.y + sh[1] - org.y),
-                                (float)(p.z + sh[2] - org.z), a.rcov[c.spec_s[t]]);
+          const double rx = p.x + sh[0] - org.x, ry = p.y + sh[1] - org.y, rz = p.z + sh[2] - org.z;
+          const float hx = (float)rx, hy = (float)ry, hz = (float)rz;
+          cpos[e] = make_float4(hx, hy, hz, a.rcov[c.spec_s[t]]);
+          clo[e] = make_float4((float)(rx - (double)hx), (float)(ry - (double)hy), (float)(rz - (double)hz), 0.f);
           if (BWD) cg[e] = a.dEdcn[t];
         }
         __syncthreads();
         for (int h = wv; h < nh; h += NW) {  // home atom h0 + h: wave-uniform
           const int sidx = h0 + h;
           const double4 pd = c.pos4[sidx];
-          const float px = (float)(pd.x - org.x), py = (float)(pd.y - org.y), pz = (float)(pd.z - org.z);
-          const float ri = a.rcov[c.spec_s[sidx]];
-          const float gs = BWD ? a.dEdcn[sidx] : 0.f;
-          float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+          const double hxd = pd.x - org.x, hyd = pd.y - org.y, hzd = pd.z - org.z;
+          const float px = (float)hxd, py = (float)hyd, pz = (float)hzd;
+          const double lx = hxd - (double)px, ly = hyd - (double)py, lz = hzd - (double)pz;
+          const double ri = (double)a.rcov[c.spec_s[sidx]];
+          const double gs = BWD ? a.dEdcn[sidx] : 0.0;
+          double s0 = 0.0, s1 = 0.0, s2 = 0.0;
           for (int e0 = 0; e0 < nc; e0 += kCnStep * 64) {
             // in-range candidates of [e0, e0 + 512) -> wl[wv][0 .. cnt)
             int cnt = 0;
@@ -431,24 +467,26 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
             __builtin_amdgcn_wave_barrier();  // the list is written and read by lanes of one wave
             for (int k = lane; k < cnt; k += 64) {
               const int e = wl[wv][k];
-              const float4 q = cpos[e];
-              const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
-              float f, w;
-              cn_term(vx * vx + vy * vy + vz * vz, ri + q.w, a.rcut, &f, &w);  // 0, 0 at r = 0
+              const float4 q = cpos[e], ql = clo[e];
+              const double vx = ((double)q.x - (double)px) + ((double)ql.x - lx);
+              const double vy = ((double)q.y - (double)py) + ((double)ql.y - ly);
+              const double vz = ((double)q.z - (double)pz) + ((double)ql.z - lz);
+              double f, w;
+              cn_term(vx * vx + vy * vy + vz * vz, ri + (double)q.w, a.rcut, &f, &w);  // 0, 0 at r = 0
               if (!BWD) {
                 s0 += f;
               } else {
-                const float wt = w * (gs + cg[e]);
+                const double wt = w * (gs + cg[e]);
                 s0 -= wt * vx; s1 -= wt * vy; s2 -= wt * vz;
-                const float wvv = w * gs;
+                const double wvv = w * gs;
                 vir[0] += wvv * vx * vx; vir[1] += wvv * vy * vy; vir[2] += wvv * vz * vz;
                 vir[3] += wvv * vx * vy; vir[4] += wvv * vx * vz; vir[5] += wvv * vy * vz;
               }
             }
             __builtin_amdgcn_wave_barrier();  // the next step overwrites the list
           }
-          s0 = wave_sum_f(s0);
-          if (BWD) { s1 = wave_sum_f(s1); s2 = wave_sum_f(s2); }
+          s0 = wave_sum_d(s0);
+          if (BWD) { s1 = wave_sum_d(s1); s2 = wave_sum_d(s2); }
           if (lane == 0) {  // one wave per home atom and pass: plain updates
             hacc[h][0] += s0;
             if (BWD) { hacc[h][1] += s1; hacc[h][2] += s2; }
@@ -466,19 +504,15 @@ __global__ void __launch_bounds__(256) k_cn_bins(CellArgs c, CnArgs a) {
         const int sl = h0 + h;
         if (a.gout) {
           const int64_t i = a.slot_atom ? a.slot_atom[sl] : sl;
-          for (int d = 0; d < 3; ++d) a.gout[3 * i + d] = (a.gpos[3 * sl + d] + (double)hacc[h][BWD ? d : 0]) * kA2B;
+          for (int d = 0; d < 3; ++d) a.gout[3 * i + d] = (a.gpos[3 * sl + d] + hacc[h][BWD ? d : 0]) * kA2B;
         } else {
-          for (int d = 0; d < 3; ++d) a.gpos[3 * sl + d] += (double)hacc[h][BWD ? d : 0];
+          for (int d = 0; d < 3; ++d) a.gpos[3 * sl + d] += hacc[h][BWD ? d : 0];
         }
       }
     }
     __syncthreads();  // the next home pass clears hacc
   }
-  if (BWD) {
-    double v[6];
-    for (int i = 0; i < 6; ++i) v[i] = vir[i];
-    block_sum_store<6>(v, red, a.partials + (size_t)blockIdx.x * 8);
-  }
+  if (BWD) block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
 }
 
 // Edge-list forms: pairs given by (src, dst, shift in Angstrom).
@@ -498,11 +532,11 @@ __global__ void __launch_bounds__(256) k_cn_edges(EdgeArgs e, CnArgs a, const do
   if (k >= e.n_edges) return;
   double v[3];
   edge_vec(e, k, pos4, v);
-  const float r2 = (float)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   const int64_t i = e.src[k], j = e.dst[k];
-  float f, fpr;
-  cn_term(r2, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fpr);
-  if (f != 0.f) atomicAdd(a.cn + i, f);
+  double f, fpr;
+  cn_term(r2, (double)a.rcov[spec[i]] + (double)a.rcov[spec[j]], a.rcut, &f, &fpr);
+  if (f != 0.0) atomicAdd(a.cn + i, f);
 }
 
 __global__ void __launch_bounds__(256) k_cn_bwd_edges(EdgeArgs e, CnArgs a, const double4* __restrict__ pos4,
@@ -513,11 +547,11 @@ __global__ void __launch_bounds__(256) k_cn_bwd_edges(EdgeArgs e, CnArgs a, cons
   if (k < e.n_edges) {
     double v[3];
     edge_vec(e, k, pos4, v);
-    const float r2 = (float)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double r2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
     const int64_t i = e.src[k], j = e.dst[k];
-    float f, fpr;
-    cn_term(r2, a.rcov[spec[i]] + a.rcov[spec[j]], a.rcut, &f, &fpr);
-    const double w = (double)(a.dEdcn[i] * fpr);
+    double f, fpr;
+    cn_term(r2, (double)a.rcov[spec[i]] + (double)a.rcov[spec[j]], a.rcut, &f, &fpr);
+    const double w = a.dEdcn[i] * fpr;
     const double g0 = w * v[0], g1 = w * v[1], g2 = w * v[2];  // dE/dv, Hartree/Bohr
     if (w != 0.0) {
       atomicAdd(a.gpos + 3 * j + 0, g0);
@@ -713,17 +747,25 @@ int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* p
 
 int cn_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, 256)); }
 
+// Mean candidates of a home bin: atoms per bin times the neighbour bins.
+static bool cn_small_cap(const CellArgs& c, int nbins) {
+  const double nbr = (2.0 * c.sr[0] + 1) * (2.0 * c.sr[1] + 1) * (2.0 * c.sr[2] + 1);
+  return (double)c.n / std::max(nbins, 1) * nbr <= 0.95 * kCnCapSmall;
+}
+
 int launch_cn_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   if (c.n <= 0) return FD3_OK;
   ProfScope _ps("k_cn_cells", s);
-  hipLaunchKernelGGL(k_cn_bins<false>, dim3(nbins), dim3(256), 0, s, c, a);
+  if (cn_small_cap(c, nbins)) hipLaunchKernelGGL((k_cn_bins<false, kCnCapSmall>), dim3(nbins), dim3(256), 0, s, c, a);
+  else hipLaunchKernelGGL((k_cn_bins<false, kCnCapLarge>), dim3(nbins), dim3(256), 0, s, c, a);
   FD3_LAUNCH_CHECK("k_cn_cells");
   return FD3_OK;
 }
 
 int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   ProfScope _ps("k_cn_bwd_cells", s);
-  hipLaunchKernelGGL(k_cn_bins<true>, dim3(nbins), dim3(256), 0, s, c, a);
+  if (cn_small_cap(c, nbins)) hipLaunchKernelGGL((k_cn_bins<true, kCnCapSmall>), dim3(nbins), dim3(256), 0, s, c, a);
+  else hipLaunchKernelGGL((k_cn_bins<true, kCnCapLarge>), dim3(nbins), dim3(256), 0, s, c, a);
   FD3_LAUNCH_CHECK("k_cn_bwd_cells");
   return FD3_OK;
 }

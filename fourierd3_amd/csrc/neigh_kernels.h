@@ -45,10 +45,10 @@ struct CellArgs {
 };
 
 struct CnArgs {
-  const float* rcov;  // (nS) Bohr
-  float rcut;         // Bohr
-  float* cn;          // (n) out (fwd)
-  const float* dEdcn; // (n) in (bwd)
+  const float* rcov;   // (nS) Bohr
+  double rcut;         // Bohr
+  double* cn;          // (n) out (fwd)
+  const double* dEdcn; // (n) in (bwd)
   double* gpos;       // (n, 3) in/out (bwd), Hartree/Bohr
   double* partials;   // (blocks, 8) virial (bwd)
   double* gout;       // (n, 3) or NULL (cell kernels, bwd): the final dE/dr in atom order, Hartree/Angstrom

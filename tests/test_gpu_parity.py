@@ -179,19 +179,40 @@ def test_calculator_matches_oracle(case):
     e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
     de, df, ds = _errors(res["energy"], res["forces"], res["stress"], e_ref, f_ref, s_ref, box.n_atoms)
     print(f"[errors] {case}: E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
-    e_tol, f_tol, s_tol = E_TOL, F_TOL, S_TOL
-    if case.startswith("mixed"):
-        # Random elements up to Z = 36 with the synthetic C6 table put forces of
-        # several eV/A on heavy atoms 1 A apart.  The reference algorithm at its
-        # own default dtype (float32; the oracle rerun in float32) misses the
-        # FP64 forces by up to ~1e-4 eV/A here, so the bar is: no worse than
-        # that restatement, and never looser than the stated tolerances.
-        o32 = OracleFourierD3(box.numbers, box.cell, dtype=torch.float32)
-        e32, f32, s32 = o32.calculate(box.positions, box.cell)
-        de32, df32, ds32 = _errors(e32, f32, s32, e_ref, f_ref, s_ref, box.n_atoms)
-        print(f"[errors] {case} float32 restatement: E {de32:.3e} F {df32:.3e} S {ds32:.3e}", flush=True)
-        e_tol, f_tol, s_tol = max(E_TOL, de32), max(F_TOL, df32), max(S_TOL, ds32)
-    assert de < e_tol and df < f_tol and ds < s_tol
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
+def test_heavy_element_triclinic_box_with_param_gradients():
+    """A 2000-atom cut of the differentiation workload: Z uniform on 1..36
+    (36 species: the generic k-space kernel), cell strained by up to 10 %, atoms
+    as close as 1 A, so CN forces reach ~60 eV/A as sums of much larger terms.
+    Cell-list path with FD3_NEED_PARAMS: energy, forces, stress and
+    dE/d(s6, s8, a1, a2) against one FP64 oracle autograd pass, at the stated
+    tolerances (dE/dtheta to 1e-6 eV per atom and unit parameter)."""
+    _say("Z=1..36 2000-atom triclinic box, E/F/S and dE/dtheta")
+    box = W.strained(W.mixed_z_box(2000, zmax=36, seed=5), 0.1, seed=5)
+    calc = FourierD3ASECalculator(device=_dev())
+    calc._build_model(_atoms(box))
+    assert calc._model.n_species == 36
+    pos = torch.tensor(box.positions, dtype=torch.float64, device=_dev())
+    res = calc.evaluate(pos, box.cell, need_grad=True, need_params=True)
+    torch.cuda.synchronize()
+    n = box.n_atoms
+    vol_b = abs(np.linalg.det(box.cell)) * ANGSTROM_TO_BOHR ** 3
+    e = float(res["energy"][0]) * HARTREE_TO_EV
+    f = -res["grad_pos"].cpu().numpy() * HARTREE_TO_EV
+    st = res["virial"].cpu().numpy() * HARTREE_TO_EV / vol_b * ANGSTROM_TO_BOHR ** 3
+    gp = res["grad_params"].cpu().numpy() * HARTREE_TO_EV
+    o = OracleFourierD3(box.numbers, box.cell)
+    prm = o.params.clone().requires_grad_(True)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell, params=prm)
+    gp_ref = prm.grad.numpy()
+    de, df, ds = _errors(e, f, st, e_ref, f_ref, s_ref, n)
+    dgp = np.abs(gp - gp_ref)
+    print(f"[errors] z36 2k: max |F| {np.abs(f_ref).max():.3e} eV/A; E {de:.3e} eV/atom  F {df:.3e} eV/A  "
+          f"S {ds:.3e} eV/A^3  dE/dtheta {dgp / n} eV/atom", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+    assert (dgp < E_TOL * n).all()
 
 
 @pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
@@ -720,14 +741,13 @@ def test_many_species_box_at_full_size_is_translation_invariant():
     print(f"[errors] z36 shift: dE {abs(r1['energy'] - r2['energy']) / n:.3e} eV/atom "
           f"dF {np.abs(r1['forces'] - r2['forces']).max():.3e} dS {np.abs(r1['stress'] - r2['stress']).max():.3e}",
           flush=True)
-    # Atoms of this random box sit as close as 1 A, so forces reach ~100 eV/A.
-    # The reference algorithm in float32 (the oracle rerun at float32) misses
-    # the FP64 forces by ~1e-4 of the largest force on the 400-atom box of the
-    # same kind; the shifted forces here must agree 5x tighter than that.
+    # Atoms of this random box sit as close as 1 A, so forces reach ~100 eV/A;
+    # the CN pair terms and the CN -> C6 chain run in double, so the shifted
+    # forces agree to the stated force tolerance.
     fmax = np.abs(r1["forces"]).max()
     print(f"[errors] z36 max |F| {fmax:.3e} eV/A", flush=True)
     assert abs(r1["energy"] - r2["energy"]) / n < E_TOL
-    assert np.abs(r1["forces"] - r2["forces"]).max() < F_TOL + 2e-5 * fmax
+    assert np.abs(r1["forces"] - r2["forces"]).max() < F_TOL
     assert np.abs(r1["stress"] - r2["stress"]).max() < S_TOL
 
 

@@ -39,6 +39,7 @@ EXPORTS = (
     "fd3_eval_d4_workspace", "fd3_eval_d4",
     "fd3_eval_celllist_batch_workspace", "fd3_eval_celllist_batch",
     "fd3_eval_celllist_noalloc",
+    "fd3_batch_create", "fd3_batch_workspace", "fd3_batch_run", "fd3_batch_info", "fd3_batch_destroy",
     "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
     "fd3_spread", "fd3_gather",
     "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
@@ -114,6 +115,12 @@ _SIG = {
                                           C.c_int, _P, C.c_int, _P, C.c_size_t, C.POINTER(Outputs), _P]),
     "fd3_eval_celllist_noalloc": (C.c_int, [C.POINTER(Model), C.POINTER(Geom), C.c_int64, _P, C.c_int, _P, C.c_int,
                                             C.POINTER(Outputs), _P]),
+    "fd3_batch_create": (C.c_int, [C.c_int32, C.POINTER(Model), C.c_int32, C.POINTER(C.c_int32), C.POINTER(Geom),
+                                   C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_void_p)]),
+    "fd3_batch_workspace": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "fd3_batch_run": (C.c_int, [C.c_void_p, _P, C.c_int, _P, _P, C.c_size_t, C.POINTER(Outputs), _P]),
+    "fd3_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "fd3_batch_destroy": (None, [C.c_void_p]),
     "fd3_neighbours_workspace": (C.c_int, [C.c_int64, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_size_t)]),
     "fd3_neighbours_count": (C.c_int, [C.c_int64, _P, C.c_int, C.POINTER(C.c_double), C.c_double, _P, C.c_size_t,
                                        _P, _P]),

@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -13,6 +14,7 @@
 #include "ewald.h"
 #include "fft_plan.h"
 #include "fused.h"
+#include "launch.h"
 #include "mesh_kernels.h"
 #include "neigh_kernels.h"
 
@@ -44,7 +46,26 @@ static std::string g_prof_only;  // record only this kernel name when non-empty
 static std::vector<ProfRec> g_prof_recs;
 static std::vector<std::pair<const char*, hipEvent_t>> g_prof_open;
 
-bool prof_enabled() { return g_prof; }
+// ---------------------------------------------------- launch recording ---
+Recorder*& current_recorder() {
+  static thread_local Recorder* r = nullptr;
+  return r;
+}
+
+int lds_attr(const void* fn, size_t sh) {
+  if (sh <= 64 * 1024) return FD3_OK;
+  static std::mutex mu;
+  static std::vector<std::pair<const void*, size_t>> done;
+  std::lock_guard<std::mutex> lock(mu);
+  for (auto& d : done)
+    if (d.first == fn && d.second >= sh) return FD3_OK;
+  FD3_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
+  done.push_back({fn, sh});
+  return FD3_OK;
+}
+
+// no events while launches are only being recorded
+bool prof_enabled() { return g_prof && !current_recorder(); }
 
 void prof_record(const char* name, hipStream_t s, bool start) {
   if (!g_prof_only.empty() && g_prof_only != name) return;
@@ -245,11 +266,11 @@ struct ReduceJobs {
   int n;
 };
 
-__global__ void __launch_bounds__(256) k_reduce_rows(ReduceJobs jb) {
+__device__ void reduce_rows_body(const BlockInfo& bk, const ReduceJobs& jb) {
   __shared__ double sh[16][17];
   int j = 0;
-  while (j + 1 < jb.n && (int)blockIdx.x >= jb.b0[j + 1]) ++j;
-  const int lb = blockIdx.x - jb.b0[j];
+  while (j + 1 < jb.n && (int)bk.id.x >= jb.b0[j + 1]) ++j;
+  const int lb = bk.id.x - jb.b0[j];
   const int stride = jb.stride[j], ncols = jb.ncols[j];
   const int col = threadIdx.x & 15, lane = threadIdx.x >> 4;
   const int r0 = lb * jb.rpb[j], r1 = min(jb.rows[j], r0 + jb.rpb[j]);
@@ -283,15 +304,14 @@ static void reduce_partials_add(ReduceJobs& jb, const double** src, int* rows, i
 static int reduce_partials_run(const ReduceJobs& jb, hipStream_t s) {
   if (jb.n == 0) return FD3_OK;
   ProfScope _ps("k_reduce_rows", s);
-  hipLaunchKernelGGL(k_reduce_rows, dim3(jb.b0[jb.n]), dim3(256), 0, s, jb);
-  FD3_LAUNCH_CHECK("k_reduce_rows");
-  return FD3_OK;
+  return launch<ReduceJobs, reduce_rows_body, 256>("k_reduce_rows", dim3(jb.b0[jb.n]), 0, s, jb);
 }
 
 // One block of 1024 threads: wave w sums columns w and w + 16 of the 31
 // partial columns (rows <= 256 after k_reduce_rows), thread 0 assembles.
-__global__ void __launch_bounds__(1024) k_finalize(FinalArgs a) {
+__device__ void finalize_body(const BlockInfo&, const FinalArgs& fa) {
   __shared__ double tot[32];
+  FinalArgs a = fa;
   // ks 0..10 -> tot[0..10]; c6 0..4 -> tot[11..15]; ga 0..8 -> tot[16..24]; cnv 0..5 -> tot[25..30]
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int k = w; k < 31; k += 16) {
@@ -518,9 +538,7 @@ static int run_finalize(const Common& c, bool grad, bool params, bool have_cnv, 
   fa.grad_cell = grad ? out->grad_cell : nullptr;
   fa.grad_params = params ? out->grad_params : nullptr;
   ProfScope _ps("k_finalize", s);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, fa);
-  FD3_LAUNCH_CHECK("k_finalize");
-  return FD3_OK;
+  return launch<FinalArgs, finalize_body, 1024>("k_finalize", dim3(1), 0, s, fa);
 }
 
 // Device cell update of the NPT entry point: the cell (Angstrom, row i =

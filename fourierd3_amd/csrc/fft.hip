@@ -17,6 +17,7 @@
 
 #include "fd3_internal.h"
 #include "fft_plan.h"
+#include "launch.h"
 #include "fft_lds.h"
 
 namespace fd3 {
@@ -38,22 +39,38 @@ inline int init_gtab_blocks(const PairInit& pi) {
   return (pi.nS * (pi.nS + 1) / 2 * kGtabN + kInitGI - 1) / kInitGI;
 }
 
-__global__ void __launch_bounds__(256) k_init_tables(float2* twx, float2* twy, float2* twz, float* scx, float* scy,
-                                                     float* scz, float4* htab, float4* gtab, KsPair* pc, PairInit pi,
-                                                     int ngb, int nx, int ny, int nz, int order, int use_sinc) {
+struct InitTablesArgs {
+  float2 *twx, *twy, *twz;
+  float *scx, *scy, *scz;
+  float4* htab;
+  float4* gtab;
+  KsPair* pc;
+  PairInit pi;
+  int ngb, nx, ny, nz, order, use_sinc;
+};
+
+__device__ void init_tables_body(const BlockInfo& bk, const InitTablesArgs& ia) {
+  float2 *twx = ia.twx, *twy = ia.twy, *twz = ia.twz;
+  float *scx = ia.scx, *scy = ia.scy, *scz = ia.scz;
+  float4* htab = ia.htab;
+  float4* gtab = ia.gtab;
+  KsPair* pc = ia.pc;
+  const PairInit& pi = ia.pi;
+  const int ngb = ia.ngb, nx = ia.nx, ny = ia.ny, nz = ia.nz, order = ia.order, use_sinc = ia.use_sinc;
+  const unsigned bx = bk.id.x;
   __shared__ double f[2][256];
   const int tid = threadIdx.x;
-  if (blockIdx.x < kInitHB) {  // h table
-    const int i = blockIdx.x * 42 + tid / 6, j = tid % 6;
+  if (bx < kInitHB) {  // h table
+    const int i = bx * 42 + tid / 6, j = tid % 6;
     if (tid < 252 && i < kHtabN) h_exact(htab_node(i, j), &f[0][tid], &f[1][tid]);
     __syncthreads();
-    const int ii = blockIdx.x * 42 + tid;
+    const int ii = bx * 42 + tid;
     if (tid < 42 && ii <= kHtabN) htab_fit(ii, &f[0][6 * tid], &f[1][6 * tid], htab + 3 * ii);
     return;
   }
-  if ((int)blockIdx.x < kInitHB + ngb) {  // per-pair G tables
+  if ((int)bx < kInitHB + ngb) {  // per-pair G tables
     const int np = pi.nS * (pi.nS + 1) / 2;
-    const int it = (blockIdx.x - kInitHB) * kInitGI + tid / (kGtabDeg + 1), j = tid % (kGtabDeg + 1);
+    const int it = (bx - kInitHB) * kInitGI + tid / (kGtabDeg + 1), j = tid % (kGtabDeg + 1);
     if (it < np * kGtabN) {
       const int pt = it / kGtabN, i = it - pt * kGtabN;  // pair pt = (X, Y >= X) in row order
       int X = 0, r0 = 0;
@@ -62,11 +79,11 @@ __global__ void __launch_bounds__(256) k_init_tables(float2* twx, float2* twy, f
       f[0][tid] = g_exact(gtab_node(pi.gdk, i, j), (double)pk.Rs * kHtabDx, pk.p6, pk.p8);
     }
     __syncthreads();
-    const int io = (blockIdx.x - kInitHB) * kInitGI + tid;
+    const int io = (bx - kInitHB) * kInitGI + tid;
     if (tid < kInitGI && io < np * kGtabN) gtab_fit(&f[0][(kGtabDeg + 1) * tid], gtab + 2 * io);
     return;
   }
-  const int t = (blockIdx.x - kInitHB - ngb) * blockDim.x + tid;
+  const int t = (bx - kInitHB - ngb) * blockDim.x + tid;
   if (pi.qq && t < pi.nS * pi.nS) pc[t] = ks_pair(pi.qq[t], pi.s6, pi.s8, pi.a1, pi.a2);
   auto tw = [](float2* dst, int k, int n) {
     double s, c;
@@ -130,14 +147,24 @@ __global__ void __launch_bounds__(256) k_c2r_z(const float2* __restrict__ A, flo
 // ------------------------------------------------------------ y passes ---
 // forward: A[c][x][kz][0..ny) (x in tile) -> B[c][kz][ky][x]; grid (nx/TX, M+1, C);
 // blocks of 2^LOG threads.
+struct YPassArgs {
+  const float2* in;
+  float2* out;
+  int nx, ny, M1, TX;
+  const float2* twy;
+};
+
 template <int DIR, int LOG>
-__global__ void __launch_bounds__(1 << LOG) k_c2c_y(const float2* __restrict__ in, float2* __restrict__ out, int nx,
-                                                    int ny, int M1, int TX, const float2* __restrict__ twy) {
+__device__ void c2c_y_body(const BlockInfo& bk, const YPassArgs& ya) {
   constexpr int U = kYLines >> LOG;     // a whole tile in one batch of loads
   constexpr int MAXB = kYLines >> (LOG + 2);
   extern __shared__ float2 lds[];
+  const float2* __restrict__ in = ya.in;
+  float2* __restrict__ out = ya.out;
+  const float2* __restrict__ twy = ya.twy;
+  const int nx = ya.nx, ny = ya.ny, M1 = ya.M1, TX = ya.TX;
   const int ls = ny + 1;
-  const int x0 = blockIdx.x * TX, kz = blockIdx.y, c = blockIdx.z;
+  const int x0 = bk.id.x * TX, kz = bk.id.y, c = bk.id.z;
   float2* tws = lds + TX * ls;
   stage_table(tws, twy, ny);
   const int lTX = ilog2(TX);
@@ -291,8 +318,9 @@ __device__ inline void kspace_accumulate(double (&acc)[11], const KPoint& p, flo
 
 // Block sums of the 7 (or, with parameter gradients, 11) partials; the
 // unused parameter slots are written as zeros.  Ends with a barrier.
-__device__ inline void kspace_reduce(double (&acc)[11], double* red, const KspaceArgs& a, bool params) {
-  const int bid = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+__device__ inline void kspace_reduce(const BlockInfo& bk, double (&acc)[11], double* red, const KspaceArgs& a,
+                                     bool params) {
+  const int bid = (bk.id.z * bk.n.y + bk.id.y) * bk.n.x + bk.id.x;
   double* out = a.partials + (size_t)bid * 16;
   if (params) {
     block_sum_store<11>(acc, red, out);
@@ -314,14 +342,15 @@ __device__ inline void kspace_reduce(double (&acc)[11], double* red, const Kspac
 // LDS: [tables | x twiddles | lines | line bases]; the block sums reuse the
 // lines at the end.
 template <int NS, int RT, bool PARAMS>
-__global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
+__device__ void kspace_reg_body(const BlockInfo& bk, const KspaceArgs& a_in) {
+  KspaceArgs a = a_in;
   ks_refresh(a);
   constexpr int MAXB = ks_maxb(NS * RT);
   constexpr int NP = NS * (NS + 1) / 2;  // unordered species pairs
   extern __shared__ float4 lds4[];
   __shared__ KsPair pcl[PARAMS ? NP : 1];
   const int nx = a.nx, NK = a.NK;
-  const int kz = blockIdx.y, q0 = blockIdx.z * RT;
+  const int kz = bk.id.y, q0 = bk.id.z * RT;
   const int rt = min(RT, a.r - q0);
   const int nlines = NS * RT * NK;
   // padded lines when lds_fft takes its unrolled variant (same rule on the host)
@@ -351,7 +380,7 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
   for (int i = 0; i < 11; ++i) acc[i] = 0.0;
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
   for (int it = 0; it < a.KT; ++it) {
-    const int ky0 = (blockIdx.x * a.KT + it) * NK;
+    const int ky0 = (bk.id.x * a.KT + it) * NK;
     if (it) __syncthreads();  // the previous tile is done with S and lbase
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
     __syncthreads();
@@ -443,16 +472,17 @@ __global__ void __launch_bounds__(256) k_kspace_reg(KspaceArgs a) {
     lds_fft<1, MAXB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
   __syncthreads();  // the lines are free: reuse them for the block sums
-  kspace_reduce(acc, (double*)S, a, PARAMS);
+  kspace_reduce(bk, acc, (double*)S, a, PARAMS);
 }
 
 // Any number of species: phi accumulates in a second LDS buffer P.
-__global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
+__device__ void kspace_body(const BlockInfo& bk, const KspaceArgs& a_in) {
+  KspaceArgs a = a_in;
   ks_refresh(a);
   extern __shared__ float4 lds4[];
   __shared__ double red[16 * 16];
   const int nx = a.nx, nS = a.nS, RT = a.RT, NK = a.NK;
-  const int kz = blockIdx.y, q0 = blockIdx.z * RT;
+  const int kz = bk.id.y, q0 = bk.id.z * RT;
   const int rt = min(RT, a.r - q0);
   const int nlines = nS * RT * NK;
   const int ls = nx + 1;
@@ -470,7 +500,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
   for (int i = 0; i < 11; ++i) acc[i] = 0.0;
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
   for (int it = 0; it < a.KT; ++it) {
-    const int ky0 = (blockIdx.x * a.KT + it) * NK;
+    const int ky0 = (bk.id.x * a.KT + it) * NK;
     if (it) __syncthreads();
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
     __syncthreads();
@@ -532,7 +562,7 @@ __global__ void __launch_bounds__(256) k_kspace(KspaceArgs a) {
     __syncthreads();
     lds_fft<1, kKsGenericMaxB, 256>(P, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
-  kspace_reduce(acc, red, a, params);
+  kspace_reduce(bk, acc, red, a, params);
 }
 
 // --------------------------------------------------------------- host ---
@@ -642,18 +672,26 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
   const int n = std::max(std::max(std::max(p.nx, p.ny), p.nz), pi.nS * pi.nS);
   const int ngb = init_gtab_blocks(pi);
   ProfScope _ps("k_init_tables", s);
-  hipLaunchKernelGGL(k_init_tables, dim3(kInitHB + ngb + (n + 255) / 256), dim3(256), 0, s, t.twx, t.twy, t.twz,
-                     t.scx, t.scy, t.scz, t.htab, t.gtab, t.pc, pi, ngb, p.nx, p.ny, p.nz, order, use_sinc);
-  FD3_LAUNCH_CHECK("k_init_tables");
-  return FD3_OK;
+  const InitTablesArgs ia{t.twx, t.twy, t.twz, t.scx, t.scy, t.scz, t.htab, t.gtab, t.pc, pi,
+                          ngb, p.nx, p.ny, p.nz, order, use_sinc};
+  return launch<InitTablesArgs, init_tables_body, 256>("k_init_tables", dim3(kInitHB + ngb + (n + 255) / 256), 0, s,
+                                                       ia);
 }
 
-// The y passes may use more than 64 KB of dynamic LDS, which must be allowed per kernel.
-static int y_lds_attr(size_t sh) {
-  if (sh <= (size_t)64 * 1024) return FD3_OK;
-  FD3_HIP(hipFuncSetAttribute((const void*)k_c2c_y<-1, kYLog>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  FD3_HIP(hipFuncSetAttribute((const void*)k_c2c_y<1, kYLog>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-  return FD3_OK;
+// One y pass (dir -1: A -> spec, +1: spec -> A); more than 64 KB of dynamic
+// LDS is allowed per kernel by launch().
+static int launch_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
+  const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
+                    (size_t)p.TX * sizeof(long long);
+  const dim3 grid(p.nx / p.TX, p.M1, p.C);
+  const YPassArgs ya{in, out, p.nx, p.ny, p.M1, p.TX, t.twy};
+  constexpr int NT = 1 << kYLog;
+  if (dir < 0) {
+    ProfScope _ps("k_c2c_y_fwd", s);
+    return launch<YPassArgs, c2c_y_body<-1, kYLog>, NT>("k_c2c_y_fwd", grid, sh, s, ya);
+  }
+  ProfScope _ps("k_c2c_y_inv", s);
+  return launch<YPassArgs, c2c_y_body<1, kYLog>, NT>("k_c2c_y_inv", grid, sh, s, ya);
 }
 
 int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float2* A, float2* spec, hipStream_t s) {
@@ -665,16 +703,7 @@ int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float
                        t.twz);
     FD3_LAUNCH_CHECK("k_r2c_z");
   }
-  {
-    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
-                      (size_t)p.TX * sizeof(long long);
-    ProfScope _ps("k_c2c_y_fwd", s);
-    if ((rc = y_lds_attr(sh))) return rc;
-    hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, A, spec, p.nx, p.ny, p.M1, p.TX,
-                       t.twy);
-    FD3_LAUNCH_CHECK("k_c2c_y fwd");
-  }
-  return FD3_OK;
+  return launch_y(p, t, A, spec, -1, s);
 }
 
 int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream_t s) {
@@ -692,15 +721,7 @@ int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream
 
 int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, float2* A, float* out, hipStream_t s) {
   int rc;
-  {
-    const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
-                      (size_t)p.TX * sizeof(long long);
-    ProfScope _ps("k_c2c_y_inv", s);
-    if ((rc = y_lds_attr(sh))) return rc;
-    hipLaunchKernelGGL((k_c2c_y<1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, spec, A, p.nx, p.ny, p.M1, p.TX,
-                       t.twy);
-    FD3_LAUNCH_CHECK("k_c2c_y inv");
-  }
+  if ((rc = launch_y(p, t, spec, A, 1, s))) return rc;
   {
     const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
     ProfScope _ps("k_c2r_z", s);
@@ -712,22 +733,7 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
 }
 
 int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
-  int rc;
-  const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
-                    (size_t)p.TX * sizeof(long long);
-  if (dir < 0) {
-    ProfScope _ps("k_c2c_y_fwd", s);
-    if ((rc = y_lds_attr(sh))) return rc;
-    hipLaunchKernelGGL((k_c2c_y<-1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
-                       t.twy);
-  } else {
-    ProfScope _ps("k_c2c_y_inv", s);
-    if ((rc = y_lds_attr(sh))) return rc;
-    hipLaunchKernelGGL((k_c2c_y<1, kYLog>), dim3(p.nx / p.TX, p.M1, p.C), dim3(1 << kYLog), sh, s, in, out, p.nx, p.ny, p.M1, p.TX,
-                       t.twy);
-  }
-  FD3_LAUNCH_CHECK("k_c2c_y");
-  return FD3_OK;
+  return launch_y(p, t, in, out, dir, s);
 }
 
 int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
@@ -746,24 +752,19 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
   const size_t sh = p.ks_lds;
   const dim3 grid(p.ny / p.NK / p.KT, p.M1, (p.r + p.RT - 1) / p.RT);
   ProfScope _ps("k_kspace", s);
-#define FD3_KS(NS_, RT_)                                                                         \
-  case NS_ * 8 + RT_:                                                                            \
-    if (need_params) hipLaunchKernelGGL((k_kspace_reg<NS_, RT_, true>), grid, dim3(256), sh, s, a); \
-    else hipLaunchKernelGGL((k_kspace_reg<NS_, RT_, false>), grid, dim3(256), sh, s, a);            \
-    break;
+#define FD3_KS(NS_, RT_)                                                                              \
+  case NS_ * 8 + RT_:                                                                                 \
+    if (need_params) return launch<KspaceArgs, kspace_reg_body<NS_, RT_, true>, 256>("k_kspace", grid, sh, s, a); \
+    return launch<KspaceArgs, kspace_reg_body<NS_, RT_, false>, 256>("k_kspace", grid, sh, s, a);
   switch (p.ks_reg * 8 + p.RT) {
     FD3_KS(1, 1) FD3_KS(1, 2) FD3_KS(1, 3) FD3_KS(1, 4)
     FD3_KS(2, 1) FD3_KS(2, 2) FD3_KS(2, 3) FD3_KS(2, 4)
     FD3_KS(3, 1) FD3_KS(3, 2) FD3_KS(3, 3) FD3_KS(3, 4)
     FD3_KS(4, 1) FD3_KS(4, 2) FD3_KS(4, 3) FD3_KS(4, 4)
     default:
-      if (sh > 64 * 1024) FD3_HIP(hipFuncSetAttribute((const void*)k_kspace,
-                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh));
-      hipLaunchKernelGGL(k_kspace, grid, dim3(256), sh, s, a);
+      return launch<KspaceArgs, kspace_body, 256>("k_kspace", grid, sh, s, a);
   }
 #undef FD3_KS
-  FD3_LAUNCH_CHECK("k_kspace");
-  return FD3_OK;
 }
 
 }  // namespace fd3

@@ -22,6 +22,7 @@
 #include "fd3_internal.h"
 #include "fft_lds.h"
 #include "fused.h"
+#include "launch.h"
 #include "mesh_kernels.h"
 #include "neigh_kernels.h"
 
@@ -45,18 +46,18 @@ void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b)
 }
 
 // ------------------------------------------------------------ column sort ---
-__global__ void k_col_prep(FusedArgs a) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void col_prep_body(const BlockInfo& bk, const FusedArgs& a) {
+  const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
-  if (a.dyn)
-    for (int i = 0; i < 9; ++i) a.inv[i] = a.dyn->inv[i];
+  double inv[9];
+  for (int i = 0; i < 9; ++i) inv[i] = a.dyn ? a.dyn->inv[i] : a.inv[i];
   const double4 r = a.pos4[s];
   const int X = a.spec_s[s];
   const int ns[3] = {a.nx, a.ny, a.nz};
   int b0[3];
   float xo[3];
   for (int d = 0; d < 3; ++d) {
-    const double f = r.x * a.inv[0 * 3 + d] + r.y * a.inv[1 * 3 + d] + r.z * a.inv[2 * 3 + d];
+    const double f = r.x * inv[0 * 3 + d] + r.y * inv[1 * 3 + d] + r.z * inv[2 * 3 + d];
     double x;
     b0[d] = wrap(interp_base((double)ns[d] * f, a.order, &x), ns[d]);
     xo[d] = (float)x;
@@ -68,8 +69,8 @@ __global__ void k_col_prep(FusedArgs a) {
   atomicAdd(a.cb.colcount + key, 1);
 }
 
-__global__ void k_col_scatter(FusedArgs a) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void col_scatter_body(const BlockInfo& bk, const FusedArgs& a) {
+  const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const int key = a.cb.key[s];
   const int pos = a.cb.colstart[key] + atomicAdd(a.cb.colcur + key, 1);
@@ -83,15 +84,11 @@ int col_sort(const FusedArgs& a, hipStream_t s) {
   int rc;
   ProfScope _ps("k_col_sort", s);
   if ((rc = launch_fill_i32(a.cb.colcount, 2 * ncol, s))) return rc;  // and colcur
-  if (a.n > 0) {
-    hipLaunchKernelGGL(k_col_prep, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
-    FD3_LAUNCH_CHECK("k_col_prep");
-  }
+  if (a.n > 0 && (rc = launch<FusedArgs, col_prep_body, 256>("k_col_prep", dim3(cdiv(a.n, 256)), 0, s, a)))
+    return rc;
   if ((rc = scan_exclusive(a.cb.colcount, (int64_t)ncol, a.cb.colstart, a.cb.scantmp, nullptr, s))) return rc;
-  if (a.n > 0) {
-    hipLaunchKernelGGL(k_col_scatter, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
-    FD3_LAUNCH_CHECK("k_col_scatter");
-  }
+  if (a.n > 0 && (rc = launch<FusedArgs, col_scatter_body, 256>("k_col_scatter", dim3(cdiv(a.n, 256)), 0, s, a)))
+    return rc;
   return FD3_OK;
 }
 
@@ -189,13 +186,23 @@ constexpr int kZLogSpread = 8, kZLogGather = 8;
 static_assert(kZLogSpread == kZLogGather, "one z tile for both passes");
 // grid (ny/TY, nx, C); LDS: TY lines of M+1 complex (= nz + 2 floats).
 // MAXB: butterflies per thread of lds_fft (TY * M <= 4 MAXB 2^LOG).
+struct ZPassArgs {
+  FusedArgs a;
+  float2* A;  // z-pass spectrum A[c][x][kz][y] (written by the spread, read by the gather)
+  int TY;
+  const float2* twz;
+};
+
 template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
-__global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* __restrict__ A, int TY,
-                                                      const float2* __restrict__ twz) {
+__device__ void spread_r2c_z_body(const BlockInfo& bk, const ZPassArgs& za) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
+  const FusedArgs& a = za.a;
+  float2* __restrict__ A = za.A;
+  const float2* __restrict__ twz = za.twz;
+  const int TY = za.TY;
   const int nz = a.nz, M = nz >> 1, ls = M + 1, p = P > 0 ? P : a.order;
-  const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
+  const int y0 = bk.id.x * TY, x = bk.id.y, c = bk.id.z;
   const int X = c / a.r, q = c - X * a.r;
   float2* tws = lds + TY * ls;
   stage_table(tws, twz, nz);
@@ -246,12 +253,15 @@ __global__ void __launch_bounds__(1 << LOG) k_spread_r2c_z(FusedArgs a, float2* 
 
 // ------------------------------------------------------ z pass + gather ---
 template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
-__global__ void __launch_bounds__(1 << LOG) k_c2r_z_gather(FusedArgs a, const float2* __restrict__ A, int TY,
-                                                      const float2* __restrict__ twz) {
+__device__ void c2r_z_gather_body(const BlockInfo& bk, const ZPassArgs& za) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
+  const FusedArgs& a = za.a;
+  const float2* __restrict__ A = za.A;
+  const float2* __restrict__ twz = za.twz;
+  const int TY = za.TY;
   const int nz = a.nz, M = nz >> 1, ls = M + 1, p = P > 0 ? P : a.order;
-  const int y0 = blockIdx.x * TY, x = blockIdx.y, c = blockIdx.z;
+  const int y0 = bk.id.x * TY, x = bk.id.y, c = bk.id.z;
   const int X = c / a.r, q = c - X * a.r;
   float2* tws = lds + TY * ls;
   stage_table(tws, twz, nz);
@@ -330,19 +340,20 @@ int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float
   if (TY < 0) return FD3_ERR_UNSUPPORTED;
   const size_t sh = (size_t)TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_spread_r2c_z", s);
-  const dim3 grid(p.ny / TY, p.nx, p.C), block(1 << kZLogSpread);
-  if (maxb == 4) hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 0, 4>), grid, block, sh, s, a, A, TY, t.twz);
-  else if (a.order == 5) hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 5, 2>), grid, block, sh, s, a, A, TY, t.twz);
-  else hipLaunchKernelGGL((k_spread_r2c_z<kZLogSpread, 0, 2>), grid, block, sh, s, a, A, TY, t.twz);
-  FD3_LAUNCH_CHECK("k_spread_r2c_z");
-  return FD3_OK;
+  const dim3 grid(p.ny / TY, p.nx, p.C);
+  constexpr int NT = 1 << kZLogSpread;
+  const ZPassArgs za{a, A, TY, t.twz};
+  if (maxb == 4) return launch<ZPassArgs, spread_r2c_z_body<kZLogSpread, 0, 4>, NT>("k_spread_r2c_z", grid, sh, s, za);
+  if (a.order == 5)
+    return launch<ZPassArgs, spread_r2c_z_body<kZLogSpread, 5, 2>, NT>("k_spread_r2c_z", grid, sh, s, za);
+  return launch<ZPassArgs, spread_r2c_z_body<kZLogSpread, 0, 2>, NT>("k_spread_r2c_z", grid, sh, s, za);
 }
 
 // accA / accG of each atom from the gather's partial sums: p x nodes, two
 // y slots, r channels (column order in, slot order out).  A fixed order of
 // additions, no atomics.
-__global__ void __launch_bounds__(256) k_gather_sum(FusedArgs a) {
-  const int64_t ai = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void gather_sum_body(const BlockInfo& bk, const FusedArgs& a) {
+  const int64_t ai = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (ai >= a.n) return;
   const int s = a.cb.rec_i[ai].w, r = a.r, nk = 2 * a.order;
   const float4* pp = a.cb.part + ai;
@@ -366,15 +377,16 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
   if (TY < 0) return FD3_ERR_UNSUPPORTED;
   const size_t sh = (size_t)TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_c2r_z_gather", s);
-  const dim3 grid(p.ny / TY, p.nx, p.C), block(1 << kZLogGather);
-  if (maxb == 4) hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 0, 4>), grid, block, sh, s, a, A, TY, t.twz);
-  else if (a.order == 5) hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 5, 2>), grid, block, sh, s, a, A, TY, t.twz);
-  else hipLaunchKernelGGL((k_c2r_z_gather<kZLogGather, 0, 2>), grid, block, sh, s, a, A, TY, t.twz);
-  FD3_LAUNCH_CHECK("k_c2r_z_gather");
-  if (a.n > 0) {
-    hipLaunchKernelGGL(k_gather_sum, dim3(cdiv(a.n, 256)), dim3(256), 0, s, a);
-    FD3_LAUNCH_CHECK("k_gather_sum");
-  }
+  const dim3 grid(p.ny / TY, p.nx, p.C);
+  constexpr int NT = 1 << kZLogGather;
+  const ZPassArgs za{a, const_cast<float2*>(A), TY, t.twz};
+  int rc;
+  if (maxb == 4) rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 0, 4>, NT>("k_c2r_z_gather", grid, sh, s, za);
+  else if (a.order == 5)
+    rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 5, 2>, NT>("k_c2r_z_gather", grid, sh, s, za);
+  else rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 0, 2>, NT>("k_c2r_z_gather", grid, sh, s, za);
+  if (rc) return rc;
+  if (a.n > 0) return launch<FusedArgs, gather_sum_body, 256>("k_gather_sum", dim3(cdiv(a.n, 256)), 0, s, a);
   return FD3_OK;
 }
 
@@ -385,21 +397,21 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
 //   dE/dCN   = sum_p L_p (g_p - sum_p' L_p' g_p') (-8)(CN' - ref_p) a_X,   g_p = sum_q dE/dc_q V[X,p,q]
 // (adjoint of src/fourierd3/core.py:469-478, 530-532 and of cncorr, :383-384).
 // Block partials (16 doubles): sum_i f_i (x) dE/dr_i, f = r cell^-1.
-__global__ void __launch_bounds__(256) k_c6_bwd(C6BwdArgs a) {
+__device__ void c6_bwd_body(const BlockInfo& bk, const C6BwdArgs& a) {
   __shared__ double red[16 * 9];
-  if (a.dyn)
-    for (int i = 0; i < 9; ++i) a.inv[i] = a.dyn->inv[i];
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double inv[9];
+  for (int i = 0; i < 9; ++i) inv[i] = a.dyn ? a.dyn->inv[i] : a.inv[i];
+  const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   double part[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (s < a.n) {
     const int X = a.spec_s[s];
     const double ns[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
     const double4 r = a.pos4[s];
     double gr[3], f[3];
-    for (int d = 0; d < 3; ++d) f[d] = r.x * a.inv[0 * 3 + d] + r.y * a.inv[1 * 3 + d] + r.z * a.inv[2 * 3 + d];
+    for (int d = 0; d < 3; ++d) f[d] = r.x * inv[0 * 3 + d] + r.y * inv[1 * 3 + d] + r.z * inv[2 * 3 + d];
     for (int b = 0; b < 3; ++b) {
       gr[b] = 0.0;
-      for (int d = 0; d < 3; ++d) gr[b] += (double)a.accG[3 * s + d] * ns[d] * a.inv[b * 3 + d];
+      for (int d = 0; d < 3; ++d) gr[b] += (double)a.accG[3 * s + d] * ns[d] * inv[b * 3 + d];
       a.gpos[3 * s + b] = gr[b];
     }
     for (int i = 0; i < 3; ++i)
@@ -432,14 +444,12 @@ __global__ void __launch_bounds__(256) k_c6_bwd(C6BwdArgs a) {
     }
     a.dEdcn[s] = dcn * ca;
   }
-  block_sum_store<9>(part, red, a.partials + (size_t)blockIdx.x * 16);
+  block_sum_store<9>(part, red, a.partials + (size_t)bk.id.x * 16);
 }
 
 int launch_c6_bwd(const C6BwdArgs& a, hipStream_t s) {
   ProfScope _ps("k_c6_bwd", s);
-  hipLaunchKernelGGL(k_c6_bwd, dim3(c6_blocks(a.n)), dim3(256), 0, s, a);
-  FD3_LAUNCH_CHECK("k_c6_bwd");
-  return FD3_OK;
+  return launch<C6BwdArgs, c6_bwd_body, 256>("k_c6_bwd", dim3(c6_blocks(a.n)), 0, s, a);
 }
 
 }  // namespace fd3

@@ -7,6 +7,7 @@
 // slot_atom[s] maps a slot back to the caller's atom index.
 
 #include "fd3_internal.h"
+#include "launch.h"
 #include "mesh_kernels.h"
 
 namespace fd3 {
@@ -31,9 +32,9 @@ __global__ void k_prep_pos(const void* pos, int dtype, const int32_t* __restrict
 // mesh self-interaction of exactly the charges that were spread.
 // partials per block (8 doubles): [0] sum_i sum_q lambda_q c_iq^2 phi_XX(0),
 // [1..4] the same sum with phi_XX(0) replaced by d phi_XX(0)/d(s6, s8, a1, a2).
-__global__ void __launch_bounds__(256) k_c6_fwd(C6Args a) {
+__device__ void c6_fwd_body(const BlockInfo& bk, const C6Args& a) {
   __shared__ double red[16 * 8];
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   double acc[5] = {0, 0, 0, 0, 0};
   if (s < a.n) {
     const int X = a.spec_s[s];
@@ -61,7 +62,7 @@ __global__ void __launch_bounds__(256) k_c6_fwd(C6Args a) {
     acc[3] = t * dphidR * sqrt(3.0 * (double)qqx);
     acc[4] = t * dphidR;
   }
-  block_sum_store<5>(acc, red, a.partials + (size_t)blockIdx.x * 8);
+  block_sum_store<5>(acc, red, a.partials + (size_t)bk.id.x * 8);
 }
 
 // Mesh coordinate setup of one atom: base node and 1-D weights per axis.
@@ -179,40 +180,32 @@ __global__ void k_unpermute(const double* __restrict__ gpos, const int32_t* __re
 }
 
 template <typename T>
-__global__ void k_fill(T* __restrict__ p, size_t n, T v) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+struct FillArgs {
+  T* p;
+  size_t n;
+  T v;
+};
+
+template <typename T>
+__device__ void fill_body(const BlockInfo& bk, const FillArgs<T>& a) {
+  for (size_t i = (size_t)bk.id.x * blockDim.x + threadIdx.x; i < a.n; i += (size_t)bk.n.x * blockDim.x) a.p[i] = a.v;
+}
+
+template <typename T>
+static int launch_fill(T* p, size_t n, size_t max_blocks, hipStream_t s) {
+  if (!n) return FD3_OK;
+  const size_t blocks = std::min<size_t>((n + 255) / 256, max_blocks);
+  return launch<FillArgs<T>, fill_body<T>, 256>("k_fill", dim3((unsigned)blocks), 0, s, FillArgs<T>{p, n, T(0)});
 }
 
 // ------------------------------------------------------------------ host ---
 int launch_fill_f32(float* p, size_t n, hipStream_t s) {
-  if (!n) return FD3_OK;
-  const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
   ProfScope _ps("k_fill", s);
-  hipLaunchKernelGGL(k_fill<float>, dim3(blocks), dim3(256), 0, s, p, n, 0.0f);
-  FD3_LAUNCH_CHECK("k_fill");
-  return FD3_OK;
+  return launch_fill(p, n, 65536, s);
 }
-int launch_fill_f64(double* p, size_t n, hipStream_t s) {
-  if (!n) return FD3_OK;
-  const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_fill<double>, dim3(blocks), dim3(256), 0, s, p, n, 0.0);
-  FD3_LAUNCH_CHECK("k_fill");
-  return FD3_OK;
-}
-int launch_fill_i32(int32_t* p, size_t n, hipStream_t s) {
-  if (!n) return FD3_OK;
-  const size_t blocks = std::min<size_t>((n + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_fill<int32_t>, dim3(blocks), dim3(256), 0, s, p, n, 0);
-  FD3_LAUNCH_CHECK("k_fill");
-  return FD3_OK;
-}
-
-int launch_fill_i64(int64_t* p, size_t n, hipStream_t s) {
-  if (!n) return FD3_OK;
-  hipLaunchKernelGGL(k_fill<int64_t>, dim3(1), dim3(256), 0, s, p, n, (int64_t)0);
-  FD3_LAUNCH_CHECK("k_fill");
-  return FD3_OK;
-}
+int launch_fill_f64(double* p, size_t n, hipStream_t s) { return launch_fill(p, n, 65536, s); }
+int launch_fill_i32(int32_t* p, size_t n, hipStream_t s) { return launch_fill(p, n, 65536, s); }
+int launch_fill_i64(int64_t* p, size_t n, hipStream_t s) { return launch_fill(p, n, 1, s); }
 
 int launch_prep_pos(const void* pos, int dtype, const int32_t* species, int64_t n, double4* pos4, int32_t* spec_s,
                     hipStream_t s) {
@@ -227,9 +220,7 @@ int c6_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, 256)); }
 
 int launch_c6_fwd(const C6Args& a, hipStream_t s) {
   ProfScope _ps("k_c6_fwd", s);
-  hipLaunchKernelGGL(k_c6_fwd, dim3(c6_blocks(a.n)), dim3(256), 0, s, a);
-  FD3_LAUNCH_CHECK("k_c6_fwd");
-  return FD3_OK;
+  return launch<C6Args, c6_fwd_body, 256>("k_c6_fwd", dim3(c6_blocks(a.n)), 0, s, a);
 }
 
 static int lanes_per_atom(int p) { return p * p <= 32 ? 32 : 64; }

@@ -17,6 +17,7 @@
 #include <climits>
 
 #include "fd3_internal.h"
+#include "launch.h"
 #include "neigh_kernels.h"
 
 namespace fd3 {
@@ -119,13 +120,24 @@ __device__ inline void for_each_neighbour(const CellArgs& c, int64_t s, F&& fn) 
 }
 
 // --------------------------------------------------------- build kernels ---
-__global__ void k_bin(CellArgs c, const void* pos, int dtype, int64_t n, double4* __restrict__ posw,
-                      int4* __restrict__ flo, int32_t* __restrict__ abin, int32_t* __restrict__ bcount) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+struct BinArgs {
+  CellArgs c;
+  const void* pos;
+  int dtype;
+  int64_t n;
+  double4* posw;
+  int4* flo;
+  int32_t* abin;
+  int32_t* bcount;
+};
+
+__device__ void bin_body(const BlockInfo& bk, const BinArgs& a) {
+  const int64_t i = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  CellArgs c = a.c;
   cell_refresh(c);
   double r[3];
-  load_pos(pos, dtype, i, r);
+  load_pos(a.pos, a.dtype, i, r);
   double fw[3];
   int fl[3], b[3];
   for (int d = 0; d < 3; ++d) {
@@ -142,24 +154,32 @@ __global__ void k_bin(CellArgs c, const void* pos, int dtype, int64_t n, double4
   const double x = fw[0] * c.cell[0] + fw[1] * c.cell[3] + fw[2] * c.cell[6];
   const double y = fw[0] * c.cell[1] + fw[1] * c.cell[4] + fw[2] * c.cell[7];
   const double z = fw[0] * c.cell[2] + fw[1] * c.cell[5] + fw[2] * c.cell[8];
-  posw[i] = make_double4(x, y, z, 0.0);
-  flo[i] = make_int4(fl[0], fl[1], fl[2], 0);
+  a.posw[i] = make_double4(x, y, z, 0.0);
+  a.flo[i] = make_int4(fl[0], fl[1], fl[2], 0);
   const int lin = (b[0] * c.nb[1] + b[1]) * c.nb[2] + b[2];
-  abin[i] = lin;
-  atomicAdd(bcount + lin, 1);
+  a.abin[i] = lin;
+  atomicAdd(a.bcount + lin, 1);
 }
 
 // Exclusive scan of n ints into out[0..n] (out[n] = total); one block of 1024.
 // OutT = int64_t for the neighbour-pair offsets, whose total may pass 2^31.
 template <typename OutT>
-__global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, int64_t n, OutT* __restrict__ out,
-                                                int64_t* __restrict__ total64) {
+struct Scan1Args {
+  const int32_t* in;
+  int64_t n;
+  OutT* out;
+  int64_t* total64;
+};
+
+template <typename OutT>
+__device__ void scan1_body(const BlockInfo&, const Scan1Args<OutT>& a) {
   __shared__ int64_t part[1024];
   const int t = threadIdx.x;
+  const int64_t n = a.n;
   const int64_t chunk = (n + 1023) / 1024;
   const int64_t lo = t * chunk, hi = min(n, lo + chunk);
   int64_t s = 0;
-  for (int64_t i = lo; i < hi; ++i) s += in[i];
+  for (int64_t i = lo; i < hi; ++i) s += a.in[i];
   part[t] = s;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -170,52 +190,58 @@ __global__ void __launch_bounds__(1024) k_scan1(const int32_t* __restrict__ in, 
   }
   int64_t run = t ? part[t - 1] : 0;
   for (int64_t i = lo; i < hi; ++i) {
-    out[i] = (OutT)run;
-    run += in[i];
+    a.out[i] = (OutT)run;
+    run += a.in[i];
   }
   if (t == 1023) {
-    out[n] = (OutT)part[1023];
-    if (total64) total64[0] = part[1023];
+    a.out[n] = (OutT)part[1023];
+    if (a.total64) a.total64[0] = part[1023];
   }
 }
 
 // Device-wide exclusive scan in two steps: per-block scans of 1024
-// elements, then the offset add (k_scan_add).
-__global__ void __launch_bounds__(256) k_scan_blocks(const int32_t* __restrict__ in, int64_t n,
-                                                     int32_t* __restrict__ out, int32_t* __restrict__ bsum) {
+// elements, then the offset add (scan_add_body).
+struct ScanArgs {
+  const int32_t* in;
+  int64_t n;
+  int32_t* out;
+  int32_t* bsum;
+  int64_t* total64;
+};
+
+__device__ void scan_blocks_body(const BlockInfo& bk, const ScanArgs& a) {
   __shared__ int32_t sh[256];
-  const int64_t base = (int64_t)blockIdx.x * 1024 + 4 * threadIdx.x;
+  const int64_t base = (int64_t)bk.id.x * 1024 + 4 * threadIdx.x;
   int32_t v[4];
   int32_t t = 0;
   for (int k = 0; k < 4; ++k) {
-    v[k] = base + k < n ? in[base + k] : 0;
+    v[k] = base + k < a.n ? a.in[base + k] : 0;
     t += v[k];
   }
   sh[threadIdx.x] = t;
   __syncthreads();
   for (int off = 1; off < 256; off <<= 1) {
-    const int32_t a = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+    const int32_t x = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
     __syncthreads();
-    sh[threadIdx.x] += a;
+    sh[threadIdx.x] += x;
     __syncthreads();
   }
   int32_t run = sh[threadIdx.x] - t;
   for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = run;
+    if (base + k < a.n) a.out[base + k] = run;
     run += v[k];
   }
-  if (threadIdx.x == 255) bsum[blockIdx.x] = sh[255];
+  if (threadIdx.x == 255) a.bsum[bk.id.x] = sh[255];
 }
 
 // Block b of 1024 elements adds the sum of the block totals before it (each
 // block reduces that prefix itself, so no separate scan of the totals is
 // launched); the last block also writes out[n] = total.
-__global__ void __launch_bounds__(1024) k_scan_add(int32_t* __restrict__ out, int64_t n,
-                                                   const int32_t* __restrict__ bsum, int64_t* __restrict__ total64) {
+__device__ void scan_add_body(const BlockInfo& bk, const ScanArgs& a) {
   __shared__ int64_t red[16];
-  const int64_t b = blockIdx.x;
+  const int64_t b = bk.id.x;
   int64_t v = 0;
-  for (int64_t i = threadIdx.x; i < b; i += 1024) v += bsum[i];
+  for (int64_t i = threadIdx.x; i < b; i += 1024) v += a.bsum[i];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
@@ -224,69 +250,90 @@ __global__ void __launch_bounds__(1024) k_scan_add(int32_t* __restrict__ out, in
 #pragma unroll
   for (int w = 0; w < 16; ++w) off += red[w];
   const int64_t i = b * 1024 + threadIdx.x;
-  if (i < n) out[i] += (int32_t)off;
-  if (b == (int64_t)gridDim.x - 1 && threadIdx.x == 0) {
-    const int64_t tot = off + bsum[b];
-    out[n] = (int32_t)tot;
-    if (total64) total64[0] = tot;
+  if (i < a.n) a.out[i] += (int32_t)off;
+  if (b == (int64_t)bk.n.x - 1 && threadIdx.x == 0) {
+    const int64_t tot = off + a.bsum[b];
+    a.out[a.n] = (int32_t)tot;
+    if (a.total64) a.total64[0] = tot;
   }
 }
 
 int scan_exclusive(const int32_t* in, int64_t n, int32_t* out, int32_t* tmp, int64_t* total, hipStream_t s) {
   const int64_t nb = std::max<int64_t>(1, cdiv(n, 1024));
-  int32_t* bsum = tmp;
-  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(256), 0, s, in, n, out, bsum);
-  FD3_LAUNCH_CHECK("k_scan_blocks");
-  hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(1024), 0, s, out, n, bsum, total);
-  FD3_LAUNCH_CHECK("k_scan_add");
-  return FD3_OK;
+  const ScanArgs a{in, n, out, tmp, total};
+  int rc;
+  if ((rc = launch<ScanArgs, scan_blocks_body, 256>("k_scan_blocks", dim3((unsigned)nb), 0, s, a))) return rc;
+  return launch<ScanArgs, scan_add_body, 1024>("k_scan_add", dim3((unsigned)nb), 0, s, a);
 }
 
 size_t scan_tmp_elems(int64_t n) { return (size_t)std::max<int64_t>(1, cdiv(n, 1024)); }
 
-__global__ void k_scatter(const int32_t* __restrict__ abin, int64_t n, const int32_t* __restrict__ bstart,
-                          int32_t* __restrict__ bcur, int32_t* __restrict__ slot_atom) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int b = abin[i];
-  const int slot = bstart[b] + atomicAdd(bcur + b, 1);
-  slot_atom[slot] = (int32_t)i;
+struct ScatterArgs {
+  const int32_t* abin;
+  int64_t n;
+  const int32_t* bstart;
+  int32_t* bcur;
+  int32_t* slot_atom;
+};
+
+__device__ void scatter_body(const BlockInfo& bk, const ScatterArgs& a) {
+  const int64_t i = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int b = a.abin[i];
+  const int slot = a.bstart[b] + atomicAdd(a.bcur + b, 1);
+  a.slot_atom[slot] = (int32_t)i;
 }
 
 // Deterministic order inside each bin: ascending atom index.  One wave per
 // bin: each atom's slot is lo + (number of atoms of the bin with a smaller
 // index), counted against 64 atoms at a time through lane shuffles.  Reads the
 // scattered (arbitrary) order from `in`, writes the sorted one to `out`.
-__global__ void __launch_bounds__(256) k_sort_bins(const int32_t* __restrict__ bstart, int nbins,
-                                                   const int32_t* __restrict__ in, int32_t* __restrict__ out) {
+struct SortBinsArgs {
+  const int32_t* bstart;
+  int nbins;
+  const int32_t* in;
+  int32_t* out;
+};
+
+__device__ void sort_bins_body(const BlockInfo& bk, const SortBinsArgs& a) {
   const int lane = threadIdx.x & 63;
-  const int b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (b >= nbins) return;  // uniform per wave
-  const int lo = bstart[b], cnt = bstart[b + 1] - lo;
+  const int b = bk.id.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (b >= a.nbins) return;  // uniform per wave
+  const int lo = a.bstart[b], cnt = a.bstart[b + 1] - lo;
   for (int i0 = 0; i0 < cnt; i0 += 64) {
     const bool mine = i0 + lane < cnt;
-    const int vi = mine ? in[lo + i0 + lane] : INT_MAX;
+    const int vi = mine ? a.in[lo + i0 + lane] : INT_MAX;
     int rank = 0;
     for (int j0 = 0; j0 < cnt; j0 += 64) {
-      const int vj = j0 + lane < cnt ? in[lo + j0 + lane] : INT_MAX;
+      const int vj = j0 + lane < cnt ? a.in[lo + j0 + lane] : INT_MAX;
       const int m = min(64, cnt - j0);
       for (int t = 0; t < m; ++t) rank += __shfl(vj, t) < vi ? 1 : 0;
     }
-    if (mine) out[lo + rank] = vi;
+    if (mine) a.out[lo + rank] = vi;
   }
 }
 
-__global__ void k_fill_sorted(const int32_t* __restrict__ slot_atom, int64_t n, const double4* __restrict__ posw,
-                              const int4* __restrict__ flo, const int32_t* __restrict__ abin,
-                              const int32_t* __restrict__ species, double4* __restrict__ pos4,
-                              int4* __restrict__ flo_s, int32_t* __restrict__ sbin, int32_t* __restrict__ spec_s) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const int i = slot_atom[s];
-  pos4[s] = posw[i];
-  flo_s[s] = flo[i];
-  sbin[s] = abin[i];
-  if (species) spec_s[s] = species[i];
+struct FillSortedArgs {
+  const int32_t* slot_atom;
+  int64_t n;
+  const double4* posw;
+  const int4* flo;
+  const int32_t* abin;
+  const int32_t* species;
+  double4* pos4;
+  int4* flo_s;
+  int32_t* sbin;
+  int32_t* spec_s;
+};
+
+__device__ void fill_sorted_body(const BlockInfo& bk, const FillSortedArgs& a) {
+  const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
+  if (s >= a.n) return;
+  const int i = a.slot_atom[s];
+  a.pos4[s] = a.posw[i];
+  a.flo_s[s] = a.flo[i];
+  a.sbin[s] = a.abin[i];
+  if (a.species) a.spec_s[s] = a.species[i];
 }
 
 // ----------------------------------------------------------- CN kernels ---
@@ -347,9 +394,17 @@ __device__ inline double wave_sum_d(double v) {
   return v;
 }
 
-// At most 128 VGPRs in the backward form: four waves per SIMD, not three.
+// Launched through the _w4 wrappers: at most 128 VGPRs in the backward form,
+// four waves per SIMD, not three.
+struct CnBinsArgs {
+  CellArgs c;
+  CnArgs a;
+};
+
 template <bool BWD, int kCnCap>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_cn_bins(CellArgs c, CnArgs a) {
+__device__ void cn_bins_body(const BlockInfo& bk, const CnBinsArgs& args) {
+  CellArgs c = args.c;
+  const CnArgs& a = args.a;
   cell_refresh(c);
   constexpr int NW = 4;                   // waves per block
   // candidate image positions relative to the bin's first atom as float
@@ -364,10 +419,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
   __shared__ int nsh[kNbrMax];            // packed lattice shift of neighbour bin k
   __shared__ unsigned short wl[NW][kCnStep * 64];  // per wave: hit candidates
   __shared__ double red[16 * 6];
-  const int b = blockIdx.x;
+  const int b = bk.id.x;
   const int hs = c.bstart[b], he = c.bstart[b + 1];
   if (hs == he) {  // empty bin (uniform across the block): zero virial partials
-    if (BWD && threadIdx.x < 8) a.partials[(size_t)blockIdx.x * 8 + threadIdx.x] = 0.0;
+    if (BWD && threadIdx.x < 8) a.partials[(size_t)b * 8 + threadIdx.x] = 0.0;
     return;
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -512,7 +567,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     }
     __syncthreads();  // the next home pass clears hacc
   }
-  if (BWD) block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
+  if (BWD) block_sum_store<6>(vir, red, a.partials + (size_t)b * 8);
 }
 
 // Edge-list forms: pairs given by (src, dst, shift in Angstrom).
@@ -729,20 +784,17 @@ int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* p
   if ((rc = launch_fill_i32(b.bcount, 2 * (size_t)g.nbins, s))) return rc;  // and bcur
   const CellArgs c = cell_args(g, b, n);
   ProfScope _ps("k_celllist_build", s);
-  hipLaunchKernelGGL(k_bin, dim3(cdiv(n, 256)), dim3(256), 0, s, c, pos, dtype, n, b.posw, b.flo, b.abin, b.bcount);
-  FD3_LAUNCH_CHECK("k_bin");
-  hipLaunchKernelGGL(k_scan1<int32_t>, dim3(1), dim3(1024), 0, s, b.bcount, (int64_t)g.nbins, b.bstart,
-                     (int64_t*)nullptr);
-  FD3_LAUNCH_CHECK("k_scan1");
-  // scatter into sbin (free until k_fill_sorted), then sort each bin into slot_atom
-  hipLaunchKernelGGL(k_scatter, dim3(cdiv(n, 256)), dim3(256), 0, s, b.abin, n, b.bstart, b.bcur, b.sbin);
-  FD3_LAUNCH_CHECK("k_scatter");
-  hipLaunchKernelGGL(k_sort_bins, dim3(cdiv(g.nbins, 4)), dim3(256), 0, s, b.bstart, g.nbins, b.sbin, b.slot_atom);
-  FD3_LAUNCH_CHECK("k_sort_bins");
-  hipLaunchKernelGGL(k_fill_sorted, dim3(cdiv(n, 256)), dim3(256), 0, s, b.slot_atom, n, b.posw, b.flo, b.abin,
-                     species, b.pos4, b.flo_s, b.sbin, b.spec_s);
-  FD3_LAUNCH_CHECK("k_fill_sorted");
-  return FD3_OK;
+  const BinArgs ba{c, pos, dtype, n, b.posw, b.flo, b.abin, b.bcount};
+  if ((rc = launch<BinArgs, bin_body, 256>("k_bin", dim3(cdiv(n, 256)), 0, s, ba))) return rc;
+  const Scan1Args<int32_t> sa{b.bcount, (int64_t)g.nbins, b.bstart, nullptr};
+  if ((rc = launch<Scan1Args<int32_t>, scan1_body<int32_t>, 1024>("k_scan1", dim3(1), 0, s, sa))) return rc;
+  // scatter into sbin (free until fill_sorted), then sort each bin into slot_atom
+  const ScatterArgs sc{b.abin, n, b.bstart, b.bcur, b.sbin};
+  if ((rc = launch<ScatterArgs, scatter_body, 256>("k_scatter", dim3(cdiv(n, 256)), 0, s, sc))) return rc;
+  const SortBinsArgs so{b.bstart, g.nbins, b.sbin, b.slot_atom};
+  if ((rc = launch<SortBinsArgs, sort_bins_body, 256>("k_sort_bins", dim3(cdiv(g.nbins, 4)), 0, s, so))) return rc;
+  const FillSortedArgs fs{b.slot_atom, n, b.posw, b.flo, b.abin, species, b.pos4, b.flo_s, b.sbin, b.spec_s};
+  return launch<FillSortedArgs, fill_sorted_body, 256>("k_fill_sorted", dim3(cdiv(n, 256)), 0, s, fs);
 }
 
 int cn_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, 256)); }
@@ -756,18 +808,18 @@ static bool cn_small_cap(const CellArgs& c, int nbins) {
 int launch_cn_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   if (c.n <= 0) return FD3_OK;
   ProfScope _ps("k_cn_cells", s);
-  if (cn_small_cap(c, nbins)) hipLaunchKernelGGL((k_cn_bins<false, kCnCapSmall>), dim3(nbins), dim3(256), 0, s, c, a);
-  else hipLaunchKernelGGL((k_cn_bins<false, kCnCapLarge>), dim3(nbins), dim3(256), 0, s, c, a);
-  FD3_LAUNCH_CHECK("k_cn_cells");
-  return FD3_OK;
+  const CnBinsArgs ca{c, a};
+  if (cn_small_cap(c, nbins))
+    return launch<CnBinsArgs, cn_bins_body<false, kCnCapSmall>, 256, true>("k_cn_cells", dim3(nbins), 0, s, ca);
+  return launch<CnBinsArgs, cn_bins_body<false, kCnCapLarge>, 256, true>("k_cn_cells", dim3(nbins), 0, s, ca);
 }
 
 int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   ProfScope _ps("k_cn_bwd_cells", s);
-  if (cn_small_cap(c, nbins)) hipLaunchKernelGGL((k_cn_bins<true, kCnCapSmall>), dim3(nbins), dim3(256), 0, s, c, a);
-  else hipLaunchKernelGGL((k_cn_bins<true, kCnCapLarge>), dim3(nbins), dim3(256), 0, s, c, a);
-  FD3_LAUNCH_CHECK("k_cn_bwd_cells");
-  return FD3_OK;
+  const CnBinsArgs ca{c, a};
+  if (cn_small_cap(c, nbins))
+    return launch<CnBinsArgs, cn_bins_body<true, kCnCapSmall>, 256, true>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
+  return launch<CnBinsArgs, cn_bins_body<true, kCnCapLarge>, 256, true>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
 }
 
 int launch_cn_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, hipStream_t s) {
@@ -792,9 +844,8 @@ int nb_count(const CellArgs& c, int32_t* count_atom, int64_t* offset, int64_t* t
   }
   hipLaunchKernelGGL(k_nb_count, dim3(cdiv(c.n, 256)), dim3(256), 0, s, c, count_atom);
   FD3_LAUNCH_CHECK("k_nb_count");
-  hipLaunchKernelGGL(k_scan1<int64_t>, dim3(1), dim3(1024), 0, s, count_atom, c.n, offset, total);
-  FD3_LAUNCH_CHECK("k_scan1");
-  return FD3_OK;
+  const Scan1Args<int64_t> sa{count_atom, c.n, offset, total};
+  return launch<Scan1Args<int64_t>, scan1_body<int64_t>, 1024>("k_scan1", dim3(1), 0, s, sa);
 }
 
 int nb_fill(const CellArgs& c, const int64_t* offset, int64_t capacity, int64_t* src, int64_t* dst, int32_t* sh,

@@ -196,8 +196,9 @@ int fd3_eval_d4_npt(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, 
                     const int32_t* species, int flags, void* workspace, size_t workspace_bytes,
                     const fd3_outputs* out, void* stream);
 
-/* fd3_eval_celllist_batch: n_boxes independent periodic boxes that share one
- *   model (one species set): box b owns atoms [atom_offsets[b], atom_offsets[b+1])
+/* fd3_eval_celllist_batch: the serial form of a batch, n_boxes independent
+ *   periodic boxes that share one model (one species set), one
+ *   fd3_eval_celllist call after another (fd3_batch_* merges them instead): box b owns atoms [atom_offsets[b], atom_offsets[b+1])
  *   of the packed positions (N_total, 3) and species (N_total) [device], has its
  *   own geometry geoms[b] (its own cell and mesh) and writes outs[b].  Same
  *   results as n_boxes fd3_eval_celllist calls; atom_offsets (n_boxes + 1),
@@ -212,6 +213,40 @@ int fd3_eval_celllist_batch_workspace(const fd3_model* m, int32_t n_boxes, const
 int fd3_eval_celllist_batch(const fd3_model* m, int32_t n_boxes, const fd3_geom* geoms, const int64_t* atom_offsets,
                             const void* positions, int pos_dtype, const int32_t* species, int flags, void* workspace,
                             size_t workspace_bytes, const fd3_outputs* outs, void* stream);
+
+/* ---- batched evaluation (the batched form of src/fourierd3/calculator.py:125-192) ----
+ * Many independent periodic boxes, of one or more species sets, evaluated by
+ * one merged launch sequence: each stage of the cell-list pipeline is one
+ * launch over the blocks of every box (instead of one launch per box and
+ * stage), with each box's mesh, cell list and partial sums packed into one
+ * workspace.  Box b uses models[model_of_box[b]] (so two species sets, each
+ * with its own decomposition, share a batch), the geometry geoms[b] (its own
+ * cell and mesh; mesh methods only) and atoms [atom_offsets[b],
+ * atom_offsets[b+1]) of the packed positions (N_total, 3) and species
+ * (N_total) [device]; species ids are compact within the box's model.
+ * Results equal n_boxes fd3_eval_celllist calls.  No counterpart in the
+ * reference, which evaluates one box per calculate() call.
+ *
+ * fd3_batch_create plans on the host (no device work; the model table
+ *   pointers must stay valid for the plan's life) and allocates the plan's
+ *   pinned host copy of its launch tables.
+ * fd3_batch_workspace: bytes of device workspace a run needs.
+ * fd3_batch_run: enqueues the copy of the launch tables into the workspace
+ *   and the merged launches on `stream`; the tables are rebuilt on the host
+ *   only when positions, species, workspace or outs change, so a run
+ *   captured in a HIP graph replays as captured (keep one plan per graph).
+ *   outs: n_boxes host structs of device pointers.
+ * fd3_batch_info: merged launches per run and the launches n_boxes single-box
+ *   calls would make.
+ */
+typedef struct fd3_batch fd3_batch;
+int fd3_batch_create(int32_t n_models, const fd3_model* models, int32_t n_boxes, const int32_t* model_of_box,
+                     const fd3_geom* geoms, const int64_t* atom_offsets, int flags, fd3_batch** plan);
+int fd3_batch_workspace(const fd3_batch* plan, size_t* bytes);
+int fd3_batch_run(fd3_batch* plan, const void* positions, int pos_dtype, const int32_t* species, void* workspace,
+                  size_t workspace_bytes, const fd3_outputs* outs, void* stream);
+int fd3_batch_info(const fd3_batch* plan, int32_t* merged_launches, int32_t* box_launches);
+void fd3_batch_destroy(fd3_batch* plan);
 
 /* fd3_eval_celllist_noalloc: fd3_eval_celllist with no workspace argument
  *   and no allocation of any kind.  No counterpart in the reference: it is for

@@ -238,7 +238,9 @@ class FourierD3ASECalculator(_Base):
 
     def evaluate_batch(self, positions: torch.Tensor, species: torch.Tensor, offsets, cells,
                        need_grad: bool = True, need_params: bool = False) -> dict:
-        """E+F+stress of many boxes in one call (``fd3_eval_celllist_batch``).
+        """E+F+stress of many boxes of this model's species set, one box after
+        another (``fd3_eval_celllist_batch``, the serial form; ``plan_batch``
+        merges the boxes' launches instead).
 
         positions (N_total, 3) Angstrom and species (N_total,) int32 compact
         ids (positions in the model's ascending atomic-number order) are
@@ -281,36 +283,31 @@ class FourierD3ASECalculator(_Base):
             L.ptr(self._ws), self._ws.numel(), outs, stream))
         return out
 
-    def calculate_batch(self, atoms_list) -> list:
-        """Energy (eV), forces (eV/A) and stress (eV/A^3) of every ``atoms`` in
-        ``atoms_list`` (one species set, the one this calculator's model was
-        built for, or built here from the first entry) with one batched call."""
-        if self._model is None:
-            self._build_model(atoms_list[0])
-        model = self._model
-        zmap = {int(z): i for i, z in enumerate(model.species_unique)}
-        offs = [0]
-        pos, spec, cells = [], [], []
+    def plan_batch(self, atoms_list, need_params: bool = False):
+        """A ``BoxBatch`` of the boxes in ``atoms_list`` with this calculator's
+        settings: any mix of species sets (one model and decomposition per
+        set), each box on the mesh the reference's rule gives its own cell,
+        evaluated with one merged launch sequence (``fd3_batch_*``)."""
+        from .batch import BoxBatch
         for at in atoms_list:
-            z = np.asarray(at.numbers)
-            if any(int(v) not in zmap for v in np.unique(z)):
-                raise ValueError("every box of a batch must use the species set of the model")
-            pos.append(np.asarray(at.positions, dtype=np.float64))
-            spec.append(np.array([zmap[int(v)] for v in z], dtype=np.int32))
-            cells.append(_cell_array(at))
-            offs.append(offs[-1] + len(z))
-        p = torch.tensor(np.concatenate(pos), device=self.device)
-        sp = torch.tensor(np.concatenate(spec), device=self.device)
-        res = self.evaluate_batch(p, sp, offs, cells)
-        e = (res["energy"] * self.HARTREE_TO_EV).cpu().numpy()
-        f = (-res["grad_pos"] * self.HARTREE_TO_EV).cpu().numpy()
-        v = (res["virial"] * self.HARTREE_TO_EV).cpu().numpy()
-        out = []
-        for b, cell in enumerate(cells):
-            vol_b = abs(np.linalg.det(cell)) * self.angstrom_to_bohr ** 3
-            out.append({"energy": float(e[b]), "forces": f[offs[b]:offs[b + 1]],
-                        "stress": v[b] / vol_b * self.angstrom_to_bohr ** 3})
-        return out
+            if not bool(np.asarray(at.pbc).all()):
+                raise AssertionError("particle-mesh only supports 3d pbc")
+        return BoxBatch([np.asarray(at.numbers) for at in atoms_list], [_cell_array(at) for at in atoms_list],
+                        device=self.device, need_params=need_params, mesh_spacing=self.mesh_spacing,
+                        r_cut=self.r_cut, c6tol=self.c6tol, method=self.method, params=self.params,
+                        xcfunc=self.xcfunc, interpolation_nodes=self.interpolation_nodes, dtype=self.dtype)
+
+    def calculate_batch(self, atoms_list) -> list:
+        """Energy (eV), forces (eV/A) and 3x3 stress (eV/A^3) of every ``atoms``
+        in ``atoms_list`` in one batched evaluation (``plan_batch``); the
+        result of box b equals ``calculate`` on a fresh calculator for it."""
+        if self.cnfunc != "smooth_cut":
+            raise NotImplementedError("the batched form evaluates cnfunc='smooth_cut'")
+        batch = self.plan_batch(atoms_list)
+        pos = torch.tensor(np.concatenate([np.asarray(at.positions, dtype=np.float64) for at in atoms_list]),
+                           device=self.device)
+        batch.evaluate(pos)
+        return batch.results()
 
     def calculate(self, atoms=None, properties=None, system_changes=all_changes):
         """Energy (eV), forces (eV/A) and 3x3 stress (eV/A^3) of ``atoms``

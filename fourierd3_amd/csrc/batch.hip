@@ -258,12 +258,7 @@ int fd3_batch_create(int32_t n_models, const fd3_model* models, int32_t n_boxes,
   p->n_records = 0;
   for (auto& r : recs) p->n_records += (int)r.recs.size();
   p->ws_bytes = p->blob_off + p->blob_cap;
-  const hipError_t e = hipHostMalloc((void**)&p->pinned, p->blob_cap);
-  if (e != hipSuccess) {
-    delete p;
-    return hip_fail(e, "hipHostMalloc (batch table)");
-  }
-  *plan = p;
+  *plan = p;  // the pinned table copy is allocated by the first run: planning touches no device
   return FD3_OK;
 }
 
@@ -309,6 +304,13 @@ int fd3_batch_run(fd3_batch* p, const void* positions, int pos_dtype, const int3
   bool same = p->k_pos == positions && p->k_spec == species && p->k_ws == workspace && p->k_dtype == pos_dtype &&
               (int)p->k_outs.size() == nb;
   for (int b = 0; same && b < nb; ++b) same = std::memcmp(&p->k_outs[b], &outs[b], sizeof(fd3_outputs)) == 0;
+  if (!p->pinned) {
+    const hipError_t e = hipHostMalloc((void**)&p->pinned, p->blob_cap);
+    if (e != hipSuccess) {
+      p->pinned = nullptr;
+      return hip_fail(e, "hipHostMalloc (batch launch tables)");
+    }
+  }
   if (!same) {
     std::vector<Recorder> recs;
     int rc = record_boxes(*p, positions, pos_dtype, species, (char*)workspace, outs, recs);

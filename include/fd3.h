@@ -228,8 +228,8 @@ int fd3_eval_celllist_batch(const fd3_model* m, int32_t n_boxes, const fd3_geom*
  * reference, which evaluates one box per calculate() call.
  *
  * fd3_batch_create plans on the host (no device work; the model table
- *   pointers must stay valid for the plan's life) and allocates the plan's
- *   pinned host copy of its launch tables.
+ *   pointers must stay valid for the plan's life).  The first run allocates
+ *   the plan's pinned host copy of its launch tables (freed by destroy).
  * fd3_batch_workspace: bytes of device workspace a run needs.
  * fd3_batch_run: enqueues the copy of the launch tables into the workspace
  *   and the merged launches on `stream`; the tables are rebuilt on the host

@@ -668,6 +668,92 @@ def test_batched_boxes_match_single_box_calls():
     assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
+def test_batching_workload_matches_single_box_calls_and_replays():
+    """The batching workload: 256 boxes with log-uniform 50..5000 atoms, half
+    SiO2-like and half H/C/N/O (two species sets with their own
+    decompositions), cells strained by up to 10 %, each box on the mesh its
+    own cell gives (16^3 .. 128^3).  One merged launch sequence
+    (fd3_batch_run) gives every box what its own fd3_eval_celllist call
+    gives; the smallest box of each composition is checked against the FP64
+    oracle at the stated tolerances; a HIP graph of the whole batch replays
+    like eager runs."""
+    from fourierd3_amd.batch import BoxBatch
+    _say("batching workload: 256 boxes, two species sets")
+    dev = _dev()
+    boxes = W.batch_boxes(256, seed=0)
+    bb = BoxBatch([b.numbers for b in boxes], [b.cell for b in boxes], device=dev)
+    info = bb.info()
+    print(f"[errors] batch launches: {info}", flush=True)
+    assert info["n_models"] == 2 and info["merged_launches"] * 50 < info["box_launches"]
+    pos = torch.tensor(np.concatenate([b.positions for b in boxes]), dtype=torch.float64, device=dev)
+    out = bb.evaluate(pos)
+    torch.cuda.synchronize()
+    got = {k: v.clone() for k, v in out.items()}
+    # every box through its own single-box call, on the same mesh
+    f64 = dict(dtype=torch.float64, device=dev)
+    ws = None
+    de = df = dv = 0.0
+    for b, box in enumerate(boxes):
+        m = bb.models[int(bb.model_of_box[b])]
+        o0, o1 = int(bb.offsets[b]), int(bb.offsets[b + 1])
+        ms = m._model_struct()
+        g = bb.geoms[b]
+        nb = C.c_size_t()
+        L.check(L.lib().fd3_eval_celllist_workspace(C.byref(ms), C.byref(g), o1 - o0, L.FD3_NEED_GRAD, C.byref(nb)))
+        if ws is None or ws.numel() < nb.value:
+            ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+        e1, g1, v1 = torch.empty(1, **f64), torch.empty((o1 - o0, 3), **f64), torch.empty((3, 3), **f64)
+        o = L.Outputs()
+        o.energy, o.grad_pos, o.virial = L.ptr(e1), L.ptr(g1), L.ptr(v1)
+        L.check(L.lib().fd3_eval_celllist(C.byref(ms), C.byref(g), o1 - o0, L.ptr(pos[o0:o1]), L.FD3_DTYPE_F64,
+                                          L.ptr(bb.species[o0:o1]), L.FD3_NEED_GRAD, L.ptr(ws), ws.numel(),
+                                          C.byref(o), _stream()))
+        torch.cuda.synchronize()
+        de = max(de, abs(float(got["energy"][b]) - float(e1[0])) * HARTREE_TO_EV / (o1 - o0))
+        df = max(df, (got["grad_pos"][o0:o1] - g1).abs().max().item() * HARTREE_TO_EV)
+        dv = max(dv, (got["virial"][b] - v1).abs().max().item() * HARTREE_TO_EV)
+    print(f"[errors] batch vs single-box calls, 256 boxes: E {de:.3e} eV/atom  F {df:.3e} eV/A  "
+          f"virial {dv:.3e} eV", flush=True)
+    # the spreading's float adds run in a varying order: agreement to float rounding
+    assert de < 1e-9 and df < 1e-6 and dv < 1e-6
+    # oracle on the smallest box of each composition
+    res = bb.results()
+    for kind in (14, 1):
+        b = min((i for i, x in enumerate(boxes) if kind in x.numbers), key=lambda i: boxes[i].n_atoms)
+        box = boxes[b]
+        o = OracleFourierD3(box.numbers, box.cell)
+        assert list(o.ns) == [int(v) for v in bb.geoms[b].ns]
+        e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+        e, f, st = _errors(res[b]["energy"], res[b]["forces"], res[b]["stress"], e_ref, f_ref, s_ref, box.n_atoms)
+        print(f"[errors] batch box {b} ({box.n_atoms} atoms, mesh {list(o.ns)}): E {e:.3e} eV/atom  F {f:.3e} eV/A  "
+              f"S {st:.3e} eV/A^3", flush=True)
+        assert e < E_TOL and f < F_TOL and st < S_TOL
+    # one HIP graph for the whole batch, replayed with the atoms moved in place
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        bb.evaluate(pos)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(6)
+    for rep in range(3):
+        pos.add_(torch.tensor(rng.normal(0.0, 0.01, tuple(pos.shape)), **f64))
+        for v in out.values():
+            v.zero_()
+        graph.replay()
+        torch.cuda.synchronize()
+        rep_out = {k: v.clone() for k, v in out.items()}
+        bb.evaluate(pos)
+        torch.cuda.synchronize()
+        d_e = (rep_out["energy"] - out["energy"]).abs().max().item()
+        d_g = (rep_out["grad_pos"] - out["grad_pos"]).abs().max().item()
+        print(f"[errors] batch graph replay {rep}: dE {d_e:.3e} Ha  dgrad {d_g:.3e} Ha/A", flush=True)
+        # float adds of the spreading in a varying order: agreement to float
+        # rounding, as for the single-box graph
+        assert d_e <= 1e-7 * out["energy"].abs().max().item()
+        assert d_g <= 1e-4 * out["grad_pos"].abs().max().item()
+
+
 @pytest.mark.parametrize("case", ["sio2_triclinic", "hcno_triclinic"])
 def test_ewald_calculator_matches_oracle(case):
     """method='ewald': the direct structure-factor sum over the full k grid."""

@@ -1,23 +1,35 @@
 """Benchmark: D3(BJ) SPME energy + forces + stress throughput on one MI355X.
 
-Workload: the 100k-atom H/C/N/O cubic box (ratio 2 : 1 : 0.3 : 0.7,
-1.0 g/cm^3, 1.0 A minimum separation, seed 0), r_cut = 6 A, SPME with 5
-B-spline nodes, 1.2 A mesh spacing (mesh 256^3), PBE BJ parameters,
-c6tol = 1 %.  One step = one full evaluation through the HIP library: cell
-list + CN, CN -> C6 charges, spreading, 3-D FFT, reciprocal-space kernel,
-inverse FFT, gathering with the adjoint C6/CN chain -> energy, forces and
-the 3x3 stress.  Positions are resident in HBM before the timed region.
+Headline workload (the ``value`` of the JSON line): the 100k-atom H/C/N/O
+cubic box (ratio 2 : 1 : 0.3 : 0.7, 1.0 g/cm^3, 1.0 A minimum separation,
+seed 0), r_cut = 6 A, SPME with 5 B-spline nodes, 1.2 A mesh spacing (mesh
+256^3), PBE BJ parameters, c6tol = 1 %.  One step = one full evaluation
+through the HIP library: cell list + CN, CN -> C6 charges, spreading, 3-D
+FFT, reciprocal-space kernel, inverse FFT, gathering with the adjoint C6/CN
+chain -> energy, forces and the 3x3 stress.  Positions are resident in HBM
+before the timed region.
 
-Prints one JSON line; see --help.  Extra legs: the per-kernel roofline of the
-dominant kernel (HIP events on the launch stream, recorded over the timed
-steps), the FP64 CPU oracle on a bounded sample (cpu_baseline) and the
-reference algorithm written with plain torch ops on the GPU (plain_gpu).
+Extra legs (keys of the same JSON line; each can be switched off):
+  roofline       the dominant kernel's achieved HBM bandwidth (HIP events on
+                 the launch stream over the timed steps)
+  plain_gpu      the reference algorithm written with plain torch ops on the GPU
+  rank_bracket   the headline box at c6tol 0.02 (rank 4) and 1e-4 (rank 6)
+  c1             the 648-atom SiO2 box (64^3) on the GPU, and cpu_baseline
+                 (the FP64 oracle on the host at that box, median of 5 runs
+                 after a warm-up, with and without its neighbour list)
+  batch_c4       256 boxes of 50..5000 atoms, SiO2 and H/C/N/O mixed, in one
+                 merged launch sequence (eager and HIP-graph replay) against a
+                 serial loop of single-box calls and plain torch on a sample
+  md_c5          1M-atom SiO2 box (mesh 512^3): 100 steps of seeded Gaussian
+                 displacements replayed from one captured HIP graph
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
+import statistics
 import sys
 import time
 
@@ -32,7 +44,7 @@ HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
 
 
 def _args():
-    ap = argparse.ArgumentParser(description=__doc__)
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -43,10 +55,17 @@ def _args():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain-gpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel HIP events")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-rank-bracket", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-box batching leg")
+    ap.add_argument("--no-md", action="store_true", help="skip the 1M-atom MD leg")
+    ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for HBM counters)")
     return ap.parse_args()
+
+
+def _say(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def kernel_bytes(name: str, plan: dict) -> float | None:
@@ -54,13 +73,9 @@ def kernel_bytes(name: str, plan: dict) -> float | None:
     M = mesh points, Mh = half-spectrum points, C = channels, N = atoms."""
     C, M, Mh, N = plan["C"], plan["M"], plan["Mh"], plan["N"]
     table = {
-        "k_r2c_z": 4 * M * C + 8 * Mh * C,       # read real mesh, write half spectrum
         "k_c2c_y_fwd": 16 * Mh * C,              # read + write complex half spectrum
         "k_kspace": 16 * Mh * C,                 # read S, write dE/drho spectrum (x passes fused)
         "k_c2c_y_inv": 16 * Mh * C,
-        "k_c2r_z": 8 * Mh * C + 4 * M * C,
-        "k_fill": 4 * M * C,                     # zero the real mesh
-        "k_spread": 4 * M * C,                   # mesh written once (accumulated in L2)
         # spreading fused into the z pass: the real mesh never reaches HBM
         "k_spread_r2c_z": 8 * Mh * C + 36 * N,   # write half spectrum; read positions, species, charges
         "k_c2r_z_gather": 8 * Mh * C + 36 * N,   # read half spectrum; per-atom records in, accumulators out
@@ -68,85 +83,85 @@ def kernel_bytes(name: str, plan: dict) -> float | None:
     return table.get(name)
 
 
-def run_gpu(args):
+def _calc_for(box, c6tol=1.0, cnfunc="smooth_cut"):
+    from types import SimpleNamespace
+
     import torch
 
     from fourierd3_amd import FourierD3ASECalculator
+    calc = FourierD3ASECalculator(device=torch.device("cuda:0"), c6tol=c6tol, cnfunc=cnfunc)
+    calc._build_model(SimpleNamespace(numbers=box.numbers, cell=box.cell, pbc=np.array([True] * 3)))
+    return calc
+
+
+def run_box(box, steps, warmup, c6tol=1.0, cnfunc="smooth_cut", profile=True, traffic_key=None):
+    """Time `steps` E+F+stress evaluations of one box; per-kernel times from
+    separate instrumented steps, and the roofline of the dominant kernel from
+    HIP events around its launches inside the timed steps."""
+    import torch
+
     from fourierd3_amd import _lib as L
-    from fourierd3_amd import workloads as W
 
     dev = torch.device("cuda:0")
-    box = W.hcno_box(args.atoms, seed=0)
-    calc = FourierD3ASECalculator(device=dev, c6tol=args.c6tol, cnfunc=args.cnfunc)
-    from types import SimpleNamespace
-    calc._build_model(SimpleNamespace(numbers=box.numbers, cell=box.cell, pbc=np.array([True] * 3)))
+    calc = _calc_for(box, c6tol, cnfunc)
     model = calc._model
     pos = torch.tensor(box.positions, dtype=torch.float64, device=dev)
     f64 = dict(dtype=torch.float64, device=dev)
     out = {"energy": torch.empty(1, **f64), "grad_pos": torch.empty((box.n_atoms, 3), **f64),
            "virial": torch.empty((3, 3), **f64), "grad_params": torch.empty(4, **f64)}
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         calc.evaluate(pos, box.cell, out=out)
     torch.cuda.synchronize()
     ns = [int(v) for v in model.ns_mesh]
-    M = ns[0] * ns[1] * ns[2]
-    plan = {"C": model.n_species * model.n_rank, "M": M, "Mh": ns[0] * ns[1] * (ns[2] // 2 + 1), "N": box.n_atoms}
-    def instrumented(steps):
-        # HIP events on the launch stream around every kernel
+    plan = {"C": model.n_species * model.n_rank, "M": ns[0] * ns[1] * ns[2],
+            "Mh": ns[0] * ns[1] * (ns[2] // 2 + 1), "N": box.n_atoms}
+
+    def instrumented(n):
         L.profile_only(None)
         L.profile_enable(True)
-        for _ in range(steps):
+        for _ in range(n):
             calc.evaluate(pos, box.cell, out=out)
         torch.cuda.synchronize()
         rep = L.profile_report()
         L.profile_enable(False)
         return rep
 
-    prof = {}
-    prof_steps = min(args.steps, 10)
     roof_name = None
-    if not args.no_profile:
-        # the roofline kernel: the longest one with an algorithmic byte count
+    if profile:
         pick = instrumented(2)
         roof_name = max((k for k in pick if kernel_bytes(k, plan)), key=lambda k: pick[k][0], default=None)
-    # timed steps; only the roofline kernel carries an event pair per launch
     if roof_name:
         L.profile_only(roof_name)
         L.profile_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         calc.evaluate(pos, box.cell, out=out)
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    dt = (t1 - t0) / args.steps
+    dt = (time.perf_counter() - t0) / steps
     timed = {}
     if roof_name:
         timed = L.profile_report()
         L.profile_enable(False)
         L.profile_only(None)
-    # per-kernel times from separate, instrumented steps after the timed ones
-    if not args.no_profile:
-        prof = instrumented(prof_steps)
-    energy_ev = float(out["energy"].item()) * 27.21138505
+    prof_steps = min(steps, 10)
+    prof = instrumented(prof_steps) if profile else {}
     kernels = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / prof_steps} for k, v in prof.items()}
     roof = None
     if roof_name and roof_name in timed:
-        name = roof_name
-        avg_s = timed[name][0] / timed[name][1] / 1e3
-        b = kernel_bytes(name, plan)
+        avg_s = timed[roof_name][0] / timed[roof_name][1] / 1e3
+        b = kernel_bytes(roof_name, plan)
         achieved = b / avg_s / 1e9
         traffic = None
         tf = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tf):
+        if traffic_key and os.path.exists(tf):
             with open(tf) as fh:
-                traffic = json.load(fh).get(f"{name}@{args.atoms}")
-        roof = {"kernel": name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                traffic = json.load(fh).get(f"{roof_name}@{traffic_key}")
+        roof = {"kernel": roof_name, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved * 1e9 / HBM_PEAK, "traffic": traffic, "algorithmic_bytes": b,
-                "avg_ms": avg_s * 1e3, "launches_timed": timed[name][1]}
-    info = {"box": box, "model": model, "energy_ev": energy_ev, "dt": dt, "kernels": kernels, "roofline": roof,
-            "plan": plan}
-    return info
+                "avg_ms": avg_s * 1e3, "launches_timed": timed[roof_name][1]}
+    return {"calc": calc, "model": model, "dt": dt, "kernels": kernels, "roofline": roof, "plan": plan,
+            "energy_ev": float(out["energy"].item()) * 27.21138505}
 
 
 def plain_gpu(box, model, steps=3):
@@ -290,35 +305,277 @@ def plain_gpu(box, model, steps=3):
             "energy_ev": float(out[0]), "what": "reference algorithm in plain torch ops on the GPU (FP32, autograd)"}
 
 
-def cpu_baseline(budget_s: float):
-    """FP64 oracle on a bounded sample of the workload: a 12.5k-atom box of the
-    same composition, density and separation (1/8 of the atoms, mesh 128^3),
-    timed for whole calculator calls (neighbour list + E + F + stress)."""
+
+
+def _cpu_info():
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline_c1(runs=5):
+    """The FP64 oracle (torch on the host) at the CPU configuration: 648 SiO2
+    atoms (216 Si + 432 O, 2.2 g/cm^3, 1.5 A minimum separation, seed 0), mesh
+    64^3, whole calculator calls (neighbour list + E + F + stress); one
+    warm-up, then the median of `runs`, also without the neighbour list."""
     import torch
 
     from fourierd3_amd import workloads as W
     from oracle.fd3_oracle import OracleFourierD3
     from oracle.neighbours import neighbour_list_kdtree
 
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    cores, model = _cpu_info()
     torch.set_num_threads(cores)
-    box = W.hcno_box(12_500, seed=0)
+    box = W.sio2_box()
     o = OracleFourierD3(box.numbers, box.cell)
-    runs, t_all, t_nl = 0, 0.0, 0.0
-    while runs < 1 or (t_all < budget_s and runs < 5):
+    t_all, t_calc = [], []
+    for k in range(runs + 1):
         t0 = time.perf_counter()
         edges = neighbour_list_kdtree(box.positions, box.cell, 6.0)
         t1 = time.perf_counter()
         o.calculate(box.positions, box.cell, edges=edges)
         t2 = time.perf_counter()
-        t_all += t2 - t0
-        t_nl += t1 - t0
-        runs += 1
-    return {"value": box.n_atoms * runs / t_all, "unit": "atom-evals/s", "cores": torch.get_num_threads(),
-            "kind": "port",
-            "sample": f"FP64 oracle (torch CPU), {runs} full calculator evaluation(s) of a 12.5k-atom H/C/N/O box "
-                      f"of the benchmark's composition and density (mesh 128^3), neighbour list included; "
-                      f"without it {box.n_atoms * runs / (t_all - t_nl):.4g} atom-evals/s"}
+        if k:  # the first run is the warm-up
+            t_all.append(t2 - t0)
+            t_calc.append(t2 - t1)
+    med, med_c = statistics.median(t_all), statistics.median(t_calc)
+    return {"value": box.n_atoms / med, "unit": "atom-evals/s", "cores": torch.get_num_threads(), "kind": "port",
+            "cpu_model": model, "ms_per_eval": med * 1e3, "value_without_neighbour_list": box.n_atoms / med_c,
+            "sample": f"FP64 oracle (torch CPU) on the 648-atom SiO2 box (mesh {[int(v) for v in o.ns]}), median of {runs} "
+                      f"whole calculator calls after one warm-up, neighbour list included "
+                      f"(without it: {med_c * 1e3:.1f} ms)"}
+
+
+def cpu_sample_hcno(budget_s=10.0):
+    """The FP64 oracle on a 12.5k-atom H/C/N/O box of the headline composition
+    and density (mesh 128^3): one warm-up, then runs within the budget."""
+    import torch
+
+    from fourierd3_amd import workloads as W
+    from oracle.fd3_oracle import OracleFourierD3
+    from oracle.neighbours import neighbour_list_kdtree
+
+    cores, _ = _cpu_info()
+    torch.set_num_threads(cores)
+    box = W.hcno_box(12_500, seed=0)
+    o = OracleFourierD3(box.numbers, box.cell)
+    ts = []
+    for k in range(6):
+        t0 = time.perf_counter()
+        o.calculate(box.positions, box.cell, edges=neighbour_list_kdtree(box.positions, box.cell, 6.0))
+        if k:
+            ts.append(time.perf_counter() - t0)
+        if sum(ts) > budget_s:
+            break
+    med = statistics.median(ts) if ts else float("nan")
+    return {"value": box.n_atoms / med, "unit": "atom-evals/s", "cores": torch.get_num_threads(), "kind": "port",
+            "sample": f"FP64 oracle, 12.5k-atom H/C/N/O box (mesh {[int(v) for v in o.ns]}), median of {len(ts)} run(s) "
+                      "after one warm-up"}
+
+
+def rank_bracket(box, steps, warmup):
+    """The headline box at higher ranks of the decomposition: the mesh work
+    scales with C = nS * r, and the real D3 table's rank at c6tol = 1 % is
+    unknown (the synthetic table gives 2)."""
+    out = []
+    for tol in (0.02, 1e-4):
+        info = run_box(box, steps, warmup, c6tol=tol)
+        m = info["model"]
+        out.append({"c6tol": tol, "rank": int(m.n_rank), "channels": int(m.n_rank * m.n_species),
+                    "ms_per_step": info["dt"] * 1e3, "value": box.n_atoms / info["dt"],
+                    "roofline": info["roofline"], "energy_eV": info["energy_ev"]})
+        del info
+    return out
+
+
+def c1_gpu(steps=50, warmup=5):
+    from fourierd3_amd import workloads as W
+    box = W.sio2_box()
+    info = run_box(box, steps, warmup, profile=False)
+    return {"workload": "SiO2 648 atoms, mesh 64^3, SPME E+F+stress", "ms_per_eval": info["dt"] * 1e3,
+            "value": box.n_atoms / info["dt"], "unit": "atom-evals/s"}
+
+
+def batch_c4(steps=10, plain_sample=16):
+    """256 boxes (log-uniform 50..5000 atoms, half SiO2-like, half H/C/N/O,
+    cells strained by up to 10 %, each on its own mesh by the reference rule)
+    in one merged launch sequence, eager and as one HIP-graph replay, against
+    the same boxes as single-box calls one after another (eager and captured)
+    and against the reference algorithm in plain torch on every 16th box."""
+    import ctypes as C
+
+    import torch
+
+    from fourierd3_amd import FourierD3
+    from fourierd3_amd import _lib as L
+    from fourierd3_amd import workloads as W
+    from fourierd3_amd.batch import BoxBatch
+
+    dev = torch.device("cuda:0")
+    boxes = W.batch_boxes(256, seed=0)
+    n_tot = sum(b.n_atoms for b in boxes)
+    bb = BoxBatch([b.numbers for b in boxes], [b.cell for b in boxes], device=dev)
+    pos = torch.tensor(np.concatenate([b.positions for b in boxes]), dtype=torch.float64, device=dev)
+
+    def timed(fn, n):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    dt_eager = timed(lambda: bb.evaluate(pos), steps)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    g_batch = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g_batch, stream=side):
+        bb.evaluate(pos)
+    torch.cuda.synchronize()
+    dt_graph = timed(g_batch.replay, steps)
+    # per-stage times of the merged launches
+    L.profile_only(None)
+    L.profile_enable(True)
+    bb.evaluate(pos)
+    torch.cuda.synchronize()
+    stages = {k: round(v[0], 4) for k, v in L.profile_report().items()}
+    L.profile_enable(False)
+    # the serial form: one fd3_eval_celllist call per box
+    f64 = dict(dtype=torch.float64, device=dev)
+    ws_bytes = 0
+    structs = [m._model_struct() for m in bb.models]
+    for b in range(bb.n_boxes):
+        nb_ = C.c_size_t()
+        n = int(bb.offsets[b + 1] - bb.offsets[b])
+        L.check(L.lib().fd3_eval_celllist_workspace(C.byref(structs[int(bb.model_of_box[b])]), C.byref(bb.geoms[b]),
+                                                    n, L.FD3_NEED_GRAD, C.byref(nb_)))
+        ws_bytes = max(ws_bytes, nb_.value)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    e1 = torch.empty(bb.n_boxes, **f64)
+    g1 = torch.empty((n_tot, 3), **f64)
+    v1 = torch.empty((bb.n_boxes, 3, 3), **f64)
+    outs = []
+    for b in range(bb.n_boxes):
+        o = L.Outputs()
+        o.energy = L.ptr(e1) + 8 * b
+        o.grad_pos = L.ptr(g1) + 24 * int(bb.offsets[b])
+        o.virial = L.ptr(v1) + 72 * b
+        outs.append(o)
+
+    def serial():
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        for b in range(bb.n_boxes):
+            o0, o1 = int(bb.offsets[b]), int(bb.offsets[b + 1])
+            L.check(L.lib().fd3_eval_celllist(
+                C.byref(structs[int(bb.model_of_box[b])]), C.byref(bb.geoms[b]), o1 - o0,
+                L.ptr(pos) + 24 * o0, L.FD3_DTYPE_F64, L.ptr(bb.species) + 4 * o0, L.FD3_NEED_GRAD, L.ptr(ws),
+                ws.numel(), C.byref(outs[b]), stream))
+
+    dt_serial = timed(serial, max(2, steps // 3))
+    g_serial = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g_serial, stream=side):
+        serial()
+    torch.cuda.synchronize()
+    dt_serial_graph = timed(g_serial.replay, max(2, steps // 3))
+    # the batch's results equal the serial calls'
+    bb.evaluate(pos)
+    serial()
+    torch.cuda.synchronize()
+    dev_e = ((bb.out["energy"] - e1).abs() / torch.tensor([b.n_atoms for b in boxes], **f64)).max().item() * 27.21138505
+    dev_f = (bb.out["grad_pos"] - g1).abs().max().item() * 27.21138505
+    # plain torch on a sample of the boxes
+    pg_atoms, pg_time = 0, 0.0
+    for b in range(0, bb.n_boxes, max(1, bb.n_boxes // plain_sample)):
+        box = boxes[b]
+        m = FourierD3(box.numbers, box.cell, device=dev, verbose=False)
+        r = plain_gpu(box, m, steps=2)
+        pg_atoms += box.n_atoms
+        pg_time += r["ms_per_step"] / 1e3
+        del m
+    info = bb.info()
+    res = {"workload": "256 boxes, 50..5000 atoms log-uniform, SiO2-like / H/C/N/O 50:50, cells strained <= 10 %, "
+                       "per-box SPME meshes 16^3..128^3, E+F+stress",
+           "n_boxes": bb.n_boxes, "n_atoms": n_tot, "species_sets": info["n_models"],
+           "merged_launches": info["merged_launches"], "single_box_launches": info["box_launches"],
+           "value": n_tot / dt_graph, "unit": "atom-evals/s", "ms_per_batch_graph": dt_graph * 1e3,
+           "ms_per_batch_eager": dt_eager * 1e3, "value_eager": n_tot / dt_eager,
+           "serial_loop": {"ms_per_batch": dt_serial * 1e3, "value": n_tot / dt_serial,
+                           "ms_per_batch_graph": dt_serial_graph * 1e3, "value_graph": n_tot / dt_serial_graph},
+           "plain_gpu_sample": {"boxes": len(range(0, bb.n_boxes, max(1, bb.n_boxes // plain_sample))),
+                                "atoms": pg_atoms, "value": pg_atoms / pg_time, "unit": "atom-evals/s"},
+           "speedup_vs_serial_loop": dt_serial / dt_graph, "speedup_vs_plain_gpu": (n_tot / dt_graph) /
+                                                                                   (pg_atoms / pg_time),
+           "max_dev_vs_serial": {"E_eV_per_atom": dev_e, "F_eV_per_A": dev_f},
+           "stage_ms": stages, "workspace_bytes": info["workspace_bytes"]}
+    del g_batch, g_serial, bb
+    return res
+
+
+def md_c5(n_steps=100):
+    """1M-atom SiO2-like box (999,999 atoms, 2.2 g/cm^3, edge 247.3 A, mesh
+    512^3 by the reference rule): one E+F+stress evaluation captured in a HIP
+    graph, replayed for `n_steps` MD-like steps, each after seeded Gaussian
+    displacements (sigma 0.01 A) of every atom; the graph's results are
+    compared with eager evaluations of the same positions along the way."""
+    import torch
+
+    from fourierd3_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    box = W.sio2_box(999_999, seed=0)
+    gen_s = time.perf_counter() - t0
+    calc = _calc_for(box)
+    model = calc._model
+    f64 = dict(dtype=torch.float64, device=dev)
+    n = box.n_atoms
+    pos = torch.tensor(box.positions, **f64)
+    out = {"energy": torch.empty(1, **f64), "grad_pos": torch.empty((n, 3), **f64),
+           "virial": torch.empty((3, 3), **f64), "grad_params": torch.empty(4, **f64)}
+    eager = {k: torch.empty_like(v) for k, v in out.items()}
+    calc.evaluate(pos, box.cell, out=out)
+    torch.cuda.synchronize()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph, stream=side):
+        calc.evaluate(pos, box.cell, out=out)
+    torch.cuda.synchronize()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    dev_e, dev_f, checks = 0.0, 0.0, []
+    torch.cuda.synchronize()
+    t_loop = 0.0
+    for step in range(1, n_steps + 1):
+        t0 = time.perf_counter()
+        pos.add_(torch.randn((n, 3), generator=gen, **f64), alpha=0.01)
+        graph.replay()
+        torch.cuda.synchronize()
+        t_loop += time.perf_counter() - t0
+        if step in (1, n_steps // 2, n_steps):
+            calc.evaluate(pos, box.cell, out=eager)
+            torch.cuda.synchronize()
+            de = abs(float(out["energy"][0]) - float(eager["energy"][0])) * 27.21138505 / n
+            df = (out["grad_pos"] - eager["grad_pos"]).abs().max().item() * 27.21138505
+            dev_e, dev_f = max(dev_e, de), max(dev_f, df)
+            checks.append(step)
+    ms = t_loop / n_steps * 1e3
+    return {"workload": f"SiO2-like {n} atoms, edge {box.cell[0, 0]:.1f} A, mesh "
+                        f"{[int(v) for v in model.ns_mesh]}, rank {model.n_rank}; {n_steps} steps of seeded N(0, 0.01 A) "
+                        "displacements, each step one replay of a captured E+F+stress graph",
+            "ms_per_step": ms, "value": n / (ms / 1e3), "unit": "atom-evals/s",
+            "graph_vs_eager": {"steps_checked": checks, "max_dE_eV_per_atom": dev_e, "max_dF_eV_per_A": dev_f},
+            "generation_s": gen_s}
 
 
 def main():
@@ -327,10 +584,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    info = run_gpu(args)
-    model, box = info["model"], info["box"]
+    import torch
+
+    from fourierd3_amd import workloads as W
+
+    _say("headline box")
+    box = W.hcno_box(args.atoms, seed=0)
+    info = run_box(box, args.steps, args.warmup, c6tol=args.c6tol, cnfunc=args.cnfunc,
+                   profile=not args.no_profile, traffic_key=args.atoms)
+    model = info["model"]
     if args.calibrate_copy:
-        import torch
         src = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0").fill_(1.0)
         dst = torch.empty_like(src)
         dst.copy_(src)
@@ -357,15 +620,35 @@ def main():
         "roofline": info["roofline"],
         "kernels": info["kernels"],
     }
-    if not args.no_plain_gpu:
+    calc = info.pop("calc")
+    del calc, info
+    torch.cuda.empty_cache()
+
+    def leg(name, fn):
+        _say(name)
         try:
-            pg = plain_gpu(box, model)
-            res["plain_gpu"] = pg
-            res["speedup_vs_plain_gpu"] = res["value"] / pg["value"]
+            res[name] = fn()
         except Exception as exc:  # report, do not fail the benchmark line
-            res["plain_gpu"] = {"error": repr(exc)[:300]}
+            res[name] = {"error": repr(exc)[:300]}
+        torch.cuda.empty_cache()
+
+    if not args.no_plain_gpu:
+        from fourierd3_amd import FourierD3
+        leg("plain_gpu", lambda: plain_gpu(box, FourierD3(box.numbers, box.cell, verbose=False)))
+        if "value" in res["plain_gpu"]:
+            res["speedup_vs_plain_gpu"] = res["value"] / res["plain_gpu"]["value"]
+    if not args.no_rank_bracket and args.cnfunc == "smooth_cut":
+        leg("rank_bracket", lambda: rank_bracket(box, max(5, args.steps // 2), args.warmup))
+    leg("c1", c1_gpu)
+    if not args.no_batch and args.cnfunc == "smooth_cut":
+        leg("batch_c4", batch_c4)
+    if not args.no_md and args.cnfunc == "smooth_cut":
+        leg("md_c5", lambda: md_c5(args.md_steps))
     if not args.no_cpu_baseline:
-        res["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+        leg("cpu_baseline", cpu_baseline_c1)
+        leg("cpu_baseline_hcno12k", cpu_sample_hcno)
+        if isinstance(res.get("c1"), dict) and "value" in res["c1"] and "value" in res["cpu_baseline"]:
+            res["c1"]["speedup_vs_cpu_baseline"] = res["c1"]["value"] / res["cpu_baseline"]["value"]
     print(json.dumps(res), flush=True)
 
 

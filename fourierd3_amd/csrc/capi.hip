@@ -266,7 +266,7 @@ struct ReduceJobs {
   int n;
 };
 
-__device__ void reduce_rows_body(const BlockInfo& bk, const ReduceJobs& jb) {
+__device__ __forceinline__ void reduce_rows_body(const BlockInfo& bk, const ReduceJobs& jb) {
   __shared__ double sh[16][17];
   int j = 0;
   while (j + 1 < jb.n && (int)bk.id.x >= jb.b0[j + 1]) ++j;
@@ -309,7 +309,7 @@ static int reduce_partials_run(const ReduceJobs& jb, hipStream_t s) {
 
 // One block of 1024 threads: wave w sums columns w and w + 16 of the 31
 // partial columns (rows <= 256 after k_reduce_rows), thread 0 assembles.
-__device__ void finalize_body(const BlockInfo&, const FinalArgs& fa) {
+__device__ __forceinline__ void finalize_body(const BlockInfo&, const FinalArgs& fa) {
   __shared__ double tot[32];
   FinalArgs a = fa;
   // ks 0..10 -> tot[0..10]; c6 0..4 -> tot[11..15]; ga 0..8 -> tot[16..24]; cnv 0..5 -> tot[25..30]

@@ -49,7 +49,7 @@ struct InitTablesArgs {
   int ngb, nx, ny, nz, order, use_sinc;
 };
 
-__device__ void init_tables_body(const BlockInfo& bk, const InitTablesArgs& ia) {
+__device__ __forceinline__ void init_tables_body(const BlockInfo& bk, const InitTablesArgs& ia) {
   float2 *twx = ia.twx, *twy = ia.twy, *twz = ia.twz;
   float *scx = ia.scx, *scy = ia.scy, *scz = ia.scz;
   float4* htab = ia.htab;
@@ -155,7 +155,7 @@ struct YPassArgs {
 };
 
 template <int DIR, int LOG>
-__device__ void c2c_y_body(const BlockInfo& bk, const YPassArgs& ya) {
+__device__ __forceinline__ void c2c_y_body(const BlockInfo& bk, const YPassArgs& ya) {
   constexpr int U = kYLines >> LOG;     // a whole tile in one batch of loads
   constexpr int MAXB = kYLines >> (LOG + 2);
   extern __shared__ float2 lds[];
@@ -247,10 +247,18 @@ __device__ inline float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
 
-__device__ inline void ks_refresh(KspaceArgs& a) {
-  if (!a.dyn) return;
-  for (int i = 0; i < 9; ++i) a.recip[i] = uniform_f(a.dyn->recip_f[i]);
-  a.neg_inv_vol = uniform_f(a.dyn->neg_inv_vol_f);
+struct KsCell {
+  float recip[9];
+  float neg_inv_vol;
+};
+
+// Only the cell-dependent values are copied out of the arguments (the rest
+// stay in the argument block and load as scalars where they are used).
+__device__ inline KsCell ks_cell(const KspaceArgs& a) {
+  KsCell c;
+  for (int i = 0; i < 9; ++i) c.recip[i] = a.dyn ? uniform_f(a.dyn->recip_f[i]) : a.recip[i];
+  c.neg_inv_vol = a.dyn ? uniform_f(a.dyn->neg_inv_vol_f) : a.neg_inv_vol;
+  return c;
 }
 
 // Dynamic LDS of both k-space kernels: [h table | x twiddles | lines ... |
@@ -274,14 +282,14 @@ __device__ inline void kspace_line_bases(const KspaceArgs& a, long long* lbase, 
 struct KPoint {
   float kv0, kv1, kv2, kn, isc;
 };
-__device__ inline KPoint kspace_point(const KspaceArgs& a, int kx, int ky, int kz) {
+__device__ inline KPoint kspace_point(const KspaceArgs& a, const KsCell& c, int kx, int ky, int kz) {
   const float mx = (float)(kx < (a.nx + 1) / 2 ? kx : kx - a.nx);
   const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
   const float mz = (float)kz;
   KPoint p;
-  p.kv0 = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
-  p.kv1 = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
-  p.kv2 = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
+  p.kv0 = mx * c.recip[0] + my * c.recip[3] + mz * c.recip[6];
+  p.kv1 = mx * c.recip[1] + my * c.recip[4] + mz * c.recip[7];
+  p.kv2 = mx * c.recip[2] + my * c.recip[5] + mz * c.recip[8];
   p.kn = __builtin_amdgcn_sqrtf(p.kv0 * p.kv0 + p.kv1 * p.kv1 + p.kv2 * p.kv2);
   p.isc = 1.0f;
   if (a.use_sinc) p.isc = __builtin_amdgcn_rcpf(fmaxf(a.scx[kx] * a.scy[ky] * a.scz[kz], 1e-10f));
@@ -342,9 +350,8 @@ __device__ inline void kspace_reduce(const BlockInfo& bk, double (&acc)[11], dou
 // LDS: [tables | x twiddles | lines | line bases]; the block sums reuse the
 // lines at the end.
 template <int NS, int RT, bool PARAMS>
-__device__ void kspace_reg_body(const BlockInfo& bk, const KspaceArgs& a_in) {
-  KspaceArgs a = a_in;
-  ks_refresh(a);
+__device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const KspaceArgs& a) {
+  const KsCell kc = ks_cell(a);
   constexpr int MAXB = ks_maxb(NS * RT);
   constexpr int NP = NS * (NS + 1) / 2;  // unordered species pairs
   extern __shared__ float4 lds4[];
@@ -388,7 +395,7 @@ __device__ void kspace_reg_body(const BlockInfo& bk, const KspaceArgs& a_in) {
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const int kxe = pad ? lpad(kx) : kx;  // element kx of a line
-      const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
+      const KPoint kp = kspace_point(a, kc, kx, ky0 + kyi, kz);
       // G and dG/d|k| (or dG/dw) of every pair first, then the spectra
       float gp[NP], dgp[NP], p_acc[4] = {0.f, 0.f, 0.f, 0.f};
       float h6v[PARAMS ? NP : 1], d6v[PARAMS ? NP : 1], h8v[PARAMS ? NP : 1], d8v[PARAMS ? NP : 1];
@@ -449,7 +456,7 @@ __device__ void kspace_reg_body(const BlockInfo& bk, const KspaceArgs& a_in) {
       // ph holds sum_Y G S_Y; the 1/sinc^2p of G~ goes into f and the accumulation
       float e_acc = 0.f, f[RT];
 #pragma unroll
-      for (int q = 0; q < RT; ++q) f[q] = lam[q] * a.neg_inv_vol * kp.isc;
+      for (int q = 0; q < RT; ++q) f[q] = lam[q] * kc.neg_inv_vol * kp.isc;
 #pragma unroll
       for (int X = 0; X < NS; ++X)
 #pragma unroll
@@ -476,9 +483,8 @@ __device__ void kspace_reg_body(const BlockInfo& bk, const KspaceArgs& a_in) {
 }
 
 // Any number of species: phi accumulates in a second LDS buffer P.
-__device__ void kspace_body(const BlockInfo& bk, const KspaceArgs& a_in) {
-  KspaceArgs a = a_in;
-  ks_refresh(a);
+__device__ __forceinline__ void kspace_body(const BlockInfo& bk, const KspaceArgs& a) {
+  const KsCell kc = ks_cell(a);
   extern __shared__ float4 lds4[];
   __shared__ double red[16 * 16];
   const int nx = a.nx, nS = a.nS, RT = a.RT, NK = a.NK;
@@ -507,7 +513,7 @@ __device__ void kspace_body(const BlockInfo& bk, const KspaceArgs& a_in) {
     lds_fft<-1, kKsGenericMaxB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
-      const KPoint kp = kspace_point(a, kx, ky0 + kyi, kz);
+      const KPoint kp = kspace_point(a, kc, kx, ky0 + kyi, kz);
       float d_acc = 0.f, p_acc[4] = {0.f, 0.f, 0.f, 0.f};
       // phi accumulates in P (this thread owns column (kyi, kx) of every line)
       for (int l = 0; l < nS * RT; ++l) P[(l * NK + kyi) * ls + kx] = make_float2(0.f, 0.f);
@@ -547,7 +553,7 @@ __device__ void kspace_body(const BlockInfo& bk, const KspaceArgs& a_in) {
           const float lam = a.eigs[q0 + qi];
           const float2 sx = S[ix], ph = P[ix];
           e_acc += lam * (sx.x * ph.x + sx.y * ph.y);
-          const float f = lam * a.neg_inv_vol * kp.isc;
+          const float f = lam * kc.neg_inv_vol * kp.isc;
           float2 v = make_float2(f * ph.x, f * ph.y);
           if (kz == 0 && ky0 + kyi == 0 && kx == 0) {  // k = 0: kept out of the mesh (see KspaceArgs::k0)
             a.k0[X * a.r + q0 + qi] = v;
@@ -607,7 +613,7 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   // RT: ranks per tile; the register variant takes any RT <= 4, the generic
   // one a power of two <= 8 (a last partial tile is padded with zero lines)
   int RT = 1;
-  if (p->ks_reg) RT = std::min(r, 4);
+  if (p->ks_reg) RT = std::min(r, std::max(1, 8 / nS));  // nS * RT <= 8 spectra per thread: no register spills
   else
     while (RT < std::min(r, 8)) RT <<= 1;
   int NK = std::max(1, std::min(p->ny, 256 / p->nx));

@@ -46,7 +46,7 @@ void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b)
 }
 
 // ------------------------------------------------------------ column sort ---
-__device__ void col_prep_body(const BlockInfo& bk, const FusedArgs& a) {
+__device__ __forceinline__ void col_prep_body(const BlockInfo& bk, const FusedArgs& a) {
   const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   double inv[9];
@@ -69,7 +69,7 @@ __device__ void col_prep_body(const BlockInfo& bk, const FusedArgs& a) {
   atomicAdd(a.cb.colcount + key, 1);
 }
 
-__device__ void col_scatter_body(const BlockInfo& bk, const FusedArgs& a) {
+__device__ __forceinline__ void col_scatter_body(const BlockInfo& bk, const FusedArgs& a) {
   const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const int key = a.cb.key[s];
@@ -194,7 +194,7 @@ struct ZPassArgs {
 };
 
 template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
-__device__ void spread_r2c_z_body(const BlockInfo& bk, const ZPassArgs& za) {
+__device__ __forceinline__ void spread_r2c_z_body(const BlockInfo& bk, const ZPassArgs& za) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
   const FusedArgs& a = za.a;
@@ -253,7 +253,7 @@ __device__ void spread_r2c_z_body(const BlockInfo& bk, const ZPassArgs& za) {
 
 // ------------------------------------------------------ z pass + gather ---
 template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
-__device__ void c2r_z_gather_body(const BlockInfo& bk, const ZPassArgs& za) {
+__device__ __forceinline__ void c2r_z_gather_body(const BlockInfo& bk, const ZPassArgs& za) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
   const FusedArgs& a = za.a;
@@ -352,7 +352,7 @@ int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float
 // accA / accG of each atom from the gather's partial sums: p x nodes, two
 // y slots, r channels (column order in, slot order out).  A fixed order of
 // additions, no atomics.
-__device__ void gather_sum_body(const BlockInfo& bk, const FusedArgs& a) {
+__device__ __forceinline__ void gather_sum_body(const BlockInfo& bk, const FusedArgs& a) {
   const int64_t ai = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (ai >= a.n) return;
   const int s = a.cb.rec_i[ai].w, r = a.r, nk = 2 * a.order;
@@ -397,7 +397,7 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
 //   dE/dCN   = sum_p L_p (g_p - sum_p' L_p' g_p') (-8)(CN' - ref_p) a_X,   g_p = sum_q dE/dc_q V[X,p,q]
 // (adjoint of src/fourierd3/core.py:469-478, 530-532 and of cncorr, :383-384).
 // Block partials (16 doubles): sum_i f_i (x) dE/dr_i, f = r cell^-1.
-__device__ void c6_bwd_body(const BlockInfo& bk, const C6BwdArgs& a) {
+__device__ __forceinline__ void c6_bwd_body(const BlockInfo& bk, const C6BwdArgs& a) {
   __shared__ double red[16 * 9];
   double inv[9];
   for (int i = 0; i < 9; ++i) inv[i] = a.dyn ? a.dyn->inv[i] : a.inv[i];

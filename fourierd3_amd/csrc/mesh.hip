@@ -32,7 +32,7 @@ __global__ void k_prep_pos(const void* pos, int dtype, const int32_t* __restrict
 // mesh self-interaction of exactly the charges that were spread.
 // partials per block (8 doubles): [0] sum_i sum_q lambda_q c_iq^2 phi_XX(0),
 // [1..4] the same sum with phi_XX(0) replaced by d phi_XX(0)/d(s6, s8, a1, a2).
-__device__ void c6_fwd_body(const BlockInfo& bk, const C6Args& a) {
+__device__ __forceinline__ void c6_fwd_body(const BlockInfo& bk, const C6Args& a) {
   __shared__ double red[16 * 8];
   const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   double acc[5] = {0, 0, 0, 0, 0};
@@ -187,7 +187,7 @@ struct FillArgs {
 };
 
 template <typename T>
-__device__ void fill_body(const BlockInfo& bk, const FillArgs<T>& a) {
+__device__ __forceinline__ void fill_body(const BlockInfo& bk, const FillArgs<T>& a) {
   for (size_t i = (size_t)bk.id.x * blockDim.x + threadIdx.x; i < a.n; i += (size_t)bk.n.x * blockDim.x) a.p[i] = a.v;
 }
 

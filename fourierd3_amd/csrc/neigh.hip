@@ -131,7 +131,7 @@ struct BinArgs {
   int32_t* bcount;
 };
 
-__device__ void bin_body(const BlockInfo& bk, const BinArgs& a) {
+__device__ __forceinline__ void bin_body(const BlockInfo& bk, const BinArgs& a) {
   const int64_t i = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   CellArgs c = a.c;
@@ -172,7 +172,7 @@ struct Scan1Args {
 };
 
 template <typename OutT>
-__device__ void scan1_body(const BlockInfo&, const Scan1Args<OutT>& a) {
+__device__ __forceinline__ void scan1_body(const BlockInfo&, const Scan1Args<OutT>& a) {
   __shared__ int64_t part[1024];
   const int t = threadIdx.x;
   const int64_t n = a.n;
@@ -209,7 +209,7 @@ struct ScanArgs {
   int64_t* total64;
 };
 
-__device__ void scan_blocks_body(const BlockInfo& bk, const ScanArgs& a) {
+__device__ __forceinline__ void scan_blocks_body(const BlockInfo& bk, const ScanArgs& a) {
   __shared__ int32_t sh[256];
   const int64_t base = (int64_t)bk.id.x * 1024 + 4 * threadIdx.x;
   int32_t v[4];
@@ -237,7 +237,7 @@ __device__ void scan_blocks_body(const BlockInfo& bk, const ScanArgs& a) {
 // Block b of 1024 elements adds the sum of the block totals before it (each
 // block reduces that prefix itself, so no separate scan of the totals is
 // launched); the last block also writes out[n] = total.
-__device__ void scan_add_body(const BlockInfo& bk, const ScanArgs& a) {
+__device__ __forceinline__ void scan_add_body(const BlockInfo& bk, const ScanArgs& a) {
   __shared__ int64_t red[16];
   const int64_t b = bk.id.x;
   int64_t v = 0;
@@ -276,7 +276,7 @@ struct ScatterArgs {
   int32_t* slot_atom;
 };
 
-__device__ void scatter_body(const BlockInfo& bk, const ScatterArgs& a) {
+__device__ __forceinline__ void scatter_body(const BlockInfo& bk, const ScatterArgs& a) {
   const int64_t i = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int b = a.abin[i];
@@ -295,7 +295,7 @@ struct SortBinsArgs {
   int32_t* out;
 };
 
-__device__ void sort_bins_body(const BlockInfo& bk, const SortBinsArgs& a) {
+__device__ __forceinline__ void sort_bins_body(const BlockInfo& bk, const SortBinsArgs& a) {
   const int lane = threadIdx.x & 63;
   const int b = bk.id.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (b >= a.nbins) return;  // uniform per wave
@@ -326,7 +326,7 @@ struct FillSortedArgs {
   int32_t* spec_s;
 };
 
-__device__ void fill_sorted_body(const BlockInfo& bk, const FillSortedArgs& a) {
+__device__ __forceinline__ void fill_sorted_body(const BlockInfo& bk, const FillSortedArgs& a) {
   const int64_t s = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (s >= a.n) return;
   const int i = a.slot_atom[s];
@@ -402,7 +402,7 @@ struct CnBinsArgs {
 };
 
 template <bool BWD, int kCnCap>
-__device__ void cn_bins_body(const BlockInfo& bk, const CnBinsArgs& args) {
+__device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsArgs& args) {
   CellArgs c = args.c;
   const CnArgs& a = args.a;
   cell_refresh(c);

@@ -40,6 +40,7 @@ EXPORTS = (
     "fd3_eval_celllist_batch_workspace", "fd3_eval_celllist_batch",
     "fd3_eval_celllist_noalloc",
     "fd3_batch_create", "fd3_batch_workspace", "fd3_batch_run", "fd3_batch_info", "fd3_batch_destroy",
+    "fd3_tune_set", "fd3_tune_clear", "fd3_plan_tiles",
     "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
     "fd3_spread", "fd3_gather",
     "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
@@ -121,6 +122,9 @@ _SIG = {
     "fd3_batch_run": (C.c_int, [C.c_void_p, _P, C.c_int, _P, _P, C.c_size_t, C.POINTER(Outputs), _P]),
     "fd3_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "fd3_batch_destroy": (None, [C.c_void_p]),
+    "fd3_tune_set": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
+    "fd3_tune_clear": (C.c_int, []),
+    "fd3_plan_tiles": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "fd3_neighbours_workspace": (C.c_int, [C.c_int64, C.POINTER(C.c_double), C.c_double, C.POINTER(C.c_size_t)]),
     "fd3_neighbours_count": (C.c_int, [C.c_int64, _P, C.c_int, C.POINTER(C.c_double), C.c_double, _P, C.c_size_t,
                                        _P, _P]),
@@ -152,7 +156,23 @@ def lib():
             f.restype = res
             f.argtypes = args
         _lib = h
+        _load_tuning(h)
     return _lib
+
+
+TUNING_PATH = Path(__file__).resolve().parent / "data" / "tuning.json"
+
+
+def _load_tuning(h) -> None:
+    """Register the autotuner's measured tiles (fourierd3_amd/data/tuning.json,
+    written by ``python -m fourierd3_amd.autotune``) with the library."""
+    import json
+    if not TUNING_PATH.exists():
+        return
+    for e in json.loads(TUNING_PATH.read_text()).get("entries", []):
+        ns = (C.c_int32 * 3)(*e["ns"])
+        tiles = (C.c_int32 * 4)(*e["tiles"])
+        h.fd3_tune_set(ns, int(e["n_species"]), int(e["n_rank"]), tiles)
 
 
 def check(status: int) -> None:

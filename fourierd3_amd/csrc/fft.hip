@@ -15,6 +15,11 @@
 // The real-to-complex z transform packs a real line of length nz into nz/2
 // complex values, transforms them and separates even/odd halves.
 
+#include <array>
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "fd3_internal.h"
 #include "fft_plan.h"
 #include "launch.h"
@@ -573,6 +578,23 @@ __device__ __forceinline__ void kspace_body(const BlockInfo& bk, const KspaceArg
 
 // --------------------------------------------------------------- host ---
 constexpr int kYLog = 10;  // y passes: 1024-thread blocks
+
+// Tile overrides measured by the autotuner, per (mesh, species, rank):
+// z-pass lines TY, y-pass lines TX, k-space ranks per tile RT and tiles per
+// block KT (0 = the plan's own choice).  An override that does not fit the
+// kernels' limits for its plan is ignored.
+using TuneKey = std::tuple<int, int, int, int, int>;
+static std::mutex g_tune_mu;
+static std::map<TuneKey, std::array<int, 4>> g_tune;
+
+static bool tune_lookup(const FftPlan& p, std::array<int, 4>* t) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  const auto it = g_tune.find(TuneKey{p.nx, p.ny, p.nz, p.nS, p.r});
+  if (it == g_tune.end()) return false;
+  *t = it->second;
+  return true;
+}
+
 int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   for (int i = 0; i < 3; ++i) {
     const int n = ns[i];
@@ -593,6 +615,11 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   p->TX = std::max(1, std::min(p->nx, kYLines / p->ny));
   p->TX = 1 << (31 - __builtin_clz(p->TX));
   p->LPB = std::max(1, budget / (p->nx + 1));
+  std::array<int, 4> tune{0, 0, 0, 0};
+  const bool tuned = tune_lookup(*p, &tune);
+  auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+  if (tuned && pow2(tune[0]) && tune[0] <= p->ny && (size_t)tune[0] * p->M <= (size_t)kFftLines) p->TY = tune[0];
+  if (tuned && pow2(tune[1]) && tune[1] <= p->nx && (size_t)tune[1] * p->ny <= (size_t)kYLines) p->TX = tune[1];
   // k-space: the h table, the x twiddles, one (nS <= kKsRegMaxS) or two LDS
   // buffers of nS*RT*NK lines of nx+1 complex, and the line bases, within
   // 62 KB (the block also holds 2 KB of static LDS)
@@ -621,6 +648,9 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
     const int maxb = p->ks_reg ? ks_maxb(nS * rt_) : kKsGenericMaxB;
     return ks_bytes(rt_, nk_) <= ks_budget && (size_t)nS * rt_ * nk_ * p->nx <= (size_t)1024 * maxb;
   };
+  if (tuned && tune[2] >= 1 && tune[2] <= std::min(r, p->ks_reg ? 4 : 8) && (p->ks_reg || pow2(tune[2])) &&
+      ks_fits(tune[2], 1))
+    RT = tune[2];
   while (RT > 1 && !ks_fits(RT, NK)) RT = p->ks_reg ? RT - 1 : RT >> 1;
   while (NK > 1 && !ks_fits(RT, NK)) NK >>= 1;
   if (!ks_fits(RT, NK)) {
@@ -633,6 +663,7 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   const int zb = (r + RT - 1) / RT;
   int KT = 1;
   while (KT < 4 && (p->ny / NK) % (2 * KT) == 0 && (size_t)(p->ny / NK / (2 * KT)) * p->M1 * zb >= 8192) KT <<= 1;
+  if (tuned && pow2(tune[3]) && tune[3] <= 4 && (p->ny / NK) % tune[3] == 0) KT = tune[3];
   p->KT = KT;
   p->ks_blocks = (p->ny / NK / KT) * p->M1 * zb;
   // register budget of lds_fft<.., kFftMaxB>: lines * n <= kFftLines
@@ -774,3 +805,37 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
 }
 
 }  // namespace fd3
+
+using namespace fd3;
+
+extern "C" {
+
+int fd3_tune_set(const int32_t ns[3], int32_t n_species, int32_t n_rank, const int32_t tiles[4]) {
+  if (!ns || !tiles || n_species < 1 || n_rank < 1) {
+    set_error("bad tuning entry");
+    return FD3_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune[TuneKey{ns[0], ns[1], ns[2], n_species, n_rank}] = {tiles[0], tiles[1], tiles[2], tiles[3]};
+  return FD3_OK;
+}
+
+int fd3_tune_clear(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune.clear();
+  return FD3_OK;
+}
+
+int fd3_plan_tiles(const int32_t ns[3], int32_t n_species, int32_t n_rank, int32_t tiles[4]) {
+  if (!ns || !tiles || n_species < 1 || n_rank < 1) {
+    set_error("plan tiles: NULL argument or no species / ranks");
+    return FD3_ERR_ARG;
+  }
+  FftPlan p;
+  const int rc = fft_plan_make(ns, n_species, n_rank, &p);
+  if (rc) return rc;
+  tiles[0] = p.TY; tiles[1] = p.TX; tiles[2] = p.RT; tiles[3] = p.KT;
+  return FD3_OK;
+}
+
+}  // extern "C"

@@ -248,6 +248,18 @@ int fd3_batch_run(fd3_batch* plan, const void* positions, int pos_dtype, const i
 int fd3_batch_info(const fd3_batch* plan, int32_t* merged_launches, int32_t* box_launches);
 void fd3_batch_destroy(fd3_batch* plan);
 
+/* ---- tile tuning (no counterpart in the reference) ----
+ * Tile shapes of the mesh passes per (mesh ns, species, rank): tiles[0] = y
+ * lines per z-pass block, tiles[1] = x lines per y-pass block, tiles[2] =
+ * ranks per k-space tile, tiles[3] = k-space tiles per block; 0 keeps the
+ * plan's own choice, and a value outside a kernel's limits is ignored.
+ * fd3_tune_set records an override (the package loads the autotuner's
+ * measured table at import), fd3_tune_clear drops them all, fd3_plan_tiles
+ * reports the tiles a plan would use.  Results do not depend on the tiles. */
+int fd3_tune_set(const int32_t ns[3], int32_t n_species, int32_t n_rank, const int32_t tiles[4]);
+int fd3_tune_clear(void);
+int fd3_plan_tiles(const int32_t ns[3], int32_t n_species, int32_t n_rank, int32_t tiles[4]);
+
 /* fd3_eval_celllist_noalloc: fd3_eval_celllist with no workspace argument
  *   and no allocation of any kind.  No counterpart in the reference: it is for
  *   callers that cannot allocate device memory (code under HIP graph capture,

@@ -41,6 +41,7 @@ EXPORTS = (
     "fd3_eval_celllist_noalloc",
     "fd3_batch_create", "fd3_batch_workspace", "fd3_batch_run", "fd3_batch_info", "fd3_batch_destroy",
     "fd3_tune_set", "fd3_tune_clear", "fd3_plan_tiles",
+    "fd3_eval_direct_workspace", "fd3_eval_direct",
     "fd3_neighbours_workspace", "fd3_neighbours_count", "fd3_neighbours_fill",
     "fd3_spread", "fd3_gather",
     "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
@@ -122,6 +123,10 @@ _SIG = {
     "fd3_batch_run": (C.c_int, [C.c_void_p, _P, C.c_int, _P, _P, C.c_size_t, C.POINTER(Outputs), _P]),
     "fd3_batch_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "fd3_batch_destroy": (None, [C.c_void_p]),
+    "fd3_eval_direct_workspace": (C.c_int, [C.POINTER(Model), _P, C.c_double, C.c_int64, C.c_int,
+                                            C.POINTER(C.c_size_t)]),
+    "fd3_eval_direct": (C.c_int, [C.POINTER(Model), _P, C.c_double, C.c_int64, _P, C.c_int, _P, C.c_int, _P,
+                                  C.c_size_t, C.POINTER(Outputs), _P]),
     "fd3_tune_set": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),
     "fd3_tune_clear": (C.c_int, []),
     "fd3_plan_tiles": (C.c_int, [C.POINTER(C.c_int32), C.c_int32, C.c_int32, C.POINTER(C.c_int32)]),

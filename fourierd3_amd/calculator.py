@@ -75,7 +75,7 @@ class FourierD3ASECalculator(_Base):
                  device=torch.device("cuda" if torch.cuda.is_available() else "cpu"), verbose=False,
                  params=None, c6tol: float = 1, xcfunc: str = "pbe", cnfunc="smooth_cut",
                  k_cutoff: float = 10.0, mesh_spacing: float = 1.2, interpolation_nodes: int = 5,
-                 dtype=torch.float32, **kwargs):
+                 dtype=torch.float32, r_direct: float = 20.0, **kwargs):
         super().__init__(**kwargs)
         check_dtype(dtype)  # float32 kernels; dtype=torch.float64 raises NotImplementedError
         self.device = torch.device(device) if device is not None else torch.device(
@@ -93,6 +93,7 @@ class FourierD3ASECalculator(_Base):
         self.mesh_spacing = mesh_spacing
         self.interpolation_nodes = interpolation_nodes
         self.dtype = dtype
+        self.r_direct = float(r_direct)  # method='direct' (extension): real-space image radius, Angstrom
         self._model = None
         self._ws = None
         self._numbers = None
@@ -142,7 +143,7 @@ class FourierD3ASECalculator(_Base):
             pbc=torch.as_tensor(np.asarray(atoms.pbc)), mesh_spacing=self.mesh_spacing, params=self.params,
             c6tol=self.c6tol, xcfunc=self.xcfunc, cnfunc=self.cnfunc, device=self.device, method=self.method,
             interpolation_nodes=self.interpolation_nodes, k_cutoff=self.k_cutoff, verbose=self.verbose,
-            r_cut=self.r_cut, dtype=self.dtype)
+            r_cut=self.r_cut, dtype=self.dtype, r_direct=self.r_direct)
         self._numbers = np.asarray(atoms.numbers).copy()
 
     # ------------------------------------------------------------------
@@ -157,6 +158,8 @@ class FourierD3ASECalculator(_Base):
         if self.cnfunc == "d4":
             cell_t = torch.as_tensor(np.asarray(cell_A, dtype=np.float64))
             return model._run_d4(positions, cell_t, need_grad or need_params, need_params)
+        if self.method == "direct":
+            return model._run_direct(positions, np.asarray(cell_A, dtype=np.float64), need_grad, need_params, out)
         n = positions.shape[0]
         g = model._geom(np.asarray(cell_A, dtype=np.float64))
         m = model._model_struct()

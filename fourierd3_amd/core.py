@@ -103,16 +103,19 @@ class _EdgesEnergy(torch.autograd.Function):
 
 
 class _D4Energy(torch.autograd.Function):
-    """E(positions, cell, params) of cnfunc='d4' with gradients from the adjoint
-    kernels.  The library reports dE/dpositions and the strain derivative
-    sigma = dE/d eps (cell' = cell + eps cell, r' = r + eps r); the cell
-    gradient at fixed positions follows as G = cell^-T (sigma - g^T r)^T."""
+    """E(positions, cell, params) of the positions-only entry points (cnfunc='d4'
+    with runner 'd4', the real-space sum with runner 'direct') with gradients
+    from the adjoint kernels.  The library reports dE/dpositions and the
+    strain derivative sigma = dE/d eps (cell' = cell + eps cell, r' = r + eps
+    r); the cell gradient at fixed positions follows as
+    G = cell^-T (sigma - g^T r)^T."""
 
     @staticmethod
-    def forward(ctx, positions, cell_A, params, model):
+    def forward(ctx, positions, cell_A, params, model, runner="d4"):
         need_grad = any(ctx.needs_input_grad[:2])
         need_params = bool(ctx.needs_input_grad[2])
-        res = model._run_d4(positions, cell_A, need_grad or need_params, need_params)
+        run = model._run_direct if runner == "direct" else model._run_d4
+        res = run(positions, cell_A, need_grad or need_params, need_params)
         gcell = torch.empty(0, dtype=torch.float64, device=res["energy"].device)
         if need_grad:
             pos = positions.detach().to(torch.float64)
@@ -134,14 +137,18 @@ class _D4Energy(torch.autograd.Function):
         out_pos = (gpos * g64).to(dp) if ctx.needs_input_grad[0] else None
         out_cell = (gcell * g64).to(dc).to(ctx.cell_device) if ctx.needs_input_grad[1] else None
         out_par = (gpar.to(g64.device) * g64).to(dpr).to(ctx.param_device) if ctx.needs_input_grad[2] else None
-        return out_pos, out_cell, out_par, None
+        return out_pos, out_cell, out_par, None, None
 
 
 class FourierD3(torch.nn.Module):
     """DFT-D3(BJ) dispersion energy by reciprocal-space summation (SPME/PME).
 
     Args mirror ``src/fourierd3/core.py:50-68``.  ``method`` is 'spme'
-    (P3M B-splines, 1..5 nodes), 'pme' (Lagrange, 2..7 nodes) or 'ewald';
+    (P3M B-splines, 1..5 nodes), 'pme' (Lagrange, 2..7 nodes), 'ewald', or
+    (extension) 'direct': the same pair energy summed in real space over
+    lattice images within ``r_direct`` Angstrom plus the continuum tail, in
+    double (``fd3_eval_direct``; smooth-cut CN; cost ~ N^2 x images, for
+    checks and small boxes);
     ``cnfunc`` is 'smooth_cut' or 'd4' (D4 CN in reciprocal space).  The C6 reference table is the synthetic stand-in of
     ``fourierd3_amd.tables`` unless ``FD3_C6REF`` points at a real one.
     ``ns_mesh`` (extension) overrides the mesh rule.
@@ -153,7 +160,7 @@ class FourierD3(torch.nn.Module):
                  dtype=torch.float32, method: str = "spme", params=None, cnfunc="smooth_cut",
                  interpolation_nodes: int = 5, k_cutoff: float = 10.0, r_cut: float = 6.0,
                  cncorr: Optional[torch.Tensor] = None, verbose=True, ns_mesh=None,
-                 npt_reach: Optional[float] = None) -> None:
+                 npt_reach: Optional[float] = None, r_direct: float = 20.0) -> None:
         super().__init__()
         check_dtype(dtype)
         device = torch.device(device)
@@ -169,10 +176,14 @@ class FourierD3(torch.nn.Module):
             assert bool(torch.as_tensor(pbc).all()), "particle-mesh only supports 3d pbc"
         if verbose:
             print("Assuming 3D PBC are satisfied")
-        if method not in ("spme", "pme", "ewald"):
-            raise ValueError(f"unknown method '{method}'; expected 'spme', 'pme' or 'ewald'")
+        if method not in ("spme", "pme", "ewald", "direct"):
+            raise ValueError(f"unknown method '{method}'; expected 'spme', 'pme', 'ewald' or 'direct'")
         if cnfunc not in ("smooth_cut", "d4"):
             raise ValueError(f"Unknown cnfunc '{cnfunc}'. Expected 'smooth_cut' or 'd4'.")
+        if method == "direct" and (cnfunc != "smooth_cut" or npt_reach is not None):
+            raise NotImplementedError("method='direct' evaluates cnfunc='smooth_cut' on a host cell "
+                                      "(no npt_reach)")
+        self.r_direct = float(r_direct)
 
         z = np.asarray(species, dtype=np.int64)
         uniq = np.unique(z)                                         # core.py:104
@@ -241,7 +252,9 @@ class FourierD3(torch.nn.Module):
             self.ns_mesh = T.get_ns_mesh(cell_np * A2B, mesh_spacing * A2B)   # core.py:186
         if verbose:
             print("Using mesh size", self.ns_mesh)
-        self._interp = {"spme": L.FD3_INTERP_P3M, "pme": L.FD3_INTERP_LAGRANGE, "ewald": L.FD3_INTERP_EWALD}[method]
+        # ('direct' plans no mesh; its geometry record is never passed on)
+        self._interp = {"spme": L.FD3_INTERP_P3M, "pme": L.FD3_INTERP_LAGRANGE, "ewald": L.FD3_INTERP_EWALD,
+                        "direct": L.FD3_INTERP_P3M}[method]
         self._ws = None
         # NPT: with npt_reach set (mesh methods), forward reads the cell from
         # the device (no host copy in _update_cell, so a step can be captured
@@ -422,13 +435,59 @@ class FourierD3(torch.nn.Module):
             L.check(L.lib().fd3_eval_d4(C.byref(m), C.byref(d4), C.byref(g), *rest))
         return res
 
+    def _run_direct(self, positions, cell_A, need_grad: bool, need_params: bool, out: dict | None = None):
+        """Real-space lattice sum (``fd3_eval_direct``) of positions (N, 3)
+        Angstrom on the device for the cell ``cell_A`` (3, 3) Angstrom."""
+        dev = self.device
+        pos = positions.detach()
+        if pos.device != dev:
+            raise ValueError(f"positions are on {pos.device}, the model on {dev}")
+        pos = pos.contiguous()
+        if pos.dim() != 2 or pos.shape[1] != 3:
+            raise ValueError(f"positions must be (n_atoms, 3), got {tuple(pos.shape)}")
+        n = pos.shape[0]
+        if n != self._species_i32.shape[0]:
+            raise ValueError(f"{n} positions for a model of {self._species_i32.shape[0]} atoms")
+        if pos.dtype not in (torch.float32, torch.float64):
+            pos = pos.double()
+        m = self._model_struct()
+        cell = L.cell_array(torch.as_tensor(cell_A).detach().to(torch.float64).cpu().numpy())
+        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        nbytes = C.c_size_t()
+        L.check(L.lib().fd3_eval_direct_workspace(C.byref(m), cell, self.r_direct, n, flags, C.byref(nbytes)))
+        ws = self._workspace(nbytes.value)
+        f64 = dict(dtype=torch.float64, device=dev)
+        res = out if out is not None else {
+            "energy": torch.empty(1, **f64),
+            "grad_pos": torch.empty((n, 3), **f64) if need_grad else torch.empty(0, **f64),
+            "virial": torch.empty((3, 3), **f64) if need_grad else torch.empty(0, **f64),
+            "grad_params": torch.empty(4, **f64) if need_params else torch.empty(0, **f64),
+        }
+        o = L.Outputs()
+        o.energy = L.ptr(res["energy"])
+        if "cn" in res:
+            o.cn = L.ptr(res["cn"])
+        if need_grad:
+            o.grad_pos = L.ptr(res["grad_pos"])
+            o.virial = L.ptr(res["virial"])
+        if need_params:
+            o.grad_params = L.ptr(res["grad_params"])
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        L.check(L.lib().fd3_eval_direct(C.byref(m), cell, self.r_direct, n, L.ptr(pos), L.dtype_code(pos),
+                                        L.ptr(self._species_i32), flags, L.ptr(ws), ws.numel(), C.byref(o),
+                                        stream))
+        return res
+
     def forward(self, positions: torch.Tensor, edge_index: torch.Tensor = None,
                 shifts: torch.Tensor = None) -> torch.Tensor:
         """Energy in Hartree of ``positions`` (Angstrom) with the CN neighbour
         list ``edge_index`` (2, E) and lattice ``shifts`` (E, 3) in Angstrom
-        (``src/fourierd3/core.py:480-583``); cnfunc='d4' needs positions only."""
+        (``src/fourierd3/core.py:480-583``); cnfunc='d4' and method='direct'
+        need positions only."""
         if self.cnfunc == "d4":
             return _D4Energy.apply(positions, self._cell_A, self.params, self)
+        if self.method == "direct":   # real-space sum: positions only, CN from its own cell list
+            return _D4Energy.apply(positions, self._cell_A, self.params, self, "direct")
         if edge_index is None or shifts is None:
             raise ValueError("cnfunc='smooth_cut' needs edge_index and shifts")
         return _EdgesEnergy.apply(positions, shifts, self._cell_A, self.params, self, edge_index)

@@ -14,6 +14,7 @@
 #include "ewald.h"
 #include "fft_plan.h"
 #include "fused.h"
+#include "direct.h"
 #include "launch.h"
 #include "mesh_kernels.h"
 #include "neigh_kernels.h"
@@ -132,14 +133,15 @@ int prof_report(int max_entries, char* names, double* total_ms, int64_t* counts,
   return FD3_OK;
 }
 
-int make_geom(const fd3_geom* g, Geom* out) {
-  if (!g) {
-    set_error("geometry is NULL");
+// Cell part of a geometry: Bohr cell, inverse, reciprocal rows, volume.
+static int make_cell(const double cell_A[9], Geom* out) {
+  if (!cell_A) {
+    set_error("cell is NULL");
     return FD3_ERR_ARG;
   }
   double c[9];
   for (int i = 0; i < 9; ++i) {
-    c[i] = g->cell[i] * kA2B;
+    c[i] = cell_A[i] * kA2B;
     if (!std::isfinite(c[i])) {
       set_error("cell entry %d is not finite", i);
       return FD3_ERR_ARG;
@@ -169,6 +171,16 @@ int make_geom(const fd3_geom* g, Geom* out) {
     for (int d = 0; d < 3; ++d) out->recip[i * 3 + d] = 2.0 * kPi * inv[d * 3 + i];
   out->volume = fabs(det);
   out->dyn = nullptr;
+  return FD3_OK;
+}
+
+int make_geom(const fd3_geom* g, Geom* out) {
+  if (!g) {
+    set_error("geometry is NULL");
+    return FD3_ERR_ARG;
+  }
+  int rc;
+  if ((rc = make_cell(g->cell, out))) return rc;
   out->nx = g->ns[0];
   out->ny = g->ns[1];
   out->nz = g->ns[2];
@@ -1176,6 +1188,137 @@ int fd3_eval_celllist_npt(const fd3_model* m, const fd3_geom* g, double reach, c
   }
   return eval_celllist(m, g, reach, cell, status, n, positions, pos_dtype, species, flags, workspace, ws_bytes, out,
                        (hipStream_t)stream);
+}
+
+// ------------------------------------------------------ real-space sum ---
+// The pair energy summed over lattice images within r_max (direct.hip), with
+// the cell-list CN and the same CN -> C6 chain and adjoint as the mesh path.
+constexpr int64_t kDirectMaxImages = (int64_t)1 << 22;
+
+static int direct_layout(const fd3_model* m, const double cell_A[9], double r_max_A, int64_t n, Common& c,
+                         CellGrid& cg, CellBuffers& cb, Arena& ar, DirectArgs& da) {
+  int rc;
+  if ((rc = check_model(m))) return rc;
+  if ((rc = make_cell(cell_A, &c.geom))) return rc;
+  if (n < 0) {
+    set_error("negative atom count");
+    return FD3_ERR_ARG;
+  }
+  if (!(r_max_A > 0.0) || !std::isfinite(r_max_A)) {
+    set_error("r_max must be positive and finite, got %g", r_max_A);
+    return FD3_ERR_ARG;
+  }
+  da = DirectArgs{};
+  da.rmax = r_max_A * kA2B;
+  direct_images(c.geom.cell, c.geom.inv, da.rmax, da.nimg);
+  const int64_t ncube = (int64_t)(2 * da.nimg[0] + 1) * (2 * da.nimg[1] + 1) * (2 * da.nimg[2] + 1);
+  if (ncube > kDirectMaxImages) {
+    set_error("r_max %g A needs %lld lattice images for this cell (at most %lld)", r_max_A, (long long)ncube,
+              (long long)kDirectMaxImages);
+    return FD3_ERR_UNSUPPORTED;
+  }
+  if ((rc = cell_grid_make(cell_A, m->r_cut, &cg, 1.0))) return rc;
+  const int64_t n1 = std::max<int64_t>(n, 1);
+  const int r = m->n_rank, nS = m->n_species;
+  c.cn_raw = ar.take<double>(n1);
+  c.charges = ar.take<float>(n1 * r);
+  c.dEdcn = ar.take<double>(n1);
+  c.gpos = ar.take<double>(3 * n1);
+  c.n_c6 = c6_blocks(n);
+  c.p_c6 = ar.take<double>((size_t)std::max(c.n_c6, 1) * 8);
+  c.n_cn = std::max(cg.nbins, 1);
+  c.p_cn = ar.take<double>((size_t)c.n_cn * 8);
+  da.tab = ar.take<double>((size_t)4 * nS * nS);
+  da.qsum = ar.take<double>((size_t)nS * r);
+  da.rows = ar.take<double>((size_t)16 * n1);
+  da.dEdq = ar.take<double>((size_t)r * n1);
+  cell_carve(cg, n1, ar, &cb);
+  return FD3_OK;
+}
+
+int fd3_eval_direct_workspace(const fd3_model* m, const double cell[9], double r_max, int64_t n_atoms, int flags,
+                              size_t* bytes) {
+  (void)flags;
+  if (!bytes) {
+    set_error("bytes is NULL");
+    return FD3_ERR_ARG;
+  }
+  Common c;
+  CellGrid cg;
+  CellBuffers cb;
+  Arena ar;
+  DirectArgs da;
+  const int rc = direct_layout(m, cell, r_max, n_atoms, c, cg, cb, ar, da);
+  if (rc) return rc;
+  *bytes = align_up(ar.off);
+  return FD3_OK;
+}
+
+int fd3_eval_direct(const fd3_model* m, const double cell[9], double r_max, int64_t n, const void* positions,
+                    int pos_dtype, const int32_t* species, int flags, void* workspace, size_t ws_bytes,
+                    const fd3_outputs* out, void* stream) {
+  const hipStream_t s = (hipStream_t)stream;
+  Common c;
+  CellGrid cg;
+  CellBuffers cb;
+  Arena ar(workspace, ws_bytes);
+  DirectArgs da;
+  int rc = direct_layout(m, cell, r_max, n, c, cg, cb, ar, da);
+  if (rc) return rc;
+  if (!ar.ok() || !workspace) {
+    set_error("workspace of %zu bytes is smaller than the %zu required", ws_bytes, align_up(ar.off));
+    return FD3_ERR_WORKSPACE;
+  }
+  if (!out || !out->energy) {
+    set_error("outputs or the energy pointer are NULL");
+    return FD3_ERR_ARG;
+  }
+  if (n > 0 && (!positions || !species)) {
+    set_error("input pointer is NULL");
+    return FD3_ERR_ARG;
+  }
+  const bool grad = flags & FD3_NEED_GRAD;
+  if ((rc = cell_build(cg, cb, n, positions, pos_dtype, species, s))) return rc;
+  const CellArgs ca = cell_args(cg, cb, n);
+  CnArgs cna{};
+  cna.rcov = m->rcov; cna.rcut = m->r_cut * kA2B; cna.cn = c.cn_raw; cna.dEdcn = c.dEdcn;
+  cna.gpos = c.gpos; cna.partials = c.p_cn;
+  if ((rc = launch_cn_cells(ca, cna, cg.nbins, s))) return rc;
+  const int r = m->n_rank, nS = m->n_species;
+  if (n > 0) {
+    C6Args c6a;
+    c6a.n = n; c6a.nS = nS; c6a.r = r; c6a.cn_raw = c.cn_raw; c6a.spec_s = cb.spec_s; c6a.slot_atom = cb.slot_atom;
+    c6a.cnref = m->cnref; c6a.vq = m->vq; c6a.eigs = m->eigs; c6a.qq = m->qq; c6a.cncorr = m->cncorr;
+    c6a.s6 = m->s6; c6a.s8 = m->s8; c6a.a1 = m->a1; c6a.a2 = m->a2;
+    c6a.charges = c.charges; c6a.cn_out = out->cn; c6a.partials = c.p_c6;
+    if ((rc = launch_c6_fwd(c6a, s))) return rc;
+  }
+  da.n = n; da.nS = nS; da.r = r;
+  for (int i = 0; i < 9; ++i) da.cell[i] = c.geom.cell[i];
+  da.volume = c.geom.volume;
+  da.pos4 = cb.pos4; da.spec_s = cb.spec_s; da.charges = c.charges; da.eigs = m->eigs; da.qq = m->qq;
+  da.s6 = m->s6; da.s8 = m->s8; da.a1 = m->a1; da.a2 = m->a2;
+  da.gpos = c.gpos;
+  if ((rc = launch_direct(da, s))) return rc;
+  if (grad && n > 0) {
+    C6BwdArgs ba{};
+    ba.n = n; ba.nS = nS; ba.r = r; ba.nx = 1; ba.ny = 1; ba.nz = 1;
+    for (int i = 0; i < 9; ++i) ba.inv[i] = c.geom.inv[i];
+    ba.pos4 = cb.pos4; ba.spec_s = cb.spec_s; ba.charges = c.charges;
+    ba.cn_raw = c.cn_raw; ba.cnref = m->cnref; ba.vq = m->vq; ba.eigs = m->eigs; ba.qq = m->qq;
+    ba.cncorr = m->cncorr; ba.s6 = m->s6; ba.s8 = m->s8; ba.a1 = m->a1; ba.a2 = m->a2;
+    ba.gpos = c.gpos; ba.dEdcn = c.dEdcn; ba.partials = nullptr; ba.dEdq = da.dEdq;
+    if ((rc = launch_c6_bwd(ba, s))) return rc;
+    // the CN backward adds the CN term of dE/dr and writes grad_pos (atom order, per Angstrom)
+    cna.gout = out->grad_pos; cna.slot_atom = cb.slot_atom;
+    if ((rc = launch_cn_bwd_cells(ca, cna, cg.nbins, s))) return rc;
+  }
+  da.cnv = (grad && n > 0) ? c.p_cn : nullptr;
+  da.ncnv = c.n_cn;
+  da.energy = out->energy;
+  da.virial = grad ? out->virial : nullptr;
+  da.grad_params = (flags & FD3_NEED_PARAMS) ? out->grad_params : nullptr;
+  return launch_direct_finalize(da, s);
 }
 
 // ---------------------------------------------------------- batched path ---

@@ -67,6 +67,9 @@ struct C6BwdArgs {
   double* gpos;      // (n, 3) out: mesh part of dE/dr, Hartree/Bohr
   double* dEdcn;     // (n) out
   double* partials;  // (c6_blocks(n), 16): sum_i f_i (x) dE/dr_i
+  // When set, (n, r) dE/dc of the real-space sum: replaces accA, the self
+  // term and k0; gpos and partials are then left alone (accG unused).
+  const double* dEdq = nullptr;
 };
 int launch_c6_bwd(const C6BwdArgs& a, hipStream_t s);
 

@@ -406,16 +406,18 @@ __device__ __forceinline__ void c6_bwd_body(const BlockInfo& bk, const C6BwdArgs
   if (s < a.n) {
     const int X = a.spec_s[s];
     const double ns[3] = {(double)a.nx, (double)a.ny, (double)a.nz};
-    const double4 r = a.pos4[s];
-    double gr[3], f[3];
-    for (int d = 0; d < 3; ++d) f[d] = r.x * inv[0 * 3 + d] + r.y * inv[1 * 3 + d] + r.z * inv[2 * 3 + d];
-    for (int b = 0; b < 3; ++b) {
-      gr[b] = 0.0;
-      for (int d = 0; d < 3; ++d) gr[b] += (double)a.accG[3 * s + d] * ns[d] * inv[b * 3 + d];
-      a.gpos[3 * s + b] = gr[b];
+    if (!a.dEdq) {
+      const double4 r = a.pos4[s];
+      double gr[3], f[3];
+      for (int d = 0; d < 3; ++d) f[d] = r.x * inv[0 * 3 + d] + r.y * inv[1 * 3 + d] + r.z * inv[2 * 3 + d];
+      for (int b = 0; b < 3; ++b) {
+        gr[b] = 0.0;
+        for (int d = 0; d < 3; ++d) gr[b] += (double)a.accG[3 * s + d] * ns[d] * inv[b * 3 + d];
+        a.gpos[3 * s + b] = gr[b];
+      }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) part[i * 3 + j] = f[i] * gr[j];
     }
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) part[i * 3 + j] = f[i] * gr[j];
     const float qqx = a.qq[X * a.nS + X];
     const double R = a.a1 * sqrt(3.0 * (double)qqx) + a.a2;
     const double R2 = R * R, R6 = R2 * R2 * R2;
@@ -431,8 +433,13 @@ __device__ __forceinline__ void c6_bwd_body(const BlockInfo& bk, const C6BwdArgs
     double gp[kNRef] = {0, 0, 0, 0, 0, 0, 0};
     for (int q = 0; q < a.r; ++q) {
       // with the charges the mesh and the self term used (float-rounded)
-      double dc = (double)a.accA[s * a.r + q] + (double)a.eigs[q] * (double)a.charges[s * a.r + q] * phi0;
-      if (a.k0) dc += (double)a.k0[X * a.r + q].x;
+      double dc;
+      if (a.dEdq) {
+        dc = a.dEdq[s * a.r + q];
+      } else {
+        dc = (double)a.accA[s * a.r + q] + (double)a.eigs[q] * (double)a.charges[s * a.r + q] * phi0;
+        if (a.k0) dc += (double)a.k0[X * a.r + q].x;
+      }
       for (int pp = 0; pp < kNRef; ++pp) gp[pp] += dc * (double)a.vq[(X * kNRef + pp) * a.r + q];
     }
     double gbar = 0.0;
@@ -444,7 +451,7 @@ __device__ __forceinline__ void c6_bwd_body(const BlockInfo& bk, const C6BwdArgs
     }
     a.dEdcn[s] = dcn * ca;
   }
-  block_sum_store<9>(part, red, a.partials + (size_t)bk.id.x * 16);
+  if (a.partials) block_sum_store<9>(part, red, a.partials + (size_t)bk.id.x * 16);
 }
 
 int launch_c6_bwd(const C6BwdArgs& a, hipStream_t s) {

@@ -159,6 +159,26 @@ int fd3_eval_celllist_npt(const fd3_model* m, const fd3_geom* g, double reach, c
                           const int32_t* species, int flags, void* workspace, size_t workspace_bytes,
                           const fd3_outputs* out, void* stream);
 
+/* fd3_eval_direct: real-space lattice sum of the same pair energy (no
+ *   counterpart in the reference, whose energy is reciprocal-space plus the
+ *   self term only, src/fourierd3/core.py:553-581).  For checking the mesh
+ *   methods and for small boxes:
+ *     E = -1/2 sum_{i,j,n}' C6_ij [s6/(r^6+R0^6) + 3 s8 QQ_ij/(r^8+R0^8)]
+ *   over lattice images with r = |r_j + n.cell - r_i| < r_max (Angstrom),
+ *   C6_ij = sum_q eigs_q c_iq c_jq from the cell-list CN (as
+ *   fd3_eval_celllist), R0_ij = a1 sqrt(3 QQ_ij) + a2, plus the continuum
+ *   tail beyond r_max at the mean density 1/Omega.  cell: (3,3) Angstrom,
+ *   rows = lattice vectors.  Energy; with FD3_NEED_GRAD grad_pos and virial;
+ *   with FD3_NEED_PARAMS grad_params; cn if set (grad_shifts and grad_cell
+ *   are not produced).  The image cube, 2 ceil(r_max / d_a) + 3 per axis
+ *   with d_a the plane spacings, is limited to 2^22 images.  Double
+ *   precision throughout; the cost grows as N^2 times the images. */
+int fd3_eval_direct_workspace(const fd3_model* m, const double cell[9], double r_max, int64_t n_atoms, int flags,
+                              size_t* bytes);
+int fd3_eval_direct(const fd3_model* m, const double cell[9], double r_max, int64_t n_atoms, const void* positions,
+                    int pos_dtype, const int32_t* species, int flags, void* workspace, size_t workspace_bytes,
+                    const fd3_outputs* out, void* stream);
+
 /* fd3_eval_d4: FourierD3ASECalculator.calculate / FourierD3.forward(positions)
  *   with cnfunc='d4' and method='spme'/'pme' (src/fourierd3/core.py:389-447,
  *   480-583; src/fourierd3/calculator.py:169-171): the coordination numbers

@@ -21,6 +21,7 @@
 
 #include "fd3_internal.h"
 #include "fft_lds.h"
+#include "fft_reg.h"
 #include "fused.h"
 #include "launch.h"
 #include "mesh_kernels.h"
@@ -179,6 +180,152 @@ __device__ __forceinline__ void lds_add_f32_cas(float* line, const int* z, const
   }
 }
 
+// ------------------------------------- two-pass z lines for nz = 256 ---
+// With nz = 256 (M = 128 complex points per packed real line) and TY = 16
+// lines, the z FFT runs as 128 = 8 x 16: pass A, one thread per (line, n2),
+// n2 < 16, does an 8-point FFT of x[16 n1 + n2] in registers and multiplies
+// by W128^(n2 k1); pass B, one thread per (line, k1), k1 < 8, does a
+// 16-point FFT of those values in registers, giving X[k1 + 8 k2].  The
+// values cross between the passes once through LDS, instead of once per
+// radix-4 stage, and the R2C split (or C2R merge) happens in registers on
+// the way to (or from) global memory.  Lines sit at stride kZ2Ls = 130
+// complex (2 * 130 = 4 mod 64 banks), and the exchange slot of (k1, n2) is
+// 16 k1 + ((n2 + k1) mod 16): with the lines as the fastest thread index,
+// every pass touches 64 distinct banks per 32 lanes.
+constexpr int kZ2Ls = 130;
+
+__device__ __forceinline__ int z2_slot(int k1, int n2) { return 16 * k1 + ((n2 + k1) & 15); }
+
+// k1 of lane group kk = t / 16 in pass B: k1 and 8 - k1 (the R2C partners)
+// share a wave (groups 2j and 2j + 1; k1 = 0 and 4 are their own partners).
+__device__ __forceinline__ int z2_k1(int kk) { return (0x53627140 >> (4 * kk)) & 15; }
+
+// Forward: the 16 spread lines in LDS -> half spectrum rows kz = 0..M of
+// dst (row kz at dst + kz * ny, 16 contiguous y).  256 threads.
+__device__ __forceinline__ void z2_forward_store(float2* lds, const float2* tws, float2* __restrict__ dst, int ny) {
+  const int t = threadIdx.x;
+  {
+    const int l = t & 15, n2 = t >> 4;
+    float2* L = lds + l * kZ2Ls;
+    float2 v[8];
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) v[n1] = L[16 * n1 + n2];
+    fft8r<-1>(v);
+    // W128^(n2 k1) from W128^n2, ^2n2, ^4n2 (three table reads, at most two
+    // products each: fewer registers than seven reads in flight)
+    {
+      const float2 w1 = tws[(2 * n2) & 255], w2 = tws[(4 * n2) & 255], w4 = tws[(8 * n2) & 255];
+      const float2 w3 = cmul(w1, w2);
+      v[1] = cmul(v[1], w1);
+      v[2] = cmul(v[2], w2);
+      v[3] = cmul(v[3], w3);
+      v[4] = cmul(v[4], w4);
+      v[5] = cmul(v[5], cmul(w4, w1));
+      v[6] = cmul(v[6], cmul(w4, w2));
+      v[7] = cmul(v[7], cmul(w4, w3));
+    }
+    __syncthreads();  // every pass-A read is done
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) L[z2_slot(k1, n2)] = v[k1];
+  }
+  __syncthreads();
+  if (t < 128) {  // waves 0 and 1
+    const int l = t & 15, kk = t >> 4, k1 = z2_k1(kk);
+    const float2* L = lds + l * kZ2Ls;
+    float2 z[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) z[n2] = L[z2_slot(k1, n2)];
+    fft16r<-1>(z);  // z[k2] = Z[k1 + 8 k2]
+    // Z[M - k] = Z[(8 - k1) + 8 (15 - k2)]: the partner lane's z[15 - k2]
+    // (k1 = 4: its own), or z[16 - k2] for k1 = 0
+    const int src = (l + 16 * (kk < 2 ? kk : (kk ^ 1))) & 63;
+    const float2 w1 = tws[k1];  // W256^k1
+    static_for<16>([&](auto kc) {
+      constexpr int k2 = decltype(kc)::value;
+      float2 zm = make_float2(__shfl(z[15 - k2].x, src), __shfl(z[15 - k2].y, src));
+      if (k1 == 0) zm = z[(16 - k2) & 15];
+      const float2 zk = z[k2];
+      const int k = k1 + 8 * k2;
+      if (k2 == 0 && k1 == 0) {
+        dst[l] = make_float2(zk.x + zk.y, 0.0f);
+        dst[(size_t)128 * ny + l] = make_float2(zk.x - zk.y, 0.0f);
+        return;
+      }
+      // X_k = E_k + W256^k O_k, E_k = (Z_k + conj Z_{M-k}) / 2, O_k = -i (Z_k - conj Z_{M-k}) / 2
+      const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+      const float2 O = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+      const float2 T = cmul(tw_mul<32, k2, -1>(w1), O);
+      dst[(size_t)k * ny + l] = make_float2(E.x + T.x, E.y + T.y);
+    });
+  }
+}
+
+// Inverse: half spectrum rows kz = 0..M of src -> the 16 real lines in LDS
+// (unnormalised, as c2r_merge + lds_fft<1>).  256 threads; ends with a barrier.
+__device__ __forceinline__ void z2_load_inverse(float2* lds, const float2* tws, const float2* __restrict__ src,
+                                                int ny) {
+  const int t = threadIdx.x;
+  if (t < 128) {
+    const int l = t & 15, k1 = t >> 4;
+    const float2 w1 = conjf2(tws[k1]);  // W256^-k1 (inverse sign)
+    float2 z[16];
+    // four rounds of 8 loads (X_k and X_{M-k} of four k): all 32 at once
+    // would hold 64 VGPRs beside z and halve the occupancy
+    static_for<4>([&](auto hc) {
+      constexpr int h = decltype(hc)::value;
+      float2 xa[4], xb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k1 + 8 * (4 * h + u);
+        xa[u] = src[(size_t)k * ny + l];
+        xb[u] = src[(size_t)(128 - k) * ny + l];
+      }
+      static_for<4>([&](auto uc) {
+        constexpr int u = decltype(uc)::value, k2 = 4 * h + u;
+        const float2 xk = xa[u], xm = xb[u];
+        // Z_k = (X_k + conj X_{M-k}) + i (X_k - conj X_{M-k}) w^k, w = exp(2 pi i / 256)
+        const float2 Ak = make_float2(xk.x + xm.x, xk.y - xm.y);
+        const float2 Bk = cmul(make_float2(xk.x - xm.x, xk.y + xm.y), tw_mul<32, k2, 1>(w1));
+        z[k2] = make_float2(Ak.x - Bk.y, Ak.y + Bk.x);
+        if (k2 == 0 && k1 == 0) z[0] = make_float2(xk.x + xm.x, xk.x - xm.x);  // X_0, X_M real
+      });
+      asm volatile("" ::: "memory");  // keep the rounds apart
+    });
+    fft16r<1>(z);  // over k2 -> n2
+    // W128^-(k1 n2) from W128^k1, ^2k1, ^4k1, ^8k1 (four table reads, at
+    // most three products each)
+    {
+      const float2 w1 = conjf2(tws[(2 * k1) & 255]), w2 = conjf2(tws[(4 * k1) & 255]);
+      const float2 w4 = conjf2(tws[(8 * k1) & 255]), w8 = conjf2(tws[(16 * k1) & 255]);
+      const float2 w3 = cmul(w1, w2);
+      float2 lo[8];
+      lo[0] = make_float2(1.f, 0.f); lo[1] = w1; lo[2] = w2; lo[3] = w3;
+      lo[4] = w4; lo[5] = cmul(w4, w1); lo[6] = cmul(w4, w2); lo[7] = cmul(w4, w3);
+#pragma unroll
+      for (int n2 = 1; n2 < 8; ++n2) z[n2] = cmul(z[n2], lo[n2]);
+      z[8] = cmul(z[8], w8);
+#pragma unroll
+      for (int n2 = 9; n2 < 16; ++n2) z[n2] = cmul(z[n2], cmul(w8, lo[n2 - 8]));
+    }
+    float2* L = lds + l * kZ2Ls;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) L[z2_slot(k1, n2)] = z[n2];
+  }
+  __syncthreads();
+  {
+    const int l = t & 15, n2 = t >> 4;
+    float2* L = lds + l * kZ2Ls;
+    float2 v[8];
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) v[k1] = L[z2_slot(k1, n2)];
+    fft8r<1>(v);  // over k1 -> n1: z[n2 + 16 n1]
+    __syncthreads();  // every read of the exchange slots is done
+#pragma unroll
+    for (int n1 = 0; n1 < 8; ++n1) L[n2 + 16 * n1] = v[n1];
+  }
+  __syncthreads();
+}
+
 // fused z passes: 256-thread blocks of kZLines complex values (measured
 // against 512 threads x 32 lines and 128 x 8); z_tile relies on the two
 // being equal
@@ -193,15 +340,16 @@ struct ZPassArgs {
   const float2* twz;
 };
 
-template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
+// Z2: nz = 256 and TY = 16, two-pass z FFT (z2_forward_store)
+template <int LOG, int P, int MAXB, bool Z2 = false>  // P > 0: compile-time interpolation order
 __device__ __forceinline__ void spread_r2c_z_body(const BlockInfo& bk, const ZPassArgs& za) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
   const FusedArgs& a = za.a;
   float2* __restrict__ A = za.A;
   const float2* __restrict__ twz = za.twz;
-  const int TY = za.TY;
-  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = P > 0 ? P : a.order;
+  const int TY = Z2 ? 16 : za.TY;
+  const int nz = Z2 ? 256 : a.nz, M = nz >> 1, ls = Z2 ? kZ2Ls : M + 1, p = P > 0 ? P : a.order;
   const int y0 = bk.id.x * TY, x = bk.id.y, c = bk.id.z;
   const int X = c / a.r, q = c - X * a.r;
   float2* tws = lds + TY * ls;
@@ -243,24 +391,29 @@ __device__ __forceinline__ void spread_r2c_z_body(const BlockInfo& bk, const ZPa
     }
   }
   __syncthreads();
-  lds_fft<-1, MAXB, (1 << LOG)>(lds, M, TY, ls, tws, 2);
-  z_pairs<true>(lds, TY, ls, M, tws);
-  __syncthreads();
   float2* dst = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
-  const int lT = ilog2(TY), ny = a.ny;
-  tile2_from_lds<4 * MAXB + 1, LOG>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
+  if constexpr (Z2) {
+    z2_forward_store(lds, tws, dst, a.ny);
+  } else {
+    lds_fft<-1, MAXB, (1 << LOG)>(lds, M, TY, ls, tws, 2);
+    z_pairs<true>(lds, TY, ls, M, tws);
+    __syncthreads();
+    const int lT = ilog2(TY), ny = a.ny;
+    tile2_from_lds<4 * MAXB + 1, LOG>(dst, lds, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
+  }
 }
 
 // ------------------------------------------------------ z pass + gather ---
-template <int LOG, int P, int MAXB>  // P > 0: compile-time interpolation order
+// Z2: nz = 256 and TY = 16, two-pass z FFT (z2_load_inverse)
+template <int LOG, int P, int MAXB, bool Z2 = false>  // P > 0: compile-time interpolation order
 __device__ __forceinline__ void c2r_z_gather_body(const BlockInfo& bk, const ZPassArgs& za) {
   extern __shared__ float2 lds[];
   __shared__ Segs sg;
   const FusedArgs& a = za.a;
   const float2* __restrict__ A = za.A;
   const float2* __restrict__ twz = za.twz;
-  const int TY = za.TY;
-  const int nz = a.nz, M = nz >> 1, ls = M + 1, p = P > 0 ? P : a.order;
+  const int TY = Z2 ? 16 : za.TY;
+  const int nz = Z2 ? 256 : a.nz, M = nz >> 1, ls = Z2 ? kZ2Ls : M + 1, p = P > 0 ? P : a.order;
   const int y0 = bk.id.x * TY, x = bk.id.y, c = bk.id.z;
   const int X = c / a.r, q = c - X * a.r;
   float2* tws = lds + TY * ls;
@@ -268,12 +421,18 @@ __device__ __forceinline__ void c2r_z_gather_body(const BlockInfo& bk, const ZPa
   const float2* src = A + (size_t)(c * a.nx + x) * (M + 1) * a.ny + y0;
   const int lT = ilog2(TY), ny = a.ny;
   segs_lookup(&sg, a.cb.colstart, X, x, y0, TY, a.nx, a.ny, p);
-  tile2_to_lds<4 * MAXB + 1, LOG>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
-  __syncthreads();
-  segs_compact(&sg, p);
-  z_pairs<false>(lds, TY, ls, M, tws);
-  __syncthreads();
-  lds_fft<1, MAXB, (1 << LOG)>(lds, M, TY, ls, tws, 2);
+  if constexpr (Z2) {
+    __syncthreads();  // twiddles staged, segment lookups written
+    segs_compact(&sg, p);
+    z2_load_inverse(lds, tws, src, ny);  // its barriers publish sg
+  } else {
+    tile2_to_lds<4 * MAXB + 1, LOG>(lds, src, TileShape{M + 1, lT, 1, 1, ls}, StridedLines{ny});
+    __syncthreads();
+    segs_compact(&sg, p);
+    z_pairs<false>(lds, TY, ls, M, tws);
+    __syncthreads();
+    lds_fft<1, MAXB, (1 << LOG)>(lds, M, TY, ls, tws, 2);
+  }
   const float* ldsf = (const float*)lds;
   const int total = sg.off[sg.n];
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
@@ -322,6 +481,9 @@ __device__ __forceinline__ void c2r_z_gather_body(const BlockInfo& bk, const ZPa
 // y lines per block of the fused z passes: the plan's TY, raised (within the
 // register budget of MAXB = 4) until an atom's p y nodes span at most two
 // tiles, which the gather's two partial-sum slots per x node assume.
+// The two-pass z lines: nz = 256, 16 lines per block of 256 threads.
+static bool z2_fits(const FftPlan& p, int TY) { return p.nz == 256 && TY == 16 && kZLogSpread == 8; }
+
 static int z_tile(const FusedArgs& a, const FftPlan& p, int* maxb) {
   int TY = p.TY;
   while (TY + 1 < a.order && TY < p.ny) TY <<= 1;
@@ -338,11 +500,15 @@ int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float
   int maxb;
   const int TY = z_tile(a, p, &maxb);
   if (TY < 0) return FD3_ERR_UNSUPPORTED;
-  const size_t sh = (size_t)TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
+  const bool z2 = z2_fits(p, TY);
+  const size_t sh = (size_t)TY * (z2 ? kZ2Ls : p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_spread_r2c_z", s);
   const dim3 grid(p.ny / TY, p.nx, p.C);
   constexpr int NT = 1 << kZLogSpread;
   const ZPassArgs za{a, A, TY, t.twz};
+  if (z2 && a.order == 5)
+    return launch<ZPassArgs, spread_r2c_z_body<kZLogSpread, 5, 2, true>, NT>("k_spread_r2c_z", grid, sh, s, za);
+  if (z2) return launch<ZPassArgs, spread_r2c_z_body<kZLogSpread, 0, 2, true>, NT>("k_spread_r2c_z", grid, sh, s, za);
   if (maxb == 4) return launch<ZPassArgs, spread_r2c_z_body<kZLogSpread, 0, 4>, NT>("k_spread_r2c_z", grid, sh, s, za);
   if (a.order == 5)
     return launch<ZPassArgs, spread_r2c_z_body<kZLogSpread, 5, 2>, NT>("k_spread_r2c_z", grid, sh, s, za);
@@ -375,13 +541,18 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
   int maxb;
   const int TY = z_tile(a, p, &maxb);
   if (TY < 0) return FD3_ERR_UNSUPPORTED;
-  const size_t sh = (size_t)TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
+  const bool z2 = z2_fits(p, TY);
+  const size_t sh = (size_t)TY * (z2 ? kZ2Ls : p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
   ProfScope _ps("k_c2r_z_gather", s);
   const dim3 grid(p.ny / TY, p.nx, p.C);
   constexpr int NT = 1 << kZLogGather;
   const ZPassArgs za{a, const_cast<float2*>(A), TY, t.twz};
   int rc;
-  if (maxb == 4) rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 0, 4>, NT>("k_c2r_z_gather", grid, sh, s, za);
+  if (z2 && a.order == 5)
+    rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 5, 2, true>, NT>("k_c2r_z_gather", grid, sh, s, za);
+  else if (z2)
+    rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 0, 2, true>, NT>("k_c2r_z_gather", grid, sh, s, za);
+  else if (maxb == 4) rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 0, 4>, NT>("k_c2r_z_gather", grid, sh, s, za);
   else if (a.order == 5)
     rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 5, 2>, NT>("k_c2r_z_gather", grid, sh, s, za);
   else rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 0, 2>, NT>("k_c2r_z_gather", grid, sh, s, za);

@@ -66,15 +66,16 @@ __global__ void __launch_bounds__(NT) k_many(BatchTable t) {
   BODY(bk, a);
 }
 
-// The same wrappers for bodies held to at most 128 VGPRs (four waves per SIMD).
-template <class P, void (*BODY)(const BlockInfo&, const P&), int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) k_one_w4(P a) {
+// The same wrappers for bodies held to at least WPE waves per SIMD (at most
+// 512 / WPE VGPRs): WPE = 4 (128 VGPRs) or 8 (64 VGPRs).
+template <class P, void (*BODY)(const BlockInfo&, const P&), int NT, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_one_w(P a) {
   const BlockInfo bk{make_uint3(blockIdx.x, blockIdx.y, blockIdx.z), make_uint3(gridDim.x, gridDim.y, gridDim.z)};
   BODY(bk, a);
 }
 
-template <class P, void (*BODY)(const BlockInfo&, const P&), int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 8))) k_many_w4(BatchTable t) {
+template <class P, void (*BODY)(const BlockInfo&, const P&), int NT, int WPE>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPE, 8))) k_many_w(BatchTable t) {
   BlockInfo bk;
   const P a = *batch_locate<P>(t, &bk);
   BODY(bk, a);
@@ -111,12 +112,12 @@ Recorder*& current_recorder();
 // Raise a kernel's dynamic-LDS limit once per kernel and size (above 64 KB).
 int lds_attr(const void* fn, size_t sh);
 
-template <class P, void (*BODY)(const BlockInfo&, const P&), int NT, bool W4>
+template <class P, void (*BODY)(const BlockInfo&, const P&), int NT, int WPE>
 int launch_many_impl(const BatchTable& t, unsigned nblocks, size_t sh, hipStream_t s) {
   int rc;
-  if constexpr (W4) {
-    if ((rc = lds_attr((const void*)k_many_w4<P, BODY, NT>, sh))) return rc;
-    hipLaunchKernelGGL((k_many_w4<P, BODY, NT>), dim3(nblocks), dim3(NT), sh, s, t);
+  if constexpr (WPE > 0) {
+    if ((rc = lds_attr((const void*)k_many_w<P, BODY, NT, WPE>, sh))) return rc;
+    hipLaunchKernelGGL((k_many_w<P, BODY, NT, WPE>), dim3(nblocks), dim3(NT), sh, s, t);
   } else {
     if ((rc = lds_attr((const void*)k_many<P, BODY, NT>, sh))) return rc;
     hipLaunchKernelGGL((k_many<P, BODY, NT>), dim3(nblocks), dim3(NT), sh, s, t);
@@ -125,19 +126,21 @@ int launch_many_impl(const BatchTable& t, unsigned nblocks, size_t sh, hipStream
   return FD3_OK;
 }
 
-// Launch BODY over `grid` blocks of NT threads (or record it).
-template <class P, void (*BODY)(const BlockInfo&, const P&), int NT, bool W4 = false>
+// Launch BODY over `grid` blocks of NT threads (or record it); WPE > 0 asks
+// the compiler for at least that many waves per SIMD.
+template <class P, void (*BODY)(const BlockInfo&, const P&), int NT, int WPE = 0>
 int launch(const char* tag, dim3 grid, size_t sh, hipStream_t s, const P& a) {
+  static_assert(WPE == 0 || WPE == 4 || WPE == 8, "waves per SIMD: 0, 4 or 8");
   const void* one;
-  if constexpr (W4) one = (const void*)k_one_w4<P, BODY, NT>;
+  if constexpr (WPE > 0) one = (const void*)k_one_w<P, BODY, NT, WPE>;
   else one = (const void*)k_one<P, BODY, NT>;
   if (Recorder* r = current_recorder()) {
-    r->add(tag, one, &launch_many_impl<P, BODY, NT, W4>, grid, NT, sh, &a, sizeof(P));
+    r->add(tag, one, &launch_many_impl<P, BODY, NT, WPE>, grid, NT, sh, &a, sizeof(P));
     return FD3_OK;
   }
   int rc;
   if ((rc = lds_attr(one, sh))) return rc;
-  if constexpr (W4) hipLaunchKernelGGL((k_one_w4<P, BODY, NT>), grid, dim3(NT), sh, s, a);
+  if constexpr (WPE > 0) hipLaunchKernelGGL((k_one_w<P, BODY, NT, WPE>), grid, dim3(NT), sh, s, a);
   else hipLaunchKernelGGL((k_one<P, BODY, NT>), grid, dim3(NT), sh, s, a);
   FD3_LAUNCH_CHECK(tag);
   return FD3_OK;

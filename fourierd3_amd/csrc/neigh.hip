@@ -810,16 +810,16 @@ int launch_cn_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s
   ProfScope _ps("k_cn_cells", s);
   const CnBinsArgs ca{c, a};
   if (cn_small_cap(c, nbins))
-    return launch<CnBinsArgs, cn_bins_body<false, kCnCapSmall>, 256, true>("k_cn_cells", dim3(nbins), 0, s, ca);
-  return launch<CnBinsArgs, cn_bins_body<false, kCnCapLarge>, 256, true>("k_cn_cells", dim3(nbins), 0, s, ca);
+    return launch<CnBinsArgs, cn_bins_body<false, kCnCapSmall>, 256, 4>("k_cn_cells", dim3(nbins), 0, s, ca);
+  return launch<CnBinsArgs, cn_bins_body<false, kCnCapLarge>, 256, 4>("k_cn_cells", dim3(nbins), 0, s, ca);
 }
 
 int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   ProfScope _ps("k_cn_bwd_cells", s);
   const CnBinsArgs ca{c, a};
   if (cn_small_cap(c, nbins))
-    return launch<CnBinsArgs, cn_bins_body<true, kCnCapSmall>, 256, true>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
-  return launch<CnBinsArgs, cn_bins_body<true, kCnCapLarge>, 256, true>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
+    return launch<CnBinsArgs, cn_bins_body<true, kCnCapSmall>, 256, 4>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
+  return launch<CnBinsArgs, cn_bins_body<true, kCnCapLarge>, 256, 4>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
 }
 
 int launch_cn_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, hipStream_t s) {

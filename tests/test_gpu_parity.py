@@ -340,19 +340,22 @@ def test_pme_lagrange_matches_oracle(order):
     assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
-@pytest.mark.parametrize("method,order", [("spme", 5), ("pme", 7)])
-def test_tall_box_z_tiles_match_oracle(method, order):
-    """A box stretched 32x along z gets a 1024-point z mesh: the fused z passes
-    then hold few y lines per block, and at p = 7 (Lagrange) they widen their
-    tiles so that an atom's y nodes span at most two of them."""
-    _say("tall box", method, order)
+@pytest.mark.parametrize("method,order,zs", [("spme", 5, 32), ("pme", 7, 32), ("spme", 5, 8), ("pme", 7, 8),
+                                             ("spme", 3, 8)])
+def test_tall_box_z_tiles_match_oracle(method, order, zs):
+    """A box stretched zs times along z.  zs = 32: a 1024-point z mesh, where
+    the fused z passes hold few y lines per block, and at p = 7 (Lagrange)
+    widen their tiles so that an atom's y nodes span at most two of them.
+    zs = 8: a 256-point z mesh (32 x 32 lines), which the z passes run as
+    the two-pass (8 x 16) line FFT with the R2C split in registers."""
+    _say("tall box", method, order, zs)
     b = W.sio2_box(120, seed=13)
-    stretch = np.diag([1.0, 1.0, 32.0])
+    stretch = np.diag([1.0, 1.0, float(zs)])
     box = W.Box(numbers=b.numbers, positions=b.positions @ stretch, cell=b.cell @ stretch)
     calc = FourierD3ASECalculator(device=_dev(), method=method, interpolation_nodes=order)
     calc.calculate(_atoms(box))
     o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order)
-    assert list(o.ns) == list(calc._model.ns_mesh) and int(o.ns[2]) == 1024
+    assert list(o.ns) == list(calc._model.ns_mesh) and int(o.ns[2]) == 32 * zs
     e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
     de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"], e_ref, f_ref,
                          s_ref, box.n_atoms)

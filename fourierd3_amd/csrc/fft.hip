@@ -24,6 +24,7 @@
 #include "fft_plan.h"
 #include "launch.h"
 #include "fft_lds.h"
+#include "fft_reg.h"
 
 namespace fd3 {
 
@@ -346,6 +347,103 @@ __device__ inline void kspace_reduce(const BlockInfo& bk, double (&acc)[11], dou
   }
 }
 
+// Two-pass x FFT of the k-space kernel for nx = 256 (256 = 16 x 16), 16
+// lines of stride kKs2Ls in LDS, 256 threads, one (line, column) task each.
+// Pass A: thread (l, n2) reads x[16 n1 + n2] (n1 < 16) of its line from
+// global memory, does a 16-point FFT in registers and multiplies by
+// W256^(n2 k1); pass B: thread (l, k1) gathers the 16 values of its k1 from
+// the exchange slots 17 k1 + n2 and does the second 16-point FFT, giving
+// X[k1 + 16 k2], stored in natural order.  The inverse mirrors it and writes
+// the lines back to global memory.  One LDS exchange per direction instead
+// of one per radix-4 stage; with 32 banks between lines and the slot rows
+// padded to 17, every access touches 64 distinct banks per 32 lanes, and
+// each thread's 16 addresses are one base plus immediate offsets.
+// lbase[l] < 0: a dead line (read as zeros, not written).  Lines >= nlines
+// idle but take part in the barriers.
+__device__ __forceinline__ int ks2_slot(int k1, int n2) { return 17 * k1 + n2; }
+
+// W256^(DIR m j) for j = 1..15 from four table reads (tw[m] = exp(-2 pi i m / 256)).
+template <int DIR>
+__device__ __forceinline__ void ks2_twiddle(float2 (&v)[16], const float2* tw, int m) {
+  float2 w1 = tw[m & 255], w2 = tw[(2 * m) & 255], w4 = tw[(4 * m) & 255], w8 = tw[(8 * m) & 255];
+  if (DIR > 0) { w1 = conjf2(w1); w2 = conjf2(w2); w4 = conjf2(w4); w8 = conjf2(w8); }
+  const float2 w3 = cmul(w1, w2);
+  float2 lo[8];
+  lo[0] = make_float2(1.f, 0.f); lo[1] = w1; lo[2] = w2; lo[3] = w3;
+  lo[4] = w4; lo[5] = cmul(w4, w1); lo[6] = cmul(w4, w2); lo[7] = cmul(w4, w3);
+#pragma unroll
+  for (int j = 1; j < 8; ++j) v[j] = cmul(v[j], lo[j]);
+  v[8] = cmul(v[8], w8);
+#pragma unroll
+  for (int j = 9; j < 16; ++j) v[j] = cmul(v[j], cmul(w8, lo[j - 8]));
+}
+
+// Forward: global lines -> spectra in natural order in LDS.  Ends with a barrier.
+__device__ __forceinline__ void ks2_forward(float2* S, const float2* tw, const float2* __restrict__ g,
+                                            const long long* lbase, int nlines) {
+  const int t = threadIdx.x, l = t >> 4, j = t & 15;
+  float2* L = S + l * kKs2Ls;
+  const bool on = l < nlines;
+  if (on) {
+    const long long b = lbase[l];
+    const float2* src = g + (b >= 0 ? b : 0) + j;
+    float2 v[16];
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = src[16 * n1];
+    if (b < 0)
+#pragma unroll
+      for (int n1 = 0; n1 < 16; ++n1) v[n1] = make_float2(0.f, 0.f);
+    fft16r<-1>(v);  // over n1 -> k1 (n2 = j)
+    ks2_twiddle<-1>(v, tw, j);
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) L[ks2_slot(k1, j)] = v[k1];
+  }
+  __syncthreads();
+  float2 z[16];
+  if (on) {
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) z[n2] = L[ks2_slot(j, n2)];  // k1 = j
+    fft16r<-1>(z);  // over n2 -> k2: X[j + 16 k2]
+  }
+  __syncthreads();  // every slot read is done
+  if (on)
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) L[j + 16 * k2] = z[k2];
+  __syncthreads();
+}
+
+// Inverse (unnormalised): spectra in natural order in LDS -> global lines.
+__device__ __forceinline__ void ks2_inverse(float2* S, const float2* tw, float2* __restrict__ g,
+                                            const long long* lbase, int nlines) {
+  const int t = threadIdx.x, l = t >> 4, j = t & 15;
+  float2* L = S + l * kKs2Ls;
+  const bool on = l < nlines;
+  float2 z[16];
+  if (on) {
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) z[k2] = L[j + 16 * k2];  // k1 = j
+    fft16r<1>(z);  // over k2 -> n2
+    ks2_twiddle<1>(z, tw, j);  // W256^-(k1 n2)
+  }
+  __syncthreads();  // every natural-order read is done
+  if (on)
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) L[ks2_slot(j, n2)] = z[n2];
+  __syncthreads();
+  if (on) {
+    float2 v[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = L[ks2_slot(k1, j)];  // n2 = j
+    fft16r<1>(v);  // over k1 -> n1: x[j + 16 n1]
+    const long long b = lbase[l];
+    if (b >= 0) {
+      float2* dst = g + b + j;
+#pragma unroll
+      for (int n1 = 0; n1 < 16; ++n1) dst[16 * n1] = v[n1];
+    }
+  }
+}
+
 // nS = NS <= kKsRegMaxS species and RT ranks per tile: each thread keeps the
 // NS*RT spectra of its k point in registers and writes phi back over them,
 // so the block needs a single LDS buffer.  Lines l = (X*RT + qi)*NK + kyi.
@@ -354,28 +452,35 @@ __device__ inline void kspace_reduce(const BlockInfo& bk, double (&acc)[11], dou
 // also gives h6 and h8 separately.
 // LDS: [tables | x twiddles | lines | line bases]; the block sums reuse the
 // lines at the end.
-template <int NS, int RT, bool PARAMS>
+// TWO: nx = 256 with the two-pass x FFT (ks2_forward / ks2_inverse), NK = 2.
+template <int NS, int RT, bool PARAMS, bool TWO = false>
 __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const KspaceArgs& a) {
   const KsCell kc = ks_cell(a);
   constexpr int MAXB = ks_maxb(NS * RT);
   constexpr int NP = NS * (NS + 1) / 2;  // unordered species pairs
   extern __shared__ float4 lds4[];
   __shared__ KsPair pcl[PARAMS ? NP : 1];
-  const int nx = a.nx, NK = a.NK;
+  const int nx = TWO ? 256 : a.nx, NK = TWO ? 2 : a.NK;
   const int kz = bk.id.y, q0 = bk.id.z * RT;
   const int rt = min(RT, a.r - q0);
   const int nlines = NS * RT * NK;
   // padded lines when lds_fft takes its unrolled variant (same rule on the host)
-  const bool pad = fft_fixed_tile(nx, nlines, MAXB, 256);
-  const int ls = pad ? padded_stride(nx) : nx + 1;
+  const bool pad = !TWO && fft_fixed_tile(nx, nlines, MAXB, 256);
+  const int ls = TWO ? kKs2Ls : pad ? padded_stride(nx) : nx + 1;
   const int lnx = ilog2(nx);  // nx a power of two
+  // GT: the two-pass variant reads the per-pair G tables straight from
+  // global memory (L1-resident) instead of staging them: its lines need 35 KB
+  // of LDS, and without the tables four blocks fit a CU
+  constexpr bool GT = TWO && !PARAMS;
   float4* tab = lds4;
   const int ntab = PARAMS ? 3 * (kHtabN + 1) : 2 * NP * kGtabN;
-  float2* tws = (float2*)(tab + ks_table_f4(NS));
+  float2* tws = (float2*)(tab + (TWO ? ks2_table_f4(PARAMS) : ks_table_f4(NS)));
   float2* S = tws + nx;
   long long* lbase = (long long*)(S + nlines * ls);
   const float4* src_tab = PARAMS ? a.htab : a.gtab;
-  for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[i] = src_tab[i];
+  if (!GT)
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[i] = src_tab[i];
+  const float4* gt = GT ? a.gtab : tab;
   stage_table(tws, a.twx, a.nx);
   if (PARAMS)
     for (int t = threadIdx.x; t < NP; t += blockDim.x) {  // pair t = (X, Y >= X) in row order
@@ -396,7 +501,8 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
     if (it) __syncthreads();  // the previous tile is done with S and lbase
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
     __syncthreads();
-    lds_fft<-1, MAXB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
+    if constexpr (TWO) ks2_forward(S, tws, a.spec, lbase, nlines);
+    else lds_fft<-1, MAXB, 256>(S, nx, nlines, ls, tws, 1, GlobalLines{a.spec, lbase});
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const int kxe = pad ? lpad(kx) : kx;  // element kx of a line
@@ -419,7 +525,7 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
         const float fk = kp.kn * a.gkinv;
         const int i = min((int)fk, kGtabN - 1);
         const float wk = fk - (float)i;
-        const float4* e = tab + 2 * i;
+        const float4* e = gt + 2 * i;
 #pragma unroll
         for (int pi = 0; pi < NP; ++pi) {
           gtab_eval(e[2 * kGtabN * pi], e[2 * kGtabN * pi + 1], wk, &gp[pi], &dgp[pi]);
@@ -481,7 +587,8 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
     }
     if (!a.need_grad) continue;
     __syncthreads();
-    lds_fft<1, MAXB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
+    if constexpr (TWO) ks2_inverse(S, tws, a.spec, lbase, nlines);
+    else lds_fft<1, MAXB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
   __syncthreads();  // the lines are free: reuse them for the block sums
   kspace_reduce(bk, acc, (double*)S, a, PARAMS);
@@ -659,6 +766,19 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   }
   p->RT = RT; p->NK = NK;
   p->ks_lds = ks_bytes(RT, NK);
+  // nx = 256 with at most 8 lines per ky row: the two-pass x FFT over two ky
+  // rows (16 lines of kKs2Ls complex, no padding, 256 (line, column) tasks)
+  p->ks2 = 0;
+  if (p->ks_reg && p->nx == 256 && p->ny % 2 == 0 && nS * RT <= 8) {
+    const size_t nl = (size_t)nS * RT * 2;
+    const size_t b = (size_t)ks2_table_f4(true) * sizeof(float4) + (size_t)p->nx * sizeof(float2) +
+                     std::max(nl * kKs2Ls * sizeof(float2) + nl * sizeof(long long), (size_t)16 * 16 * sizeof(double));
+    if (b <= ks_budget) {
+      p->ks2 = 1;
+      p->NK = NK = 2;
+      p->ks_lds = b;
+    }
+  }
   // tiles per block: up to 4 ky tiles, keeping at least ~8k blocks in flight
   const int zb = (r + RT - 1) / RT;
   int KT = 1;
@@ -773,6 +893,19 @@ int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, i
   return launch_y(p, t, in, out, dir, s);
 }
 
+template <int NS, int RT>
+static int ks_launch(bool two, bool params, dim3 grid, size_t sh, hipStream_t s, const KspaceArgs& a) {
+  if constexpr (NS * RT <= 8) {
+    // the plan sizes the two-pass variant's LDS with the h table, which only
+    // the parameter-gradient kernel stages
+    const size_t sh_g = sh - (size_t)(ks2_table_f4(true) - ks2_table_f4(false)) * sizeof(float4);
+    if (two && params) return launch<KspaceArgs, kspace_reg_body<NS, RT, true, true>, 256>("k_kspace", grid, sh, s, a);
+    if (two) return launch<KspaceArgs, kspace_reg_body<NS, RT, false, true>, 256>("k_kspace", grid, sh_g, s, a);
+  }
+  if (params) return launch<KspaceArgs, kspace_reg_body<NS, RT, true>, 256>("k_kspace", grid, sh, s, a);
+  return launch<KspaceArgs, kspace_reg_body<NS, RT, false>, 256>("k_kspace", grid, sh, s, a);
+}
+
 int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
                int need_params, double* partials, float2* k0, hipStream_t s) {
   KspaceArgs a;
@@ -791,8 +924,7 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
   ProfScope _ps("k_kspace", s);
 #define FD3_KS(NS_, RT_)                                                                              \
   case NS_ * 8 + RT_:                                                                                 \
-    if (need_params) return launch<KspaceArgs, kspace_reg_body<NS_, RT_, true>, 256>("k_kspace", grid, sh, s, a); \
-    return launch<KspaceArgs, kspace_reg_body<NS_, RT_, false>, 256>("k_kspace", grid, sh, s, a);
+    return ks_launch<NS_, RT_>(p.ks2 != 0, need_params != 0, grid, sh, s, a);
   switch (p.ks_reg * 8 + p.RT) {
     FD3_KS(1, 1) FD3_KS(1, 2) FD3_KS(1, 3) FD3_KS(1, 4)
     FD3_KS(2, 1) FD3_KS(2, 2) FD3_KS(2, 3) FD3_KS(2, 4)

@@ -28,6 +28,7 @@ struct FftPlan {
   int RT, NK;             // ranks and ky rows per tile of the k-space kernel
   int KT;                 // tiles (of NK ky rows) per k-space block
   int ks_reg;             // k-space variant: nS (<= kKsRegMaxS) with S and phi in registers, 0 = generic
+  int ks2;                // register variant with the two-pass (16 x 16) x FFT: nx = 256, NK = 2
   int ks_blocks;          // grid size of the k-space kernel
   size_t ks_lds;          // dynamic LDS bytes of the k-space kernel
   size_t mesh_complex() const { return (size_t)C * nx * M1 * ny; }
@@ -53,6 +54,13 @@ struct PairInit {
 // Butterflies per thread held by the register k-space variant's line FFTs:
 // its tiles hold nS * RT * NK * nx <= 1024 * ks_maxb(nS * RT) values.
 constexpr int ks_maxb(int lines_per_row) { return lines_per_row <= 8 ? 2 : 4; }
+
+// Line stride (complex) of the two-pass x FFT of the k-space kernel: 2 * 272
+// = 32 (mod 64) banks between lines.
+constexpr int kKs2Ls = 272;
+// float4 entries of its table region: the h table with parameter gradients,
+// none without (the per-pair G tables are read from global memory).
+constexpr int ks2_table_f4(bool params) { return params ? 3 * (kHtabN + 1) : 0; }
 
 // Species counts for which the k-space kernel keeps a k point's spectra in registers.
 constexpr int kKsRegMaxS = 4;

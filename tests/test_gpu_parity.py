@@ -363,6 +363,37 @@ def test_tall_box_z_tiles_match_oracle(method, order, zs):
     assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
+@pytest.mark.parametrize("method,order", [("spme", 5), ("pme", 7)])
+def test_long_x_box_kspace_matches_oracle(method, order):
+    """A box stretched 8x along x gets a 256-point x mesh, which the k-space
+    kernel transforms with the two-pass (16 x 16) x FFT over two ky rows;
+    energy, forces, stress and dE/dtheta against the oracle."""
+    _say("long x box", method, order)
+    b = W.sio2_box(120, seed=14)
+    stretch = np.diag([8.0, 1.0, 1.0])
+    box = W.strained(W.Box(numbers=b.numbers, positions=b.positions @ stretch, cell=b.cell @ stretch), 0.05, seed=3)
+    calc = FourierD3ASECalculator(device=_dev(), method=method, interpolation_nodes=order)
+    calc._build_model(_atoms(box))
+    o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order)
+    assert list(o.ns) == list(calc._model.ns_mesh) and int(o.ns[0]) == 256
+    pos = torch.tensor(box.positions, dtype=torch.float64, device=_dev())
+    res = calc.evaluate(pos, box.cell, need_grad=True, need_params=True)
+    torch.cuda.synchronize()
+    n = box.n_atoms
+    vol_b = abs(np.linalg.det(box.cell)) * ANGSTROM_TO_BOHR ** 3
+    e = float(res["energy"][0]) * HARTREE_TO_EV
+    f = -res["grad_pos"].cpu().numpy() * HARTREE_TO_EV
+    st = res["virial"].cpu().numpy() * HARTREE_TO_EV / vol_b * ANGSTROM_TO_BOHR ** 3
+    gp = res["grad_params"].cpu().numpy() * HARTREE_TO_EV
+    prm = o.params.clone().requires_grad_(True)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell, params=prm)
+    de, df, ds = _errors(e, f, st, e_ref, f_ref, s_ref, n)
+    dgp = np.abs(gp - prm.grad.numpy()) / n
+    print(f"[errors] long x box {method} p={order} mesh {list(o.ns)}: E {de:.3e} F {df:.3e} S {ds:.3e} "
+          f"dE/dtheta {dgp.max():.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL and (dgp < E_TOL).all()
+
+
 def test_param_gradients_match_oracle_autograd():
     _say("param gradients")
     box = W.sio2_box(96, seed=13)

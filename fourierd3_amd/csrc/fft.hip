@@ -192,6 +192,110 @@ __device__ __forceinline__ void c2c_y_body(const BlockInfo& bk, const YPassArgs&
   }
 }
 
+// Exchange slot of (k1, n2) in a 256-point line of the two-pass FFTs (rows
+// of 16 padded to 17).
+__device__ __forceinline__ int ks2_slot(int k1, int n2) { return 17 * k1 + n2; }
+
+// W256^(DIR m j) for j = 1..15 from four table reads (tw[m] = exp(-2 pi i m / 256)).
+template <int DIR>
+__device__ __forceinline__ void ks2_twiddle(float2 (&v)[16], const float2* tw, int m) {
+  float2 w1 = tw[m & 255], w2 = tw[(2 * m) & 255], w4 = tw[(4 * m) & 255], w8 = tw[(8 * m) & 255];
+  if (DIR > 0) { w1 = conjf2(w1); w2 = conjf2(w2); w4 = conjf2(w4); w8 = conjf2(w8); }
+  const float2 w3 = cmul(w1, w2);
+  float2 lo[8];
+  lo[0] = make_float2(1.f, 0.f); lo[1] = w1; lo[2] = w2; lo[3] = w3;
+  lo[4] = w4; lo[5] = cmul(w4, w1); lo[6] = cmul(w4, w2); lo[7] = cmul(w4, w3);
+#pragma unroll
+  for (int j = 1; j < 8; ++j) v[j] = cmul(v[j], lo[j]);
+  v[8] = cmul(v[8], w8);
+#pragma unroll
+  for (int j = 9; j < 16; ++j) v[j] = cmul(v[j], cmul(w8, lo[j - 8]));
+}
+
+// The y passes for ny = 256: NL = 16 or 32 y lines per block of 16 NL
+// threads, the two-pass (16 x 16) line FFT of the k-space kernel (pass A
+// reads the y lines from global memory, one LDS exchange, pass B), and the
+// transposed side (x contiguous, NL x values per ky row) through an LDS tile
+// T of rows of NL, rotated by the row index,
+//   T(ky, l) = NL ky + ((l + (NL / 16) ky) mod NL):
+// the pass-B stores (lanes over k1) and the row-wise copies (lanes over l)
+// both touch 64 distinct banks per 32 lanes.  T takes the place of the lines.
+template <int NL>
+__device__ __forceinline__ int y2_tile(int ky, int l) { return NL * ky + ((l + (NL / 16) * ky) & (NL - 1)); }
+
+template <int DIR, int NL>
+__device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs& ya) {
+  static_assert(NL == 16 || NL == 32, "y2: 16 or 32 lines");
+  constexpr int NT = 16 * NL, LR = NL == 16 ? 4 : 5;
+  extern __shared__ float2 lds[];
+  const float2* __restrict__ in = ya.in;
+  float2* __restrict__ out = ya.out;
+  const int nx = ya.nx, M1 = ya.M1;
+  constexpr int ny = 256;
+  const int x0 = bk.id.x * NL, kz = bk.id.y, c = bk.id.z;
+  float2* S = lds;  // NL lines of kKs2Ls, or the tile T
+  float2* tws = lds + NL * kKs2Ls;
+  stage_table(tws, ya.twy, ny);
+  const int t = threadIdx.x, l = t >> 4, j = t & 15;
+  const long long yline = ((long long)(c * nx + x0 + l) * M1 + kz) * ny;  // this thread's y line
+  // rows ky = (t >> LR) + 16 it of the transposed side, x = x0 + (t mod NL)
+  const size_t row0 = ((size_t)(c * M1 + kz) * ny + (t >> LR)) * nx + x0 + (t & (NL - 1));
+  const int trow = y2_tile<NL>(t >> LR, t & (NL - 1));  // + 16 NL it
+  static_assert(NT == 16 * NL, "");
+  if (DIR < 0) {
+    __syncthreads();  // twiddles staged
+    float2 v[16];
+    const float2* src = in + yline + j;
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = src[16 * n1];
+    fft16r<-1>(v);  // n2 = j
+    ks2_twiddle<-1>(v, tws, j);
+    float2* L = S + l * kKs2Ls;
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) L[ks2_slot(k1, j)] = v[k1];
+    __syncthreads();
+    float2 z[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) z[n2] = L[ks2_slot(j, n2)];  // k1 = j
+    fft16r<-1>(z);  // X[j + 16 k2]
+    __syncthreads();  // the lines are read: T takes their place
+    const int tb = y2_tile<NL>(j, l);  // + 16 NL k2
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) S[tb + 16 * NL * k2] = z[k2];
+    __syncthreads();
+    float2 w[16];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) w[it] = S[trow + 16 * NL * it];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) out[row0 + (size_t)16 * it * nx] = w[it];
+  } else {
+    float2 w[16];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) w[it] = in[row0 + (size_t)16 * it * nx];
+#pragma unroll
+    for (int it = 0; it < 16; ++it) S[trow + 16 * NL * it] = w[it];
+    __syncthreads();  // T filled, twiddles staged
+    const int tb = y2_tile<NL>(j, l);
+    float2 z[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) z[k2] = S[tb + 16 * NL * k2];  // X[j + 16 k2] of line l
+    fft16r<1>(z);  // over k2 -> n2
+    ks2_twiddle<1>(z, tws, j);
+    __syncthreads();  // T is read: the lines take its place
+    float2* L = S + l * kKs2Ls;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) L[ks2_slot(j, n2)] = z[n2];
+    __syncthreads();
+    float2 v[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 16; ++k1) v[k1] = L[ks2_slot(k1, j)];  // n2 = j
+    fft16r<1>(v);  // x[j + 16 n1]
+    float2* dst = out + yline + j;
+#pragma unroll
+    for (int n1 = 0; n1 < 16; ++n1) dst[16 * n1] = v[n1];
+  }
+}
+
 // ------------------------------------------------------------ x pass ---
 // In place on contiguous lines of length nx (spec layout), LPB lines per block.
 template <int DIR>
@@ -360,23 +464,6 @@ __device__ inline void kspace_reduce(const BlockInfo& bk, double (&acc)[11], dou
 // each thread's 16 addresses are one base plus immediate offsets.
 // lbase[l] < 0: a dead line (read as zeros, not written).  Lines >= nlines
 // idle but take part in the barriers.
-__device__ __forceinline__ int ks2_slot(int k1, int n2) { return 17 * k1 + n2; }
-
-// W256^(DIR m j) for j = 1..15 from four table reads (tw[m] = exp(-2 pi i m / 256)).
-template <int DIR>
-__device__ __forceinline__ void ks2_twiddle(float2 (&v)[16], const float2* tw, int m) {
-  float2 w1 = tw[m & 255], w2 = tw[(2 * m) & 255], w4 = tw[(4 * m) & 255], w8 = tw[(8 * m) & 255];
-  if (DIR > 0) { w1 = conjf2(w1); w2 = conjf2(w2); w4 = conjf2(w4); w8 = conjf2(w8); }
-  const float2 w3 = cmul(w1, w2);
-  float2 lo[8];
-  lo[0] = make_float2(1.f, 0.f); lo[1] = w1; lo[2] = w2; lo[3] = w3;
-  lo[4] = w4; lo[5] = cmul(w4, w1); lo[6] = cmul(w4, w2); lo[7] = cmul(w4, w3);
-#pragma unroll
-  for (int j = 1; j < 8; ++j) v[j] = cmul(v[j], lo[j]);
-  v[8] = cmul(v[8], w8);
-#pragma unroll
-  for (int j = 9; j < 16; ++j) v[j] = cmul(v[j], cmul(w8, lo[j - 8]));
-}
 
 // Forward: global lines -> spectra in natural order in LDS.  Ends with a barrier.
 __device__ __forceinline__ void ks2_forward(float2* S, const float2* tw, const float2* __restrict__ g,
@@ -838,6 +925,20 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
 // One y pass (dir -1: A -> spec, +1: spec -> A); more than 64 KB of dynamic
 // LDS is allowed per kernel by launch().
 static int launch_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
+  if (p.ny == 256 && p.nx % 16 == 0) {  // two-pass y lines, 32 per block when nx allows
+    const int nl = p.nx % 32 == 0 ? 32 : 16;
+    const size_t sh2 = (size_t)nl * kKs2Ls * sizeof(float2) + (size_t)p.ny * sizeof(float2);
+    const dim3 grid2(p.nx / nl, p.M1, p.C);
+    const YPassArgs ya{in, out, p.nx, p.ny, p.M1, nl, t.twy};
+    if (dir < 0) {
+      ProfScope _ps("k_c2c_y_fwd", s);
+      if (nl == 32) return launch<YPassArgs, c2c_y2_body<-1, 32>, 512>("k_c2c_y_fwd", grid2, sh2, s, ya);
+      return launch<YPassArgs, c2c_y2_body<-1, 16>, 256>("k_c2c_y_fwd", grid2, sh2, s, ya);
+    }
+    ProfScope _ps("k_c2c_y_inv", s);
+    if (nl == 32) return launch<YPassArgs, c2c_y2_body<1, 32>, 512>("k_c2c_y_inv", grid2, sh2, s, ya);
+    return launch<YPassArgs, c2c_y2_body<1, 16>, 256>("k_c2c_y_inv", grid2, sh2, s, ya);
+  }
   const size_t sh = (size_t)p.TX * (p.ny + 1) * sizeof(float2) + (size_t)p.ny * sizeof(float2) +
                     (size_t)p.TX * sizeof(long long);
   const dim3 grid(p.nx / p.TX, p.M1, p.C);

@@ -58,6 +58,7 @@ def _args():
     ap.add_argument("--no-rank-bracket", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-box batching leg")
     ap.add_argument("--no-md", action="store_true", help="skip the 1M-atom MD leg")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 648-atom SiO2 GPU leg")
     ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one 1 GiB device-to-device copy (reference kernel for HBM counters)")
@@ -639,7 +640,8 @@ def main():
             res["speedup_vs_plain_gpu"] = res["value"] / res["plain_gpu"]["value"]
     if not args.no_rank_bracket and args.cnfunc == "smooth_cut":
         leg("rank_bracket", lambda: rank_bracket(box, max(5, args.steps // 2), args.warmup))
-    leg("c1", c1_gpu)
+    if not args.no_c1:
+        leg("c1", c1_gpu)
     if not args.no_batch and args.cnfunc == "smooth_cut":
         leg("batch_c4", batch_c4)
     if not args.no_md and args.cnfunc == "smooth_cut":

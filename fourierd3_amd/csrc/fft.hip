@@ -217,11 +217,13 @@ __device__ __forceinline__ void ks2_twiddle(float2 (&v)[16], const float2* tw, i
 // reads the y lines from global memory, one LDS exchange, pass B), and the
 // transposed side (x contiguous, NL x values per ky row) through an LDS tile
 // T of rows of NL, rotated by the row index,
-//   T(ky, l) = NL ky + ((l + (NL / 16) ky) mod NL):
-// the pass-B stores (lanes over k1) and the row-wise copies (lanes over l)
-// both touch 64 distinct banks per 32 lanes.  T takes the place of the lines.
+//   T(ky, l) = NL ky + ((l + ky) mod NL):
+// LDS serves a 64-bit access 16 lanes at a time over 32 banks, so 16 lanes
+// are conflict-free when their float2 indices differ modulo 16; the pass-B
+// stores (16 lanes over k1) and the row-wise copies (16 lanes over l) both
+// are.  T takes the place of the lines.
 template <int NL>
-__device__ __forceinline__ int y2_tile(int ky, int l) { return NL * ky + ((l + (NL / 16) * ky) & (NL - 1)); }
+__device__ __forceinline__ int y2_tile(int ky, int l) { return NL * ky + ((l + ky) & (NL - 1)); }
 
 template <int DIR, int NL>
 __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs& ya) {
@@ -240,7 +242,7 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
   const long long yline = ((long long)(c * nx + x0 + l) * M1 + kz) * ny;  // this thread's y line
   // rows ky = (t >> LR) + 16 it of the transposed side, x = x0 + (t mod NL)
   const size_t row0 = ((size_t)(c * M1 + kz) * ny + (t >> LR)) * nx + x0 + (t & (NL - 1));
-  const int trow = y2_tile<NL>(t >> LR, t & (NL - 1));  // + 16 NL it
+  const int ty = t >> LR, tl = t & (NL - 1);  // tile row ty + 16 it, column tl
   static_assert(NT == 16 * NL, "");
   if (DIR < 0) {
     __syncthreads();  // twiddles staged
@@ -259,13 +261,12 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
     for (int n2 = 0; n2 < 16; ++n2) z[n2] = L[ks2_slot(j, n2)];  // k1 = j
     fft16r<-1>(z);  // X[j + 16 k2]
     __syncthreads();  // the lines are read: T takes their place
-    const int tb = y2_tile<NL>(j, l);  // + 16 NL k2
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) S[tb + 16 * NL * k2] = z[k2];
+    for (int k2 = 0; k2 < 16; ++k2) S[y2_tile<NL>(j + 16 * k2, l)] = z[k2];
     __syncthreads();
     float2 w[16];
 #pragma unroll
-    for (int it = 0; it < 16; ++it) w[it] = S[trow + 16 * NL * it];
+    for (int it = 0; it < 16; ++it) w[it] = S[y2_tile<NL>(ty + 16 * it, tl)];
 #pragma unroll
     for (int it = 0; it < 16; ++it) out[row0 + (size_t)16 * it * nx] = w[it];
   } else {
@@ -273,12 +274,11 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
 #pragma unroll
     for (int it = 0; it < 16; ++it) w[it] = in[row0 + (size_t)16 * it * nx];
 #pragma unroll
-    for (int it = 0; it < 16; ++it) S[trow + 16 * NL * it] = w[it];
+    for (int it = 0; it < 16; ++it) S[y2_tile<NL>(ty + 16 * it, tl)] = w[it];
     __syncthreads();  // T filled, twiddles staged
-    const int tb = y2_tile<NL>(j, l);
     float2 z[16];
 #pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) z[k2] = S[tb + 16 * NL * k2];  // X[j + 16 k2] of line l
+    for (int k2 = 0; k2 < 16; ++k2) z[k2] = S[y2_tile<NL>(j + 16 * k2, l)];  // X[j + 16 k2] of line l
     fft16r<1>(z);  // over k2 -> n2
     ks2_twiddle<1>(z, tws, j);
     __syncthreads();  // T is read: the lines take its place
@@ -459,9 +459,11 @@ __device__ inline void kspace_reduce(const BlockInfo& bk, double (&acc)[11], dou
 // the exchange slots 17 k1 + n2 and does the second 16-point FFT, giving
 // X[k1 + 16 k2], stored in natural order.  The inverse mirrors it and writes
 // the lines back to global memory.  One LDS exchange per direction instead
-// of one per radix-4 stage; with 32 banks between lines and the slot rows
-// padded to 17, every access touches 64 distinct banks per 32 lanes, and
-// each thread's 16 addresses are one base plus immediate offsets.
+// of one per radix-4 stage.  LDS serves a 64-bit access 16 lanes at a time
+// over 32 banks (conflict-free when the float2 indices differ modulo 16);
+// the 16 lanes of a group share a line and differ in j, and with the slot
+// rows padded to 17 (slot 17 k1 + n2) both passes' accesses are
+// conflict-free, each thread's 16 addresses one base plus immediate offsets.
 // lbase[l] < 0: a dead line (read as zeros, not written).  Lines >= nlines
 // idle but take part in the barriers.
 

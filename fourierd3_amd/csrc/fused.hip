@@ -188,13 +188,15 @@ __device__ __forceinline__ void lds_add_f32_cas(float* line, const int* z, const
 // 16-point FFT of those values in registers, giving X[k1 + 8 k2].  The
 // values cross between the passes once through LDS, instead of once per
 // radix-4 stage, and the R2C split (or C2R merge) happens in registers on
-// the way to (or from) global memory.  Lines sit at stride kZ2Ls = 130
-// complex (2 * 130 = 4 mod 64 banks), and the exchange slot of (k1, n2) is
-// 16 k1 + ((n2 + k1) mod 16): with the lines as the fastest thread index,
-// every pass touches 64 distinct banks per 32 lanes.
-constexpr int kZ2Ls = 130;
+// the way to (or from) global memory.  LDS serves a 64-bit access 16 lanes
+// at a time over 32 four-byte banks, so 16 lanes are conflict-free when
+// their float2 indices differ modulo 16: with the lines as the fastest
+// thread index and an odd line stride (kZ2Ls = 129 complex), every access
+// of a pass (fixed slot, 16 lines) is.  The exchange slot of (k1, n2) is
+// 16 k1 + n2.
+constexpr int kZ2Ls = 129;
 
-__device__ __forceinline__ int z2_slot(int k1, int n2) { return 16 * k1 + ((n2 + k1) & 15); }
+__device__ __forceinline__ int z2_slot(int k1, int n2) { return 16 * k1 + n2; }
 
 // k1 of lane group kk = t / 16 in pass B: k1 and 8 - k1 (the R2C partners)
 // share a wave (groups 2j and 2j + 1; k1 = 0 and 4 are their own partners).

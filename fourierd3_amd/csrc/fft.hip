@@ -434,6 +434,22 @@ __device__ inline void kspace_accumulate(double (&acc)[11], const KPoint& p, flo
   }
 }
 
+// The same in float, for a thread's own partial sums over its few k points
+// (a handful of terms; the block sums are double).
+__device__ inline void kspace_accumulate_f(float (&acc)[11], const KPoint& p, float w, float e_acc, float d_acc,
+                                           const float* p_acc, bool params) {
+  acc[0] += w * e_acc * p.isc;
+  if (p.kn > 0.f) {
+    const float s = w * d_acc * p.isc * __builtin_amdgcn_rcpf(p.kn);
+    acc[1] += s * p.kv0 * p.kv0; acc[2] += s * p.kv1 * p.kv1; acc[3] += s * p.kv2 * p.kv2;
+    acc[4] += s * p.kv0 * p.kv1; acc[5] += s * p.kv0 * p.kv2; acc[6] += s * p.kv1 * p.kv2;
+  }
+  if (params) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) acc[7 + i] += w * p_acc[i] * p.isc;
+  }
+}
+
 // Block sums of the 7 (or, with parameter gradients, 11) partials; the
 // unused parameter slots are written as zeros.  Ends with a barrier.
 __device__ inline void kspace_reduce(const BlockInfo& bk, double (&acc)[11], double* red, const KspaceArgs& a,
@@ -581,9 +597,11 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
   float lam[RT];
 #pragma unroll
   for (int q = 0; q < RT; ++q) lam[q] = q < rt ? a.eigs[q0 + q] : 0.0f;
-  double acc[11];
+  // each thread's partial sums over its k points (at most KT * NK * nx / 256
+  // = 8 terms) in float, the block sums in double
+  float accf[11];
 #pragma unroll
-  for (int i = 0; i < 11; ++i) acc[i] = 0.0;
+  for (int i = 0; i < 11; ++i) accf[i] = 0.0f;
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
   for (int it = 0; it < a.KT; ++it) {
     const int ky0 = (bk.id.x * a.KT + it) * NK;
@@ -664,7 +682,7 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
           e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
           S[((X * RT + q) * NK + kyi) * ls + kxe] = make_float2(f[q] * ph[X][q].x, f[q] * ph[X][q].y);
         }
-      kspace_accumulate(acc, kp, w, e_acc, d_acc, p_acc, PARAMS);
+      kspace_accumulate_f(accf, kp, w, e_acc, d_acc, p_acc, PARAMS);
     }
     if (kz == 0 && ky0 == 0 && threadIdx.x == 0) {  // k = 0 (this thread's column 0): kept out of the mesh
       for (int l = 0; l < NS * RT; ++l) {
@@ -679,6 +697,9 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
     if constexpr (TWO) ks2_inverse(S, tws, a.spec, lbase, nlines);
     else lds_fft<1, MAXB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
+  double acc[11];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) acc[i] = (double)accf[i];
   __syncthreads();  // the lines are free: reuse them for the block sums
   kspace_reduce(bk, acc, (double*)S, a, PARAMS);
 }

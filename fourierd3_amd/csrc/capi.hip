@@ -506,12 +506,13 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   const FusedArgs fa = fused_args(m, c, n, pos4, spec_s);
   float2* spec = (float2*)c.buf1;
   if ((rc = spread_r2c_z(fa, c.plan, c.tab, c.buf2, s))) return rc;
-  if ((rc = fft_y(c.plan, c.tab, c.buf2, spec, -1, s))) return rc;
+  const bool xs = c.plan.xs != 0;  // split x FFT between the y passes and the k-space kernel
+  if ((rc = fft_y(c.plan, c.tab, c.buf2, spec, -1, s, xs))) return rc;
   if ((rc = kspace_run(c.plan, c.tab, c.geom, spec, m->eigs, grad ? 1 : 0, (flags & FD3_NEED_PARAMS) ? 1 : 0, c.p_ks,
-                       c.k0, s)))
+                       c.k0, s, xs)))
     return rc;
   if (grad) {
-    if ((rc = fft_y(c.plan, c.tab, spec, c.buf2, 1, s))) return rc;
+    if ((rc = fft_y(c.plan, c.tab, spec, c.buf2, 1, s, xs))) return rc;
     if ((rc = c2r_z_gather(fa, c.plan, c.tab, c.buf2, s))) return rc;
     C6BwdArgs ba;
     ba.n = n; ba.nS = nS; ba.r = r; ba.nx = c.geom.nx; ba.ny = c.geom.ny; ba.nz = c.geom.nz;

@@ -222,24 +222,56 @@ __device__ __forceinline__ void ks2_twiddle(float2 (&v)[16], const float2* tw, i
 // are conflict-free when their float2 indices differ modulo 16; the pass-B
 // stores (16 lanes over k1) and the row-wise copies (16 lanes over l) both
 // are.  T takes the place of the lines.
+//
+// XS (split x FFT, nx = ny = 256, NL = 32): the block also runs the first
+// stage of the 256-point x FFT (256 = 16 x 16, x = 16 n1 + n2) on its tile.
+// Block b takes the x lines n2 in {2b, 2b + 1}, all n1 (tile column
+// l = 16 (n2 - 2b) + n1), and the rows are stored in the order
+// 16 n2 + k1 (tile column 16 (n2 - 2b) + k1 at row position 32 b + column):
+//   forward  Y[k1, n2] = W256^(n2 k1) sum_n1 W16^(n1 k1) x[16 n1 + n2],
+//   inverse  x[16 n1 + n2] = sum_k1 W16^-(n1 k1) W256^-(n2 k1) Y[k1, n2],
+// and the k-space kernel finishes (or starts) the transform with one
+// 16-point FFT over n2 per (row, k1), reading 16 coalesced values.  The
+// stage runs in place on T, one thread per (row, n2): 16 lanes over
+// consecutive rows hit distinct banks.
 template <int NL>
 __device__ __forceinline__ int y2_tile(int ky, int l) { return NL * ky + ((l + ky) & (NL - 1)); }
 
-template <int DIR, int NL>
+template <int DIR>
+__device__ __forceinline__ void y2_x_stage(float2* T, const float2* tws, int b) {
+  const int t = threadIdx.x, ky = t & 255, h = t >> 8;
+  float2 v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) v[i] = T[y2_tile<32>(ky, 16 * h + i)];
+  if (DIR < 0) {
+    fft16r<-1>(v);                    // over n1 -> k1
+    ks2_twiddle<-1>(v, tws, 2 * b + h);  // W256^(n2 k1)
+  } else {
+    ks2_twiddle<1>(v, tws, 2 * b + h);   // W256^-(n2 k1)
+    fft16r<1>(v);                     // over k1 -> n1
+  }
+#pragma unroll
+  for (int i = 0; i < 16; ++i) T[y2_tile<32>(ky, 16 * h + i)] = v[i];
+}
+
+template <int DIR, int NL, bool XS = false>
 __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs& ya) {
   static_assert(NL == 16 || NL == 32, "y2: 16 or 32 lines");
+  static_assert(!XS || NL == 32, "y2: the split x FFT takes 32 lines");
   constexpr int NT = 16 * NL, LR = NL == 16 ? 4 : 5;
   extern __shared__ float2 lds[];
   const float2* __restrict__ in = ya.in;
   float2* __restrict__ out = ya.out;
-  const int nx = ya.nx, M1 = ya.M1;
+  const int nx = XS ? 256 : ya.nx, M1 = ya.M1;
   constexpr int ny = 256;
   const int x0 = bk.id.x * NL, kz = bk.id.y, c = bk.id.z;
   float2* S = lds;  // NL lines of kKs2Ls, or the tile T
   float2* tws = lds + NL * kKs2Ls;
   stage_table(tws, ya.twy, ny);
   const int t = threadIdx.x, l = t >> 4, j = t & 15;
-  const long long yline = ((long long)(c * nx + x0 + l) * M1 + kz) * ny;  // this thread's y line
+  // x of this thread's y line: x0 + l, or with XS 16 n1 + n2 (l = 16 (n2 - 2b) + n1)
+  const int xl = XS ? 16 * (l & 15) + 2 * bk.id.x + (l >> 4) : x0 + l;
+  const long long yline = ((long long)(c * nx + xl) * M1 + kz) * ny;  // this thread's y line
   // rows ky = (t >> LR) + 16 it of the transposed side, x = x0 + (t mod NL)
   const size_t row0 = ((size_t)(c * M1 + kz) * ny + (t >> LR)) * nx + x0 + (t & (NL - 1));
   const int ty = t >> LR, tl = t & (NL - 1);  // tile row ty + 16 it, column tl
@@ -264,6 +296,10 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) S[y2_tile<NL>(j + 16 * k2, l)] = z[k2];
     __syncthreads();
+    if constexpr (XS) {
+      y2_x_stage<-1>(S, tws, bk.id.x);
+      __syncthreads();
+    }
     float2 w[16];
 #pragma unroll
     for (int it = 0; it < 16; ++it) w[it] = S[y2_tile<NL>(ty + 16 * it, tl)];
@@ -276,6 +312,10 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
 #pragma unroll
     for (int it = 0; it < 16; ++it) S[y2_tile<NL>(ty + 16 * it, tl)] = w[it];
     __syncthreads();  // T filled, twiddles staged
+    if constexpr (XS) {
+      y2_x_stage<1>(S, tws, bk.id.x);
+      __syncthreads();
+    }
     float2 z[16];
 #pragma unroll
     for (int k2 = 0; k2 < 16; ++k2) z[k2] = S[y2_tile<NL>(j + 16 * k2, l)];  // X[j + 16 k2] of line l
@@ -549,12 +589,180 @@ __device__ __forceinline__ void ks2_inverse(float2* S, const float2* tw, float2*
   }
 }
 
+// The same x FFT when the y pass has already run its first stage (split x
+// FFT, c2c_y2_body<.., XS>): a line holds Y[k1, n2] at 16 n2 + k1, and
+// thread (l, k1) takes its 16 values Y[k1, 0..15] (16 lanes over k1 read 128
+// contiguous bytes), does the 16-point FFT over n2 and stores X[k1 + 16 k2]
+// in natural order; the inverse mirrors it.  No twiddles, no LDS exchange.
+// Lines past the tile or of ranks past the last (live = false) read as
+// zeros and are not written.
+
+// this thread's 16 values of one line, starting at element e (= line base + k1)
+__device__ __forceinline__ void ksx_load(float2 (&v)[16], const float2* __restrict__ g, bool live, long long e) {
+  if (live) {
+    const float2* src = g + e;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = src[16 * n2];
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = make_float2(0.f, 0.f);
+  }
+}
+
+// Forward from the loaded values into the first nlines lines (stride ls)
+// of S; ends with a barrier.
+__device__ __forceinline__ void ksx_forward_regs(float2* S, int ls, const float2 (&v)[16], int nlines) {
+  const int t = threadIdx.x, l = t >> 4, j = t & 15;
+  if (l < nlines) {
+    float2 z[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) z[n2] = v[n2];
+    fft16r<-1>(z);  // over n2 -> k2: X[j + 16 k2]
+    float2* L = S + l * ls;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) L[j + 16 * k2] = z[k2];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void ksx_inverse(const float2* S, int ls, float2* __restrict__ g, bool live,
+                                           long long e) {
+  const int t = threadIdx.x, l = t >> 4, j = t & 15;
+  const float2* L = S + l * ls;
+  if (live) {
+    float2 z[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) z[k2] = L[j + 16 * k2];  // k1 = j
+    fft16r<1>(z);  // over k2 -> n2: Y[j, n2]
+    float2* dst = g + e;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) dst[16 * n2] = z[n2];
+  }
+}
+
+// One k point (column kx of tile row kyi, element kxe of a line) of the
+// register k-space kernels: G~ of every species pair, phi_X = sum_Y G~ S_Y
+// over the NS*RT spectra of the point (read from the tile lines in S,
+// overwritten with -(lambda_q/Omega) phi), and the energy, virial and
+// parameter partials added to acc (float: a handful of terms per thread).
+// Without parameter gradients G comes from the per-pair tables gt (one index
+// per k point, one Horner pass per pair, the coefficients of the next PD
+// pairs in flight while a pair is evaluated); with them from the h table
+// tab and the pair constants pcl, which also give h6 and h8 separately.
+template <int NS, int RT, bool PARAMS>
+__device__ __forceinline__ void kspace_kpoint(const KspaceArgs& a, const KsCell& kc, float2* S, int ls, int NK,
+                                              int kyi, int kx, int kxe, int ky, int kz, const float (&lam)[RT],
+                                              float w, const KsPair* pcl, const float4* tab, const float4* gt,
+                                              float (&accf)[11]) {
+  constexpr int NP = NS * (NS + 1) / 2;  // unordered species pairs
+  const KPoint kp = kspace_point(a, kc, kx, ky, kz);
+  // G and dG/d|k| (or dG/dw) of every pair first, then the spectra
+  float gp[NP], dgp[NP], p_acc[4] = {0.f, 0.f, 0.f, 0.f};
+  float h6v[PARAMS ? NP : 1], d6v[PARAMS ? NP : 1], h8v[PARAMS ? NP : 1], d8v[PARAMS ? NP : 1];
+  if constexpr (PARAMS) {
+#pragma unroll
+    for (int pi = 0; pi < NP; ++pi) {
+      float h6, d6, h8, d8;
+      green_eval_tab(pcl[pi], tab, kp.kn, &gp[pi], &dgp[pi], &h6, &d6, &h8, &d8);
+      h6v[pi] = h6; d6v[pi] = d6; h8v[pi] = h8; d8v[pi] = d8;
+      asm volatile("" : "+v"(h6v[pi]), "+v"(d6v[pi]), "+v"(h8v[pi]), "+v"(d8v[pi]));
+      // pin g and dG/dk here: left to itself the compiler defers the
+      // derivative and keeps every pair's Horner intermediates live
+      asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
+    }
+  } else {
+    const float fk = kp.kn * a.gkinv;
+    const int i = min((int)fk, kGtabN - 1);
+    const float wk = fk - (float)i;
+    const float4* e = gt + 2 * i;
+    constexpr int PD = 2;
+    float4 ca[PD + 1], cb[PD + 1];
+    static_for<PD>([&](auto dc) {
+      constexpr int d = decltype(dc)::value;
+      if constexpr (d < NP) { ca[d] = e[2 * kGtabN * d]; cb[d] = e[2 * kGtabN * d + 1]; }
+    });
+    static_for<NP>([&](auto pc) {
+      constexpr int pi = decltype(pc)::value, nxt = pi + PD;
+      if constexpr (nxt < NP) {
+        ca[nxt % (PD + 1)] = e[2 * kGtabN * nxt];
+        cb[nxt % (PD + 1)] = e[2 * kGtabN * nxt + 1];
+      }
+      gtab_eval(ca[pi % (PD + 1)], cb[pi % (PD + 1)], wk, &gp[pi], &dgp[pi]);
+      asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
+    });
+  }
+  float2 sv[NS][RT], ph[NS][RT];
+#pragma unroll
+  for (int X = 0; X < NS; ++X)
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+      sv[X][q] = S[((X * RT + q) * NK + kyi) * ls + kxe];
+      ph[X][q] = make_float2(0.f, 0.f);
+    }
+  float d_acc = 0.f;
+#pragma unroll
+  for (int X = 0, pi = 0; X < NS; ++X) {
+#pragma unroll
+    for (int Y = X; Y < NS; ++Y, ++pi) {
+      const float gs = gp[pi];
+      float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        const float2 sx = sv[X][q], sy = sv[Y][q];
+        tre += lam[q] * (sx.x * sy.x + sx.y * sy.y);
+        ph[X][q].x += gs * sy.x;
+        ph[X][q].y += gs * sy.y;
+        if (Y != X) {
+          ph[Y][q].x += gs * sx.x;
+          ph[Y][q].y += gs * sx.y;
+        }
+      }
+      if (Y != X) tre *= 2.0f;
+      d_acc += dgp[pi] * tre;
+      if (PARAMS) kspace_param_terms(pcl[pi], kp.kn, h6v[pi], d6v[pi], h8v[pi], d8v[pi], tre, p_acc);
+    }
+  }
+  if (!PARAMS) d_acc *= a.gkinv;  // dG/dw -> dG/d|k|
+  // ph holds sum_Y G S_Y; the 1/sinc^2p of G~ goes into f and the accumulation
+  float e_acc = 0.f, f[RT];
+#pragma unroll
+  for (int q = 0; q < RT; ++q) f[q] = lam[q] * kc.neg_inv_vol * kp.isc;
+#pragma unroll
+  for (int X = 0; X < NS; ++X)
+#pragma unroll
+    for (int q = 0; q < RT; ++q) {
+      e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
+      S[((X * RT + q) * NK + kyi) * ls + kxe] = make_float2(f[q] * ph[X][q].x, f[q] * ph[X][q].y);
+    }
+  kspace_accumulate_f(accf, kp, w, e_acc, d_acc, p_acc, PARAMS);
+}
+
+// The k = 0 coefficient of each spectrum of the tile (column 0 of row 0)
+// goes to k0 and is zeroed in the mesh (thread 0 of the tile holding it).
+template <int NS, int RT>
+__device__ __forceinline__ void kspace_take_k0(const KspaceArgs& a, float2* S, int ls, int NK, int q0, int rt) {
+  for (int l = 0; l < NS * RT; ++l) {
+    float2* sp = &S[l * NK * ls];
+    const int X = l / RT, q = l - X * RT;
+    if (q < rt) a.k0[X * a.r + q0 + q] = *sp;
+    *sp = make_float2(0.f, 0.f);
+  }
+}
+
+// Pair constants of the NP unordered species pairs (X, Y >= X) in row order.
+template <int NS>
+__device__ __forceinline__ void kspace_stage_pairs(const KspaceArgs& a, KsPair* pcl) {
+  constexpr int NP = NS * (NS + 1) / 2;
+  for (int t = threadIdx.x; t < NP; t += blockDim.x) {
+    int X = 0, r0 = 0;
+    while (t >= r0 + NS - X) { r0 += NS - X; ++X; }
+    pcl[t] = a.pc[X * NS + X + (t - r0)];
+  }
+}
+
 // nS = NS <= kKsRegMaxS species and RT ranks per tile: each thread keeps the
 // NS*RT spectra of its k point in registers and writes phi back over them,
 // so the block needs a single LDS buffer.  Lines l = (X*RT + qi)*NK + kyi.
-// Without parameter gradients G comes from the per-pair tables (one index
-// per k point, one Horner pass per pair); with them from the h table, which
-// also gives h6 and h8 separately.
 // LDS: [tables | x twiddles | lines | line bases]; the block sums reuse the
 // lines at the end.
 // TWO: nx = 256 with the two-pass x FFT (ks2_forward / ks2_inverse), NK = 2.
@@ -587,12 +795,7 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
     for (int i = threadIdx.x; i < ntab; i += blockDim.x) tab[i] = src_tab[i];
   const float4* gt = GT ? a.gtab : tab;
   stage_table(tws, a.twx, a.nx);
-  if (PARAMS)
-    for (int t = threadIdx.x; t < NP; t += blockDim.x) {  // pair t = (X, Y >= X) in row order
-      int X = 0, r0 = 0;
-      while (t >= r0 + NS - X) { r0 += NS - X; ++X; }
-      pcl[t] = a.pc[X * NS + X + (t - r0)];
-    }
+  if (PARAMS) kspace_stage_pairs<NS>(a, pcl);
 
   float lam[RT];
 #pragma unroll
@@ -613,85 +816,9 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
     for (int t = threadIdx.x; t < NK * nx; t += blockDim.x) {
       const int kyi = t >> lnx, kx = t & (nx - 1);
       const int kxe = pad ? lpad(kx) : kx;  // element kx of a line
-      const KPoint kp = kspace_point(a, kc, kx, ky0 + kyi, kz);
-      // G and dG/d|k| (or dG/dw) of every pair first, then the spectra
-      float gp[NP], dgp[NP], p_acc[4] = {0.f, 0.f, 0.f, 0.f};
-      float h6v[PARAMS ? NP : 1], d6v[PARAMS ? NP : 1], h8v[PARAMS ? NP : 1], d8v[PARAMS ? NP : 1];
-      if constexpr (PARAMS) {
-#pragma unroll
-        for (int pi = 0; pi < NP; ++pi) {
-          float h6, d6, h8, d8;
-          green_eval_tab(pcl[pi], tab, kp.kn, &gp[pi], &dgp[pi], &h6, &d6, &h8, &d8);
-          h6v[pi] = h6; d6v[pi] = d6; h8v[pi] = h8; d8v[pi] = d8;
-          asm volatile("" : "+v"(h6v[pi]), "+v"(d6v[pi]), "+v"(h8v[pi]), "+v"(d8v[pi]));
-          // pin g and dG/dk here: left to itself the compiler defers the
-          // derivative and keeps every pair's Horner intermediates live
-          asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
-        }
-      } else {
-        const float fk = kp.kn * a.gkinv;
-        const int i = min((int)fk, kGtabN - 1);
-        const float wk = fk - (float)i;
-        const float4* e = gt + 2 * i;
-#pragma unroll
-        for (int pi = 0; pi < NP; ++pi) {
-          gtab_eval(e[2 * kGtabN * pi], e[2 * kGtabN * pi + 1], wk, &gp[pi], &dgp[pi]);
-          asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi]));
-        }
-      }
-      float2 sv[NS][RT], ph[NS][RT];
-#pragma unroll
-      for (int X = 0; X < NS; ++X)
-#pragma unroll
-        for (int q = 0; q < RT; ++q) {
-          sv[X][q] = S[((X * RT + q) * NK + kyi) * ls + kxe];
-          ph[X][q] = make_float2(0.f, 0.f);
-        }
-      float d_acc = 0.f;
-#pragma unroll
-      for (int X = 0, pi = 0; X < NS; ++X) {
-#pragma unroll
-        for (int Y = X; Y < NS; ++Y, ++pi) {
-          const float gs = gp[pi];
-          float tre = 0.f;  // sum_q lambda_q Re(S_X conj S_Y)
-#pragma unroll
-          for (int q = 0; q < RT; ++q) {
-            const float2 sx = sv[X][q], sy = sv[Y][q];
-            tre += lam[q] * (sx.x * sy.x + sx.y * sy.y);
-            ph[X][q].x += gs * sy.x;
-            ph[X][q].y += gs * sy.y;
-            if (Y != X) {
-              ph[Y][q].x += gs * sx.x;
-              ph[Y][q].y += gs * sx.y;
-            }
-          }
-          if (Y != X) tre *= 2.0f;
-          d_acc += dgp[pi] * tre;
-          if (PARAMS) kspace_param_terms(pcl[pi], kp.kn, h6v[pi], d6v[pi], h8v[pi], d8v[pi], tre, p_acc);
-        }
-      }
-      if (!PARAMS) d_acc *= a.gkinv;  // dG/dw -> dG/d|k|
-      // ph holds sum_Y G S_Y; the 1/sinc^2p of G~ goes into f and the accumulation
-      float e_acc = 0.f, f[RT];
-#pragma unroll
-      for (int q = 0; q < RT; ++q) f[q] = lam[q] * kc.neg_inv_vol * kp.isc;
-#pragma unroll
-      for (int X = 0; X < NS; ++X)
-#pragma unroll
-        for (int q = 0; q < RT; ++q) {
-          e_acc += lam[q] * (sv[X][q].x * ph[X][q].x + sv[X][q].y * ph[X][q].y);
-          S[((X * RT + q) * NK + kyi) * ls + kxe] = make_float2(f[q] * ph[X][q].x, f[q] * ph[X][q].y);
-        }
-      kspace_accumulate_f(accf, kp, w, e_acc, d_acc, p_acc, PARAMS);
+      kspace_kpoint<NS, RT, PARAMS>(a, kc, S, ls, NK, kyi, kx, kxe, ky0 + kyi, kz, lam, w, pcl, tab, gt, accf);
     }
-    if (kz == 0 && ky0 == 0 && threadIdx.x == 0) {  // k = 0 (this thread's column 0): kept out of the mesh
-      for (int l = 0; l < NS * RT; ++l) {
-        float2* sp = &S[l * NK * ls];
-        const int X = l / RT, q = l - X * RT;
-        if (q < rt) a.k0[X * a.r + q0 + q] = *sp;
-        *sp = make_float2(0.f, 0.f);
-      }
-    }
+    if (kz == 0 && ky0 == 0 && threadIdx.x == 0) kspace_take_k0<NS, RT>(a, S, ls, NK, q0, rt);
     if (!a.need_grad) continue;
     __syncthreads();
     if constexpr (TWO) ks2_inverse(S, tws, a.spec, lbase, nlines);
@@ -700,6 +827,77 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
   double acc[11];
 #pragma unroll
   for (int i = 0; i < 11; ++i) acc[i] = (double)accf[i];
+  __syncthreads();  // the lines are free: reuse them for the block sums
+  kspace_reduce(bk, acc, (double*)S, a, PARAMS);
+}
+
+// The register k-space kernel with the split x FFT (nx = ny = 256, the y
+// passes run the first stage): blocks of 256 threads over a 1-D grid, block b
+// taking the KT consecutive tiles b KT .. b KT + KT - 1 of the
+// (ny/2) x M1 x ceil(r/RT) tiles of two ky rows x NS*RT spectra, ky pair
+// fastest.  Thread (l, j) owns column j of line l = (X*RT + qi)*2 + kyi of
+// every tile (the 16-point FFT of ksx_load / ksx_forward_regs /
+// ksx_inverse); its load is issued before the barrier that frees the lines.
+// Dynamic LDS: the 16 lines of 256 points (32 KB, the whole allocation
+// without parameter gradients, so LDS never holds the occupancy below the
+// VGPRs' 4 waves per SIMD of the 4-species, rank-2 kernel), then with them
+// the h table and the pair constants.  The block sums reuse the lines at the
+// end.  Each thread adds its tile's two k points in float and the tile sums
+// in double.  Measured on the 100k-atom box (256^3, 4 species, rank 2):
+// 0.267 ms against 0.274 ms for the same tiles with padded lines, twiddle
+// and line-base tables in LDS (35 KB), 0.297 ms for persistent blocks
+// striding over the tiles, 0.29 ms for one ky row per tile; streaming
+// (nontemporal) loads of the spectrum cost 40 %.
+template <int NS, int RT, bool PARAMS>
+__device__ __forceinline__ void kspace_xs_body(const BlockInfo& bk, const KspaceArgs& a) {
+  static_assert(NS * RT <= 8, "split x k-space: at most 8 spectra per tile");
+  constexpr int NP = NS * (NS + 1) / 2, NK = 2, ls = 256, nlines = NS * RT * NK;
+  const KsCell kc = ks_cell(a);
+  extern __shared__ float4 lds4[];
+  float2* S = (float2*)lds4;
+  float4* tab = lds4 + nlines * ls / 2;
+  KsPair* pcl = (KsPair*)(tab + (PARAMS ? 3 * (kHtabN + 1) : 0));
+  if constexpr (PARAMS) {
+    for (int i = threadIdx.x; i < 3 * (kHtabN + 1); i += blockDim.x) tab[i] = a.htab[i];
+    kspace_stage_pairs<NS>(a, pcl);
+  }
+  const int nyp = a.ny >> 1, zb = (a.r + RT - 1) / RT;
+  const int ntiles = nyp * a.M1 * zb;
+  const int xl = threadIdx.x >> 4, xj = threadIdx.x & 15;
+  const int xqi = (xl >> 1) % RT, xX = (xl >> 1) / RT;
+  constexpr int NA = PARAMS ? 11 : 7;  // partial sums carried over the tiles
+  double acc[11];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) acc[i] = 0.0;
+
+  const int u0 = bk.id.x * a.KT, u1 = min(ntiles, u0 + a.KT);
+  for (int u = u0; u < u1; ++u) {
+    const int kyp = u % nyp, rest = u / nyp, kz = rest % a.M1, q0 = rest / a.M1 * RT;
+    const int rt = min(RT, a.r - q0), ky0 = 2 * kyp;
+    const bool live = xl < nlines && xqi < rt;
+    const long long e0 = (((long long)(xX * a.r + q0 + xqi) * a.M1 + kz) * a.ny + ky0 + (xl & 1)) * 256 + xj;
+    float lam[RT];
+#pragma unroll
+    for (int q = 0; q < RT; ++q) lam[q] = q < rt ? a.eigs[q0 + q] : 0.0f;
+    const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
+    float accf[11];  // this tile's two k points in float, the tile sums in double
+#pragma unroll
+    for (int i = 0; i < 11; ++i) accf[i] = 0.0f;
+    float2 v[16];
+    ksx_load(v, a.spec, live, e0);
+    __syncthreads();  // the previous tile (and the staged tables) are done
+    ksx_forward_regs(S, ls, v, nlines);
+    for (int t = threadIdx.x; t < NK * 256; t += blockDim.x) {
+      const int kyi = t >> 8, kx = t & 255;
+      kspace_kpoint<NS, RT, PARAMS>(a, kc, S, ls, NK, kyi, kx, kx, ky0 + kyi, kz, lam, w, pcl, tab, a.gtab, accf);
+    }
+#pragma unroll
+    for (int i = 0; i < NA; ++i) acc[i] += (double)accf[i];
+    if (kz == 0 && ky0 == 0 && threadIdx.x == 0) kspace_take_k0<NS, RT>(a, S, ls, NK, q0, rt);
+    if (!a.need_grad) continue;
+    __syncthreads();
+    ksx_inverse(S, ls, a.spec, live, e0);
+  }
   __syncthreads();  // the lines are free: reuse them for the block sums
   kspace_reduce(bk, acc, (double*)S, a, PARAMS);
 }
@@ -889,6 +1087,9 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
       p->ks_lds = b;
     }
   }
+  // ny = 256 as well: the y passes run the first stage of the x FFT and the
+  // k-space kernel the second (kspace_xs_body, 32 KB of LDS per block)
+  p->xs = p->ks2 && p->ny == 256;
   // tiles per block: up to 4 ky tiles, keeping at least ~8k blocks in flight
   const int zb = (r + RT - 1) / RT;
   int KT = 1;
@@ -896,6 +1097,7 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   if (tuned && pow2(tune[3]) && tune[3] <= 4 && (p->ny / NK) % tune[3] == 0) KT = tune[3];
   p->KT = KT;
   p->ks_blocks = (p->ny / NK / KT) * p->M1 * zb;
+  if (p->xs) p->ks_blocks = (p->ny / 2 * p->M1 * zb + KT - 1) / KT;  // 1-D grid of KT-tile blocks
   // register budget of lds_fft<.., kFftMaxB>: lines * n <= kFftLines
   const size_t lim = kFftLines;
   if ((size_t)p->TY * p->M > lim || (size_t)p->TX * p->ny > (size_t)kYLines || (size_t)p->LPB * p->nx > lim) {
@@ -947,7 +1149,12 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
 
 // One y pass (dir -1: A -> spec, +1: spec -> A); more than 64 KB of dynamic
 // LDS is allowed per kernel by launch().
-static int launch_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
+static int launch_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s,
+                    bool xs) {
+  if (xs && !(p.xs && p.nx == 256 && p.ny == 256)) {
+    set_error("split x FFT planned for a mesh that is not 256 x 256 in x, y");
+    return FD3_ERR_UNSUPPORTED;
+  }
   if (p.ny == 256 && p.nx % 16 == 0) {  // two-pass y lines, 32 per block when nx allows
     const int nl = p.nx % 32 == 0 ? 32 : 16;
     const size_t sh2 = (size_t)nl * kKs2Ls * sizeof(float2) + (size_t)p.ny * sizeof(float2);
@@ -955,10 +1162,12 @@ static int launch_y(const FftPlan& p, const FftTables& t, const float2* in, floa
     const YPassArgs ya{in, out, p.nx, p.ny, p.M1, nl, t.twy};
     if (dir < 0) {
       ProfScope _ps("k_c2c_y_fwd", s);
+      if (xs) return launch<YPassArgs, c2c_y2_body<-1, 32, true>, 512>("k_c2c_y_fwd", grid2, sh2, s, ya);
       if (nl == 32) return launch<YPassArgs, c2c_y2_body<-1, 32>, 512>("k_c2c_y_fwd", grid2, sh2, s, ya);
       return launch<YPassArgs, c2c_y2_body<-1, 16>, 256>("k_c2c_y_fwd", grid2, sh2, s, ya);
     }
     ProfScope _ps("k_c2c_y_inv", s);
+    if (xs) return launch<YPassArgs, c2c_y2_body<1, 32, true>, 512>("k_c2c_y_inv", grid2, sh2, s, ya);
     if (nl == 32) return launch<YPassArgs, c2c_y2_body<1, 32>, 512>("k_c2c_y_inv", grid2, sh2, s, ya);
     return launch<YPassArgs, c2c_y2_body<1, 16>, 256>("k_c2c_y_inv", grid2, sh2, s, ya);
   }
@@ -984,7 +1193,7 @@ int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float
                        t.twz);
     FD3_LAUNCH_CHECK("k_r2c_z");
   }
-  return launch_y(p, t, A, spec, -1, s);
+  return launch_y(p, t, A, spec, -1, s, false);
 }
 
 int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream_t s) {
@@ -1002,7 +1211,7 @@ int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream
 
 int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, float2* A, float* out, hipStream_t s) {
   int rc;
-  if ((rc = launch_y(p, t, spec, A, 1, s))) return rc;
+  if ((rc = launch_y(p, t, spec, A, 1, s, false))) return rc;
   {
     const size_t sh = (size_t)p.TY * (p.M + 1) * sizeof(float2) + (size_t)p.nz * sizeof(float2);
     ProfScope _ps("k_c2r_z", s);
@@ -1013,16 +1222,18 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
   return FD3_OK;
 }
 
-int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s) {
-  return launch_y(p, t, in, out, dir, s);
+int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s, bool xs) {
+  return launch_y(p, t, in, out, dir, s, xs);
 }
 
 template <int NS, int RT>
-static int ks_launch(bool two, bool params, dim3 grid, size_t sh, hipStream_t s, const KspaceArgs& a) {
+static int ks_launch(bool two, bool xs, bool params, dim3 grid, size_t sh, hipStream_t s, const KspaceArgs& a) {
   if constexpr (NS * RT <= 8) {
     // the plan sizes the two-pass variant's LDS with the h table, which only
     // the parameter-gradient kernel stages
     const size_t sh_g = sh - (size_t)(ks2_table_f4(true) - ks2_table_f4(false)) * sizeof(float4);
+    if (xs && params) return launch<KspaceArgs, kspace_xs_body<NS, RT, true>, 256>("k_kspace", grid, sh, s, a);
+    if (xs) return launch<KspaceArgs, kspace_xs_body<NS, RT, false>, 256>("k_kspace", grid, sh, s, a);
     if (two && params) return launch<KspaceArgs, kspace_reg_body<NS, RT, true, true>, 256>("k_kspace", grid, sh, s, a);
     if (two) return launch<KspaceArgs, kspace_reg_body<NS, RT, false, true>, 256>("k_kspace", grid, sh_g, s, a);
   }
@@ -1031,7 +1242,11 @@ static int ks_launch(bool two, bool params, dim3 grid, size_t sh, hipStream_t s,
 }
 
 int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
-               int need_params, double* partials, float2* k0, hipStream_t s) {
+               int need_params, double* partials, float2* k0, hipStream_t s, bool xs) {
+  if (xs && !(p.xs && p.ks2)) {
+    set_error("split x FFT requested for a k-space plan without it");
+    return FD3_ERR_UNSUPPORTED;
+  }
   KspaceArgs a;
   a.k0 = k0;
   a.spec = spec; a.eigs = eigs; a.pc = t.pc; a.twx = t.twx; a.htab = t.htab; a.gtab = t.gtab;
@@ -1043,12 +1258,12 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
   a.nx = p.nx; a.ny = p.ny; a.nz = p.nz; a.M1 = p.M1; a.nS = p.nS; a.r = p.r; a.RT = p.RT; a.NK = p.NK; a.KT = p.KT;
   a.use_sinc = g.interp == FD3_INTERP_P3M;
   a.need_grad = need_grad; a.need_params = need_params;
-  const size_t sh = p.ks_lds;
-  const dim3 grid(p.ny / p.NK / p.KT, p.M1, (p.r + p.RT - 1) / p.RT);
+  const size_t sh = xs ? ksx_lds(p.nS * p.RT, need_params != 0) : p.ks_lds;
+  const dim3 grid = xs ? dim3(p.ks_blocks) : dim3(p.ny / p.NK / p.KT, p.M1, (p.r + p.RT - 1) / p.RT);
   ProfScope _ps("k_kspace", s);
 #define FD3_KS(NS_, RT_)                                                                              \
   case NS_ * 8 + RT_:                                                                                 \
-    return ks_launch<NS_, RT_>(p.ks2 != 0, need_params != 0, grid, sh, s, a);
+    return ks_launch<NS_, RT_>(p.ks2 != 0, xs, need_params != 0, grid, sh, s, a);
   switch (p.ks_reg * 8 + p.RT) {
     FD3_KS(1, 1) FD3_KS(1, 2) FD3_KS(1, 3) FD3_KS(1, 4)
     FD3_KS(2, 1) FD3_KS(2, 2) FD3_KS(2, 3) FD3_KS(2, 4)

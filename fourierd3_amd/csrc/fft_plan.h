@@ -29,6 +29,7 @@ struct FftPlan {
   int KT;                 // tiles (of NK ky rows) per k-space block
   int ks_reg;             // k-space variant: nS (<= kKsRegMaxS) with S and phi in registers, 0 = generic
   int ks2;                // register variant with the two-pass (16 x 16) x FFT: nx = 256, NK = 2
+  int xs;                 // ks2 with ny = 256: the y passes run the x FFT's first stage (split x FFT)
   int ks_blocks;          // grid size of the k-space kernel
   size_t ks_lds;          // dynamic LDS bytes of the k-space kernel
   size_t mesh_complex() const { return (size_t)C * nx * M1 * ny; }
@@ -58,6 +59,14 @@ constexpr int ks_maxb(int lines_per_row) { return lines_per_row <= 8 ? 2 : 4; }
 // Line stride (complex) of the two-pass x FFT of the k-space kernel: 2 * 272
 // = 32 (mod 64) banks between lines.
 constexpr int kKs2Ls = 272;
+// Dynamic LDS of the split x FFT variant (kspace_xs_body): 2 ky rows of
+// nl = nS * RT lines of 256 points (at most 32 KB, so five blocks fit the
+// 160 KB of a gfx950 CU), then with parameter gradients the h table and the
+// pair constants.
+constexpr size_t ksx_lds(int nl, bool params) {
+  return (size_t)2 * nl * 256 * sizeof(float2) +
+         (params ? (size_t)3 * (kHtabN + 1) * sizeof(float4) + (size_t)nl * nl * sizeof(KsPair) : 0);
+}
 // float4 entries of its table region: the h table with parameter gradients,
 // none without (the per-pair G tables are read from global memory).
 constexpr int ks2_table_f4(bool params) { return params ? 3 * (kHtabN + 1) : 0; }
@@ -85,8 +94,10 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
                     const PairInit* pairs = nullptr);
 // rho (real) -> A -> spec; spec holds the y/z-transformed data (x not yet transformed)
 int fft_forward_zy(const FftPlan& p, const FftTables& t, const float* rho, float2* A, float2* spec, hipStream_t s);
-// y pass alone: dir -1 A -> spec, dir +1 spec -> A
-int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s);
+// y pass alone: dir -1 A -> spec, dir +1 spec -> A.  xs (needs p.xs): the
+// pass also runs the first stage of the x FFT, for kspace_run(.., xs = true)
+int fft_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s,
+          bool xs = false);
 // plain in-place x transform of spec (dir -1 forward, +1 inverse)
 int fft_x(const FftPlan& p, const FftTables& t, float2* spec, int dir, hipStream_t s);
 // spec (x already inverse-transformed) -> A -> out (real)
@@ -94,7 +105,8 @@ int fft_inverse_yz(const FftPlan& p, const FftTables& t, const float2* spec, flo
 // fused forward-x / k-space / inverse-x; partials has p.ks_blocks * 16 doubles
 // (the pair constants come from fft_tables_init with a PairInit)
 // k0 (C): the k = 0 coefficients of dE/drho, zeroed in the spectrum.
+// xs: spec comes from (and goes back to) the y passes with the split x FFT.
 int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec, const float* eigs, int need_grad,
-               int need_params, double* partials, float2* k0, hipStream_t s);
+               int need_params, double* partials, float2* k0, hipStream_t s, bool xs = false);
 
 }  // namespace fd3

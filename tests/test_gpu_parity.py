@@ -394,6 +394,40 @@ def test_long_x_box_kspace_matches_oracle(method, order):
     assert de < E_TOL and df < F_TOL and ds < S_TOL and (dgp < E_TOL).all()
 
 
+@pytest.mark.parametrize("method,order,params", [("spme", 5, False), ("spme", 5, True), ("pme", 6, True)])
+def test_flat_box_split_x_fft_matches_oracle(method, order, params):
+    """A box stretched 8x along x and y gets a 256 x 256 x/y mesh: the y
+    passes run the first stage of the x FFT and the k-space kernel the second
+    (split x FFT); energy, forces, stress (and dE/dtheta) against the oracle."""
+    _say("flat box split x", method, order, params)
+    b = W.sio2_box(120, seed=15)
+    stretch = np.diag([8.0, 8.0, 1.0])
+    box = W.strained(W.Box(numbers=b.numbers, positions=b.positions @ stretch, cell=b.cell @ stretch), 0.05, seed=4)
+    calc = FourierD3ASECalculator(device=_dev(), method=method, interpolation_nodes=order)
+    calc._build_model(_atoms(box))
+    o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order)
+    assert list(o.ns) == list(calc._model.ns_mesh) and int(o.ns[0]) == 256 and int(o.ns[1]) == 256
+    pos = torch.tensor(box.positions, dtype=torch.float64, device=_dev())
+    res = calc.evaluate(pos, box.cell, need_grad=True, need_params=params)
+    torch.cuda.synchronize()
+    n = box.n_atoms
+    vol_b = abs(np.linalg.det(box.cell)) * ANGSTROM_TO_BOHR ** 3
+    e = float(res["energy"][0]) * HARTREE_TO_EV
+    f = -res["grad_pos"].cpu().numpy() * HARTREE_TO_EV
+    st = res["virial"].cpu().numpy() * HARTREE_TO_EV / vol_b * ANGSTROM_TO_BOHR ** 3
+    prm = o.params.clone().requires_grad_(params)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell, params=prm)
+    de, df, ds = _errors(e, f, st, e_ref, f_ref, s_ref, n)
+    msg = f"[errors] flat box {method} p={order} mesh {list(o.ns)}: E {de:.3e} F {df:.3e} S {ds:.3e}"
+    ok = de < E_TOL and df < F_TOL and ds < S_TOL
+    if params:
+        dgp = np.abs(res["grad_params"].cpu().numpy() * HARTREE_TO_EV - prm.grad.numpy()) / n
+        msg += f" dE/dtheta {dgp.max():.3e}"
+        ok = ok and bool((dgp < E_TOL).all())
+    print(msg, flush=True)
+    assert ok
+
+
 def test_param_gradients_match_oracle_autograd():
     _say("param gradients")
     box = W.sio2_box(96, seed=13)

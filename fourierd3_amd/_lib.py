@@ -26,6 +26,7 @@ FD3_DTYPE_F32 = 0
 FD3_DTYPE_F64 = 1
 FD3_NEED_GRAD = 1
 FD3_NEED_PARAMS = 2
+FD3_DETERMINISTIC = 4
 FD3_NOALLOC_SCRATCH_BYTES = 256 << 20
 
 # Every symbol include/fd3.h declares (checked by the CPU test suite).
@@ -47,6 +48,18 @@ EXPORTS = (
     "fd3_fft_workspace", "fd3_fft_r2c", "fd3_fft_c2r",
     "fd3_green_host", "fd3_green_table_host", "fd3_interp_weights_host",
 )
+
+
+def eval_flags(need_grad: bool, need_params: bool = False) -> int:
+    """Flags of an fd3_eval_* call.  Under ``torch.use_deterministic_algorithms(True)``
+    FD3_DETERMINISTIC is added: columns ordered by atom and the spread in that
+    order, so repeated calls return bit-identical results."""
+    import torch
+
+    f = (FD3_NEED_GRAD if need_grad else 0) | (FD3_NEED_PARAMS if need_params else 0)
+    if torch.are_deterministic_algorithms_enabled():
+        f |= FD3_DETERMINISTIC
+    return f
 
 
 class Model(C.Structure):

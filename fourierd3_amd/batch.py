@@ -73,7 +73,7 @@ class BoxBatch:
         self.species = torch.tensor(np.concatenate(species), dtype=torch.int32, device=self.device)
         self.model_of_box = np.asarray(model_of_box, dtype=np.int32)
         self._mstructs = (L.Model * len(self.models))(*[m._model_struct() for m in self.models])
-        self.flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        self.flags = L.eval_flags(need_grad, need_params)
         self._plan = C.c_void_p()
         mob = (C.c_int32 * self.n_boxes)(*[int(v) for v in model_of_box])
         offs_c = (C.c_int64 * (self.n_boxes + 1))(*[int(v) for v in self.offsets])

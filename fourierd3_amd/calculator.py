@@ -163,7 +163,7 @@ class FourierD3ASECalculator(_Base):
         n = positions.shape[0]
         g = model._geom(np.asarray(cell_A, dtype=np.float64))
         m = model._model_struct()
-        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        flags = L.eval_flags(need_grad, need_params)
         nbytes = C.c_size_t()
         L.check(L.lib().fd3_eval_celllist_workspace(C.byref(m), C.byref(g), n, flags, C.byref(nbytes)))
         if self._ws is None or self._ws.numel() < nbytes.value:
@@ -206,7 +206,7 @@ class FourierD3ASECalculator(_Base):
         n = positions.shape[0]
         g = model._geom(np.asarray(ref_cell_A, dtype=np.float64))
         m = model._model_struct()
-        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        flags = L.eval_flags(need_grad, need_params)
         nbytes = C.c_size_t()
         if d4:
             d4s = model._d4_struct()
@@ -262,7 +262,7 @@ class FourierD3ASECalculator(_Base):
             ns = T.get_ns_mesh(cell * a2b, self.mesh_spacing * a2b)
             geoms[b] = L.make_geom(cell, ns, model.interpolation_nodes, model._interp)
         m = model._model_struct()
-        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        flags = L.eval_flags(need_grad, need_params)
         offs_c = (C.c_int64 * len(offs))(*[int(v) for v in offs])
         nbytes = C.c_size_t()
         L.check(L.lib().fd3_eval_celllist_batch_workspace(C.byref(m), nb, geoms, offs_c, flags, C.byref(nbytes)))

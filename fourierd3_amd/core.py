@@ -332,7 +332,7 @@ class FourierD3(torch.nn.Module):
         else:
             g = self._geom(cell_A.detach().to(torch.float64).cpu().numpy())
         m = self._model_struct()
-        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        flags = L.eval_flags(need_grad, need_params)
         nbytes = C.c_size_t()
         if npt:
             L.check(L.lib().fd3_eval_edges_npt_workspace(C.byref(m), C.byref(g), self._npt_reach, n, ne, flags,
@@ -401,7 +401,7 @@ class FourierD3(torch.nn.Module):
             g = self._geom(self._npt_ref_cell)
         else:
             g = self._geom(cell_A.detach().to(torch.float64).cpu().numpy())
-        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        flags = L.eval_flags(need_grad, need_params)
         nbytes = C.c_size_t()
         if npt:
             L.check(L.lib().fd3_eval_d4_npt_workspace(C.byref(m), C.byref(d4), C.byref(g), self._npt_reach, n,
@@ -452,7 +452,7 @@ class FourierD3(torch.nn.Module):
             pos = pos.double()
         m = self._model_struct()
         cell = L.cell_array(torch.as_tensor(cell_A).detach().to(torch.float64).cpu().numpy())
-        flags = (L.FD3_NEED_GRAD if need_grad else 0) | (L.FD3_NEED_PARAMS if need_params else 0)
+        flags = L.eval_flags(need_grad, need_params)
         nbytes = C.c_size_t()
         L.check(L.lib().fd3_eval_direct_workspace(C.byref(m), cell, self.r_direct, n, flags, C.byref(nbytes)))
         ws = self._workspace(nbytes.value)

@@ -398,6 +398,7 @@ struct Common {
   double *p_ks, *p_c6, *p_ga, *p_cn, *p_red;
   int n_c6, n_ga, n_cn;
   const int32_t* status = nullptr;  // device cell check (NPT entry point)
+  int det = 0;                      // FD3_DETERMINISTIC (column order and spread)
 };
 
 static void carve_common(Arena& ar, Common& c, int64_t n, int r, int cn_rows) {
@@ -441,15 +442,17 @@ static FusedArgs fused_args(const fd3_model* m, const Common& c, int64_t n, cons
   for (int i = 0; i < 9; ++i) fa.inv[i] = c.geom.inv[i];
   fa.dyn = c.geom.dyn;
   fa.pos4 = pos4; fa.spec_s = spec_s; fa.charges = c.charges; fa.cb = c.cols;
+  fa.det = c.det;
   return fa;
 }
 
 // The part of the mesh path that needs only the model, the cell, positions
 // and species: FFT and Green's function tables, and the column sort of the
 // atoms (mesh methods).
-static int mesh_prep(const fd3_model* m, Common& c, int64_t n, const double4* pos4, const int32_t* spec_s,
+static int mesh_prep(const fd3_model* m, Common& c, int64_t n, const double4* pos4, const int32_t* spec_s, int flags,
                      hipStream_t s) {
   int rc;
+  c.det = (flags & FD3_DETERMINISTIC) != 0;
   PairInit pi;
   pi.qq = m->qq; pi.nS = m->n_species;
   pi.s6 = (float)m->s6; pi.s8 = (float)m->s8; pi.a1 = (float)m->a1; pi.a2 = (float)m->a2;
@@ -471,7 +474,8 @@ static int run_mesh_path(const fd3_model* m, Common& c, int64_t n, const double4
   int rc;
   const int r = m->n_rank, nS = m->n_species;
   const bool grad = flags & FD3_NEED_GRAD;
-  if (!prepped && (rc = mesh_prep(m, c, n, pos4, spec_s, s))) return rc;
+  c.det = (flags & FD3_DETERMINISTIC) != 0;
+  if (!prepped && (rc = mesh_prep(m, c, n, pos4, spec_s, flags, s))) return rc;
   C6Args ca;
   ca.n = n; ca.nS = nS; ca.r = r; ca.cn_raw = c.cn_raw; ca.spec_s = spec_s; ca.slot_atom = slot_atom;
   ca.cnref = m->cnref; ca.vq = m->vq; ca.eigs = m->eigs; ca.qq = m->qq; ca.cncorr = m->cncorr;
@@ -861,7 +865,7 @@ static int eval_d4(const fd3_model* m, const fd3_cn_d4* d4, const fd3_geom* g, d
     if ((rc = launch_cell_update(c, ck, cell_dev, cd, st, status_user, s))) return rc;
   }
   if ((rc = launch_prep_pos(positions, pos_dtype, species, n, pos4, spec_s, s))) return rc;
-  if ((rc = mesh_prep(m, c, n, pos4, spec_s, s))) return rc;
+  if ((rc = mesh_prep(m, c, n, pos4, spec_s, flags, s))) return rc;
   if ((rc = fill_value(db.ones, n, 1.0f, s))) return rc;
   if ((rc = d4_passes(m, d4, c, db, n, pos4, spec_s, false, s))) return rc;
   if ((rc = run_mesh_path(m, c, n, pos4, spec_s, nullptr, flags, out, s, true))) return rc;

@@ -35,6 +35,7 @@ struct FusedArgs {
   const int32_t* spec_s;
   const float* charges;  // (n, r) slot order
   ColBuffers cb;
+  int det = 0;  // FD3_DETERMINISTIC: columns ordered by slot, spread in that order
 };
 
 // sort atoms into mesh columns (positions and species only: the z passes

@@ -80,6 +80,28 @@ __device__ __forceinline__ void col_scatter_body(const BlockInfo& bk, const Fuse
 }
 
 
+// Deterministic mode: the records of each column (placed in arrival order
+// by col_scatter_body) sorted by slot, one thread per column (insertion sort:
+// columns hold a few atoms).
+__device__ __forceinline__ void col_order_body(const BlockInfo& bk, const FusedArgs& a) {
+  const int64_t key = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
+  if (key >= (int64_t)a.nS * a.nx * a.ny) return;
+  const int b = a.cb.colstart[key], m = a.cb.colcount[key];
+  int4* ri = a.cb.rec_i + b;
+  float4* rf = a.cb.rec_f + b;
+  for (int i = 1; i < m; ++i) {
+    const int4 ki = ri[i];
+    const float4 kf = rf[i];
+    int j = i - 1;
+    for (; j >= 0 && ri[j].w > ki.w; --j) {
+      ri[j + 1] = ri[j];
+      rf[j + 1] = rf[j];
+    }
+    ri[j + 1] = ki;
+    rf[j + 1] = kf;
+  }
+}
+
 int col_sort(const FusedArgs& a, hipStream_t s) {
   const size_t ncol = (size_t)a.nS * a.nx * a.ny;
   int rc;
@@ -89,6 +111,9 @@ int col_sort(const FusedArgs& a, hipStream_t s) {
     return rc;
   if ((rc = scan_exclusive(a.cb.colcount, (int64_t)ncol, a.cb.colstart, a.cb.scantmp, nullptr, s))) return rc;
   if (a.n > 0 && (rc = launch<FusedArgs, col_scatter_body, 256>("k_col_scatter", dim3(cdiv(a.n, 256)), 0, s, a)))
+    return rc;
+  if (a.det && a.n > 0 &&
+      (rc = launch<FusedArgs, col_order_body, 256>("k_col_order", dim3(cdiv((int64_t)ncol, 256)), 0, s, a)))
     return rc;
   return FD3_OK;
 }
@@ -176,6 +201,40 @@ __device__ __forceinline__ void lds_add_f32_cas(float* line, const int* z, const
         if (old == as) pend &= ~(1u << j);
         else cur[j] = __uint_as_float(old);
       }
+    }
+  }
+}
+
+// Deterministic spread (FD3_DETERMINISTIC): the block's atoms one after the
+// other in list order (columns in segment order, each column ordered by slot
+// by col_order_body); the first wave adds an atom's p x p stencil nodes of
+// this plane, lane k taking node (k / p, k mod p), with plain LDS
+// read-modify-writes (distinct nodes: no two lanes meet), or one lane doing
+// all of them when the mesh is shorter than the stencil along y or z.
+__device__ __forceinline__ void spread_ordered(float* ldsf, const Segs& sg, const FusedArgs& a, int q, int y0, int TY,
+                                            int nz, int ls, int p) {
+  if (threadIdx.x >= 64) return;
+  const int lane = threadIdx.x;
+  const bool serial = p > a.ny || p > nz;
+  const int total = sg.off[sg.n];
+  for (int i = 0; i < total; ++i) {
+    int k = 0;
+    while (i >= sg.off[k + 1]) ++k;
+    const int ai = sg.a0[k] + (i - sg.off[k]);
+    const int4 bi = a.cb.rec_i[ai];
+    const float4 xf = a.cb.rec_f[ai];
+    float w[kMaxOrder], dw[kMaxOrder], wy[kMaxOrder], wz[kMaxOrder];
+    interp_weights<float>(p, a.interp, xf.x, w, dw);
+    const float wx = pick(w, sg.jx[k], p) * a.charges[(size_t)bi.w * a.r + q];
+    interp_weights<float>(p, a.interp, xf.y, wy, dw);
+    interp_weights<float>(p, a.interp, xf.z, wz, dw);
+    const int n0 = serial ? 0 : lane, n1 = serial ? (lane == 0 ? p * p : 0) : min(lane + 1, p * p);
+    for (int node = n0; node < n1; ++node) {
+      const int jy = node / p, jz = node - jy * p;
+      const int t = (bi.y + jy) % a.ny - y0;
+      if (t < 0 || t >= TY) continue;
+      const int z = (bi.z + jz) % nz;
+      ldsf[2 * t * ls + z] += wx * pick(wy, jy, p) * pick(wz, jz, p);
     }
   }
 }
@@ -363,7 +422,11 @@ __device__ __forceinline__ void spread_r2c_z_body(const BlockInfo& bk, const ZPa
   __syncthreads();
   float* ldsf = (float*)lds;
   const int total = sg.off[sg.n];
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+  if (a.det) {
+    spread_ordered(ldsf, sg, a, q, y0, TY, nz, ls, p);
+    __syncthreads();
+  }
+  for (int i = a.det ? total : threadIdx.x; i < total; i += blockDim.x) {
     int k = 0;
     while (i >= sg.off[k + 1]) ++k;
     const int ai = sg.a0[k] + (i - sg.off[k]);

@@ -44,6 +44,14 @@ extern "C" {
 /* Flags of fd3_eval_* */
 #define FD3_NEED_GRAD 1    /* compute forces / virial / cell and shift gradients */
 #define FD3_NEED_PARAMS 2  /* compute dE/d(s6, s8, a1, a2)                        */
+/* Deterministic order (no reference counterpart: the reference's index_add_
+ * spread, src/fourierd3/mesh_interpolator_d3.py:128-133, is order-dependent
+ * on the GPU).  The atoms of each mesh column are ordered by index and the
+ * spread adds every atom's stencil in that order with plain read-modify-write
+ * steps instead of concurrent compare-and-swap adds, so repeated calls on the
+ * same inputs return bit-identical results.  Slower spread; a debugging and
+ * reproducibility mode. */
+#define FD3_DETERMINISTIC 4
 
 /* Per-species model constants.  All arrays are float32 [device].
  * Mirrors the buffers FourierD3.__init__ registers (src/fourierd3/core.py:139-182)

@@ -951,3 +951,56 @@ def test_hip_path_matches_golden_fixtures(path):
                          float(d["energy"]), d["forces"], d["stress"], len(d["numbers"]))
     print(f"[errors] golden {name}: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
     assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
+def _eval_bits(calc, box, need_params=False):
+    pos = torch.tensor(box.positions, dtype=torch.float64, device=_dev())
+    res = calc.evaluate(pos, box.cell, need_grad=True, need_params=need_params)
+    torch.cuda.synchronize()
+    return {k: v.detach().cpu().numpy().copy() for k, v in res.items() if torch.is_tensor(v)}
+
+
+@pytest.mark.parametrize("cnfunc", ["smooth_cut", "d4"])
+def test_deterministic_mode_is_bitwise_reproducible(cnfunc):
+    """Under torch.use_deterministic_algorithms(True) the library orders every
+    mesh column by atom and spreads in that order (FD3_DETERMINISTIC): three
+    evaluations of a 20k-atom H/C/N/O box return bit-identical energy, forces,
+    virial and dE/dtheta, and they agree with the default (concurrent) spread
+    at the stated tolerances."""
+    _say("deterministic mode", cnfunc)
+    box = W.strained(W.hcno_box(20_000, seed=21), 0.05, seed=21)
+    calc = FourierD3ASECalculator(device=_dev(), cnfunc=cnfunc)
+    calc._build_model(_atoms(box))
+    fast = _eval_bits(calc, box, need_params=True)
+    prev = torch.are_deterministic_algorithms_enabled()
+    torch.use_deterministic_algorithms(True)
+    try:
+        runs = [_eval_bits(calc, box, need_params=True) for _ in range(3)]
+    finally:
+        torch.use_deterministic_algorithms(prev)
+    for k in runs[0]:
+        for r in runs[1:]:
+            assert np.array_equal(runs[0][k], r[k]), f"{k} differs between deterministic runs"
+    n = box.n_atoms
+    de = abs(float(runs[0]["energy"][0]) - float(fast["energy"][0])) * HARTREE_TO_EV / n
+    df = float(np.abs(runs[0]["grad_pos"] - fast["grad_pos"]).max()) * HARTREE_TO_EV
+    print(f"[errors] deterministic vs default spread ({cnfunc}): E {de:.3e} F {df:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL
+
+
+def test_deterministic_mode_matches_oracle():
+    """The deterministic spread against the FP64 oracle (SiO2, triclinic)."""
+    _say("deterministic mode vs oracle")
+    box = W.strained(W.sio2_box(192, seed=22), 0.08, seed=22)
+    prev = torch.are_deterministic_algorithms_enabled()
+    torch.use_deterministic_algorithms(True)
+    try:
+        calc = FourierD3ASECalculator(device=_dev())
+        calc.calculate(_atoms(box))
+    finally:
+        torch.use_deterministic_algorithms(prev)
+    e_ref, f_ref, s_ref = OracleFourierD3(box.numbers, box.cell).calculate(box.positions, box.cell)
+    de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"], e_ref, f_ref, s_ref,
+                         box.n_atoms)
+    print(f"[errors] deterministic mode vs oracle: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL

@@ -182,3 +182,26 @@ def test_backward_is_first_order_only():
                                               torch.zeros((4, 3), dtype=torch.float64))
         with pytest.raises(RuntimeError, match="second derivatives"):
             torch.autograd.grad(e2, pos, create_graph=True)
+
+
+def test_eval_flags_follow_torch_deterministic_mode():
+    """FD3_DETERMINISTIC is added exactly while torch's deterministic mode is
+    on, and the header and the binding agree on the flag values."""
+    import re
+
+    import torch
+
+    text = HEADER.read_text()
+    for name in ("FD3_NEED_GRAD", "FD3_NEED_PARAMS", "FD3_DETERMINISTIC"):
+        m = re.search(rf"#define {name} (\d+)", text)
+        assert m and int(m.group(1)) == getattr(L, name), name
+    prev = torch.are_deterministic_algorithms_enabled()
+    try:
+        torch.use_deterministic_algorithms(False)
+        assert L.eval_flags(True, True) == L.FD3_NEED_GRAD | L.FD3_NEED_PARAMS
+        assert L.eval_flags(False) == 0
+        torch.use_deterministic_algorithms(True)
+        assert L.eval_flags(True, False) == L.FD3_NEED_GRAD | L.FD3_DETERMINISTIC
+        assert L.eval_flags(False, True) == L.FD3_NEED_PARAMS | L.FD3_DETERMINISTIC
+    finally:
+        torch.use_deterministic_algorithms(prev)

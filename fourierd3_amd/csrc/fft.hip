@@ -223,11 +223,11 @@ __device__ __forceinline__ void ks2_twiddle(float2 (&v)[16], const float2* tw, i
 // stores (16 lanes over k1) and the row-wise copies (16 lanes over l) both
 // are.  T takes the place of the lines.
 //
-// XS (split x FFT, nx = ny = 256, NL = 32): the block also runs the first
-// stage of the 256-point x FFT (256 = 16 x 16, x = 16 n1 + n2) on its tile.
-// Block b takes the x lines n2 in {2b, 2b + 1}, all n1 (tile column
-// l = 16 (n2 - 2b) + n1), and the rows are stored in the order
-// 16 n2 + k1 (tile column 16 (n2 - 2b) + k1 at row position 32 b + column):
+// XS (split x FFT, nx = ny = 256): the block also runs the first stage of
+// the 256-point x FFT (256 = 16 x 16, x = 16 n1 + n2) on its tile.  Block b
+// takes the x lines n2 in [H b, H b + H), H = NL / 16, all n1 (tile column
+// l = 16 (n2 - H b) + n1), and the rows are stored in the order 16 n2 + k1
+// (tile column 16 (n2 - H b) + k1 at row position NL b + column):
 //   forward  Y[k1, n2] = W256^(n2 k1) sum_n1 W16^(n1 k1) x[16 n1 + n2],
 //   inverse  x[16 n1 + n2] = sum_k1 W16^-(n1 k1) W256^-(n2 k1) Y[k1, n2],
 // and the k-space kernel finishes (or starts) the transform with one
@@ -237,27 +237,27 @@ __device__ __forceinline__ void ks2_twiddle(float2 (&v)[16], const float2* tw, i
 template <int NL>
 __device__ __forceinline__ int y2_tile(int ky, int l) { return NL * ky + ((l + ky) & (NL - 1)); }
 
-template <int DIR>
+template <int DIR, int NL>
 __device__ __forceinline__ void y2_x_stage(float2* T, const float2* tws, int b) {
+  constexpr int NH = NL / 16;  // n2 values per block
   const int t = threadIdx.x, ky = t & 255, h = t >> 8;
   float2 v[16];
 #pragma unroll
-  for (int i = 0; i < 16; ++i) v[i] = T[y2_tile<32>(ky, 16 * h + i)];
+  for (int i = 0; i < 16; ++i) v[i] = T[y2_tile<NL>(ky, 16 * h + i)];
   if (DIR < 0) {
-    fft16r<-1>(v);                    // over n1 -> k1
-    ks2_twiddle<-1>(v, tws, 2 * b + h);  // W256^(n2 k1)
+    fft16r<-1>(v);                      // over n1 -> k1
+    ks2_twiddle<-1>(v, tws, NH * b + h);  // W256^(n2 k1)
   } else {
-    ks2_twiddle<1>(v, tws, 2 * b + h);   // W256^-(n2 k1)
-    fft16r<1>(v);                     // over k1 -> n1
+    ks2_twiddle<1>(v, tws, NH * b + h);   // W256^-(n2 k1)
+    fft16r<1>(v);                       // over k1 -> n1
   }
 #pragma unroll
-  for (int i = 0; i < 16; ++i) T[y2_tile<32>(ky, 16 * h + i)] = v[i];
+  for (int i = 0; i < 16; ++i) T[y2_tile<NL>(ky, 16 * h + i)] = v[i];
 }
 
 template <int DIR, int NL, bool XS = false>
 __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs& ya) {
   static_assert(NL == 16 || NL == 32, "y2: 16 or 32 lines");
-  static_assert(!XS || NL == 32, "y2: the split x FFT takes 32 lines");
   constexpr int NT = 16 * NL, LR = NL == 16 ? 4 : 5;
   extern __shared__ float2 lds[];
   const float2* __restrict__ in = ya.in;
@@ -270,7 +270,7 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
   stage_table(tws, ya.twy, ny);
   const int t = threadIdx.x, l = t >> 4, j = t & 15;
   // x of this thread's y line: x0 + l, or with XS 16 n1 + n2 (l = 16 (n2 - 2b) + n1)
-  const int xl = XS ? 16 * (l & 15) + 2 * bk.id.x + (l >> 4) : x0 + l;
+  const int xl = XS ? 16 * (l & 15) + (NL / 16) * bk.id.x + (l >> 4) : x0 + l;
   const long long yline = ((long long)(c * nx + xl) * M1 + kz) * ny;  // this thread's y line
   // rows ky = (t >> LR) + 16 it of the transposed side, x = x0 + (t mod NL)
   const size_t row0 = ((size_t)(c * M1 + kz) * ny + (t >> LR)) * nx + x0 + (t & (NL - 1));
@@ -297,7 +297,7 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
     for (int k2 = 0; k2 < 16; ++k2) S[y2_tile<NL>(j + 16 * k2, l)] = z[k2];
     __syncthreads();
     if constexpr (XS) {
-      y2_x_stage<-1>(S, tws, bk.id.x);
+      y2_x_stage<-1, NL>(S, tws, bk.id.x);
       __syncthreads();
     }
     float2 w[16];
@@ -313,7 +313,7 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
     for (int it = 0; it < 16; ++it) S[y2_tile<NL>(ty + 16 * it, tl)] = w[it];
     __syncthreads();  // T filled, twiddles staged
     if constexpr (XS) {
-      y2_x_stage<1>(S, tws, bk.id.x);
+      y2_x_stage<1, NL>(S, tws, bk.id.x);
       __syncthreads();
     }
     float2 z[16];
@@ -1162,7 +1162,14 @@ static int launch_y(const FftPlan& p, const FftTables& t, const float2* in, floa
     const YPassArgs ya{in, out, p.nx, p.ny, p.M1, nl, t.twy};
     if (dir < 0) {
       ProfScope _ps("k_c2c_y_fwd", s);
-      if (xs) return launch<YPassArgs, c2c_y2_body<-1, 32, true>, 512>("k_c2c_y_fwd", grid2, sh2, s, ya);
+      // split x FFT: 16 lines per forward block, 32 per inverse block (the
+      // faster of the two for each direction: 0.226 / 0.230 ms forward,
+      // 0.230 / 0.220 ms inverse on the 100k-atom box)
+      if (xs)
+        return launch<YPassArgs, c2c_y2_body<-1, 16, true>, 256>("k_c2c_y_fwd", dim3(16, p.M1, p.C),
+                                                                 (size_t)16 * kKs2Ls * sizeof(float2) +
+                                                                     (size_t)p.ny * sizeof(float2),
+                                                                 s, ya);
       if (nl == 32) return launch<YPassArgs, c2c_y2_body<-1, 32>, 512>("k_c2c_y_fwd", grid2, sh2, s, ya);
       return launch<YPassArgs, c2c_y2_body<-1, 16>, 256>("k_c2c_y_fwd", grid2, sh2, s, ya);
     }

@@ -71,8 +71,11 @@ def _say(*a):
 
 def kernel_bytes(name: str, plan: dict) -> float | None:
     """Compulsory HBM bytes of one launch of `name` (algorithmic, not measured):
-    M = mesh points, Mh = half-spectrum points, C = channels, N = atoms."""
-    C, M, Mh, N = plan["C"], plan["M"], plan["Mh"], plan["N"]
+    M = mesh points, Mh = half-spectrum points, C = channels, S = species,
+    N = atoms.  With D4 coordination numbers the spread, y and gather passes
+    also run over the S species channels under the same names, so only the
+    k-space kernels (one channel count each) have a byte count there."""
+    C, M, Mh, N, S = plan["C"], plan["M"], plan["Mh"], plan["N"], plan["S"]
     table = {
         "k_c2c_y_fwd": 16 * Mh * C,              # read + write complex half spectrum
         "k_kspace": 16 * Mh * C,                 # read S, write dE/drho spectrum (x passes fused)
@@ -80,7 +83,13 @@ def kernel_bytes(name: str, plan: dict) -> float | None:
         # spreading fused into the z pass: the real mesh never reaches HBM
         "k_spread_r2c_z": 8 * Mh * C + 36 * N,   # write half spectrum; read positions, species, charges
         "k_c2r_z_gather": 8 * Mh * C + 36 * N,   # read half spectrum; per-atom records in, accumulators out
+        # D4 CN convolution of the species densities: read + write the spectrum,
+        # write (forward) or read (backward) the x-transformed density
+        "k_kspace_cn": 24 * Mh * S,
+        "k_kspace_cn_bwd": 24 * Mh * S,
     }
+    if plan.get("d4") and name not in ("k_kspace", "k_kspace_cn", "k_kspace_cn_bwd"):
+        return None
     return table.get(name)
 
 
@@ -114,8 +123,8 @@ def run_box(box, steps, warmup, c6tol=1.0, cnfunc="smooth_cut", profile=True, tr
         calc.evaluate(pos, box.cell, out=out)
     torch.cuda.synchronize()
     ns = [int(v) for v in model.ns_mesh]
-    plan = {"C": model.n_species * model.n_rank, "M": ns[0] * ns[1] * ns[2],
-            "Mh": ns[0] * ns[1] * (ns[2] // 2 + 1), "N": box.n_atoms}
+    plan = {"C": model.n_species * model.n_rank, "S": model.n_species, "M": ns[0] * ns[1] * ns[2],
+            "Mh": ns[0] * ns[1] * (ns[2] // 2 + 1), "N": box.n_atoms, "d4": cnfunc == "d4"}
 
     def instrumented(n):
         L.profile_only(None)

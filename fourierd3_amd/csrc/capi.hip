@@ -794,11 +794,11 @@ static int d4_passes(const fd3_model* m, const fd3_cn_d4* d4, Common& c, D4Bufs&
   if (bwd) fs.charges = db.wf;
   float2* spec = (float2*)c.buf1;
   if ((rc = spread_r2c_z(fs, db.pcn, c.tab, c.buf2, s))) return rc;
-  if ((rc = fft_y(db.pcn, c.tab, c.buf2, spec, -1, s))) return rc;
+  if ((rc = fft_y(db.pcn, c.tab, c.buf2, spec, -1, s, db.kp.xs))) return rc;
   const CnKsArgs ka = d4_ks_args(m, d4, c, db);
   if (!bwd && (rc = cn_tab_init(ka, ks_gtab_dk(c.plan, c.geom), s))) return rc;
   if ((rc = cn_ks_run(ka, db.kp, bwd, s))) return rc;
-  if ((rc = fft_y(db.pcn, c.tab, spec, c.buf2, 1, s))) return rc;
+  if ((rc = fft_y(db.pcn, c.tab, spec, c.buf2, 1, s, db.kp.xs))) return rc;
   if ((rc = c2r_z_gather(fa, db.pcn, c.tab, c.buf2, s))) return rc;  // and the per-atom sums
   const CnAtomArgs aa = d4_atom_args(d4, c, db, n, spec_s);
   return bwd ? cn_d4_force(aa, s) : cn_d4_fin(aa, s);

@@ -16,6 +16,7 @@
 // stores the x-transformed density spectrum.
 
 #include "cn_d4.h"
+#include "fft_reg.h"
 #include "fft_lds.h"
 
 namespace fd3 {
@@ -201,6 +202,111 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
   }
 }
 
+// The same convolution with the split x FFT (nx = ny = 256, NS <= 4
+// species): the y passes have run the first stage of the x FFT, so a thread
+// (l, k1) of a tile (kKsxNK ky rows x NS species, line l = X * kKsxNK + kyi)
+// does one 16-point FFT over n2 per direction (ksx_load / ksx_inverse,
+// fft_reg.h); the NS values of a k point sit in registers.  The forward
+// variant also stores the x-transformed density (natural order, rho_hat),
+// the backward one loads it beside the spectra (R) for the strain
+// derivative.  Block b takes the KT consecutive tiles b KT .. b KT + KT - 1,
+// ky tile fastest.  LDS: S (and R) lines of 256 points.
+constexpr int kKsxNK = 4;
+template <bool BWD, int NS>
+__global__ void __launch_bounds__(256) k_kspace_cn_xs(CnKsArgs a) {
+  if (a.dyn) {
+    for (int i = 0; i < 9; ++i)
+      a.recip[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.dyn->recip_f[i])));
+    a.inv_vol = a.dyn->inv_volume;
+  }
+  constexpr int NK = kKsxNK, ls = 256, nlines = NS * NK;
+  extern __shared__ float2 lds[];
+  __shared__ double red[4 * 6];
+  float2* S = lds;
+  float2* R = S + nlines * ls;  // backward only
+  const int nyp = a.ny / NK, ntiles = nyp * a.M1;
+  const int xl = threadIdx.x >> 4, xj = threadIdx.x & 15;
+  const int xX = xl / NK, xkyi = xl - xX * NK;
+  const bool live = xl < nlines;
+  double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int u0 = blockIdx.x * a.KT, u1 = min(ntiles, u0 + a.KT);
+  for (int u = u0; u < u1; ++u) {
+    const int kyp = u % nyp, kz = u / nyp, ky0 = kyp * NK;
+    const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
+    const long long lb = (((long long)xX * a.M1 + kz) * a.ny + ky0 + xkyi) * 256;
+    float2 v[16];
+    ksx_load(v, a.spec, live, lb + xj);
+    __syncthreads();  // the previous tile is done with S and R
+    if (live) {
+      fft16r<-1>(v);  // over n2 -> k2: X[xj + 16 k2]
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) S[xl * ls + xj + 16 * k2] = v[k2];
+      if constexpr (!BWD) {
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) a.rho_hat[lb + xj + 16 * k2] = v[k2];
+      } else {
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) R[xl * ls + xj + 16 * k2] = a.rho_hat[lb + xj + 16 * k2];
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < NK * 256; t += blockDim.x) {
+      const int kyi = t >> 8, kx = t & 255, ky = ky0 + kyi;
+      const float mx = (float)(kx < 128 ? kx : kx - 256);
+      const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
+      const float mz = (float)kz;
+      const float k0v = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
+      const float k1v = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
+      const float k2v = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
+      const float kn = sqrtf(k0v * k0v + k1v * k1v + k2v * k2v);
+      const float isc = sinc_prod(a, kx, ky, kz);
+      const float fk = kn * a.tkinv;
+      const int ti = min((int)fk, kGtabN - 1);
+      const float tw = fk - (float)ti;
+      const float4* te = a.tab + 2 * ti;
+      float2 sx[NS], px[NS], rx[BWD ? NS : 1];
+#pragma unroll
+      for (int X = 0; X < NS; ++X) {
+        sx[X] = S[(X * NK + kyi) * ls + kx];
+        if constexpr (BWD) rx[X] = R[(X * NK + kyi) * ls + kx];
+        px[X] = make_float2(0.f, 0.f);
+      }
+      float vs = 0.f;  // sum_XY dG/dk Re(conj(S_X) rho_Y)
+#pragma unroll
+      for (int X = 0; X < NS; ++X)
+#pragma unroll
+        for (int Y = X; Y < NS; ++Y) {
+          const int pt = X * NS - X * (X - 1) / 2 + (Y - X);
+          float g, dg;
+          gtab_eval(te[2 * kGtabN * pt], te[2 * kGtabN * pt + 1], tw, &g, &dg);
+          g *= isc;
+          px[X].x += g * sx[Y].x; px[X].y += g * sx[Y].y;
+          if (Y != X) { px[Y].x += g * sx[X].x; px[Y].y += g * sx[X].y; }
+          if constexpr (BWD) {
+            float t2 = sx[X].x * rx[Y].x + sx[X].y * rx[Y].y;
+            if (Y != X) t2 += sx[Y].x * rx[X].x + sx[Y].y * rx[X].y;
+            vs += dg * t2;
+          }
+        }
+#pragma unroll
+      for (int X = 0; X < NS; ++X) S[(X * NK + kyi) * ls + kx] = px[X];
+      if (BWD && kn > 0.f) {
+        // d|k|/d eps_ab = -k_a k_b / |k| under the strain cell' = (1 + eps) cell
+        const double f = -(double)w * (double)(vs * a.tkinv * isc) / kn;
+        vir[0] += f * k0v * k0v; vir[1] += f * k1v * k1v; vir[2] += f * k2v * k2v;
+        vir[3] += f * k0v * k1v; vir[4] += f * k0v * k2v; vir[5] += f * k1v * k2v;
+      }
+    }
+    __syncthreads();
+    ksx_inverse(S, ls, a.spec, live, lb + xj);
+  }
+  if (BWD) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) vir[i] *= a.inv_vol;
+    block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
+  }
+}
+
 int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
   // two butterflies per thread (tiles of 2048 values: with 4 species and
   // 256-point lines a full tile of the unrolled FFT) where the species fit,
@@ -219,6 +325,12 @@ int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
   while (kp->KT < 4 && (p.ny / NK) % (2 * kp->KT) == 0 && (size_t)(p.ny / NK / (2 * kp->KT)) * p.M1 >= 4096)
     kp->KT <<= 1;
   kp->blocks = (p.ny / NK / kp->KT) * p.M1;
+  kp->xs = p.nx == 256 && p.ny == 256 && nS <= 4;
+  if (kp->xs) {  // tiles of kKsxNK ky rows, two per block
+    kp->NK = kKsxNK;
+    kp->KT = 2;
+    kp->blocks = (p.ny / kKsxNK * p.M1 + kp->KT - 1) / kp->KT;
+  }
   const size_t tile = (size_t)nS * NK * (p.nx + 1) * sizeof(float2);
   // the backward variant stays generic: holding the stored density as well
   // takes the register form to about 170 VGPRs, and it measured slower
@@ -255,7 +367,34 @@ static int cn_ks_species(const CnKsArgs& b, const CnKsPlan& kp, dim3 grid, size_
   }
 }
 
+template <bool BWD>
+static int cn_ks_xs(const CnKsArgs& b, const CnKsPlan& kp, hipStream_t s) {
+  const size_t sh = (size_t)(BWD ? 2 : 1) * b.nS * kKsxNK * 256 * sizeof(float2);
+  const dim3 grid(kp.blocks);
+  switch (b.nS) {
+    case 1: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 1>), grid, dim3(256), sh, s, b); break;
+    case 2: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 2>), grid, dim3(256), sh, s, b); break;
+    case 3: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 3>), grid, dim3(256), sh, s, b); break;
+    default: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 4>), grid, dim3(256), sh, s, b); break;
+  }
+  return FD3_OK;
+}
+
 int cn_ks_run(const CnKsArgs& a, const CnKsPlan& kp, bool bwd, hipStream_t s) {
+  if (kp.xs) {
+    if (a.nx != 256 || a.ny != 256 || a.nS < 1 || a.nS > 4) {
+      set_error("split x D4 CN k-space kernel planned for a %d x %d mesh with %d species", a.nx, a.ny, a.nS);
+      return FD3_ERR_UNSUPPORTED;
+    }
+    CnKsArgs b = a;
+    b.NK = kp.NK;
+    b.KT = kp.KT;
+    ProfScope _ps(bwd ? "k_kspace_cn_bwd" : "k_kspace_cn", s);
+    const int rc = bwd ? cn_ks_xs<true>(b, kp, s) : cn_ks_xs<false>(b, kp, s);
+    if (rc) return rc;
+    FD3_LAUNCH_CHECK("k_kspace_cn");
+    return FD3_OK;
+  }
   const size_t sh = bwd ? kp.lds_bwd : kp.lds;
   const dim3 grid(a.ny / kp.NK / kp.KT, a.M1);
   CnKsArgs b = a;

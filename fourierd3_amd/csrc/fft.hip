@@ -589,57 +589,6 @@ __device__ __forceinline__ void ks2_inverse(float2* S, const float2* tw, float2*
   }
 }
 
-// The same x FFT when the y pass has already run its first stage (split x
-// FFT, c2c_y2_body<.., XS>): a line holds Y[k1, n2] at 16 n2 + k1, and
-// thread (l, k1) takes its 16 values Y[k1, 0..15] (16 lanes over k1 read 128
-// contiguous bytes), does the 16-point FFT over n2 and stores X[k1 + 16 k2]
-// in natural order; the inverse mirrors it.  No twiddles, no LDS exchange.
-// Lines past the tile or of ranks past the last (live = false) read as
-// zeros and are not written.
-
-// this thread's 16 values of one line, starting at element e (= line base + k1)
-__device__ __forceinline__ void ksx_load(float2 (&v)[16], const float2* __restrict__ g, bool live, long long e) {
-  if (live) {
-    const float2* src = g + e;
-#pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) v[n2] = src[16 * n2];
-  } else {
-#pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) v[n2] = make_float2(0.f, 0.f);
-  }
-}
-
-// Forward from the loaded values into the first nlines lines (stride ls)
-// of S; ends with a barrier.
-__device__ __forceinline__ void ksx_forward_regs(float2* S, int ls, const float2 (&v)[16], int nlines) {
-  const int t = threadIdx.x, l = t >> 4, j = t & 15;
-  if (l < nlines) {
-    float2 z[16];
-#pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) z[n2] = v[n2];
-    fft16r<-1>(z);  // over n2 -> k2: X[j + 16 k2]
-    float2* L = S + l * ls;
-#pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) L[j + 16 * k2] = z[k2];
-  }
-  __syncthreads();
-}
-
-__device__ __forceinline__ void ksx_inverse(const float2* S, int ls, float2* __restrict__ g, bool live,
-                                           long long e) {
-  const int t = threadIdx.x, l = t >> 4, j = t & 15;
-  const float2* L = S + l * ls;
-  if (live) {
-    float2 z[16];
-#pragma unroll
-    for (int k2 = 0; k2 < 16; ++k2) z[k2] = L[j + 16 * k2];  // k1 = j
-    fft16r<1>(z);  // over k2 -> n2: Y[j, n2]
-    float2* dst = g + e;
-#pragma unroll
-    for (int n2 = 0; n2 < 16; ++n2) dst[16 * n2] = z[n2];
-  }
-}
-
 // One k point (column kx of tile row kyi, element kxe of a line) of the
 // register k-space kernels: G~ of every species pair, phi_X = sum_Y G~ S_Y
 // over the NS*RT spectra of the point (read from the tile lines in S,
@@ -1151,7 +1100,7 @@ int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sin
 // LDS is allowed per kernel by launch().
 static int launch_y(const FftPlan& p, const FftTables& t, const float2* in, float2* out, int dir, hipStream_t s,
                     bool xs) {
-  if (xs && !(p.xs && p.nx == 256 && p.ny == 256)) {
+  if (xs && !(p.nx == 256 && p.ny == 256)) {
     set_error("split x FFT planned for a mesh that is not 256 x 256 in x, y");
     return FD3_ERR_UNSUPPORTED;
   }

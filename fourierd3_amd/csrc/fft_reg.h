@@ -175,4 +175,55 @@ __device__ __forceinline__ void fft16r(float2 (&v)[16]) {
   }
 }
 
+// The second stage of a 256-point x FFT whose first stage the y pass has
+// run (split x FFT, c2c_y2_body<.., XS> in fft.hip): a line holds Y[k1, n2]
+// at 16 n2 + k1, and thread (l, k1) takes its 16 values Y[k1, 0..15] (16
+// lanes over k1 read 128 contiguous bytes), does the 16-point FFT over n2
+// and stores X[k1 + 16 k2] in natural order; the inverse mirrors it.  No
+// twiddles, no LDS exchange.  Lines past the tile or of ranks past the last
+// (live = false) read as zeros and are not written.
+
+// this thread's 16 values of one line, starting at element e (= line base + k1)
+__device__ __forceinline__ void ksx_load(float2 (&v)[16], const float2* __restrict__ g, bool live, long long e) {
+  if (live) {
+    const float2* src = g + e;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = src[16 * n2];
+  } else {
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = make_float2(0.f, 0.f);
+  }
+}
+
+// Forward from the loaded values into the first nlines lines (stride ls)
+// of S; ends with a barrier.
+__device__ __forceinline__ void ksx_forward_regs(float2* S, int ls, const float2 (&v)[16], int nlines) {
+  const int t = threadIdx.x, l = t >> 4, j = t & 15;
+  if (l < nlines) {
+    float2 z[16];
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) z[n2] = v[n2];
+    fft16r<-1>(z);  // over n2 -> k2: X[j + 16 k2]
+    float2* L = S + l * ls;
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) L[j + 16 * k2] = z[k2];
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ void ksx_inverse(const float2* S, int ls, float2* __restrict__ g, bool live,
+                                           long long e) {
+  const int t = threadIdx.x, l = t >> 4, j = t & 15;
+  const float2* L = S + l * ls;
+  if (live) {
+    float2 z[16];
+#pragma unroll
+    for (int k2 = 0; k2 < 16; ++k2) z[k2] = L[j + 16 * k2];  // k1 = j
+    fft16r<1>(z);  // over k2 -> n2: Y[j, n2]
+    float2* dst = g + e;
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) dst[16 * n2] = z[n2];
+  }
+}
+
 }  // namespace fd3

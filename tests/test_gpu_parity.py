@@ -428,6 +428,26 @@ def test_flat_box_split_x_fft_matches_oracle(method, order, params):
     assert ok
 
 
+@pytest.mark.parametrize("case,method,order", [("sio2", "spme", 5), ("hcno", "spme", 5), ("hcno", "pme", 6)])
+def test_flat_box_split_x_fft_d4_matches_oracle(case, method, order):
+    """cnfunc='d4' on boxes stretched 8x along x and y (256 x 256 x 32 mesh):
+    the D4 CN k-space kernels (forward and backward) run the second stage of
+    the split x FFT for 2 and 4 species; E, F and stress against the oracle."""
+    _say("flat box split x d4", case, method, order)
+    b = W.sio2_box(120, seed=16) if case == "sio2" else W.hcno_box(300, seed=23)
+    stretch = np.diag([8.0, 8.0, 1.0])
+    box = W.strained(W.Box(numbers=b.numbers, positions=b.positions @ stretch, cell=b.cell @ stretch), 0.05, seed=5)
+    calc = FourierD3ASECalculator(device=_dev(), cnfunc="d4", method=method, interpolation_nodes=order)
+    calc.calculate(_atoms(box))
+    o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order, cnfunc="d4")
+    assert list(o.ns) == list(calc._model.ns_mesh) and int(o.ns[0]) == 256 and int(o.ns[1]) == 256
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"], e_ref, f_ref,
+                         s_ref, box.n_atoms)
+    print(f"[errors] flat box d4 {case} {method} p={order}: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
 def test_param_gradients_match_oracle_autograd():
     _say("param gradients")
     box = W.sio2_box(96, seed=13)

@@ -753,7 +753,7 @@ static CnAtomArgs d4_atom_args(const fd3_cn_d4* d4, const Common& c, const D4Buf
   aa.spec_s = spec_s; aa.selfcont = d4->selfcont;
   aa.accA = c.cols.accA; aa.accG = c.cols.accG;
   aa.cn = c.cn_raw; aa.g1 = db.g1; aa.dEdcn = c.dEdcn; aa.gpos = c.gpos;
-  aa.partials = c.p_cn + (size_t)db.kp.blocks * 8;
+  aa.partials = offset_or_null(c.p_cn, (size_t)db.kp.blocks * 8);
   aa.dyn = c.geom.dyn;
   return aa;
 }

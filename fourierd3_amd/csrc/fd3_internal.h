@@ -424,6 +424,13 @@ inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 inline size_t align_up(size_t v, size_t a = 256) { return (v + a - 1) / a * a; }
 
 // Simple bump allocator over a caller-provided workspace.
+// p + k, or null for a null p (a workspace query carves from a null base
+// and must not offset it).
+template <typename T>
+inline T* offset_or_null(T* p, size_t k) {
+  return p ? p + k : nullptr;
+}
+
 struct Arena {
   char* base;
   size_t cap, off;

@@ -36,7 +36,7 @@ void col_carve(const FftPlan& p, int64_t n, int order, Arena& ar, ColBuffers* b)
   b->tmp_i = ar.take<int4>(nn);
   b->tmp_f = ar.take<float4>(nn);
   b->colcount = ar.take<int32_t>(2 * ncol);  // colcount and colcur: cleared by one fill
-  b->colcur = b->colcount + ncol;
+  b->colcur = offset_or_null(b->colcount, ncol);
   b->colstart = ar.take<int32_t>(ncol + 1);
   b->scantmp = ar.take<int32_t>(scan_tmp_elems((int64_t)ncol));
   b->rec_i = ar.take<int4>(nn);

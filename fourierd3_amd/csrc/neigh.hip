@@ -746,7 +746,7 @@ void cell_carve(const CellGrid& g, int64_t n, Arena& ar, CellBuffers* b) {
   b->flo = ar.take<int4>(n);
   b->abin = ar.take<int32_t>(n);
   b->bcount = ar.take<int32_t>(2 * (size_t)g.nbins);  // bcount and bcur: cleared by one fill
-  b->bcur = b->bcount + g.nbins;
+  b->bcur = offset_or_null(b->bcount, (size_t)g.nbins);
   b->bstart = ar.take<int32_t>(g.nbins + 1);
   b->slot_atom = ar.take<int32_t>(n);
   b->pos4 = ar.take<double4>(n);

@@ -1024,3 +1024,47 @@ def test_deterministic_mode_matches_oracle():
                          box.n_atoms)
     print(f"[errors] deterministic mode vs oracle: E {de:.3e} F {df:.3e} S {ds:.3e}", flush=True)
     assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
+@pytest.mark.parametrize("case", ["single_atom", "dimer", "single_atom_d4", "single_atom_pme", "single_atom_ewald"])
+def test_smallest_boxes_match_oracle(case):
+    """One atom (only its own periodic images: CN 0, zero force by
+    translation symmetry) and a dimer in small triclinic cells, against the
+    oracle, for SPME, PME, Ewald and the D4 CN."""
+    _say("smallest box", case)
+    cell = np.array([[9.0, 0.0, 0.0], [1.5, 10.0, 0.0], [0.7, -0.9, 11.0]])
+    if case == "dimer":
+        box = W.Box(numbers=np.array([6, 8]), positions=np.array([[1.0, 2.0, 3.0], [2.1, 2.4, 3.3]]), cell=cell)
+    else:
+        box = W.Box(numbers=np.array([14]), positions=np.array([[1.0, 2.0, 3.0]]), cell=cell)
+    method = {"single_atom_pme": "pme", "single_atom_ewald": "ewald"}.get(case, "spme")
+    order = 6 if method == "pme" else 5
+    cnfunc = "d4" if case.endswith("d4") else "smooth_cut"
+    calc = FourierD3ASECalculator(device=_dev(), method=method, interpolation_nodes=order, cnfunc=cnfunc)
+    calc.calculate(_atoms(box))
+    o = OracleFourierD3(box.numbers, box.cell, method=method, interpolation_nodes=order, cnfunc=cnfunc)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    de, df, ds = _errors(calc.results["energy"], calc.results["forces"], calc.results["stress"], e_ref, f_ref, s_ref,
+                         box.n_atoms)
+    print(f"[errors] {case}: E {calc.results['energy']:.6e} eV (oracle {e_ref:.6e}); E {de:.3e} F {df:.3e} "
+          f"S {ds:.3e}", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+    if box.n_atoms == 1:
+        assert float(np.abs(calc.results["forces"]).max()) < F_TOL
+
+
+def test_empty_box_gives_zero_energy():
+    """No atoms: the library returns zero energy, virial and dE/dtheta (nothing
+    is spread, no self term) without touching the empty inputs."""
+    _say("empty box")
+    box = W.sio2_box(48, seed=3)
+    calc = FourierD3ASECalculator(device=_dev())
+    calc._build_model(_atoms(box))
+    pos = torch.empty((0, 3), dtype=torch.float64, device=_dev())
+    res = calc.evaluate(pos, box.cell, need_grad=True, need_params=True)
+    torch.cuda.synchronize()
+    print(f"[errors] empty box: E {float(res['energy'][0]):.3e} virial {res['virial'].abs().max().item():.3e} "
+          f"dE/dtheta {res['grad_params'].abs().max().item():.3e}", flush=True)
+    assert float(res["energy"][0]) == 0.0
+    assert res["virial"].abs().max().item() == 0.0 and res["grad_params"].abs().max().item() == 0.0
+    assert res["grad_pos"].shape == (0, 3)

@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <string>
 #include <vector>
 
 #include "fd3_internal.h"
@@ -91,28 +92,38 @@ int record_boxes(const fd3_batch& p, const void* pos, int dtype, const int32_t* 
 }
 
 // Merge the boxes' launch sequences: repeatedly take, among the boxes' next
-// launches, the one that comes first in a reference order (the longest
-// box's sequence; launches are identified by kernel and occurrence), and
-// launch it together with every box whose next launch is the same.  Each
-// box's own order is kept whatever the reference order is.
+// launches, the one whose stage comes first in a reference order (the
+// longest box's sequence; a stage is a launch site's tag and its
+// occurrence), and launch it together with every box whose next launch is
+// the same kernel (and occurrence of it).  Boxes that run another kernel
+// variant at that stage (another candidate capacity of the CN kernel, say)
+// rank the same and follow right after, so the stages after it still merge.
+// Each box's own order is kept whatever the reference order is.
 std::vector<MergedLaunch> merge(const std::vector<Recorder>& recs) {
   const int nb = (int)recs.size();
   using Ident = std::pair<const void*, int>;
+  using Stage = std::pair<std::string, int>;
   std::vector<std::vector<Ident>> ids(nb);
+  std::vector<std::vector<Stage>> stages(nb);
   for (int b = 0; b < nb; ++b) {
     std::map<const void*, int> occ;
-    for (const LaunchRec& r : recs[b].recs) ids[b].push_back({r.key, occ[r.key]++});
+    std::map<std::string, int> tocc;
+    for (const LaunchRec& r : recs[b].recs) {
+      ids[b].push_back({r.key, occ[r.key]++});
+      const std::string t(r.tag);
+      stages[b].push_back({t, tocc[t]++});
+    }
   }
   std::vector<int> order(nb);
   for (int b = 0; b < nb; ++b) order[b] = b;
   std::stable_sort(order.begin(), order.end(),
                    [&](int x, int y) { return recs[x].recs.size() > recs[y].recs.size(); });
-  std::map<Ident, int> rank;
+  std::map<Stage, int> rank;
   for (int b : order)
-    for (const Ident& id : ids[b])
-      if (!rank.count(id)) {
+    for (const Stage& st : stages[b])
+      if (!rank.count(st)) {
         const int r = (int)rank.size();
-        rank[id] = r;
+        rank[st] = r;
       }
   std::vector<size_t> cur(nb, 0);
   std::vector<MergedLaunch> out;
@@ -120,7 +131,7 @@ std::vector<MergedLaunch> merge(const std::vector<Recorder>& recs) {
     int best = -1, best_rank = 0;
     for (int b = 0; b < nb; ++b) {
       if (cur[b] >= ids[b].size()) continue;
-      const int r = rank[ids[b][cur[b]]];
+      const int r = rank[stages[b][cur[b]]];
       if (best < 0 || r < best_rank) { best = b; best_rank = r; }
     }
     if (best < 0) break;

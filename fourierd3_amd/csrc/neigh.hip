@@ -720,6 +720,11 @@ int cell_grid_make(const double cell_A[9], double r_cut_A, CellGrid* g, double r
                             inv[2 * 3 + a] * inv[2 * 3 + a]);
     const double d = 1.0 / nrm;
     int nb = (int)std::max(1.0, floor(d / rs));
+    // fewer than three bins of rs along this axis: bins of at least rs / 2
+    // (search reach 2), so an atom scans the images within ~2.5 rs instead
+    // of three whole-box images per axis (a 50-atom box: ~230 candidates
+    // per atom instead of ~1350)
+    if (nb < 3) nb = (int)std::max(1.0, floor(2.0 * d / rs));
     nb = std::min(nb, 256);
     g->nb[a] = nb;
     g->sr[a] = (int)ceil(rs * nb / d - 1e-12);

@@ -222,7 +222,7 @@ static int check_model(const fd3_model* m) {
     set_error("n_species and n_rank must be positive, got %d and %d", m->n_species, m->n_rank);
     return FD3_ERR_ARG;
   }
-  if (m->n_species > 64 || m->n_rank > 32) {
+  if (m->n_species > 64 || m->n_rank > kMaxRank) {
     set_error("this build supports up to 64 species and rank 32, got %d and %d", m->n_species, m->n_rank);
     return FD3_ERR_UNSUPPORTED;
   }

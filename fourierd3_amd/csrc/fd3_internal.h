@@ -13,6 +13,7 @@ constexpr double kA2B = 1.0 / 0.52917726;  // Angstrom -> Bohr, src/fourierd3/co
 constexpr double kPi = 3.14159265358979323846;
 constexpr int kNRef = 7;
 constexpr int kMaxOrder = 7;
+constexpr int kMaxRank = 32;  // ranks of the C6 decomposition the kernels take
 
 // ---------------------------------------------------------------- errors ---
 void set_error(const char* fmt, ...);

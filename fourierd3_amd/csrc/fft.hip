@@ -376,6 +376,7 @@ struct KspaceArgs {
   const float2* twx;
   const float4* htab;
   const float4* gtab;   // per-pair G tables (register kernel, no parameter gradients)
+  float2* sink;         // 256 values written by idle lanes (kspace_xsp_body), never read
   float gkinv;          // 1 / (interval width) of gtab, Bohr
   const float* scx; const float* scy; const float* scz;
   double* partials;     // (gridDim, 16)
@@ -780,6 +781,64 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
   kspace_reduce(bk, acc, (double*)S, a, PARAMS);
 }
 
+// The spectra part of one k point (column kx of tile row kyi) given G and
+// dG/dw of its species pairs: phi_X = sum_Y G S_Y rank by rank (the NS
+// spectra of one rank in registers at a time), S <- -(lambda_q/Omega) phi /
+// sinc^2p, and the energy and virial partials added to acc.
+template <int NS, int RT, int NK>
+__device__ __forceinline__ void ks_kpoint_spectra(const KspaceArgs& a, const KsCell& kc, float2* S, int kyi, int kx,
+                                                  const KPoint& kp, const float (&gp)[NS * (NS + 1) / 2],
+                                                  const float (&dgp)[NS * (NS + 1) / 2], const float (&lam)[RT],
+                                                  float w, float (&accf)[7]) {
+  constexpr int ls = 256;
+  float e_acc = 0.f, d_acc = 0.f;
+#pragma unroll
+  for (int q = 0; q < RT; ++q) {
+    float2 sv[NS], ph[NS];
+#pragma unroll
+    for (int X = 0; X < NS; ++X) {
+      sv[X] = S[((X * RT + q) * NK + kyi) * ls + kx];
+      ph[X] = make_float2(0.f, 0.f);
+    }
+    float dq = 0.f;  // sum over pairs of dG/dw Re(S_X conj S_Y) (both orders)
+#pragma unroll
+    for (int X = 0, pi = 0; X < NS; ++X) {
+#pragma unroll
+      for (int Y = X; Y < NS; ++Y, ++pi) {
+        const float gs = gp[pi];
+        const float2 sx = sv[X], sy = sv[Y];
+        const float tre = sx.x * sy.x + sx.y * sy.y;
+        ph[X].x += gs * sy.x;
+        ph[X].y += gs * sy.y;
+        if (Y != X) {
+          ph[Y].x += gs * sx.x;
+          ph[Y].y += gs * sx.y;
+          dq += (2.0f * dgp[pi]) * tre;
+        } else {
+          dq += dgp[pi] * tre;
+        }
+      }
+    }
+    float eq = 0.f;
+    const float f = lam[q] * kc.neg_inv_vol * kp.isc;
+#pragma unroll
+    for (int X = 0; X < NS; ++X) {
+      eq += sv[X].x * ph[X].x + sv[X].y * ph[X].y;
+      S[((X * RT + q) * NK + kyi) * ls + kx] = make_float2(f * ph[X].x, f * ph[X].y);
+    }
+    e_acc += lam[q] * eq;
+    d_acc += lam[q] * dq;
+    asm volatile("" ::: "memory");  // one rank's spectra in registers at a time
+  }
+  d_acc *= a.gkinv;  // dG/dw -> dG/d|k|
+  accf[0] += w * e_acc * kp.isc;
+  if (kp.kn > 0.f) {
+    const float sv = w * d_acc * kp.isc * __builtin_amdgcn_rcpf(kp.kn);
+    accf[1] += sv * kp.kv0 * kp.kv0; accf[2] += sv * kp.kv1 * kp.kv1; accf[3] += sv * kp.kv2 * kp.kv2;
+    accf[4] += sv * kp.kv0 * kp.kv1; accf[5] += sv * kp.kv0 * kp.kv2; accf[6] += sv * kp.kv1 * kp.kv2;
+  }
+}
+
 // The register k-space kernel with the split x FFT (nx = ny = 256, the y
 // passes run the first stage): blocks of 256 threads over a 1-D grid, block b
 // taking the KT consecutive tiles b KT .. b KT + KT - 1 of the
@@ -787,16 +846,16 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
 // fastest.  Thread (l, j) owns column j of line l = (X*RT + qi)*2 + kyi of
 // every tile (the 16-point FFT of ksx_load / ksx_forward_regs /
 // ksx_inverse); its load is issued before the barrier that frees the lines.
-// Dynamic LDS: the 16 lines of 256 points (32 KB, the whole allocation
-// without parameter gradients, so LDS never holds the occupancy below the
-// VGPRs' 4 waves per SIMD of the 4-species, rank-2 kernel), then with them
-// the h table and the pair constants.  The block sums reuse the lines at the
-// end.  Each thread adds its tile's two k points in float and the tile sums
-// in double.  Measured on the 100k-atom box (256^3, 4 species, rank 2):
-// 0.267 ms against 0.274 ms for the same tiles with padded lines, twiddle
-// and line-base tables in LDS (35 KB), 0.297 ms for persistent blocks
-// striding over the tiles, 0.29 ms for one ky row per tile; streaming
-// (nontemporal) loads of the spectrum cost 40 %.
+// Dynamic LDS: the 16 lines of 256 points (32 KB), then with parameter
+// gradients the h table and the pair constants.  The block sums reuse the
+// lines at the end.  Each thread adds its tile's two k points in float and
+// the tile sums in double.  It runs with parameter gradients;
+// kspace_xsp_body (below) without them.  Measured on the 100k-atom box
+// (256^3, 4 species, rank 2, without parameter gradients): 0.267 ms against
+// 0.274 ms for the same tiles with padded lines, twiddle and line-base
+// tables in LDS (35 KB), 0.297 ms for persistent blocks striding over the
+// tiles, 0.29 ms for one ky row per tile; streaming (nontemporal) loads of
+// the spectrum cost 40 %.
 template <int NS, int RT, bool PARAMS>
 __device__ __forceinline__ void kspace_xs_body(const BlockInfo& bk, const KspaceArgs& a) {
   static_assert(NS * RT <= 8, "split x k-space: at most 8 spectra per tile");
@@ -849,6 +908,178 @@ __device__ __forceinline__ void kspace_xs_body(const BlockInfo& bk, const Kspace
   }
   __syncthreads();  // the lines are free: reuse them for the block sums
   kspace_reduce(bk, acc, (double*)S, a, PARAMS);
+}
+
+// k vector and |k| of column (kx, ky, kz) with a given 1/sinc^2p.
+__device__ inline KPoint kspace_point_isc(const KspaceArgs& a, const KsCell& c, int kx, int ky, int kz, float isc) {
+  const float mx = (float)(kx < (a.nx + 1) / 2 ? kx : kx - a.nx);
+  const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
+  const float mz = (float)kz;
+  KPoint p;
+  p.kv0 = mx * c.recip[0] + my * c.recip[3] + mz * c.recip[6];
+  p.kv1 = mx * c.recip[1] + my * c.recip[4] + mz * c.recip[7];
+  p.kv2 = mx * c.recip[2] + my * c.recip[5] + mz * c.recip[8];
+  p.kn = __builtin_amdgcn_sqrtf(p.kv0 * p.kv0 + p.kv1 * p.kv1 + p.kv2 * p.kv2);
+  p.isc = isc;
+  return p;
+}
+
+// G and dG/dw of the NP species pairs at |k| = kn from per-pair tables in LDS.
+template <int NP>
+__device__ __forceinline__ void ks_gtab_lds(const float4* tab, float gkinv, float kn, float (&gp)[NP],
+                                            float (&dgp)[NP]) {
+  const float fk = kn * gkinv;
+  const int i = min((int)fk, kGtabN - 1);
+  const float wk = fk - (float)i;
+  const float4* e = tab + 2 * i;
+  constexpr int PD = 2;  // pairs whose coefficients are in flight
+  float4 ca[PD + 1], cb[PD + 1];
+  static_for<PD>([&](auto dc) {
+    constexpr int d = decltype(dc)::value;
+    if constexpr (d < NP) { ca[d] = e[2 * kGtabN * d]; cb[d] = e[2 * kGtabN * d + 1]; }
+  });
+  static_for<NP>([&](auto pc) {
+    constexpr int pi = decltype(pc)::value, nxt = pi + PD;
+    if constexpr (nxt < NP) {
+      ca[nxt % (PD + 1)] = e[2 * kGtabN * nxt];
+      cb[nxt % (PD + 1)] = e[2 * kGtabN * nxt + 1];
+    }
+    gtab_eval(ca[pi % (PD + 1)], cb[pi % (PD + 1)], wk, &gp[pi], &dgp[pi]);
+    asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi])::"memory");
+  });
+}
+
+// The split-x k-space kernel without parameter gradients (nx = ny = 256,
+// the y passes run the first stage of the x FFT), software-pipelined over
+// the block's KT tiles: each thread loads its 16 values of the next tile
+// into registers right after the forward pass has moved the current tile
+// into LDS, so that load is in flight while the current tile is filtered and
+// transformed back.  The vector-memory counter retires in issue order, so
+// nothing else in the loop may wait on it, and the compiler places its
+// waits from what every path to them has issued:
+//  * the per-pair G tables, the rank eigenvalues and the sinc factors are
+//    staged in LDS once per block, so the loop's only vector-memory
+//    operations are that load and the tile's stores;
+//  * the loop is rotated (k points, inverse and stores of tile u, then the
+//    forward pass of tile u + 1 and the load of tile u + 2), and every lane
+//    stores (GRAD) on every path, lanes of idle lines into a 256-value sink,
+//    so the wait before a forward pass leaves the 16 stores in flight.
+// Tiles, lines and thread roles as kspace_xs_body; each thread's two k
+// points (column kx of both tile rows) are taken one after the other, rank
+// by rank, with float sums per tile and double sums over the tiles.  LDS:
+// the lines (32 KB), the G tables, the eigenvalues and the sinc factors
+// (ksxp_lds); three blocks fit a CU.
+template <int NS, int RT, bool GRAD>
+__device__ __forceinline__ void kspace_xsp_body(const BlockInfo& bk, const KspaceArgs& a) {
+  static_assert(NS * RT <= 8, "split x k-space: at most 8 spectra per tile");
+  constexpr int NP = NS * (NS + 1) / 2, NK = 2, ls = 256, nlines = NS * RT * NK;
+  const KsCell kc = ks_cell(a);
+  extern __shared__ float4 lds4[];
+  float2* S = (float2*)lds4;
+  float4* tab = lds4 + nlines * ls / 2;
+  float* eig_l = (float*)(tab + 2 * NP * kGtabN);  // kMaxRank
+  float* scx_l = eig_l + kMaxRank;                  // 256
+  float* scy_l = scx_l + 256;                       // 256
+  float* scz_l = scy_l + 256;                       // M1
+  for (int i = threadIdx.x; i < 2 * NP * kGtabN; i += blockDim.x) tab[i] = a.gtab[i];
+  for (int i = threadIdx.x; i < a.r; i += blockDim.x) eig_l[i] = a.eigs[i];
+  if (a.use_sinc) {
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+      scx_l[i] = a.scx[i];
+      scy_l[i] = a.scy[i];
+    }
+    for (int i = threadIdx.x; i < a.M1; i += blockDim.x) scz_l[i] = a.scz[i];
+  }
+  const int nyp = 128, zb = (a.r + RT - 1) / RT;
+  const int ntiles = nyp * a.M1 * zb;
+  const int xl = threadIdx.x >> 4, xj = threadIdx.x & 15;
+  const int xqi = (xl >> 1) % RT, xX = (xl >> 1) / RT;
+  const int kx = threadIdx.x;  // column of this thread's two k points
+  // spectrum offset of this thread's line column in tile u, or -1 (idle:
+  // a line past the tile's spectra or of a rank past the last)
+  auto tile_elem = [&](int u) -> long long {
+    const int kyp = u % nyp, rest = u / nyp, kz = rest % a.M1, q0 = rest / a.M1 * RT;
+    if (xl >= nlines || xqi >= min(RT, a.r - q0)) return -1LL;
+    return (((long long)(xX * a.r + q0 + xqi) * a.M1 + kz) * 256 + 2 * kyp + (xl & 1)) * 256 + xj;
+  };
+  double acc[11];  // tile sums (7 used; the parameter slots stay zero)
+#pragma unroll
+  for (int i = 0; i < 11; ++i) acc[i] = 0.0;
+  const int u0 = bk.id.x * a.KT, u1 = min(ntiles, u0 + a.KT);
+  float2 v[16];
+  // idle lanes load the spectrum's first 256 values (ignored) and store
+  // into the sink
+  auto load_tile = [&](long long e) {
+    const float2* src = a.spec + (e >= 0 ? e : xj);
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = src[16 * n2];
+  };
+  const float2* Lr = S + (xl < nlines ? xl : 0) * ls;  // the line this lane transforms (any, when idle)
+  auto forward = [&](bool live) {  // the tile's column of this lane into LDS, then a barrier
+    if (xl < nlines) {
+      float2 z[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) z[n2] = v[n2];
+      fft16r<-1>(z);  // over n2 -> k2: X[xj + 16 k2]
+      float2* L = S + xl * ls;
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) L[xj + 16 * k2] = live ? z[k2] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+  };
+  if (u0 < u1) {
+    long long e0 = tile_elem(u0);
+    load_tile(e0);
+    __syncthreads();  // the staged tables
+    forward(e0 >= 0);
+    long long e1 = u0 + 1 < u1 ? tile_elem(u0 + 1) : -1LL;
+    if (u0 + 1 < u1) load_tile(e1);
+    for (int u = u0; u < u1; ++u) {
+      asm volatile("" ::: "memory");  // the load is issued before the k points
+      const int kyp = u % nyp, rest = u / nyp, kz = rest % a.M1, q0 = rest / a.M1 * RT;
+      const int rt = min(RT, a.r - q0), ky0 = 2 * kyp;
+      float lam[RT];
+#pragma unroll
+      for (int q = 0; q < RT; ++q) lam[q] = q < rt ? eig_l[q0 + q] : 0.0f;
+      const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
+      const float scxz = a.use_sinc ? scx_l[kx] * scz_l[kz] : 1.0f;
+      float accb[7];
+#pragma unroll
+      for (int i = 0; i < 7; ++i) accb[i] = 0.0f;
+#pragma unroll 1
+      for (int kyi = 0; kyi < 2; ++kyi) {
+        float isc = 1.0f;
+        if (a.use_sinc) isc = __builtin_amdgcn_rcpf(fmaxf(scxz * scy_l[ky0 + kyi], 1e-10f));
+        const KPoint kp = kspace_point_isc(a, kc, kx, ky0 + kyi, kz, isc);
+        float gp[NP], dgp[NP];
+        ks_gtab_lds<NP>(tab, a.gkinv, kp.kn, gp, dgp);
+        ks_kpoint_spectra<NS, RT, NK>(a, kc, S, kyi, kx, kp, gp, dgp, lam, w, accb);
+        asm volatile("" ::: "memory");  // one k point at a time
+      }
+#pragma unroll
+      for (int i = 0; i < 7; ++i) acc[i] += (double)accb[i];
+      if (kz == 0 && ky0 == 0 && threadIdx.x == 0) kspace_take_k0<NS, RT>(a, S, ls, NK, q0, rt);
+      if constexpr (GRAD) {
+        __syncthreads();
+        float2 z[16];
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) z[k2] = Lr[xj + 16 * k2];  // k1 = xj
+        fft16r<1>(z);  // over k2 -> n2: Y[xj, n2]
+        float2* dst = e0 >= 0 ? a.spec + e0 : a.sink + xj;
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) dst[16 * n2] = z[n2];
+      }
+      if (u + 1 < u1) {
+        __syncthreads();  // every lane is done with the lines
+        forward(e1 >= 0);
+        e0 = e1;
+        e1 = u + 2 < u1 ? tile_elem(u + 2) : -1LL;
+        if (u + 2 < u1) load_tile(e1);
+      }
+    }
+  }
+  __syncthreads();  // the lines are free: reuse them for the block sums
+  kspace_reduce(bk, acc, (double*)S, a, false);
 }
 
 // Any number of species: phi accumulates in a second LDS buffer P.
@@ -1043,7 +1274,13 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   const int zb = (r + RT - 1) / RT;
   int KT = 1;
   while (KT < 4 && (p->ny / NK) % (2 * KT) == 0 && (size_t)(p->ny / NK / (2 * KT)) * p->M1 * zb >= 8192) KT <<= 1;
-  if (tuned && pow2(tune[3]) && tune[3] <= 4 && (p->ny / NK) % tune[3] == 0) KT = tune[3];
+  // the split x kernel takes any tile count per block (1-D grid): three
+  // tiles (measured: 222 us against 229 / 231 / 261 us for 2 / 4 / 1 at the
+  // 100k-atom box, 256^3, 4 species, rank 2), fewer when that would leave
+  // fewer than ~3k blocks
+  if (p->xs) KT = std::max(1, std::min(3, (int)((size_t)p->ny / 2 * p->M1 * zb / 3072)));
+  if (tuned && p->xs && tune[3] >= 1 && tune[3] <= 64) KT = tune[3];
+  else if (tuned && pow2(tune[3]) && tune[3] <= 4 && (p->ny / NK) % tune[3] == 0) KT = tune[3];
   p->KT = KT;
   p->ks_blocks = (p->ny / NK / KT) * p->M1 * zb;
   if (p->xs) p->ks_blocks = (p->ny / 2 * p->M1 * zb + KT - 1) / KT;  // 1-D grid of KT-tile blocks
@@ -1081,6 +1318,7 @@ void fft_tables_carve(const FftPlan& p, Arena& ar, FftTables* t) {
   t->htab = ar.take<float4>(3 * (kHtabN + 1));
   t->gtab = ar.take<float4>((size_t)ks_table_f4(kKsRegMaxS));
   t->pc = ar.take<KsPair>((size_t)p.nS * p.nS);
+  t->sink = ar.take<float2>(256);
 }
 
 int fft_tables_init(const FftPlan& p, const FftTables& t, int order, int use_sinc, hipStream_t s,
@@ -1189,7 +1427,8 @@ static int ks_launch(bool two, bool xs, bool params, dim3 grid, size_t sh, hipSt
     // the parameter-gradient kernel stages
     const size_t sh_g = sh - (size_t)(ks2_table_f4(true) - ks2_table_f4(false)) * sizeof(float4);
     if (xs && params) return launch<KspaceArgs, kspace_xs_body<NS, RT, true>, 256>("k_kspace", grid, sh, s, a);
-    if (xs) return launch<KspaceArgs, kspace_xs_body<NS, RT, false>, 256>("k_kspace", grid, sh, s, a);
+    if (xs && a.need_grad) return launch<KspaceArgs, kspace_xsp_body<NS, RT, true>, 256>("k_kspace", grid, sh, s, a);
+    if (xs) return launch<KspaceArgs, kspace_xsp_body<NS, RT, false>, 256>("k_kspace", grid, sh, s, a);
     if (two && params) return launch<KspaceArgs, kspace_reg_body<NS, RT, true, true>, 256>("k_kspace", grid, sh, s, a);
     if (two) return launch<KspaceArgs, kspace_reg_body<NS, RT, false, true>, 256>("k_kspace", grid, sh_g, s, a);
   }
@@ -1205,7 +1444,7 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
   }
   KspaceArgs a;
   a.k0 = k0;
-  a.spec = spec; a.eigs = eigs; a.pc = t.pc; a.twx = t.twx; a.htab = t.htab; a.gtab = t.gtab;
+  a.spec = spec; a.eigs = eigs; a.pc = t.pc; a.twx = t.twx; a.htab = t.htab; a.gtab = t.gtab; a.sink = t.sink;
   a.gkinv = (float)(1.0 / ks_gtab_dk(p, g));
   a.scx = t.scx; a.scy = t.scy; a.scz = t.scz; a.partials = partials;
   for (int i = 0; i < 9; ++i) a.recip[i] = (float)g.recip[i];
@@ -1214,7 +1453,7 @@ int kspace_run(const FftPlan& p, const FftTables& t, const Geom& g, float2* spec
   a.nx = p.nx; a.ny = p.ny; a.nz = p.nz; a.M1 = p.M1; a.nS = p.nS; a.r = p.r; a.RT = p.RT; a.NK = p.NK; a.KT = p.KT;
   a.use_sinc = g.interp == FD3_INTERP_P3M;
   a.need_grad = need_grad; a.need_params = need_params;
-  const size_t sh = xs ? ksx_lds(p.nS * p.RT, need_params != 0) : p.ks_lds;
+  const size_t sh = !xs ? p.ks_lds : need_params ? ksx_lds(p.nS * p.RT, true) : ksxp_lds(p.nS, p.nS * p.RT, p.M1);
   const dim3 grid = xs ? dim3(p.ks_blocks) : dim3(p.ny / p.NK / p.KT, p.M1, (p.r + p.RT - 1) / p.RT);
   ProfScope _ps("k_kspace", s);
 #define FD3_KS(NS_, RT_)                                                                              \

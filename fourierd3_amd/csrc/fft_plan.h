@@ -42,6 +42,7 @@ struct FftTables {
   float4* htab;  // 3 * (kHtabN + 1) entries: polynomial table of h6, h8
   float4* gtab;  // (pairs X <= Y, kGtabN, 2): per-pair G tables of the register k-space kernel
   KsPair* pc;    // (nS, nS) per-pair constants of the k-space kernel
+  float2* sink;  // 256: written, never read (stores of the split x k-space kernel's idle lanes)
 };
 
 // Inputs of the per-pair constants (device table qq = sqrt(Q_X) sqrt(Q_Y)).
@@ -66,6 +67,13 @@ constexpr int kKs2Ls = 272;
 constexpr size_t ksx_lds(int nl, bool params) {
   return (size_t)2 * nl * 256 * sizeof(float2) +
          (params ? (size_t)3 * (kHtabN + 1) * sizeof(float4) + (size_t)nl * nl * sizeof(KsPair) : 0);
+}
+// Dynamic LDS of the pipelined split x kernel without parameter gradients
+// (kspace_xsp_body): the lines of nl spectra, the per-pair G tables of nS
+// species, the rank eigenvalues, sinc_x and sinc_y (256 each) and sinc_z (M1).
+inline size_t ksxp_lds(int nS, int nl, int M1) {
+  return (size_t)2 * nl * 256 * sizeof(float2) + (size_t)nS * (nS + 1) * kGtabN * sizeof(float4) +
+         (size_t)(kMaxRank + 512 + M1) * sizeof(float);
 }
 // float4 entries of its table region: the h table with parameter gradients,
 // none without (the per-pair G tables are read from global memory).

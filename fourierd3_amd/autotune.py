@@ -13,7 +13,9 @@ Each tile moves one group of kernels, so the search is coordinate-wise:
   tiles[0]  y lines per z-pass block   k_spread_r2c_z + k_c2r_z_gather
   tiles[1]  x lines per y-pass block   k_c2c_y_fwd + k_c2c_y_inv
   tiles[2]  ranks per k-space tile     k_kspace            (searched jointly
-  tiles[3]  k-space tiles per block    k_kspace             with tiles[2])
+  tiles[3]  k-space tiles per block    k_kspace             with tiles[2];
+                                                            any count up to 64
+                                                            for the split x kernel)
 
 A candidate the library would not take (``fd3_plan_tiles`` reports a
 different tile) is skipped.  A key keeps its tuned tiles only if a full
@@ -147,7 +149,7 @@ def tune_box(box, c6tol: float, reps: int = 5, log=print) -> dict:
         elif g == 1:
             cands = [[best[0], v, best[2], best[3]] for v in _pow2_upto(ns[0])]
         else:
-            cands = [[best[0], best[1], rt, kt] for rt in range(1, min(r, 8) + 1) for kt in (1, 2, 4)]
+            cands = [[best[0], best[1], rt, kt] for rt in range(1, min(r, 8) + 1) for kt in (1, 2, 3, 4, 8)]
         scored = []
         for c in cands:
             ms = measure(c, g)

@@ -23,7 +23,10 @@ def test_default_tiles(clean_table):
     ty, tx, rt, kt = A.plan_tiles(NS, 4, 2)
     # z lines: M = 128 complex per line, 2048 values per block; y lines: 8192 per block
     assert (ty, tx) == (16, 32)
-    assert rt == 2 and kt in (1, 2, 4)
+    # the split x k-space kernel (nx = ny = 256) takes three tiles per block
+    assert rt == 2 and kt == 3
+    # other meshes: 1, 2 or 4 ky tiles per block
+    assert A.plan_tiles([128, 128, 128], 4, 2)[3] in (1, 2, 4)
     # four species: at most 8 spectra per thread -> rank tile 2
     assert A.plan_tiles(NS, 4, 6)[2] == 2
     assert A.plan_tiles(NS, 2, 6)[2] == 4
@@ -40,18 +43,25 @@ def test_override_taken_and_cleared(clean_table):
     assert A.plan_tiles(NS, 4, 2) == base
 
 
-@pytest.mark.parametrize("tiles", [
-    [3, 0, 0, 0],      # not a power of two
-    [64, 0, 0, 0],     # 64 lines x 128 exceed the z-pass register budget
-    [0, 64, 0, 0],     # 64 lines x 256 exceed the y-pass budget
-    [0, 0, 3, 0],      # more ranks per tile than the model has
-    [0, 0, 0, 8],      # more than 4 k tiles per block
-    [0, 0, 0, 3],      # not a power of two
+@pytest.mark.parametrize("ns,tiles", [
+    (NS, [3, 0, 0, 0]),                # not a power of two
+    (NS, [64, 0, 0, 0]),               # 64 lines x 128 exceed the z-pass register budget
+    (NS, [0, 64, 0, 0]),               # 64 lines x 256 exceed the y-pass budget
+    (NS, [0, 0, 3, 0]),                # more ranks per tile than the model has
+    (NS, [0, 0, 0, 65]),               # split x k-space kernel: at most 64 tiles per block
+    ([128, 128, 128], [0, 0, 0, 8]),   # other meshes: at most 4 k tiles per block
+    ([128, 128, 128], [0, 0, 0, 3]),   # ... and a power of two
 ])
-def test_invalid_override_ignored(clean_table, tiles):
-    base = A.plan_tiles(NS, 4, 2)
-    A.set_tiles(NS, 4, 2, tiles)
-    assert A.plan_tiles(NS, 4, 2) == base
+def test_invalid_override_ignored(clean_table, ns, tiles):
+    base = A.plan_tiles(ns, 4, 2)
+    A.set_tiles(ns, 4, 2, tiles)
+    assert A.plan_tiles(ns, 4, 2) == base
+
+
+@pytest.mark.parametrize("kt", [1, 2, 5, 8, 64])
+def test_split_x_kspace_takes_any_tile_count(clean_table, kt):
+    A.set_tiles(NS, 4, 2, [0, 0, 0, kt])
+    assert A.plan_tiles(NS, 4, 2)[3] == kt
 
 
 def test_zero_keeps_plan_choice(clean_table):

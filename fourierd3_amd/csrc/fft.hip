@@ -750,13 +750,23 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
   float lam[RT];
 #pragma unroll
   for (int q = 0; q < RT; ++q) lam[q] = q < rt ? a.eigs[q0 + q] : 0.0f;
-  // each thread's partial sums over its k points (at most KT * NK * nx / 256
-  // = 8 terms) in float, the block sums in double
+  // each thread's partial sums over its k points: without parameter
+  // gradients in float per tile (NK * nx / 256 terms) and in double over the
+  // tiles; with them (11 sums) in float over the block's tiles, at most
+  // KT * NK * nx / 256 <= 8 terms, which keeps the kernel at three waves per
+  // SIMD; the block sums in double
+  double acc[11];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) acc[i] = 0.0;
   float accf[11];
 #pragma unroll
   for (int i = 0; i < 11; ++i) accf[i] = 0.0f;
   const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
   for (int it = 0; it < a.KT; ++it) {
+    if constexpr (!PARAMS) {
+#pragma unroll
+      for (int i = 0; i < 7; ++i) accf[i] = 0.0f;
+    }
     const int ky0 = (bk.id.x * a.KT + it) * NK;
     if (it) __syncthreads();  // the previous tile is done with S and lbase
     kspace_line_bases(a, lbase, nlines, RT, rt, ky0, kz, q0);
@@ -768,15 +778,20 @@ __device__ __forceinline__ void kspace_reg_body(const BlockInfo& bk, const Kspac
       const int kxe = pad ? lpad(kx) : kx;  // element kx of a line
       kspace_kpoint<NS, RT, PARAMS>(a, kc, S, ls, NK, kyi, kx, kxe, ky0 + kyi, kz, lam, w, pcl, tab, gt, accf);
     }
+    if constexpr (!PARAMS) {
+#pragma unroll
+      for (int i = 0; i < 7; ++i) acc[i] += (double)accf[i];
+    }
     if (kz == 0 && ky0 == 0 && threadIdx.x == 0) kspace_take_k0<NS, RT>(a, S, ls, NK, q0, rt);
     if (!a.need_grad) continue;
     __syncthreads();
     if constexpr (TWO) ks2_inverse(S, tws, a.spec, lbase, nlines);
     else lds_fft<1, MAXB, 256>(S, nx, nlines, ls, tws, 1, LdsLines{}, GlobalLines{a.spec, lbase});
   }
-  double acc[11];
+  if constexpr (PARAMS) {
 #pragma unroll
-  for (int i = 0; i < 11; ++i) acc[i] = (double)accf[i];
+    for (int i = 0; i < 11; ++i) acc[i] = (double)accf[i];
+  }
   __syncthreads();  // the lines are free: reuse them for the block sums
   kspace_reduce(bk, acc, (double*)S, a, PARAMS);
 }
@@ -1173,6 +1188,7 @@ __device__ __forceinline__ void kspace_body(const BlockInfo& bk, const KspaceArg
 
 // --------------------------------------------------------------- host ---
 constexpr int kYLog = 10;  // y passes: 1024-thread blocks
+constexpr int kBatchKT = 8;  // k-space tiles per block of a merged batch launch (meshes other than 256 x 256 x/y)
 
 // Tile overrides measured by the autotuner, per (mesh, species, rank):
 // z-pass lines TY, y-pass lines TX, k-space ranks per tile RT and tiles per
@@ -1274,13 +1290,20 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   const int zb = (r + RT - 1) / RT;
   int KT = 1;
   while (KT < 4 && (p->ny / NK) % (2 * KT) == 0 && (size_t)(p->ny / NK / (2 * KT)) * p->M1 * zb >= 8192) KT <<= 1;
+  // planned for a merged batch launch (a recorder is installed): the blocks
+  // of all boxes share one grid, so small meshes take up to kBatchKT tiles
+  // per block, which spreads each block's table staging and partial-sum row
+  // over more tiles (the 256-box batching workload: k-space 1.71 -> 1.30 ms,
+  // partial-sum reduction 0.26 -> 0.06 ms at 4 tiles, 1.26 / 0.04 ms at 8)
+  if (current_recorder())
+    while (KT < kBatchKT && (p->ny / NK) % (2 * KT) == 0) KT <<= 1;
   // the split x kernel takes any tile count per block (1-D grid): three
   // tiles (measured: 222 us against 229 / 231 / 261 us for 2 / 4 / 1 at the
   // 100k-atom box, 256^3, 4 species, rank 2), fewer when that would leave
   // fewer than ~3k blocks
   if (p->xs) KT = std::max(1, std::min(3, (int)((size_t)p->ny / 2 * p->M1 * zb / 3072)));
   if (tuned && p->xs && tune[3] >= 1 && tune[3] <= 64) KT = tune[3];
-  else if (tuned && pow2(tune[3]) && tune[3] <= 4 && (p->ny / NK) % tune[3] == 0) KT = tune[3];
+  else if (tuned && pow2(tune[3]) && tune[3] <= 16 && (p->ny / NK) % tune[3] == 0) KT = tune[3];
   p->KT = KT;
   p->ks_blocks = (p->ny / NK / KT) * p->M1 * zb;
   if (p->xs) p->ks_blocks = (p->ny / 2 * p->M1 * zb + KT - 1) / KT;  // 1-D grid of KT-tile blocks

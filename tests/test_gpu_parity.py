@@ -802,8 +802,10 @@ def test_batching_workload_matches_single_box_calls_and_replays():
         dv = max(dv, (got["virial"][b] - v1).abs().max().item() * HARTREE_TO_EV)
     print(f"[errors] batch vs single-box calls, 256 boxes: E {de:.3e} eV/atom  F {df:.3e} eV/A  "
           f"virial {dv:.3e} eV", flush=True)
-    # the spreading's float adds run in a varying order: agreement to float rounding
-    assert de < 1e-9 and df < 1e-6 and dv < 1e-6
+    # the spreading's float adds run in a varying order, and the batch sums
+    # the k-space partials over other blocks (more tiles per block): agreement
+    # to float rounding, ~1e-7 of a small box's energy (~0.1 eV/atom)
+    assert de < 2e-8 and df < 1e-6 and dv < 1e-6
     # oracle on the smallest box of each composition
     res = bb.results()
     for kind in (14, 1):

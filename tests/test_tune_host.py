@@ -49,8 +49,9 @@ def test_override_taken_and_cleared(clean_table):
     (NS, [0, 64, 0, 0]),               # 64 lines x 256 exceed the y-pass budget
     (NS, [0, 0, 3, 0]),                # more ranks per tile than the model has
     (NS, [0, 0, 0, 65]),               # split x k-space kernel: at most 64 tiles per block
-    ([128, 128, 128], [0, 0, 0, 8]),   # other meshes: at most 4 k tiles per block
-    ([128, 128, 128], [0, 0, 0, 3]),   # ... and a power of two
+    ([128, 128, 128], [0, 0, 0, 32]),  # other meshes: at most 16 k tiles per block
+    ([128, 128, 128], [0, 0, 0, 3]),   # ... a power of two
+    ([32, 32, 32], [0, 0, 0, 8]),      # ... that divides the tile rows (32 / 8 ky rows per tile)
 ])
 def test_invalid_override_ignored(clean_table, ns, tiles):
     base = A.plan_tiles(ns, 4, 2)

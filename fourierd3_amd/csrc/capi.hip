@@ -738,6 +738,7 @@ static CnKsArgs d4_ks_args(const fd3_model* m, const fd3_cn_d4* d4, const Common
   ka.nx = c.geom.nx; ka.ny = c.geom.ny; ka.nz = c.geom.nz; ka.M1 = c.plan.M1; ka.nS = m->n_species;
   ka.NK = db.kp.NK; ka.KT = db.kp.KT; ka.use_sinc = c.geom.interp == FD3_INTERP_P3M;
   ka.dyn = c.geom.dyn;
+  ka.sink = c.tab.sink;
   return ka;
 }
 

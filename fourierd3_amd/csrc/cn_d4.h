@@ -25,6 +25,7 @@ struct CnKsArgs {
   float recip[9];      // rows b_i of 2 pi cell^-T (Bohr^-1)
   int nx, ny, nz, M1, nS, NK, KT, use_sinc;
   const CellDyn* dyn = nullptr;  // when set, recip and inv_vol are read from it
+  float2* sink = nullptr;        // 256 values written by idle lanes (split x kernels), never read
 };
 
 struct CnKsPlan {

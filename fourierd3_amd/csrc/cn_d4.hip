@@ -307,6 +307,179 @@ __global__ void __launch_bounds__(256) k_kspace_cn_xs(CnKsArgs a) {
   }
 }
 
+// k_kspace_cn_xs software-pipelined over the block's KT tiles, as the
+// energy's k-space kernel (kspace_xsp_body, fft.hip): the per-pair G^CN
+// tables and the sinc factors are staged in LDS once per block, each lane's
+// 16 values of the next tile are loaded into registers right after the
+// forward pass has put the current tile into LDS, the loop is rotated so
+// that each wait for a prefetched tile comes after the previous tile's
+// stores, and every lane stores on every path (lanes of idle lines into
+// a.sink).  The backward variant reads the stored density of its four k
+// points straight into registers (the next tile's after this tile's k
+// points), not through LDS, so three blocks fit a CU.  LDS: the S lines,
+// the tables, sinc_x, sinc_y (256 each) and sinc_z (M1): cn_ksp_lds.
+template <bool BWD, int NS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) k_kspace_cnp_xs(CnKsArgs a) {
+  if (a.dyn) {
+    for (int i = 0; i < 9; ++i)
+      a.recip[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.dyn->recip_f[i])));
+    a.inv_vol = a.dyn->inv_volume;
+  }
+  constexpr int NK = kKsxNK, ls = 256, nlines = NS * NK, NP = NS * (NS + 1) / 2;
+  extern __shared__ float4 lds4[];
+  __shared__ double red[4 * 6];
+  float2* S = (float2*)lds4;
+  float4* tab = lds4 + nlines * ls / 2;
+  float* scx_l = (float*)(tab + 2 * NP * kGtabN);  // 256
+  float* scy_l = scx_l + 256;                       // 256
+  float* scz_l = scy_l + 256;                       // M1
+  for (int i = threadIdx.x; i < 2 * NP * kGtabN; i += blockDim.x) tab[i] = a.tab[i];
+  if (a.use_sinc) {
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+      scx_l[i] = a.scx[i];
+      scy_l[i] = a.scy[i];
+    }
+    for (int i = threadIdx.x; i < a.M1; i += blockDim.x) scz_l[i] = a.scz[i];
+  }
+  const int nyp = a.ny / NK, ntiles = nyp * a.M1;
+  const int xl = threadIdx.x >> 4, xj = threadIdx.x & 15;
+  const int xX = xl / NK, xkyi = xl - xX * NK;
+  const bool live = xl < nlines;
+  const int kx = threadIdx.x;  // column of this thread's four k points (rows 0..3 of the tile)
+  // first element of this lane's line in tile u (idle lanes: -1)
+  auto tile_lb = [&](int u) -> long long {
+    const int kyp = u % nyp, kz = u / nyp;
+    return live ? (((long long)xX * a.M1 + kz) * a.ny + kyp * NK + xkyi) * 256 : -1LL;
+  };
+  double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int u0 = blockIdx.x * a.KT, u1 = min(ntiles, u0 + a.KT);
+  float2 v[16];
+  float2 rr[BWD ? NK * NS : 1];  // backward: the stored density at the four k points, [kyi][X]
+  auto load_tile = [&](long long lb) {  // idle lanes read the spectrum's first values (ignored)
+    const float2* src = a.spec + (lb >= 0 ? lb + xj : xj);
+#pragma unroll
+    for (int n2 = 0; n2 < 16; ++n2) v[n2] = src[16 * n2];
+  };
+  auto load_r = [&](int u) {
+    const int kyp = u % nyp, kz = u / nyp;
+#pragma unroll
+    for (int kyi = 0; kyi < NK; ++kyi)
+#pragma unroll
+      for (int X = 0; X < NS; ++X)
+        rr[BWD ? kyi * NS + X : 0] = a.rho_hat[(((long long)X * a.M1 + kz) * a.ny + kyp * NK + kyi) * 256 + kx];
+  };
+  auto forward = [&](long long lb) {  // into LDS (and the forward variant's density out), then a barrier
+    if (live) {
+      float2 z[16];
+#pragma unroll
+      for (int n2 = 0; n2 < 16; ++n2) z[n2] = v[n2];
+      fft16r<-1>(z);  // over n2 -> k2: X[xj + 16 k2]
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) S[xl * ls + xj + 16 * k2] = z[k2];
+      if constexpr (!BWD) {
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) a.rho_hat[lb + xj + 16 * k2] = z[k2];
+      }
+    } else if constexpr (!BWD) {
+#pragma unroll
+      for (int k2 = 0; k2 < 16; ++k2) a.sink[xj + 16 * k2] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+  };
+  if (u0 < u1) {
+    long long lb0 = tile_lb(u0);
+    if constexpr (BWD) load_r(u0);
+    load_tile(lb0);
+    __syncthreads();  // the staged tables
+    forward(lb0);
+    long long lb1 = u0 + 1 < u1 ? tile_lb(u0 + 1) : -1LL;
+    if (u0 + 1 < u1) load_tile(lb1);
+    for (int u = u0; u < u1; ++u) {
+      asm volatile("" ::: "memory");  // the load is issued before the k points
+      const int kyp = u % nyp, kz = u / nyp, ky0 = kyp * NK;
+      const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
+      const float scxz = a.use_sinc ? scx_l[kx] * scz_l[kz] : 1.0f;
+#pragma unroll
+      for (int kyi = 0; kyi < NK; ++kyi) {
+        const int ky = ky0 + kyi;
+        const float mx = (float)(kx < 128 ? kx : kx - 256);
+        const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
+        const float mz = (float)kz;
+        const float k0v = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
+        const float k1v = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
+        const float k2v = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
+        const float kn = sqrtf(k0v * k0v + k1v * k1v + k2v * k2v);
+        const float isc = a.use_sinc ? 1.0f / fmaxf(scxz * scy_l[ky], 1e-10f) : 1.0f;
+        float gp[NP], dgp[NP];  // G^CN and dG^CN/dw of the species pairs (row order)
+        ks_gtab_lds<NP>(tab, a.tkinv, kn, gp, dgp);
+        float2 sx[NS], px[NS];
+#pragma unroll
+        for (int X = 0; X < NS; ++X) {
+          sx[X] = S[(X * NK + kyi) * ls + kx];
+          px[X] = make_float2(0.f, 0.f);
+        }
+        float vs = 0.f;  // sum_XY dG/dk Re(conj(S_X) rho_Y)
+#pragma unroll
+        for (int X = 0; X < NS; ++X)
+#pragma unroll
+          for (int Y = X; Y < NS; ++Y) {
+            const int pt = X * NS - X * (X - 1) / 2 + (Y - X);
+            const float g = gp[pt] * isc, dg = dgp[pt];
+            px[X].x += g * sx[Y].x; px[X].y += g * sx[Y].y;
+            if (Y != X) { px[Y].x += g * sx[X].x; px[Y].y += g * sx[X].y; }
+            if constexpr (BWD) {
+              const float2 rX = rr[kyi * NS + X], rY = rr[kyi * NS + Y];
+              float t2 = sx[X].x * rY.x + sx[X].y * rY.y;
+              if (Y != X) t2 += sx[Y].x * rX.x + sx[Y].y * rX.y;
+              vs += dg * t2;
+            }
+          }
+#pragma unroll
+        for (int X = 0; X < NS; ++X) S[(X * NK + kyi) * ls + kx] = px[X];
+        if (BWD && kn > 0.f) {
+          // d|k|/d eps_ab = -k_a k_b / |k| under the strain cell' = (1 + eps) cell
+          const double f = -(double)w * (double)(vs * a.tkinv * isc) / kn;
+          vir[0] += f * k0v * k0v; vir[1] += f * k1v * k1v; vir[2] += f * k2v * k2v;
+          vir[3] += f * k0v * k1v; vir[4] += f * k0v * k2v; vir[5] += f * k1v * k2v;
+        }
+        asm volatile("" ::: "memory");  // one k point at a time
+      }
+      if constexpr (BWD) {
+        if (u + 1 < u1) load_r(u + 1);
+      }
+      __syncthreads();
+      {
+        float2 z[16];
+        const float2* L = S + (live ? xl : 0) * ls;
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) z[k2] = L[xj + 16 * k2];  // k1 = xj
+        fft16r<1>(z);  // over k2 -> n2: Y[xj, n2]
+        float2* dst = live ? a.spec + lb0 + xj : a.sink + xj;
+#pragma unroll
+        for (int n2 = 0; n2 < 16; ++n2) dst[16 * n2] = z[n2];
+      }
+      if (u + 1 < u1) {
+        __syncthreads();  // every lane is done with the lines
+        forward(lb1);
+        lb0 = lb1;
+        lb1 = u + 2 < u1 ? tile_lb(u + 2) : -1LL;
+        if (u + 2 < u1) load_tile(lb1);
+      }
+    }
+  }
+  if (BWD) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) vir[i] *= a.inv_vol;
+    block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
+  }
+}
+
+// Dynamic LDS of k_kspace_cnp_xs.
+static size_t cn_ksp_lds(int nS, int M1) {
+  return (size_t)nS * kKsxNK * 256 * sizeof(float2) + (size_t)nS * (nS + 1) * kGtabN * sizeof(float4) +
+         (size_t)(512 + M1) * sizeof(float);
+}
+
 int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
   // two butterflies per thread (tiles of 2048 values: with 4 species and
   // 256-point lines a full tile of the unrolled FFT) where the species fit,
@@ -326,9 +499,9 @@ int cn_ks_plan(const FftPlan& p, int nS, CnKsPlan* kp) {
     kp->KT <<= 1;
   kp->blocks = (p.ny / NK / kp->KT) * p.M1;
   kp->xs = p.nx == 256 && p.ny == 256 && nS <= 4;
-  if (kp->xs) {  // tiles of kKsxNK ky rows, two per block
+  if (kp->xs) {  // tiles of kKsxNK ky rows, three per block (k_kspace_cnp_xs)
     kp->NK = kKsxNK;
-    kp->KT = 2;
+    kp->KT = 3;
     kp->blocks = (p.ny / kKsxNK * p.M1 + kp->KT - 1) / kp->KT;
   }
   const size_t tile = (size_t)nS * NK * (p.nx + 1) * sizeof(float2);
@@ -369,13 +542,13 @@ static int cn_ks_species(const CnKsArgs& b, const CnKsPlan& kp, dim3 grid, size_
 
 template <bool BWD>
 static int cn_ks_xs(const CnKsArgs& b, const CnKsPlan& kp, hipStream_t s) {
-  const size_t sh = (size_t)(BWD ? 2 : 1) * b.nS * kKsxNK * 256 * sizeof(float2);
+  const size_t sh = cn_ksp_lds(b.nS, b.M1);
   const dim3 grid(kp.blocks);
   switch (b.nS) {
-    case 1: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 1>), grid, dim3(256), sh, s, b); break;
-    case 2: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 2>), grid, dim3(256), sh, s, b); break;
-    case 3: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 3>), grid, dim3(256), sh, s, b); break;
-    default: hipLaunchKernelGGL((k_kspace_cn_xs<BWD, 4>), grid, dim3(256), sh, s, b); break;
+    case 1: hipLaunchKernelGGL((k_kspace_cnp_xs<BWD, 1>), grid, dim3(256), sh, s, b); break;
+    case 2: hipLaunchKernelGGL((k_kspace_cnp_xs<BWD, 2>), grid, dim3(256), sh, s, b); break;
+    case 3: hipLaunchKernelGGL((k_kspace_cnp_xs<BWD, 3>), grid, dim3(256), sh, s, b); break;
+    default: hipLaunchKernelGGL((k_kspace_cnp_xs<BWD, 4>), grid, dim3(256), sh, s, b); break;
   }
   return FD3_OK;
 }

@@ -939,31 +939,6 @@ __device__ inline KPoint kspace_point_isc(const KspaceArgs& a, const KsCell& c, 
   return p;
 }
 
-// G and dG/dw of the NP species pairs at |k| = kn from per-pair tables in LDS.
-template <int NP>
-__device__ __forceinline__ void ks_gtab_lds(const float4* tab, float gkinv, float kn, float (&gp)[NP],
-                                            float (&dgp)[NP]) {
-  const float fk = kn * gkinv;
-  const int i = min((int)fk, kGtabN - 1);
-  const float wk = fk - (float)i;
-  const float4* e = tab + 2 * i;
-  constexpr int PD = 2;  // pairs whose coefficients are in flight
-  float4 ca[PD + 1], cb[PD + 1];
-  static_for<PD>([&](auto dc) {
-    constexpr int d = decltype(dc)::value;
-    if constexpr (d < NP) { ca[d] = e[2 * kGtabN * d]; cb[d] = e[2 * kGtabN * d + 1]; }
-  });
-  static_for<NP>([&](auto pc) {
-    constexpr int pi = decltype(pc)::value, nxt = pi + PD;
-    if constexpr (nxt < NP) {
-      ca[nxt % (PD + 1)] = e[2 * kGtabN * nxt];
-      cb[nxt % (PD + 1)] = e[2 * kGtabN * nxt + 1];
-    }
-    gtab_eval(ca[pi % (PD + 1)], cb[pi % (PD + 1)], wk, &gp[pi], &dgp[pi]);
-    asm volatile("" : "+v"(gp[pi]), "+v"(dgp[pi])::"memory");
-  });
-}
-
 // The split-x k-space kernel without parameter gradients (nx = ny = 256,
 // the y passes run the first stage of the x FFT), software-pipelined over
 // the block's KT tiles: each thread loads its 16 values of the next tile

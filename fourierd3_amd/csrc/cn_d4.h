@@ -31,7 +31,7 @@ struct CnKsArgs {
 struct CnKsPlan {
   int NK, KT, blocks, maxb;
   bool reg;             // species count <= 4: the register-resident forward kernel
-  bool xs;              // nx = ny = 256, species <= 4: split x FFT (k_kspace_cn_xs), the
+  bool xs;              // nx = ny = 256, species <= 4: split x FFT (k_kspace_cnp_xs), the
                         // y passes run the x FFT's first stage
   size_t lds, lds_bwd;  // dynamic LDS of the forward and backward variants
 };

@@ -205,109 +205,14 @@ __global__ void __launch_bounds__(256) k_kspace_cn(CnKsArgs a) {
 // The same convolution with the split x FFT (nx = ny = 256, NS <= 4
 // species): the y passes have run the first stage of the x FFT, so a thread
 // (l, k1) of a tile (kKsxNK ky rows x NS species, line l = X * kKsxNK + kyi)
-// does one 16-point FFT over n2 per direction (ksx_load / ksx_inverse,
-// fft_reg.h); the NS values of a k point sit in registers.  The forward
-// variant also stores the x-transformed density (natural order, rho_hat),
-// the backward one loads it beside the spectra (R) for the strain
-// derivative.  Block b takes the KT consecutive tiles b KT .. b KT + KT - 1,
-// ky tile fastest.  LDS: S (and R) lines of 256 points.
+// does one 16-point FFT over n2 per direction; the NS values of a k point
+// sit in registers.  The forward variant also stores the x-transformed
+// density (natural order, rho_hat), the backward one reads it at its k
+// points for the strain derivative.  Block b takes the KT consecutive tiles
+// b KT .. b KT + KT - 1, ky tile fastest.
 constexpr int kKsxNK = 4;
-template <bool BWD, int NS>
-__global__ void __launch_bounds__(256) k_kspace_cn_xs(CnKsArgs a) {
-  if (a.dyn) {
-    for (int i = 0; i < 9; ++i)
-      a.recip[i] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(a.dyn->recip_f[i])));
-    a.inv_vol = a.dyn->inv_volume;
-  }
-  constexpr int NK = kKsxNK, ls = 256, nlines = NS * NK;
-  extern __shared__ float2 lds[];
-  __shared__ double red[4 * 6];
-  float2* S = lds;
-  float2* R = S + nlines * ls;  // backward only
-  const int nyp = a.ny / NK, ntiles = nyp * a.M1;
-  const int xl = threadIdx.x >> 4, xj = threadIdx.x & 15;
-  const int xX = xl / NK, xkyi = xl - xX * NK;
-  const bool live = xl < nlines;
-  double vir[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  const int u0 = blockIdx.x * a.KT, u1 = min(ntiles, u0 + a.KT);
-  for (int u = u0; u < u1; ++u) {
-    const int kyp = u % nyp, kz = u / nyp, ky0 = kyp * NK;
-    const float w = (kz == 0 || kz == a.M1 - 1) ? 1.0f : 2.0f;
-    const long long lb = (((long long)xX * a.M1 + kz) * a.ny + ky0 + xkyi) * 256;
-    float2 v[16];
-    ksx_load(v, a.spec, live, lb + xj);
-    __syncthreads();  // the previous tile is done with S and R
-    if (live) {
-      fft16r<-1>(v);  // over n2 -> k2: X[xj + 16 k2]
-#pragma unroll
-      for (int k2 = 0; k2 < 16; ++k2) S[xl * ls + xj + 16 * k2] = v[k2];
-      if constexpr (!BWD) {
-#pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) a.rho_hat[lb + xj + 16 * k2] = v[k2];
-      } else {
-#pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) R[xl * ls + xj + 16 * k2] = a.rho_hat[lb + xj + 16 * k2];
-      }
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < NK * 256; t += blockDim.x) {
-      const int kyi = t >> 8, kx = t & 255, ky = ky0 + kyi;
-      const float mx = (float)(kx < 128 ? kx : kx - 256);
-      const float my = (float)(ky < (a.ny + 1) / 2 ? ky : ky - a.ny);
-      const float mz = (float)kz;
-      const float k0v = mx * a.recip[0] + my * a.recip[3] + mz * a.recip[6];
-      const float k1v = mx * a.recip[1] + my * a.recip[4] + mz * a.recip[7];
-      const float k2v = mx * a.recip[2] + my * a.recip[5] + mz * a.recip[8];
-      const float kn = sqrtf(k0v * k0v + k1v * k1v + k2v * k2v);
-      const float isc = sinc_prod(a, kx, ky, kz);
-      const float fk = kn * a.tkinv;
-      const int ti = min((int)fk, kGtabN - 1);
-      const float tw = fk - (float)ti;
-      const float4* te = a.tab + 2 * ti;
-      float2 sx[NS], px[NS], rx[BWD ? NS : 1];
-#pragma unroll
-      for (int X = 0; X < NS; ++X) {
-        sx[X] = S[(X * NK + kyi) * ls + kx];
-        if constexpr (BWD) rx[X] = R[(X * NK + kyi) * ls + kx];
-        px[X] = make_float2(0.f, 0.f);
-      }
-      float vs = 0.f;  // sum_XY dG/dk Re(conj(S_X) rho_Y)
-#pragma unroll
-      for (int X = 0; X < NS; ++X)
-#pragma unroll
-        for (int Y = X; Y < NS; ++Y) {
-          const int pt = X * NS - X * (X - 1) / 2 + (Y - X);
-          float g, dg;
-          gtab_eval(te[2 * kGtabN * pt], te[2 * kGtabN * pt + 1], tw, &g, &dg);
-          g *= isc;
-          px[X].x += g * sx[Y].x; px[X].y += g * sx[Y].y;
-          if (Y != X) { px[Y].x += g * sx[X].x; px[Y].y += g * sx[X].y; }
-          if constexpr (BWD) {
-            float t2 = sx[X].x * rx[Y].x + sx[X].y * rx[Y].y;
-            if (Y != X) t2 += sx[Y].x * rx[X].x + sx[Y].y * rx[X].y;
-            vs += dg * t2;
-          }
-        }
-#pragma unroll
-      for (int X = 0; X < NS; ++X) S[(X * NK + kyi) * ls + kx] = px[X];
-      if (BWD && kn > 0.f) {
-        // d|k|/d eps_ab = -k_a k_b / |k| under the strain cell' = (1 + eps) cell
-        const double f = -(double)w * (double)(vs * a.tkinv * isc) / kn;
-        vir[0] += f * k0v * k0v; vir[1] += f * k1v * k1v; vir[2] += f * k2v * k2v;
-        vir[3] += f * k0v * k1v; vir[4] += f * k0v * k2v; vir[5] += f * k1v * k2v;
-      }
-    }
-    __syncthreads();
-    ksx_inverse(S, ls, a.spec, live, lb + xj);
-  }
-  if (BWD) {
-#pragma unroll
-    for (int i = 0; i < 6; ++i) vir[i] *= a.inv_vol;
-    block_sum_store<6>(vir, red, a.partials + (size_t)blockIdx.x * 8);
-  }
-}
 
-// k_kspace_cn_xs software-pipelined over the block's KT tiles, as the
+// The kernel is software-pipelined over the block's KT tiles, as the
 // energy's k-space kernel (kspace_xsp_body, fft.hip): the per-pair G^CN
 // tables and the sinc factors are staged in LDS once per block, each lane's
 // 16 values of the next tile are loaded into registers right after the

@@ -428,6 +428,55 @@ def test_flat_box_split_x_fft_matches_oracle(method, order, params):
     assert ok
 
 
+@pytest.mark.parametrize("case", ["sio2", "hcno_rank3"])
+def test_pipelined_kspace_tiles_match_oracle(case):
+    """The split-x k-space kernel without parameter gradients runs several
+    tiles per block with the next tile prefetched (three at the 256^3
+    headline mesh).  On the 256 x 256 x 32 flat box the plan gives one tile
+    per block, so the tile count is forced through the tuning table: 2, 3
+    (2176 tiles: a partial last block) and 5; SiO2 (2 species, rank 3: 12
+    of the 16 lines live) and H/C/N/O at rank 3 (4 species: the second rank
+    tile holds one rank, its other lines idle).  Energy, forces and stress
+    against the oracle for every tile count."""
+    _say("pipelined k-space tiles", case)
+    if case == "sio2":
+        b, tol = W.sio2_box(120, seed=15), 1.0
+    else:
+        b, tol = W.hcno_box(160, seed=6), 0.5
+    stretch = np.diag([8.0, 8.0, 1.0])
+    box = W.strained(W.Box(numbers=b.numbers, positions=b.positions @ stretch, cell=b.cell @ stretch), 0.05, seed=8)
+    calc = FourierD3ASECalculator(device=_dev(), c6tol=tol)
+    calc._build_model(_atoms(box))
+    m = calc._model
+    ns = [int(v) for v in m.ns_mesh]
+    assert ns[0] == 256 and ns[1] == 256
+    assert case != "hcno_rank3" or (m.n_species == 4 and m.n_rank == 3)
+    o = OracleFourierD3(box.numbers, box.cell, c6tol=tol)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    pos = torch.tensor(box.positions, dtype=torch.float64, device=_dev())
+    vol_b = abs(np.linalg.det(box.cell)) * ANGSTROM_TO_BOHR ** 3
+    key = ((C.c_int32 * 3)(*ns), int(m.n_species), int(m.n_rank))
+    try:
+        for kt in (2, 3, 5):
+            L.check(L.lib().fd3_tune_clear())
+            L.check(L.lib().fd3_tune_set(key[0], key[1], key[2], (C.c_int32 * 4)(0, 0, 0, kt)))
+            got = (C.c_int32 * 4)()
+            L.check(L.lib().fd3_plan_tiles(key[0], key[1], key[2], got))
+            assert got[3] == kt
+            res = calc.evaluate(pos, box.cell, need_grad=True)
+            torch.cuda.synchronize()
+            e = float(res["energy"][0]) * HARTREE_TO_EV
+            f = -res["grad_pos"].cpu().numpy() * HARTREE_TO_EV
+            st = res["virial"].cpu().numpy() * HARTREE_TO_EV / vol_b * ANGSTROM_TO_BOHR ** 3
+            de, df, ds = _errors(e, f, st, e_ref, f_ref, s_ref, box.n_atoms)
+            print(f"[errors] {case} {ns} rank {m.n_rank}, {kt} k-space tiles per block: E {de:.3e} F {df:.3e} "
+                  f"S {ds:.3e}", flush=True)
+            assert de < E_TOL and df < F_TOL and ds < S_TOL
+    finally:
+        L.check(L.lib().fd3_tune_clear())
+        L._load_tuning(L.lib())
+
+
 @pytest.mark.parametrize("case,method,order", [("sio2", "spme", 5), ("hcno", "spme", 5), ("hcno", "pme", 6)])
 def test_flat_box_split_x_fft_d4_matches_oracle(case, method, order):
     """cnfunc='d4' on boxes stretched 8x along x and y (256 x 256 x 32 mesh):

@@ -582,7 +582,9 @@ int spread_r2c_z(const FusedArgs& a, const FftPlan& p, const FftTables& t, float
 
 // accA / accG of each atom from the gather's partial sums: p x nodes, two
 // y slots, r channels (column order in, slot order out).  A fixed order of
-// additions, no atomics.
+// additions, no atomics.  ORD > 0: p = ORD, and the 2 ORD partials of a
+// channel are loaded together (one memory wait per channel, not per partial).
+template <int ORD>
 __device__ __forceinline__ void gather_sum_body(const BlockInfo& bk, const FusedArgs& a) {
   const int64_t ai = (int64_t)bk.id.x * blockDim.x + threadIdx.x;
   if (ai >= a.n) return;
@@ -591,9 +593,19 @@ __device__ __forceinline__ void gather_sum_body(const BlockInfo& bk, const Fused
   float g0 = 0.f, g1 = 0.f, g2 = 0.f;
   for (int q = 0; q < r; ++q) {
     float A = 0.f;
-    for (int k = 0; k < nk; ++k) {
-      const float4 v = pp[((size_t)k * r + q) * a.n];
-      A += v.x; g0 += v.y; g1 += v.z; g2 += v.w;
+    if constexpr (ORD > 0) {
+      float4 v[2 * ORD];
+#pragma unroll
+      for (int k = 0; k < 2 * ORD; ++k) v[k] = pp[((size_t)k * r + q) * a.n];
+#pragma unroll
+      for (int k = 0; k < 2 * ORD; ++k) {
+        A += v[k].x; g0 += v[k].y; g1 += v[k].z; g2 += v[k].w;
+      }
+    } else {
+      for (int k = 0; k < nk; ++k) {
+        const float4 v = pp[((size_t)k * r + q) * a.n];
+        A += v.x; g0 += v.y; g1 += v.z; g2 += v.w;
+      }
     }
     a.cb.accA[(size_t)s * r + q] = A;
   }
@@ -622,7 +634,9 @@ int c2r_z_gather(const FusedArgs& a, const FftPlan& p, const FftTables& t, const
     rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 5, 2>, NT>("k_c2r_z_gather", grid, sh, s, za);
   else rc = launch<ZPassArgs, c2r_z_gather_body<kZLogGather, 0, 2>, NT>("k_c2r_z_gather", grid, sh, s, za);
   if (rc) return rc;
-  if (a.n > 0) return launch<FusedArgs, gather_sum_body, 256>("k_gather_sum", dim3(cdiv(a.n, 256)), 0, s, a);
+  if (a.n > 0 && a.order == 5)
+    return launch<FusedArgs, gather_sum_body<5>, 256>("k_gather_sum", dim3(cdiv(a.n, 256)), 0, s, a);
+  if (a.n > 0) return launch<FusedArgs, gather_sum_body<0>, 256>("k_gather_sum", dim3(cdiv(a.n, 256)), 0, s, a);
   return FD3_OK;
 }
 

@@ -492,14 +492,29 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
           if (BWD) cg[e] = a.dEdcn[t];
         }
         __syncthreads();
+        // single pass: the home atoms are the candidates of the centre
+        // neighbour bin (zero shift), in home order from slot hc
+        const int hc = single ? bcnt[(nbr - 1) >> 1] + (h0 - hs) : 0;
         for (int h = wv; h < nh; h += NW) {  // home atom h0 + h: wave-uniform
           const int sidx = h0 + h;
-          const double4 pd = c.pos4[sidx];
-          const double hxd = pd.x - org.x, hyd = pd.y - org.y, hzd = pd.z - org.z;
-          const float px = (float)hxd, py = (float)hyd, pz = (float)hzd;
-          const double lx = hxd - (double)px, ly = hyd - (double)py, lz = hzd - (double)pz;
-          const double ri = (double)a.rcov[c.spec_s[sidx]];
-          const double gs = BWD ? a.dEdcn[sidx] : 0.0;
+          float px, py, pz;
+          double lx, ly, lz, ri, gs;
+          if (single) {
+            const float4 q = cpos[hc + h], ql = clo[hc + h];
+            px = q.x; py = q.y; pz = q.z;
+            lx = ql.x; ly = ql.y; lz = ql.z;
+            ri = (double)q.w;
+            gs = BWD ? cg[hc + h] : 0.0;
+          } else {
+            const double4 pd = c.pos4[sidx];
+            const double hxd = pd.x - org.x, hyd = pd.y - org.y, hzd = pd.z - org.z;
+            px = (float)hxd; py = (float)hyd; pz = (float)hzd;
+            lx = (double)(float)(hxd - (double)px);
+            ly = (double)(float)(hyd - (double)py);
+            lz = (double)(float)(hzd - (double)pz);
+            ri = (double)a.rcov[c.spec_s[sidx]];
+            gs = BWD ? a.dEdcn[sidx] : 0.0;
+          }
           double s0 = 0.0, s1 = 0.0, s2 = 0.0;
           for (int e0 = 0; e0 < nc; e0 += kCnStep * 64) {
             // in-range candidates of [e0, e0 + 512) -> wl[wv][0 .. cnt)
@@ -508,15 +523,15 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
 #pragma unroll
             for (int it = 0; it < kCnStep; ++it) {
               if (it >= nit) break;  // uniform
+              // a lane past the end reads the last candidate and never hits
               const int e = e0 + it * 64 + lane;
-              bool hit = false;
-              if (e < nc) {
-                const float4 q = cpos[e];
-                const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
-                hit = vx * vx + vy * vy + vz * vz < rc2;  // includes the atom itself at r = 0
-              }
-              const unsigned long long bal = __ballot(hit);
-              if (hit) wl[wv][cnt + __popcll(bal & ((1ull << lane) - 1ull))] = (unsigned short)e;
+              const float4 q = cpos[min(e, nc - 1)];
+              const float vx = q.x - px, vy = q.y - py, vz = q.z - pz;
+              const bool hit = (vx * vx + vy * vy + vz * vz < rc2) & (e < nc);  // includes the atom itself at r = 0
+              const unsigned long long bal = __builtin_amdgcn_ballot_w64(hit);
+              const unsigned below =
+                  __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+              if (hit) wl[wv][cnt + below] = (unsigned short)e;
               cnt += __popcll(bal);
             }
             __builtin_amdgcn_wave_barrier();  // the list is written and read by lanes of one wave

@@ -341,7 +341,7 @@ __device__ __forceinline__ void fill_sorted_body(const BlockInfo& bk, const Fill
 // (candidates) are staged in LDS with their lattice shift applied, so every
 // candidate is read from HBM once per home bin instead of once per home atom.
 // Each wave then takes home atoms in turn: its 64 lanes test |r| < cutoff
-// for up to 512 candidates (a few instructions each), compact the hits into a
+// for up to 640 candidates (a few instructions each), compact the hits into a
 // list in LDS with a ballot, and evaluate the CN term on that list with all
 // lanes busy; the home atom's sums stay in registers until the wave moves to
 // the next atom.  (Evaluating the term inside the distance loop would run it
@@ -359,7 +359,10 @@ __device__ __forceinline__ void fill_sorted_body(const BlockInfo& bk, const Fill
 constexpr int kCnCapSmall = 640, kCnCapLarge = 1536;
 constexpr int kNbrMax = 343;   // neighbour bins of a single-pass plan (reach <= 3)
 constexpr int kCnHomes = 128;  // home atoms per pass
-constexpr int kCnStep = 8;     // ballot steps (of 64 candidates) per hit list
+// ballot steps (of 64 candidates) per hit list: kCnCapSmall / 64, so the
+// candidates of a single pass form one list (a second list of a few
+// leftover candidates would cost a whole FP64 term step for a few lanes)
+constexpr int kCnStep = kCnCapSmall / 64;
 
 struct NbrBin {
   int wb;            // wrapped bin index
@@ -440,9 +443,20 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
       bcnt[k + 1] = c.bstart[nb.wb + 1] - lo;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-      bcnt[0] = 0;
-      for (int k = 0; k < nbr; ++k) bcnt[k + 1] += bcnt[k];
+    if (threadIdx.x < 64) {  // wave 0: prefix sums of the counts, 64 bins at a time
+      int carry = 0;
+      for (int k0 = 0; k0 < nbr; k0 += 64) {
+        const int k = k0 + lane;
+        int v = k < nbr ? bcnt[k + 1] : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+          const int u = __shfl_up(v, o, 64);
+          if (lane >= o) v += u;
+        }
+        if (k < nbr) bcnt[k + 1] = carry + v;
+        carry += __shfl(v, 63, 64);
+      }
+      if (lane == 0) bcnt[0] = 0;
     }
     __syncthreads();
     single = bcnt[nbr] <= kCnCap;
@@ -517,7 +531,7 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
           }
           double s0 = 0.0, s1 = 0.0, s2 = 0.0;
           for (int e0 = 0; e0 < nc; e0 += kCnStep * 64) {
-            // in-range candidates of [e0, e0 + 512) -> wl[wv][0 .. cnt)
+            // in-range candidates of [e0, e0 + 640) -> wl[wv][0 .. cnt)
             int cnt = 0;
             const int nit = min(kCnStep, (nc - e0 + 63) >> 6);  // steps holding candidates
 #pragma unroll

@@ -350,13 +350,15 @@ __device__ __forceinline__ void fill_sorted_body(const BlockInfo& bk, const Fill
 // Backward: dE/dr_s -= sum_t (g_s + g_t) f'(r) v / r,
 //           virial  += sum_t g_s f'(r) v v^T / r        (directed pairs s -> t)
 // with g = dE/dCN (src/fourierd3/core.py:348-386 and its adjoint).
-// Candidates staged per pass: CAP is a template argument, 640 when the
-// neighbour bins of a home bin hold fewer than ~600 on average (large boxes:
-// 27 bins of ~20 atoms at 1 g/cm^3 H/C/N/O), which keeps the kernel's LDS
-// within five blocks per CU, and 1536 for small boxes with many images per
-// bin.  Home bins whose candidates overflow CAP take one neighbour bin per
-// pass instead (correct for any count, slower).
-constexpr int kCnCapSmall = 640, kCnCapLarge = 1536;
+// Candidates staged per pass: CAP = 640 (27 neighbour bins of ~20 atoms at
+// 1 g/cm^3 H/C/N/O fit one pass; LDS for five blocks per CU).  The
+// candidates of a home bin form one list, neighbour bin after neighbour bin;
+// a longer list (small boxes with many images per bin, dense regions) is
+// staged in passes of CAP candidates.  Plans with more than kNbrMax
+// neighbour bins stage one neighbour bin at a time instead.  (A second
+// variant staging up to 1536 candidates at once ran two blocks per CU and
+// was slower on every measured workload.)
+constexpr int kCnCapSmall = 640;
 constexpr int kNbrMax = 343;   // neighbour bins of a single-pass plan (reach <= 3)
 constexpr int kCnHomes = 128;  // home atoms per pass
 // ballot steps (of 64 candidates) per hit list: kCnCapSmall / 64, so the
@@ -433,7 +435,7 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
   // origin of the block's float coordinates: differences stay within a few
   // bin widths, so float keeps ~1e-7 relative precision on pair vectors
   const double4 org = c.pos4[hs];
-  bool single = false;
+  bool concat = false, single = false;  // one list; a list that fits one pass
   if (nbr <= kNbrMax) {
     for (int k = threadIdx.x; k < nbr; k += blockDim.x) {
       const NbrBin nb = nbr_bin(c, b, k);
@@ -459,6 +461,7 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
       if (lane == 0) bcnt[0] = 0;
     }
     __syncthreads();
+    concat = true;
     single = bcnt[nbr] <= kCnCap;
   }
   const float rc2 = (float)c.rc2;
@@ -467,12 +470,13 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
     const int nh = min(kCnHomes, he - h0);
     for (int h = threadIdx.x; h < nh; h += blockDim.x)
       for (int d = 0; d < (BWD ? 3 : 1); ++d) hacc[h][d] = 0.0;
-    // candidate passes: the whole list at once, or one neighbour bin (chunk) at a time
-    const int npass_bins = single ? 1 : nbr;
+    // candidate passes of up to kCnCap: over the whole list, or over one
+    // neighbour bin at a time
+    const int npass_bins = concat ? 1 : nbr;
     for (int kb = 0; kb < npass_bins; ++kb) {
       int count;
       NbrBin fixed;
-      if (single) {
+      if (concat) {
         count = bcnt[nbr];
       } else {
         fixed = nbr_bin(c, b, kb);
@@ -484,16 +488,17 @@ __device__ __forceinline__ void cn_bins_body(const BlockInfo& bk, const CnBinsAr
         for (int e = threadIdx.x; e < nc; e += blockDim.x) {
           double sh[3];
           int t;
-          if (single) {
-            int lo = 0, hi = nbr - 1;  // the neighbour bin k with bcnt[k] <= e < bcnt[k + 1]
+          if (concat) {
+            const int eg = off0 + e;   // position in the list
+            int lo = 0, hi = nbr - 1;  // the neighbour bin k with bcnt[k] <= eg < bcnt[k + 1]
             while (lo < hi) {
               const int mid = (lo + hi + 1) >> 1;
-              if (bcnt[mid] <= e) lo = mid; else hi = mid - 1;
+              if (bcnt[mid] <= eg) lo = mid; else hi = mid - 1;
             }
             const int ps = nsh[lo];
             const int Sx = (ps >> 16) - 128, Sy = ((ps >> 8) & 255) - 128, Sz = (ps & 255) - 128;
             for (int d = 0; d < 3; ++d) sh[d] = Sx * c.cell[d] + Sy * c.cell[3 + d] + Sz * c.cell[6 + d];
-            t = nb0[lo] + (e - bcnt[lo]);
+            t = nb0[lo] + (eg - bcnt[lo]);
           } else {
             for (int d = 0; d < 3; ++d) sh[d] = fixed.sh[d];
             t = c.bstart[fixed.wb] + off0 + e;
@@ -833,27 +838,17 @@ int cell_build(const CellGrid& g, const CellBuffers& b, int64_t n, const void* p
 
 int cn_blocks(int64_t n) { return (int)std::max<int64_t>(1, cdiv(n, 256)); }
 
-// Mean candidates of a home bin: atoms per bin times the neighbour bins.
-static bool cn_small_cap(const CellArgs& c, int nbins) {
-  const double nbr = (2.0 * c.sr[0] + 1) * (2.0 * c.sr[1] + 1) * (2.0 * c.sr[2] + 1);
-  return (double)c.n / std::max(nbins, 1) * nbr <= 0.95 * kCnCapSmall;
-}
-
 int launch_cn_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   if (c.n <= 0) return FD3_OK;
   ProfScope _ps("k_cn_cells", s);
   const CnBinsArgs ca{c, a};
-  if (cn_small_cap(c, nbins))
-    return launch<CnBinsArgs, cn_bins_body<false, kCnCapSmall>, 256, 4>("k_cn_cells", dim3(nbins), 0, s, ca);
-  return launch<CnBinsArgs, cn_bins_body<false, kCnCapLarge>, 256, 4>("k_cn_cells", dim3(nbins), 0, s, ca);
+  return launch<CnBinsArgs, cn_bins_body<false, kCnCapSmall>, 256, 4>("k_cn_cells", dim3(nbins), 0, s, ca);
 }
 
 int launch_cn_bwd_cells(const CellArgs& c, const CnArgs& a, int nbins, hipStream_t s) {
   ProfScope _ps("k_cn_bwd_cells", s);
   const CnBinsArgs ca{c, a};
-  if (cn_small_cap(c, nbins))
-    return launch<CnBinsArgs, cn_bins_body<true, kCnCapSmall>, 256, 4>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
-  return launch<CnBinsArgs, cn_bins_body<true, kCnCapLarge>, 256, 4>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
+  return launch<CnBinsArgs, cn_bins_body<true, kCnCapSmall>, 256, 4>("k_cn_bwd_cells", dim3(nbins), 0, s, ca);
 }
 
 int launch_cn_edges(const EdgeArgs& e, const CnArgs& a, const double4* pos4, const int32_t* spec, hipStream_t s) {

@@ -182,6 +182,60 @@ def test_calculator_matches_oracle(case):
     assert de < E_TOL and df < F_TOL and ds < S_TOL
 
 
+def _cluster_box(seed=13):
+    """30 A cube: a dense carbon cluster (600 atoms in a 12 A cube, 0.35 per
+    A^3) in a sparse H/C/N/O gas (900 atoms), every pair >= 1 A apart."""
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(seed)
+    L0 = 30.0
+    blob = W._rsa_cubic(600, 12.0, 1.0, rng) + 9.0
+    pts = [blob]
+    n_gas = 900
+    while sum(len(p) for p in pts) < 600 + n_gas:
+        have = np.concatenate(pts)
+        cand = rng.random((4096, 3)) * L0
+        d, _ = cKDTree(have, boxsize=L0).query(cand)
+        cand = cand[d >= 1.0]
+        keep = []
+        for c in cand:  # mutual separation of the accepted candidates
+            if all(np.linalg.norm((c - k + L0 / 2) % L0 - L0 / 2) >= 1.0 for k in keep):
+                keep.append(c)
+            if len(keep) == 600 + n_gas - len(have):
+                break
+        pts.append(np.array(keep).reshape(-1, 3))
+    pos = np.concatenate(pts)
+    gas = rng.choice([1, 6, 7, 8], size=n_gas, p=[0.5, 0.25, 0.075, 0.175])
+    numbers = np.concatenate([np.full(600, 6), gas])
+    return W.Box(numbers=numbers, positions=pos, cell=np.eye(3) * L0)
+
+
+def test_dense_cluster_takes_several_cn_passes():
+    """The CN cell kernels stage a home bin's candidates (the atoms of its 27
+    neighbour bins) in LDS in passes of 640.  In a dense cluster in a sparse
+    gas (324 candidates per home bin on average) the home bin at the cluster
+    holds more than 640, so it takes two passes over its list while the
+    others take one.  Energy, forces and stress against the FP64 oracle."""
+    _say("dense cluster, CN passes over 640 candidates")
+    box = _cluster_box()
+    calc = FourierD3ASECalculator(device=_dev())
+    # cell-list bins along each axis: floor(L / r_cut) = 5 of 6 A, reach 1
+    nb = int(np.floor(box.cell[0, 0] / calc.r_cut))
+    assert nb == 5
+    idx = np.floor(box.positions / (box.cell[0, 0] / nb)).astype(int) % nb
+    cnt = np.zeros((nb, nb, nb), int)
+    np.add.at(cnt, (idx[:, 0], idx[:, 1], idx[:, 2]), 1)
+    cand = sum(np.roll(cnt, (a, b, c), axis=(0, 1, 2)) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1))
+    assert box.n_atoms / nb ** 3 * 27 < 640 < cand.max(), (box.n_atoms / nb ** 3 * 27, cand.max())
+    calc.calculate(_atoms(box))
+    res = calc.results
+    o = OracleFourierD3(box.numbers, box.cell)
+    e_ref, f_ref, s_ref = o.calculate(box.positions, box.cell)
+    de, df, ds = _errors(res["energy"], res["forces"], res["stress"], e_ref, f_ref, s_ref, box.n_atoms)
+    print(f"[errors] cluster: max candidates {cand.max()} in {int((cand > 640).sum())} bins; "
+          f"E {de:.3e} eV/atom  F {df:.3e} eV/A  S {ds:.3e} eV/A^3", flush=True)
+    assert de < E_TOL and df < F_TOL and ds < S_TOL
+
+
 def test_heavy_element_triclinic_box_with_param_gradients():
     """A 2000-atom cut of the differentiation workload: Z uniform on 1..36
     (36 species: the generic k-space kernel), cell strained by up to 10 %, atoms

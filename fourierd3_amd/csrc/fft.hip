@@ -281,7 +281,7 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
     float2 v[16];
     const float2* src = in + yline + j;
 #pragma unroll
-    for (int n1 = 0; n1 < 16; ++n1) v[n1] = src[16 * n1];
+    for (int n1 = 0; n1 < 16; ++n1) v[n1] = FD3_LD_NT(src + 16 * n1);  // streaming: read once
     fft16r<-1>(v);  // n2 = j
     ks2_twiddle<-1>(v, tws, j);
     float2* L = S + l * kKs2Ls;
@@ -308,7 +308,7 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
   } else {
     float2 w[16];
 #pragma unroll
-    for (int it = 0; it < 16; ++it) w[it] = in[row0 + (size_t)16 * it * nx];
+    for (int it = 0; it < 16; ++it) w[it] = FD3_LD_NT(in + row0 + (size_t)16 * it * nx);  // streaming
 #pragma unroll
     for (int it = 0; it < 16; ++it) S[y2_tile<NL>(ty + 16 * it, tl)] = w[it];
     __syncthreads();  // T filled, twiddles staged
@@ -332,7 +332,7 @@ __device__ __forceinline__ void c2c_y2_body(const BlockInfo& bk, const YPassArgs
     fft16r<1>(v);  // x[j + 16 n1]
     float2* dst = out + yline + j;
 #pragma unroll
-    for (int n1 = 0; n1 < 16; ++n1) dst[16 * n1] = v[n1];
+    for (int n1 = 0; n1 < 16; ++n1) FD3_ST_NT(dst + 16 * n1, v[n1]);
   }
 }
 
@@ -1057,7 +1057,7 @@ __device__ __forceinline__ void kspace_xsp_body(const BlockInfo& bk, const Kspac
         fft16r<1>(z);  // over k2 -> n2: Y[xj, n2]
         float2* dst = e0 >= 0 ? a.spec + e0 : a.sink + xj;
 #pragma unroll
-        for (int n2 = 0; n2 < 16; ++n2) dst[16 * n2] = z[n2];
+        for (int n2 = 0; n2 < 16; ++n2) FD3_ST_NT(dst + 16 * n2, z[n2]);  // streaming
       }
       if (u + 1 < u1) {
         __syncthreads();  // every lane is done with the lines

@@ -308,15 +308,16 @@ __device__ __forceinline__ void z2_forward_store(float2* lds, const float2* tws,
       const float2 zk = z[k2];
       const int k = k1 + 8 * k2;
       if (k2 == 0 && k1 == 0) {
-        dst[l] = make_float2(zk.x + zk.y, 0.0f);
-        dst[(size_t)128 * ny + l] = make_float2(zk.x - zk.y, 0.0f);
+        // streaming stores (the y pass reads each value once)
+        FD3_ST_NT(dst + l, make_float2(zk.x + zk.y, 0.0f));
+        FD3_ST_NT(dst + (size_t)128 * ny + l, make_float2(zk.x - zk.y, 0.0f));
         return;
       }
       // X_k = E_k + W256^k O_k, E_k = (Z_k + conj Z_{M-k}) / 2, O_k = -i (Z_k - conj Z_{M-k}) / 2
       const float2 E = make_float2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
       const float2 O = make_float2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
       const float2 T = cmul(tw_mul<32, k2, -1>(w1), O);
-      dst[(size_t)k * ny + l] = make_float2(E.x + T.x, E.y + T.y);
+      FD3_ST_NT(dst + (size_t)k * ny + l, make_float2(E.x + T.x, E.y + T.y));
     });
   }
 }

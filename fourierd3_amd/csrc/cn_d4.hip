@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
       for (int k2 = 0; k2 < 16; ++k2) S[xl * ls + xj + 16 * k2] = z[k2];
       if constexpr (!BWD) {
 #pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) a.rho_hat[lb + xj + 16 * k2] = z[k2];
+        for (int k2 = 0; k2 < 16; ++k2) FD3_ST_NT(a.rho_hat + lb + xj + 16 * k2, z[k2]);  // read back much later
       }
     } else if constexpr (!BWD) {
 #pragma unroll

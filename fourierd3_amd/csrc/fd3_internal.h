@@ -19,8 +19,9 @@ constexpr int kMaxRank = 32;  // ranks of the C6 decomposition the kernels take
 // (float2).  Macros: through an inline function the store lost its
 // nontemporal flag.  The mesh passes of the 256^2 plans use them where
 // measured faster: the spread's stores and the forward y pass's loads of the
-// z-pass spectrum, the inverse y pass's loads and stores, and the split x
-// k-space kernel's stores (the forward y pass then reads part of its input
+// z-pass spectrum, the inverse y pass's loads and stores, the split x
+// k-space kernel's stores, and the D4 CN forward's stores of the density
+// that its backward reads (the forward y pass then reads part of its input
 // from the Infinity Cache).  Nontemporal loads in the k-space kernel and the
 // gather, and stores in the forward y pass, measured slower.
 #define FD3_LD_NT(p) __builtin_bit_cast(float2, __builtin_nontemporal_load((const double*)(p)))

@@ -149,7 +149,7 @@ def tune_box(box, c6tol: float, reps: int = 5, log=print) -> dict:
         elif g == 1:
             cands = [[best[0], v, best[2], best[3]] for v in _pow2_upto(ns[0])]
         else:
-            cands = [[best[0], best[1], rt, kt] for rt in range(1, min(r, 8) + 1) for kt in (1, 2, 3, 4, 8)]
+            cands = [[best[0], best[1], rt, kt] for rt in range(1, min(r, 8) + 1) for kt in (1, 2, 3, 4, 5, 6, 8)]
         scored = []
         for c in cands:
             ms = measure(c, g)

@@ -1272,11 +1272,11 @@ int fft_plan_make(const int32_t ns[3], int nS, int r, FftPlan* p) {
   // partial-sum reduction 0.26 -> 0.06 ms at 4 tiles, 1.26 / 0.04 ms at 8)
   if (current_recorder())
     while (KT < kBatchKT && (p->ny / NK) % (2 * KT) == 0) KT <<= 1;
-  // the split x kernel takes any tile count per block (1-D grid): three
-  // tiles (measured: 222 us against 229 / 231 / 261 us for 2 / 4 / 1 at the
-  // 100k-atom box, 256^3, 4 species, rank 2), fewer when that would leave
-  // fewer than ~3k blocks
-  if (p->xs) KT = std::max(1, std::min(3, (int)((size_t)p->ny / 2 * p->M1 * zb / 3072)));
+  // the split x kernel takes any tile count per block (1-D grid): five
+  // tiles (measured with the nontemporal y passes beside it: 210 us against
+  // 229 / 223 / 217 / 211 us for 2 / 3 / 4 / 6 at the 100k-atom box, 256^3,
+  // 4 species, rank 2), fewer when that would leave fewer than ~3k blocks
+  if (p->xs) KT = std::max(1, std::min(5, (int)((size_t)p->ny / 2 * p->M1 * zb / 3072)));
   if (tuned && p->xs && tune[3] >= 1 && tune[3] <= 64) KT = tune[3];
   else if (tuned && pow2(tune[3]) && tune[3] <= 16 && (p->ny / NK) % tune[3] == 0) KT = tune[3];
   p->KT = KT;

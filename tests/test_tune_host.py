@@ -23,8 +23,8 @@ def test_default_tiles(clean_table):
     ty, tx, rt, kt = A.plan_tiles(NS, 4, 2)
     # z lines: M = 128 complex per line, 2048 values per block; y lines: 8192 per block
     assert (ty, tx) == (16, 32)
-    # the split x k-space kernel (nx = ny = 256) takes three tiles per block
-    assert rt == 2 and kt == 3
+    # the split x k-space kernel (nx = ny = 256) takes five tiles per block
+    assert rt == 2 and kt == 5
     # other meshes: 1, 2 or 4 ky tiles per block
     assert A.plan_tiles([128, 128, 128], 4, 2)[3] in (1, 2, 4)
     # four species: at most 8 spectra per thread -> rank tile 2
